@@ -1,0 +1,306 @@
+/*
+ * fp3d.h -- C-ABI of libfp3d_hip.so: the MI355X (gfx950) kernels of the
+ * full-potential nonlinear solve on unstructured P1 tetrahedra.
+ *
+ * Conventions (every entry point):
+ *   - returns int status: 0 = ok, FP3D_EINVAL = bad argument, FP3D_EHIP = a
+ *     HIP runtime error, FP3D_ELIMIT = a documented limit exceeded (e.g.
+ *     more than 256 colours).  fp3d_last_error() returns the message of the
+ *     last failure on the calling thread.
+ *   - array arguments of the device entry points are DEVICE pointers (HBM),
+ *     owned by the caller; outputs are overwritten in place (the reference's
+ *     "preallocated output" convention); nothing is allocated on the device
+ *     except where a function says so.
+ *   - `stream` is a hipStream_t passed as void* (0 = the null stream).
+ *   - connectivity is int32 (elements (n_tets,4), face neighbours (n_tets,4),
+ *     upstream ids, CSR column indices, elem_to_csr (n_tets,16)); the CSR
+ *     row pointer is int32 as well (nnz < 2^31 for every supported mesh).
+ *   - floating point state is float64; the *_f32 variants take float32
+ *     matrix values (vectors stay float64).
+ *   - `elem_index` (nullable) maps a launch position k to the element id
+ *     e = elem_index[k]; NULL means e = k (colour-major storage).
+ *
+ * Reference interface replaced by each function is cited as
+ * pyfp3d/<file>:<line> of swayli94/UP3D.
+ */
+#ifndef FP3D_H
+#define FP3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP3D_OK 0
+#define FP3D_EINVAL (-1)
+#define FP3D_EHIP (-2)
+#define FP3D_ELIMIT (-3)
+
+/* ---------------------------------------------------------------- misc */
+const char *fp3d_last_error(void);
+int fp3d_version(void);
+/* number of HIP devices visible (0 on a machine without a GPU) */
+int fp3d_device_count(int *count);
+int fp3d_stream_sync(void *stream);
+
+/* ------------------------------------------------- physics (elementwise)
+ * pyfp3d/physics/isentropic.py:235-258 density_field,
+ * :292-314 limit_q2_field, :317-334 mach_squared_field,
+ * :366-380 density_derivative_wrt_q_sq_field. */
+int fp3d_density_field(int64_t n, const double *q2, double m_inf, double gamma,
+                       double *out, void *stream);
+int fp3d_mach_squared_field(int64_t n, const double *q2, double m_inf,
+                            double gamma, double *out, void *stream);
+/* q2cap = q2_at_mach(m_cap, m_inf, gamma) computed by the host (+inf at
+ * m_inf == 0, pyfp3d/physics/isentropic.py:261-289). */
+int fp3d_limit_q2_field(int64_t n, const double *q2, double q2cap, double *out,
+                        void *stream);
+int fp3d_density_derivative_field(int64_t n, const double *q2, double m_inf,
+                                  double gamma, double *out, void *stream);
+
+/* ------------------------------------------------------ element sweeps */
+/* grad(phi)_e = sum_i phi[elem[e,i]] B[e,i,:], q2_e = |grad|^2.
+ * pyfp3d/kernels/gradient.py:32-64 element_velocity_q2. */
+int fp3d_element_velocity_q2(int64_t n_tets, const int32_t *elements,
+                             const double *B, const double *phi,
+                             double *grad_out, double *q2_out, void *stream);
+
+/* Fused state sweep of NewtonWorkspace.eval_residual
+ * (pyfp3d/solve/newton.py:345-351): grad, q2n = q2/u2, q2l = min(q2n, q2cap),
+ * rho = density(q2l), lim = (q2l == q2n).  lim_out may be NULL. */
+int fp3d_element_state(int64_t n_tets, const int32_t *elements, const double *B,
+                       const double *phi, double u2, double m_inf, double gamma,
+                       double q2cap, double *grad_out, double *q2n_out,
+                       double *q2l_out, double *rho_out, uint8_t *lim_out,
+                       void *stream);
+
+/* ------------------------------------------------------------ upwinding */
+/* Multi-hop upstream walk u(e); pyfp3d/kernels/upwind.py:68-125
+ * upstream_elements (face_neighbors int32 (n,4), -1 = boundary). */
+int fp3d_upstream_elements(int64_t n_tets, const int32_t *face_neighbors,
+                           const double *centroids, const double *nodes,
+                           const int32_t *elements, const double *grad,
+                           int max_hops, int32_t *out, void *stream);
+/* pyfp3d/kernels/upwind.py:456-500 rho_tilde_sweep. */
+int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,
+                         const int32_t *upstream, double m_inf, double m_crit,
+                         double C, double gamma, double rho_floor,
+                         double *nu_out, double *rho_tilde_out, void *stream);
+/* Walk + sweep fused in one launch (UpwindOperator.rho_tilde walk mode,
+ * pyfp3d/kernels/upwind.py:811-815); upstream_out may be NULL. */
+int fp3d_upwind_rho_tilde(int64_t n_tets, const int32_t *face_neighbors,
+                          const double *centroids, const double *nodes,
+                          const int32_t *elements, const double *grad,
+                          const double *q2, const double *rho, int max_hops,
+                          double m_inf, double m_crit, double C, double gamma,
+                          double rho_floor, int32_t *upstream_out,
+                          double *nu_out, double *rho_tilde_out, void *stream);
+/* pyfp3d/kernels/upwind.py:503-541 classify_upwind_branches. */
+int fp3d_classify_upwind_branches(int64_t n, const double *q2, const double *rho,
+                                  const int32_t *upstream, double m_inf,
+                                  double m_crit, double C, double gamma,
+                                  double rho_floor, int8_t *branch_out,
+                                  void *stream);
+/* pyfp3d/kernels/upwind.py:544-593 rho_tilde_frozen_sweep. */
+int fp3d_rho_tilde_frozen_sweep(int64_t n, const double *q2, const double *rho,
+                                const int32_t *upstream, const int8_t *branch,
+                                double m_inf, double m_crit, double C,
+                                double gamma, double rho_floor, double *nu_out,
+                                double *rho_tilde_out, void *stream);
+/* pyfp3d/kernels/upwind.py:596-643 rho_tilde_frozen_sensitivities_sweep. */
+int fp3d_rho_tilde_frozen_sensitivities(int64_t n, const double *q2,
+                                        const double *rho,
+                                        const int32_t *upstream,
+                                        const int8_t *branch, double m_inf,
+                                        double m_crit, double C, double gamma,
+                                        double rho_floor, double *se_out,
+                                        double *su_out, void *stream);
+/* pyfp3d/kernels/upwind.py:646-730 rho_tilde_sensitivities_sweep. */
+int fp3d_rho_tilde_sensitivities(int64_t n, const double *q2, const double *rho,
+                                 const int32_t *upstream, double m_inf,
+                                 double m_crit, double C, double gamma,
+                                 double rho_floor, double *se_out,
+                                 double *su_out, void *stream);
+/* Monitors of UpwindOperator.rho_tilde (pyfp3d/kernels/upwind.py:816-818):
+ * out3 = {nu_max, n_nu_positive, n_rho_tilde_eq_floor} (doubles), computed
+ * by a deterministic two-pass reduction; `work` is device scratch of
+ * fp3d_reduce_work_doubles() doubles. */
+int fp3d_upwind_monitors(int64_t n, const double *nu, const double *rho_tilde,
+                         double rho_floor, double *work, double *out3,
+                         void *stream);
+/* count of lim == 0 (speed-limited elements) into out1 (double) */
+int fp3d_count_limited(int64_t n, const uint8_t *lim, double *work,
+                       double *out1, void *stream);
+int fp3d_reduce_work_doubles(void);
+
+/* ------------------------------------------------- coloured assembly
+ * Colour classes are launched in order; within a class no two elements
+ * share a node, so every scatter is a plain read-modify-write (no atomics)
+ * and the per-node / per-nnz accumulation order is the colour sequence:
+ * bit-deterministic.  color_offsets is a HOST array (n_colors+1). */
+/* pyfp3d/kernels/residual.py:43-76 assemble_residual_colored (R zeroed). */
+int fp3d_assemble_residual_colored(int n_colors, const int64_t *color_offsets,
+                                   const int32_t *elem_index,
+                                   const int32_t *elements, const double *B,
+                                   const double *V, const double *rho_tilde,
+                                   const double *grad, int64_t n_nodes,
+                                   double *R, void *stream);
+/* pyfp3d/kernels/jacobian.py:105-129 assemble_matrix_data_colored
+ * (data must be zeroed by the caller, as in the reference). */
+int fp3d_assemble_matrix_colored(int n_colors, const int64_t *color_offsets,
+                                 const int32_t *elem_index,
+                                 const int32_t *elem_to_csr, const double *B,
+                                 const double *V, const double *rho_tilde,
+                                 double *data, void *stream);
+/* pyfp3d/kernels/jacobian.py:132-180 assemble_jacobian_data_colored; s_e is
+ * multiplied by lim[e] first when lim != NULL
+ * (pyfp3d/kernels/jacobian.py:323-324). */
+int fp3d_assemble_jacobian_colored(int n_colors, const int64_t *color_offsets,
+                                   const int32_t *elem_index,
+                                   const int32_t *elem_to_csr, const double *B,
+                                   const double *V, const double *rho_tilde,
+                                   const double *grad, const double *s_e,
+                                   const uint8_t *lim, double inv_u2,
+                                   double *data, void *stream);
+/* pyfp3d/kernels/jacobian.py:183-228 fill_term3_coo (rows/cols int32). */
+int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
+                        const int32_t *elements, const double *B,
+                        const double *V, const double *grad,
+                        const int32_t *upstream, const double *s_u,
+                        double inv_u2, int32_t *rows, int32_t *cols,
+                        double *vals, void *stream);
+
+/* ----------------------------- Term 3 as a device operator (no COO merge)
+ * The upstream coupling of the Newton Jacobian (pyfp3d/kernels/jacobian.py
+ * :183-228, 340-357; there a COO added to J with scipy every Newton step)
+ * applied matrix-free inside the Krylov matvec.  fp3d_term3_build selects
+ * the active elements (s_u != 0 after the limiter mask lim[upstream[e]]),
+ * stores their coefficients and a row-gather table built from the static
+ * node -> (4*element + local) incidence CSR (inc_ptr int32[n_nodes+1],
+ * inc_ent int32[4*n_tets], entries of a node in ascending element order),
+ * so fp3d_csr_spmv_term3 adds Term 3 row by row in a fixed order: no
+ * atomics, bitwise repeatable.  Rows with free_mask[row] == 0 (nullable)
+ * get no Term-3 entries.  `t3` is caller-owned device memory of
+ * fp3d_term3_workspace_bytes bytes; *n_active_out (host) receives the
+ * active count (the call synchronises the stream once to read it). */
+int64_t fp3d_term3_workspace_bytes(int64_t n_tets, int64_t n_nodes);
+int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
+                     const int32_t *inc_ptr, const int32_t *inc_ent,
+                     const double *B, const double *V, const double *grad,
+                     const int32_t *upstream, const double *s_u,
+                     const uint8_t *lim, const uint8_t *free_mask,
+                     double inv_u2, void *t3, int64_t *n_active_out,
+                     void *stream);
+
+/* ------------------------------------------------------ sparse / BLAS-1
+ * CSR SpMV, G lanes of a wave64 per row (G chosen from nnz / n_rows). */
+int fp3d_csr_spmv(int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                  const int32_t *indices, const double *data, const double *x,
+                  double *y, void *stream);
+int fp3d_csr_spmv_f32(int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                      const int32_t *indices, const float *data,
+                      const double *x, double *y, void *stream);
+/* y = A x + Term3 x (t3 from fp3d_term3_build, skipped when t3 == NULL or
+ * n_active == 0); rows with free_mask[row] == 0 give y[row] = x[row].
+ * data is float64 (is_f32 == 0) or float32 (is_f32 == 1). */
+int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                        const int32_t *indices, const void *data, int is_f32,
+                        const int32_t *elements, const int32_t *upstream,
+                        const void *t3, int64_t n_active, int64_t n_tets,
+                        const uint8_t *free_mask, const double *x, double *y,
+                        void *stream);
+/* Dirichlet principal-submatrix embedding (pyfp3d/solve/linear.py:20-58,
+ * apply_dirichlet) in place on CSR data: rows and columns of non-free dofs
+ * become identity, so Krylov vectors with zero Dirichlet entries see
+ * exactly A[free][:, free]; optional float32 copy into data32 (nullable). */
+int fp3d_csr_mask_dirichlet(int64_t n_rows, const int32_t *indptr,
+                            const int32_t *indices, const uint8_t *free_mask,
+                            double *data, float *data32, void *stream);
+int fp3d_csr_to_f32(int64_t nnz, const double *data, float *out, void *stream);
+int fp3d_fill(int64_t n, double value, double *x, void *stream);
+/* y = a x + b y */
+int fp3d_axpby(int64_t n, double a, const double *x, double b, double *y,
+               void *stream);
+/* y = a x */
+int fp3d_scale(int64_t n, double a, const double *x, double *y, void *stream);
+/* y = mask ? x : 0 */
+int fp3d_mask_copy(int64_t n, const uint8_t *mask, const double *x, double *y,
+                   void *stream);
+/* out[k] = v_k . w, k < nv (v_k = V + k*ld), deterministic two-pass
+ * reduction; work = fp3d_multi_dot_work_doubles(nv) doubles, out on device. */
+int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
+                   const double *w, double *work, double *out, void *stream);
+int fp3d_multi_dot_work_doubles(int nv);
+/* w += sign * sum_k h[k] v_k  (h on device, nv <= 256) */
+int fp3d_multi_axpy(int64_t n, int nv, const double *V, int64_t ld,
+                    const double *h, double sign, double *w, void *stream);
+/* out2 = {||x||_2, ||x||_inf} on device */
+int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
+               void *stream);
+
+/* ----------------------------------------------------- AMG V-cycle kernels
+ * Chebyshev smoothing (degree >= 1) of A x = b on D^-1 A with eigenvalue
+ * interval [lmin, lmax]; zero_init != 0 starts from x = 0 (x is written).
+ * r, d0, d1 are n-vectors of scratch. */
+int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
+                         const int32_t *indices, const void *data, int is_f32,
+                         const double *dinv, const double *b, double *x,
+                         double *r, double *d0, double *d1, int degree,
+                         double lmin, double lmax, int zero_init,
+                         void *stream);
+/* r = b - A x */
+int fp3d_amg_residual(int64_t n, int64_t nnz, const int32_t *indptr,
+                      const int32_t *indices, const void *data, int is_f32,
+                      const double *b, const double *x, double *r,
+                      void *stream);
+/* bc = R r (R = P^T stored as CSR, nc rows) */
+int fp3d_amg_restrict(int64_t nc, int64_t nnz, const int32_t *indptr,
+                      const int32_t *indices, const void *data, int is_f32,
+                      const double *r, double *bc, void *stream);
+/* x += P xc */
+int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
+                         const int32_t *indices, const void *data, int is_f32,
+                         const double *xc, double *x, void *stream);
+/* x = M b, M dense row-major n x n (the coarsest-level pseudo-inverse) */
+int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
+                      void *stream);
+
+/* --------------------------------------------------- host-side set-up
+ * (plain host pointers; run once per mesh, as in the reference) */
+/* pyfp3d/mesh/coloring.py:65-95 _greedy_coloring_kernel + :129-145
+ * color_partition_csr.  colors int32[n_tets]; *n_colors_out; returns
+ * FP3D_ELIMIT above 256 colours (the reference's ValueError). */
+int fp3d_host_greedy_coloring(int64_t n_tets, const int32_t *elements,
+                              int64_t n_nodes, int32_t *colors,
+                              int32_t *n_colors_out);
+/* pyfp3d/kernels/jacobian.py:60-74 build_csr_pattern: sorted, deduplicated
+ * node-node pattern.  Call with indices == NULL to get *nnz_out and
+ * indptr, then again with indices allocated. */
+int fp3d_host_csr_pattern(int64_t n_tets, const int32_t *elements,
+                          int64_t n_nodes, int32_t *indptr, int32_t *indices,
+                          int64_t *nnz_out);
+/* pyfp3d/kernels/jacobian.py:77-102 build_elem_to_csr (int32 (n,16)). */
+int fp3d_host_elem_to_csr(int64_t n_tets, const int32_t *elements,
+                          const int32_t *indptr, const int32_t *indices,
+                          int32_t *out);
+/* pyfp3d/mesh/metrics.py:207-278 build_face_adjacency (int32 (n,4)). */
+int fp3d_host_face_adjacency(int64_t n_tets, const int32_t *elements,
+                             int32_t *face_neighbors);
+/* pyfp3d/mesh/metrics.py:176-204 precompute_element_geometry; returns
+ * FP3D_EINVAL with the failing element in *bad_elem on a degenerate tet. */
+int fp3d_host_element_geometry(int64_t n_tets, const int32_t *elements,
+                               const double *nodes, double *B, double *V,
+                               int64_t *bad_elem);
+/* Smoothed-aggregation set-up helpers (pyfp3d/solve/linear.py:61-81 uses
+ * pyamg.smoothed_aggregation_solver): standard aggregation on the pattern
+ * of A (symmetric strength, theta = 0).  agg int32[n] (-1 = isolated);
+ * returns the aggregate count in *n_agg. */
+int fp3d_host_standard_aggregation(int64_t n, const int32_t *indptr,
+                                   const int32_t *indices, const double *data,
+                                   int32_t *agg, int64_t *n_agg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FP3D_H */

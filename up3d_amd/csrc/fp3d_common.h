@@ -1,0 +1,48 @@
+// Internal helpers shared by the HIP translation units of libfp3d_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/fp3d.h"
+
+namespace fp3d {
+
+void set_error(const std::string &msg);
+
+// Launch geometry: 256 work-items = 4 wave64 wavefronts per workgroup.
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int per_block = kBlock) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+int check_launch(const char *what);
+
+// Blocks used by the deterministic two-pass reductions; the partial-sum
+// layout depends only on n, never on timing, so results are bitwise
+// repeatable.
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace fp3d
+
+#define FP3D_REQUIRE(cond, msg)        \
+  do {                                 \
+    if (!(cond)) {                     \
+      fp3d::set_error(msg);            \
+      return FP3D_EINVAL;              \
+    }                                  \
+  } while (0)
+
+#define FP3D_HIP(call)                                                   \
+  do {                                                                   \
+    hipError_t _e = (call);                                              \
+    if (_e != hipSuccess) {                                              \
+      fp3d::set_error(std::string(#call) + ": " + hipGetErrorString(_e)); \
+      return FP3D_EHIP;                                                  \
+    }                                                                    \
+  } while (0)
+
+#define FP3D_STREAM(s) ((hipStream_t)(s))

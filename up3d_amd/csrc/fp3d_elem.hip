@@ -1,0 +1,862 @@
+// Element kernels of the full-potential solve on P1 tetrahedra (gfx950).
+//
+// Every kernel here is HBM-bound integer/FP64 gather-scatter work: one
+// work-item per element, wave64, no MFMA.  Arithmetic is written in the
+// reference's evaluation order and the library is compiled with
+// -ffp-contract=off, so -- apart from pow() -- each element's numbers are
+// bit-identical to an IEEE restatement of the same formulas.  The coloured
+// scatters are launched colour by colour with plain read-modify-writes
+// (colour classes share no node), which fixes the accumulation order and
+// makes every assembly bitwise repeatable, with no atomics.
+#include <cmath>
+#include <mutex>
+
+#include "fp3d_common.h"
+
+namespace fp3d {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return FP3D_EHIP;
+  }
+  return FP3D_OK;
+}
+
+// ---------------------------------------------------------------- physics
+// pyfp3d/physics/isentropic.py:29-58 density_isentropic (vacuum floor 1e-10)
+__device__ __forceinline__ double d_density(double q2, double m, double g) {
+  double ex = 1.0 / (g - 1.0);
+  double base = 1.0 + 0.5 * (g - 1.0) * (m * m) * (1.0 - q2);
+  if (base < 1e-10) base = 1e-10;
+  return pow(base, ex);
+}
+// pyfp3d/physics/isentropic.py:79-97 mach_number_squared
+__device__ __forceinline__ double d_mach2(double q2, double m, double g) {
+  double num = q2 * (m * m);
+  double den = 1.0 + 0.5 * (g - 1.0) * (m * m) * (1.0 - q2);
+  return num / den;
+}
+// pyfp3d/physics/isentropic.py:163-180 density_derivative_wrt_q_sq
+__device__ __forceinline__ double d_drho(double q2, double m, double g) {
+  double rho = d_density(q2, m, g);
+  return -0.5 * (m * m) * pow(rho, 2.0 - g);
+}
+// pyfp3d/physics/isentropic.py:183-206 mach_squared_derivative_wrt_q_sq
+__device__ __forceinline__ double d_dmach2(double q2, double m, double g) {
+  double D = 1.0 + 0.5 * (g - 1.0) * (m * m) * (1.0 - q2);
+  return (m * m) * (1.0 + 0.5 * (g - 1.0) * (m * m)) / (D * D);
+}
+// nu switch C * max(0, 1 - Mc^2 / max(M^2, Mc^2))
+__device__ __forceinline__ double d_nu(double m2, double mc2, double C) {
+  double mm = m2 > mc2 ? m2 : mc2;
+  double v = 1.0 - mc2 / mm;
+  return C * (v > 0.0 ? v : 0.0);
+}
+
+__global__ void k_density_field(int64_t n, const double *q2, double m,
+                                double g, double *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d_density(q2[i], m, g);
+}
+__global__ void k_mach2_field(int64_t n, const double *q2, double m, double g,
+                              double *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d_mach2(q2[i], m, g);
+}
+__global__ void k_limit_q2(int64_t n, const double *q2, double cap,
+                           double *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double v = q2[i];
+    out[i] = v < cap ? v : cap;
+  }
+}
+__global__ void k_drho_field(int64_t n, const double *q2, double m, double g,
+                             double *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = d_drho(q2[i], m, g);
+}
+
+// ---------------------------------------------------------- element sweeps
+__device__ __forceinline__ void elem_grad(int64_t e, const int32_t *elements,
+                                          const double *B, const double *phi,
+                                          double &gx, double &gy, double &gz) {
+  const int4 nd = *reinterpret_cast<const int4 *>(elements + 4 * e);
+  const double *b = B + 12 * e;
+  double p0 = phi[nd.x], p1 = phi[nd.y], p2 = phi[nd.z], p3 = phi[nd.w];
+  gx = 0.0; gy = 0.0; gz = 0.0;
+  gx += p0 * b[0]; gy += p0 * b[1]; gz += p0 * b[2];
+  gx += p1 * b[3]; gy += p1 * b[4]; gz += p1 * b[5];
+  gx += p2 * b[6]; gy += p2 * b[7]; gz += p2 * b[8];
+  gx += p3 * b[9]; gy += p3 * b[10]; gz += p3 * b[11];
+}
+
+__global__ __launch_bounds__(kBlock) void k_velocity_q2(
+    int64_t n, const int32_t *__restrict__ elements,
+    const double *__restrict__ B, const double *__restrict__ phi,
+    double *__restrict__ grad, double *__restrict__ q2) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double gx, gy, gz;
+  elem_grad(e, elements, B, phi, gx, gy, gz);
+  grad[3 * e] = gx;
+  grad[3 * e + 1] = gy;
+  grad[3 * e + 2] = gz;
+  q2[e] = gx * gx + gy * gy + gz * gz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_element_state(
+    int64_t n, const int32_t *__restrict__ elements,
+    const double *__restrict__ B, const double *__restrict__ phi, double u2,
+    double m, double g, double cap, double *__restrict__ grad,
+    double *__restrict__ q2n_out, double *__restrict__ q2l_out,
+    double *__restrict__ rho_out, uint8_t *__restrict__ lim_out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double gx, gy, gz;
+  elem_grad(e, elements, B, phi, gx, gy, gz);
+  grad[3 * e] = gx;
+  grad[3 * e + 1] = gy;
+  grad[3 * e + 2] = gz;
+  double q2 = gx * gx + gy * gy + gz * gz;
+  double q2n = q2 / u2;
+  double q2l = q2n < cap ? q2n : cap;
+  q2n_out[e] = q2n;
+  q2l_out[e] = q2l;
+  rho_out[e] = d_density(q2l, m, g);
+  if (lim_out) lim_out[e] = (q2l == q2n) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- upwinding
+// pyfp3d/kernels/upwind.py:68-125: step to the face neighbour with the most
+// negative centroid displacement along grad[e] (strict <, first face wins a
+// tie) until the accumulated reach covers 0.8 of e's streamwise extent or
+// max_hops steps were taken; u(e) = e when no neighbour lies upstream.
+__device__ __forceinline__ int32_t walk_upstream(
+    int64_t e, const int32_t *__restrict__ fn, const double *__restrict__ cen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    double gx, double gy, double gz, int max_hops) {
+  double pmin = 1e300, pmax = -1e300;
+  for (int k = 0; k < 4; ++k) {
+    int32_t nd = elements[4 * e + k];
+    double p = nodes[3 * nd] * gx + nodes[3 * nd + 1] * gy + nodes[3 * nd + 2] * gz;
+    if (p < pmin) pmin = p;
+    if (p > pmax) pmax = p;
+  }
+  double target = 0.8 * (pmax - pmin);
+  int32_t cur = (int32_t)e;
+  double reach = 0.0;
+  int hops = 0;
+  while (hops < max_hops && reach < target) {
+    int32_t best = -1;
+    double best_disp = 0.0;
+    double cx = cen[3 * (int64_t)cur], cy = cen[3 * (int64_t)cur + 1],
+           cz = cen[3 * (int64_t)cur + 2];
+    for (int f = 0; f < 4; ++f) {
+      int32_t nb = fn[4 * (int64_t)cur + f];
+      if (nb >= 0) {
+        double d = (cen[3 * (int64_t)nb] - cx) * gx +
+                   (cen[3 * (int64_t)nb + 1] - cy) * gy +
+                   (cen[3 * (int64_t)nb + 2] - cz) * gz;
+        if (d < best_disp) {
+          best_disp = d;
+          best = nb;
+        }
+      }
+    }
+    if (best < 0) break;
+    cur = best;
+    reach += -best_disp;
+    hops += 1;
+  }
+  return cur;
+}
+
+__global__ __launch_bounds__(kBlock) void k_upstream(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ cen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, int max_hops, int32_t *__restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  out[e] = walk_upstream(e, fn, cen, nodes, elements, grad[3 * e],
+                         grad[3 * e + 1], grad[3 * e + 2], max_hops);
+}
+
+// pyfp3d/kernels/upwind.py:483-500
+__device__ __forceinline__ void sweep_one(int64_t e, int32_t u, const double *q2,
+                                          const double *rho, double m,
+                                          double mc2, double C, double g,
+                                          double floor_, double &nu_o,
+                                          double &rt_o) {
+  double m2 = d_mach2(q2[e], m, g);
+  double m2u = d_mach2(q2[u], m, g);
+  double nu_e = d_nu(m2, mc2, C);
+  double nu_u = d_nu(m2u, mc2, C);
+  double nu = nu_e >= nu_u ? nu_e : nu_u;
+  nu_o = nu;
+  double rt = rho[e] - nu * (rho[e] - rho[u]);
+  rt_o = rt > floor_ ? rt : floor_;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rho_tilde(
+    int64_t n, const double *__restrict__ q2, const double *__restrict__ rho,
+    const int32_t *__restrict__ up, double m, double mc2, double C, double g,
+    double floor_, double *__restrict__ nu_out, double *__restrict__ rt_out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double nu, rt;
+  sweep_one(e, up[e], q2, rho, m, mc2, C, g, floor_, nu, rt);
+  nu_out[e] = nu;
+  rt_out[e] = rt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_upwind_fused(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ cen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, const double *__restrict__ q2,
+    const double *__restrict__ rho, int max_hops, double m, double mc2,
+    double C, double g, double floor_, int32_t *__restrict__ up_out,
+    double *__restrict__ nu_out, double *__restrict__ rt_out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int32_t u = walk_upstream(e, fn, cen, nodes, elements, grad[3 * e],
+                            grad[3 * e + 1], grad[3 * e + 2], max_hops);
+  if (up_out) up_out[e] = u;
+  double nu, rt;
+  sweep_one(e, u, q2, rho, m, mc2, C, g, floor_, nu, rt);
+  nu_out[e] = nu;
+  rt_out[e] = rt;
+}
+
+// pyfp3d/kernels/upwind.py:503-541
+__global__ __launch_bounds__(kBlock) void k_classify(
+    int64_t n, const double *__restrict__ q2, const double *__restrict__ rho,
+    const int32_t *__restrict__ up, double m, double mc2, double C, double g,
+    double floor_, int8_t *__restrict__ br) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int32_t u = up[e];
+  if (u == e) {
+    br[e] = rho[e] <= floor_ ? 3 : 0;
+    return;
+  }
+  double nu_e = d_nu(d_mach2(q2[e], m, g), mc2, C);
+  double nu_u = d_nu(d_mach2(q2[u], m, g), mc2, C);
+  double nu = nu_e >= nu_u ? nu_e : nu_u;
+  double rt = rho[e] - nu * (rho[e] - rho[u]);
+  if (rt <= floor_) br[e] = 3;
+  else if (nu == 0.0) br[e] = 0;
+  else if (nu_e >= nu_u) br[e] = 1;
+  else br[e] = 2;
+}
+
+// pyfp3d/kernels/upwind.py:544-593
+__global__ __launch_bounds__(kBlock) void k_frozen(
+    int64_t n, const double *__restrict__ q2, const double *__restrict__ rho,
+    const int32_t *__restrict__ up, const int8_t *__restrict__ br, double m,
+    double mc2, double C, double g, double floor_, double *__restrict__ nu_out,
+    double *__restrict__ rt_out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int b = br[e];
+  if (b == 3) {
+    nu_out[e] = 0.0;
+    rt_out[e] = floor_;
+    return;
+  }
+  if (b == 0) {
+    nu_out[e] = 0.0;
+    rt_out[e] = rho[e];
+    return;
+  }
+  int32_t u = up[e];
+  double nu = (b == 1) ? d_nu(d_mach2(q2[e], m, g), mc2, C)
+                       : d_nu(d_mach2(q2[u], m, g), mc2, C);
+  nu_out[e] = nu;
+  rt_out[e] = rho[e] - nu * (rho[e] - rho[u]);
+}
+
+// dnu/dq2 on the active switch branch: C (Mc^2 / M^4) dM^2/dq2
+__device__ __forceinline__ double d_dnu(double q2, double m2, double m,
+                                        double mc2, double C, double g) {
+  return C * (mc2 / (m2 * m2)) * d_dmach2(q2, m, g);
+}
+
+// pyfp3d/kernels/upwind.py:596-643
+__global__ __launch_bounds__(kBlock) void k_frozen_sens(
+    int64_t n, const double *__restrict__ q2, const double *__restrict__ rho,
+    const int32_t *__restrict__ up, const int8_t *__restrict__ br, double m,
+    double mc2, double C, double g, double *__restrict__ se,
+    double *__restrict__ su) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int b = br[e];
+  if (b == 3) {
+    se[e] = 0.0;
+    su[e] = 0.0;
+    return;
+  }
+  double drho_e = d_drho(q2[e], m, g);
+  if (b == 0) {
+    se[e] = drho_e;
+    su[e] = 0.0;
+    return;
+  }
+  int32_t u = up[e];
+  double drho_u = d_drho(q2[u], m, g);
+  double djump = rho[e] - rho[u];
+  if (b == 1) {
+    double m2 = d_mach2(q2[e], m, g);
+    double nu = d_nu(m2, mc2, C);
+    double dnu = m2 > mc2 ? d_dnu(q2[e], m2, m, mc2, C, g) : 0.0;
+    se[e] = drho_e * (1.0 - nu) - djump * dnu;
+    su[e] = nu * drho_u;
+  } else {
+    double m2u = d_mach2(q2[u], m, g);
+    double nu = d_nu(m2u, mc2, C);
+    double dnu = m2u > mc2 ? d_dnu(q2[u], m2u, m, mc2, C, g) : 0.0;
+    se[e] = drho_e * (1.0 - nu);
+    su[e] = nu * drho_u - djump * dnu;
+  }
+}
+
+// pyfp3d/kernels/upwind.py:646-730
+__global__ __launch_bounds__(kBlock) void k_sens(
+    int64_t n, const double *__restrict__ q2, const double *__restrict__ rho,
+    const int32_t *__restrict__ up, double m, double mc2, double C, double g,
+    double floor_, double *__restrict__ se, double *__restrict__ su) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int32_t u = up[e];
+  double rho_e = rho[e];
+  double drho_e = d_drho(q2[e], m, g);
+  if (u == e) {
+    if (rho_e <= floor_) {
+      se[e] = 0.0;
+      su[e] = 0.0;
+    } else {
+      se[e] = drho_e;
+      su[e] = 0.0;
+    }
+    return;
+  }
+  double m2 = d_mach2(q2[e], m, g);
+  double m2u = d_mach2(q2[u], m, g);
+  double nu_e = d_nu(m2, mc2, C);
+  double nu_u = d_nu(m2u, mc2, C);
+  double nu = nu_e >= nu_u ? nu_e : nu_u;
+  double rho_u = rho[u];
+  double rt = rho_e - nu * (rho_e - rho_u);
+  if (rt <= floor_) {
+    se[e] = 0.0;
+    su[e] = 0.0;
+    return;
+  }
+  if (nu == 0.0) {
+    se[e] = drho_e;
+    su[e] = 0.0;
+    return;
+  }
+  double drho_u = d_drho(q2[u], m, g);
+  double djump = rho_e - rho_u;
+  if (nu_e >= nu_u) {
+    double dnu = m2 > mc2 ? d_dnu(q2[e], m2, m, mc2, C, g) : 0.0;
+    se[e] = drho_e * (1.0 - nu) - djump * dnu;
+    su[e] = nu * drho_u;
+  } else {
+    double dnu = d_dnu(q2[u], m2u, m, mc2, C, g);
+    se[e] = drho_e * (1.0 - nu);
+    su[e] = nu * drho_u - djump * dnu;
+  }
+}
+
+// ------------------------------------------------- deterministic reductions
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// pass 1: per-block {max nu, count nu>0, count rt==floor} -> work[3*b+k]
+__global__ __launch_bounds__(kBlock) void k_monitors_p1(
+    int64_t n, const double *__restrict__ nu, const double *__restrict__ rt,
+    double floor_, double *__restrict__ work) {
+  __shared__ double s[3][kBlock / 64];
+  double vmax = -INFINITY, cpos = 0.0, cflr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = nu[i];
+    vmax = v > vmax ? v : vmax;
+    cpos += v > 0.0 ? 1.0 : 0.0;
+    cflr += rt[i] == floor_ ? 1.0 : 0.0;
+  }
+  vmax = wave_max(vmax);
+  cpos = wave_sum(cpos);
+  cflr = wave_sum(cflr);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s[0][w] = vmax;
+    s[1][w] = cpos;
+    s[2][w] = cflr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = s[0][0], b = s[1][0], c = s[2][0];
+    for (int k = 1; k < kBlock / 64; ++k) {
+      a = s[0][k] > a ? s[0][k] : a;
+      b += s[1][k];
+      c += s[2][k];
+    }
+    work[3 * blockIdx.x] = a;
+    work[3 * blockIdx.x + 1] = b;
+    work[3 * blockIdx.x + 2] = c;
+  }
+}
+__global__ void k_monitors_p2(int nb, const double *__restrict__ work,
+                              double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double a = -INFINITY, b = 0.0, c = 0.0;
+  for (int k = 0; k < nb; ++k) {
+    a = work[3 * k] > a ? work[3 * k] : a;
+    b += work[3 * k + 1];
+    c += work[3 * k + 2];
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+__global__ __launch_bounds__(kBlock) void k_count_zero_p1(
+    int64_t n, const uint8_t *__restrict__ m, double *__restrict__ work) {
+  __shared__ double s[kBlock / 64];
+  double c = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c += m[i] == 0 ? 1.0 : 0.0;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) t += s[k];
+    work[blockIdx.x] = t;
+  }
+}
+__global__ void k_sum_p2(int nb, const double *__restrict__ work,
+                         double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int k = 0; k < nb; ++k) t += work[k];
+  out[0] = t;
+}
+
+inline int reduce_blocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b > kReduceBlocks) b = kReduceBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ------------------------------------------------------ coloured assembly
+// pyfp3d/kernels/residual.py:64-76 (one launch per colour class)
+__global__ __launch_bounds__(kBlock) void k_residual_color(
+    int64_t k0, int64_t k1, const int32_t *__restrict__ eidx,
+    const int32_t *__restrict__ elements, const double *__restrict__ B,
+    const double *__restrict__ V, const double *__restrict__ rt,
+    const double *__restrict__ grad, double *__restrict__ R) {
+  int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= k1) return;
+  int64_t e = eidx ? (int64_t)eidx[k] : k;
+  double w = rt[e] * V[e];
+  double gx = grad[3 * e], gy = grad[3 * e + 1], gz = grad[3 * e + 2];
+  const int4 nd = *reinterpret_cast<const int4 *>(elements + 4 * e);
+  const double *b = B + 12 * e;
+  R[nd.x] += w * (gx * b[0] + gy * b[1] + gz * b[2]);
+  R[nd.y] += w * (gx * b[3] + gy * b[4] + gz * b[5]);
+  R[nd.z] += w * (gx * b[6] + gy * b[7] + gz * b[8]);
+  R[nd.w] += w * (gx * b[9] + gy * b[10] + gz * b[11]);
+}
+
+// pyfp3d/kernels/jacobian.py:117-129
+__global__ __launch_bounds__(kBlock) void k_matrix_color(
+    int64_t k0, int64_t k1, const int32_t *__restrict__ eidx,
+    const int32_t *__restrict__ e2c, const double *__restrict__ B,
+    const double *__restrict__ V, const double *__restrict__ rt,
+    double *__restrict__ data) {
+  int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= k1) return;
+  int64_t e = eidx ? (int64_t)eidx[k] : k;
+  double w = rt[e] * V[e];
+  double b[12];
+#pragma unroll
+  for (int t = 0; t < 12; ++t) b[t] = B[12 * e + t];
+  const int32_t *slot = e2c + 16 * e;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double bi0 = b[3 * i], bi1 = b[3 * i + 1], bi2 = b[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      data[slot[4 * i + j]] +=
+          w * (bi0 * b[3 * j] + bi1 * b[3 * j + 1] + bi2 * b[3 * j + 2]);
+  }
+}
+
+// pyfp3d/kernels/jacobian.py:146-180 (Term 1 kept as its own expression so
+// J == A bitwise where s_e == 0)
+__global__ __launch_bounds__(kBlock) void k_jacobian_color(
+    int64_t k0, int64_t k1, const int32_t *__restrict__ eidx,
+    const int32_t *__restrict__ e2c, const double *__restrict__ B,
+    const double *__restrict__ V, const double *__restrict__ rt,
+    const double *__restrict__ grad, const double *__restrict__ s_e,
+    const uint8_t *__restrict__ lim, double inv_u2,
+    double *__restrict__ data) {
+  int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= k1) return;
+  int64_t e = eidx ? (int64_t)eidx[k] : k;
+  double w1 = rt[e] * V[e];
+  double g0 = grad[3 * e], g1 = grad[3 * e + 1], g2 = grad[3 * e + 2];
+  double se = s_e[e];
+  if (lim) se = se * (lim[e] ? 1.0 : 0.0);
+  double w2 = 2.0 * inv_u2 * se * V[e];
+  double b[12];
+#pragma unroll
+  for (int t = 0; t < 12; ++t) b[t] = B[12 * e + t];
+  const int32_t *slot = e2c + 16 * e;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double bi0 = b[3 * i], bi1 = b[3 * i + 1], bi2 = b[3 * i + 2];
+    double gi = g0 * bi0 + g1 * bi1 + g2 * bi2;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double bj0 = b[3 * j], bj1 = b[3 * j + 1], bj2 = b[3 * j + 2];
+      double val = w1 * (bi0 * bj0 + bi1 * bj1 + bi2 * bj2);
+      if (w2 != 0.0) val += w2 * gi * (g0 * bj0 + g1 * bj1 + g2 * bj2);
+      data[slot[4 * i + j]] += val;
+    }
+  }
+}
+
+// pyfp3d/kernels/jacobian.py:197-228 (fixed slot slice [16a, 16a+16))
+__global__ __launch_bounds__(kBlock) void k_term3_coo(
+    int64_t n_active, const int32_t *__restrict__ active,
+    const int32_t *__restrict__ elements, const double *__restrict__ B,
+    const double *__restrict__ V, const double *__restrict__ grad,
+    const int32_t *__restrict__ up, const double *__restrict__ s_u,
+    double inv_u2, int32_t *__restrict__ rows, int32_t *__restrict__ cols,
+    double *__restrict__ vals) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_active) return;
+  int64_t e = active[a];
+  int64_t u = up[e];
+  double ge0 = grad[3 * e], ge1 = grad[3 * e + 1], ge2 = grad[3 * e + 2];
+  double gu0 = grad[3 * u], gu1 = grad[3 * u + 1], gu2 = grad[3 * u + 2];
+  double w = 2.0 * inv_u2 * s_u[e] * V[e];
+  int64_t base = 16 * a;
+  for (int i = 0; i < 4; ++i) {
+    double gi = ge0 * B[12 * e + 3 * i] + ge1 * B[12 * e + 3 * i + 1] +
+                ge2 * B[12 * e + 3 * i + 2];
+    int32_t ri = elements[4 * e + i];
+    double wi = w * gi;
+    for (int k = 0; k < 4; ++k) {
+      int64_t idx = base + 4 * i + k;
+      rows[idx] = ri;
+      cols[idx] = elements[4 * u + k];
+      vals[idx] = wi * (gu0 * B[12 * u + 3 * k] + gu1 * B[12 * u + 3 * k + 1] +
+                        gu2 * B[12 * u + 3 * k + 2]);
+    }
+  }
+}
+
+__global__ void k_fill(int64_t n, double v, double *x) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+}  // namespace fp3d
+
+using namespace fp3d;
+
+// ======================================================================
+// C-ABI
+// ======================================================================
+extern "C" {
+
+const char *fp3d_last_error(void) { return fp3d::g_err.c_str(); }
+int fp3d_version(void) { return 1; }
+
+int fp3d_device_count(int *count) {
+  FP3D_REQUIRE(count, "count is NULL");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return FP3D_OK;
+}
+
+int fp3d_stream_sync(void *stream) {
+  FP3D_HIP(hipStreamSynchronize(FP3D_STREAM(stream)));
+  return FP3D_OK;
+}
+
+int fp3d_reduce_work_doubles(void) { return 3 * kReduceBlocks; }
+
+#define ELEMWISE(kernel, n, ...)                                       \
+  do {                                                                 \
+    FP3D_REQUIRE(n >= 0, #kernel ": negative size");                   \
+    if (n == 0) return FP3D_OK;                                        \
+    hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kBlock), 0,     \
+                       FP3D_STREAM(stream), n, __VA_ARGS__);           \
+    return check_launch(#kernel);                                      \
+  } while (0)
+
+int fp3d_density_field(int64_t n, const double *q2, double m_inf, double gamma,
+                       double *out, void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && out), "density_field: NULL pointer");
+  ELEMWISE(k_density_field, n, q2, m_inf, gamma, out);
+}
+int fp3d_mach_squared_field(int64_t n, const double *q2, double m_inf,
+                            double gamma, double *out, void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && out), "mach_squared_field: NULL pointer");
+  ELEMWISE(k_mach2_field, n, q2, m_inf, gamma, out);
+}
+int fp3d_limit_q2_field(int64_t n, const double *q2, double q2cap, double *out,
+                        void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && out), "limit_q2_field: NULL pointer");
+  ELEMWISE(k_limit_q2, n, q2, q2cap, out);
+}
+int fp3d_density_derivative_field(int64_t n, const double *q2, double m_inf,
+                                  double gamma, double *out, void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && out), "density_derivative_field: NULL");
+  ELEMWISE(k_drho_field, n, q2, m_inf, gamma, out);
+}
+
+int fp3d_element_velocity_q2(int64_t n_tets, const int32_t *elements,
+                             const double *B, const double *phi,
+                             double *grad_out, double *q2_out, void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (elements && B && phi && grad_out && q2_out),
+               "element_velocity_q2: NULL pointer");
+  ELEMWISE(k_velocity_q2, n_tets, elements, B, phi, grad_out, q2_out);
+}
+
+int fp3d_element_state(int64_t n_tets, const int32_t *elements, const double *B,
+                       const double *phi, double u2, double m_inf, double gamma,
+                       double q2cap, double *grad_out, double *q2n_out,
+                       double *q2l_out, double *rho_out, uint8_t *lim_out,
+                       void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (elements && B && phi && grad_out && q2n_out &&
+                               q2l_out && rho_out),
+               "element_state: NULL pointer");
+  FP3D_REQUIRE(u2 > 0.0, "element_state: u_inf^2 must be > 0");
+  ELEMWISE(k_element_state, n_tets, elements, B, phi, u2, m_inf, gamma, q2cap,
+           grad_out, q2n_out, q2l_out, rho_out, lim_out);
+}
+
+int fp3d_upstream_elements(int64_t n_tets, const int32_t *face_neighbors,
+                           const double *centroids, const double *nodes,
+                           const int32_t *elements, const double *grad,
+                           int max_hops, int32_t *out, void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (face_neighbors && centroids && nodes &&
+                               elements && grad && out),
+               "upstream_elements: NULL pointer");
+  FP3D_REQUIRE(max_hops >= 0, "upstream_elements: max_hops < 0");
+  ELEMWISE(k_upstream, n_tets, face_neighbors, centroids, nodes, elements, grad,
+           max_hops, out);
+}
+
+int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,
+                         const int32_t *upstream, double m_inf, double m_crit,
+                         double C, double gamma, double rho_floor,
+                         double *nu_out, double *rho_tilde_out, void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && rho && upstream && nu_out && rho_tilde_out),
+               "rho_tilde_sweep: NULL pointer");
+  ELEMWISE(k_rho_tilde, n, q2, rho, upstream, m_inf, m_crit * m_crit, C, gamma,
+           rho_floor, nu_out, rho_tilde_out);
+}
+
+int fp3d_upwind_rho_tilde(int64_t n_tets, const int32_t *face_neighbors,
+                          const double *centroids, const double *nodes,
+                          const int32_t *elements, const double *grad,
+                          const double *q2, const double *rho, int max_hops,
+                          double m_inf, double m_crit, double C, double gamma,
+                          double rho_floor, int32_t *upstream_out,
+                          double *nu_out, double *rho_tilde_out, void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (face_neighbors && centroids && nodes &&
+                               elements && grad && q2 && rho && nu_out &&
+                               rho_tilde_out),
+               "upwind_rho_tilde: NULL pointer");
+  ELEMWISE(k_upwind_fused, n_tets, face_neighbors, centroids, nodes, elements,
+           grad, q2, rho, max_hops, m_inf, m_crit * m_crit, C, gamma,
+           rho_floor, upstream_out, nu_out, rho_tilde_out);
+}
+
+int fp3d_classify_upwind_branches(int64_t n, const double *q2, const double *rho,
+                                  const int32_t *upstream, double m_inf,
+                                  double m_crit, double C, double gamma,
+                                  double rho_floor, int8_t *branch_out,
+                                  void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && rho && upstream && branch_out),
+               "classify_upwind_branches: NULL pointer");
+  ELEMWISE(k_classify, n, q2, rho, upstream, m_inf, m_crit * m_crit, C, gamma,
+           rho_floor, branch_out);
+}
+
+int fp3d_rho_tilde_frozen_sweep(int64_t n, const double *q2, const double *rho,
+                                const int32_t *upstream, const int8_t *branch,
+                                double m_inf, double m_crit, double C,
+                                double gamma, double rho_floor, double *nu_out,
+                                double *rho_tilde_out, void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && rho && upstream && branch && nu_out &&
+                          rho_tilde_out),
+               "rho_tilde_frozen_sweep: NULL pointer");
+  ELEMWISE(k_frozen, n, q2, rho, upstream, branch, m_inf, m_crit * m_crit, C,
+           gamma, rho_floor, nu_out, rho_tilde_out);
+}
+
+int fp3d_rho_tilde_frozen_sensitivities(int64_t n, const double *q2,
+                                        const double *rho,
+                                        const int32_t *upstream,
+                                        const int8_t *branch, double m_inf,
+                                        double m_crit, double C, double gamma,
+                                        double rho_floor, double *se_out,
+                                        double *su_out, void *stream) {
+  (void)rho_floor;
+  FP3D_REQUIRE(n == 0 || (q2 && rho && upstream && branch && se_out && su_out),
+               "rho_tilde_frozen_sensitivities: NULL pointer");
+  ELEMWISE(k_frozen_sens, n, q2, rho, upstream, branch, m_inf,
+           m_crit * m_crit, C, gamma, se_out, su_out);
+}
+
+int fp3d_rho_tilde_sensitivities(int64_t n, const double *q2, const double *rho,
+                                 const int32_t *upstream, double m_inf,
+                                 double m_crit, double C, double gamma,
+                                 double rho_floor, double *se_out,
+                                 double *su_out, void *stream) {
+  FP3D_REQUIRE(n == 0 || (q2 && rho && upstream && se_out && su_out),
+               "rho_tilde_sensitivities: NULL pointer");
+  ELEMWISE(k_sens, n, q2, rho, upstream, m_inf, m_crit * m_crit, C, gamma,
+           rho_floor, se_out, su_out);
+}
+
+int fp3d_upwind_monitors(int64_t n, const double *nu, const double *rho_tilde,
+                         double rho_floor, double *work, double *out3,
+                         void *stream) {
+  FP3D_REQUIRE(nu && rho_tilde && work && out3, "upwind_monitors: NULL");
+  int nb = reduce_blocks(n);
+  hipLaunchKernelGGL(k_monitors_p1, dim3(nb), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, nu, rho_tilde, rho_floor, work);
+  hipLaunchKernelGGL(k_monitors_p2, dim3(1), dim3(64), 0, FP3D_STREAM(stream),
+                     nb, work, out3);
+  return check_launch("upwind_monitors");
+}
+
+int fp3d_count_limited(int64_t n, const uint8_t *lim, double *work,
+                       double *out1, void *stream) {
+  FP3D_REQUIRE(lim && work && out1, "count_limited: NULL");
+  int nb = reduce_blocks(n);
+  hipLaunchKernelGGL(k_count_zero_p1, dim3(nb), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, lim, work);
+  hipLaunchKernelGGL(k_sum_p2, dim3(1), dim3(64), 0, FP3D_STREAM(stream), nb,
+                     work, out1);
+  return check_launch("count_limited");
+}
+
+static int check_offsets(int n_colors, const int64_t *off) {
+  FP3D_REQUIRE(n_colors >= 0 && (n_colors == 0 || off),
+               "colour offsets missing");
+  for (int c = 0; c < n_colors; ++c)
+    FP3D_REQUIRE(off[c] >= 0 && off[c + 1] >= off[c],
+                 "colour offsets not non-decreasing");
+  return FP3D_OK;
+}
+
+int fp3d_assemble_residual_colored(int n_colors, const int64_t *color_offsets,
+                                   const int32_t *elem_index,
+                                   const int32_t *elements, const double *B,
+                                   const double *V, const double *rho_tilde,
+                                   const double *grad, int64_t n_nodes,
+                                   double *R, void *stream) {
+  int rc = check_offsets(n_colors, color_offsets);
+  if (rc) return rc;
+  FP3D_REQUIRE(R && n_nodes >= 0, "assemble_residual: R missing");
+  if (n_nodes > 0)
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n_nodes)), dim3(kBlock), 0,
+                       FP3D_STREAM(stream), n_nodes, 0.0, R);
+  for (int c = 0; c < n_colors; ++c) {
+    int64_t k0 = color_offsets[c], k1 = color_offsets[c + 1];
+    if (k1 == k0) continue;
+    hipLaunchKernelGGL(k_residual_color, dim3(grid_for(k1 - k0)), dim3(kBlock),
+                       0, FP3D_STREAM(stream), k0, k1, elem_index, elements, B,
+                       V, rho_tilde, grad, R);
+  }
+  return check_launch("assemble_residual_colored");
+}
+
+int fp3d_assemble_matrix_colored(int n_colors, const int64_t *color_offsets,
+                                 const int32_t *elem_index,
+                                 const int32_t *elem_to_csr, const double *B,
+                                 const double *V, const double *rho_tilde,
+                                 double *data, void *stream) {
+  int rc = check_offsets(n_colors, color_offsets);
+  if (rc) return rc;
+  FP3D_REQUIRE(elem_to_csr && B && V && rho_tilde && data,
+               "assemble_matrix: NULL pointer");
+  for (int c = 0; c < n_colors; ++c) {
+    int64_t k0 = color_offsets[c], k1 = color_offsets[c + 1];
+    if (k1 == k0) continue;
+    hipLaunchKernelGGL(k_matrix_color, dim3(grid_for(k1 - k0)), dim3(kBlock),
+                       0, FP3D_STREAM(stream), k0, k1, elem_index, elem_to_csr,
+                       B, V, rho_tilde, data);
+  }
+  return check_launch("assemble_matrix_colored");
+}
+
+int fp3d_assemble_jacobian_colored(int n_colors, const int64_t *color_offsets,
+                                   const int32_t *elem_index,
+                                   const int32_t *elem_to_csr, const double *B,
+                                   const double *V, const double *rho_tilde,
+                                   const double *grad, const double *s_e,
+                                   const uint8_t *lim, double inv_u2,
+                                   double *data, void *stream) {
+  int rc = check_offsets(n_colors, color_offsets);
+  if (rc) return rc;
+  FP3D_REQUIRE(elem_to_csr && B && V && rho_tilde && grad && s_e && data,
+               "assemble_jacobian: NULL pointer");
+  for (int c = 0; c < n_colors; ++c) {
+    int64_t k0 = color_offsets[c], k1 = color_offsets[c + 1];
+    if (k1 == k0) continue;
+    hipLaunchKernelGGL(k_jacobian_color, dim3(grid_for(k1 - k0)), dim3(kBlock),
+                       0, FP3D_STREAM(stream), k0, k1, elem_index, elem_to_csr,
+                       B, V, rho_tilde, grad, s_e, lim, inv_u2, data);
+  }
+  return check_launch("assemble_jacobian_colored");
+}
+
+int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
+                        const int32_t *elements, const double *B,
+                        const double *V, const double *grad,
+                        const int32_t *upstream, const double *s_u,
+                        double inv_u2, int32_t *rows, int32_t *cols,
+                        double *vals, void *stream) {
+  FP3D_REQUIRE(n_active == 0 || (active && elements && B && V && grad &&
+                                 upstream && s_u && rows && cols && vals),
+               "fill_term3_coo: NULL pointer");
+  ELEMWISE(k_term3_coo, n_active, active, elements, B, V, grad, upstream, s_u,
+           inv_u2, rows, cols, vals);
+}
+
+int fp3d_fill(int64_t n, double value, double *x, void *stream) {
+  FP3D_REQUIRE(n == 0 || x, "fill: NULL");
+  ELEMWISE(k_fill, n, value, x);
+}
+
+}  // extern "C"

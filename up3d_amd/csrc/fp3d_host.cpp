@@ -1,0 +1,294 @@
+// Host-side, once-per-mesh set-up of the full-potential solve: element
+// colouring, CSR pattern and scatter map, face adjacency, element geometry,
+// and the aggregation step of the smoothed-aggregation AMG set-up.
+// Plain C++ (no device code); exported through the same C-ABI as the
+// kernels.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fp3d.h"
+
+namespace fp3d {
+void set_error(const std::string &msg);
+}
+
+#define HREQ(cond, msg)      \
+  do {                       \
+    if (!(cond)) {           \
+      fp3d::set_error(msg);  \
+      return FP3D_EINVAL;    \
+    }                        \
+  } while (0)
+
+namespace {
+
+// node -> incident elements, each list in ascending element order
+// (pyfp3d/mesh/coloring.py:45-62 node_to_element_csr: stable argsort)
+void node_incidence(int64_t n_tets, const int32_t *el, int64_t n_nodes,
+                    std::vector<int64_t> &off, std::vector<int32_t> &lst) {
+  off.assign(n_nodes + 1, 0);
+  for (int64_t e = 0; e < n_tets; ++e)
+    for (int i = 0; i < 4; ++i) off[el[4 * e + i] + 1]++;
+  for (int64_t v = 0; v < n_nodes; ++v) off[v + 1] += off[v];
+  lst.resize(off[n_nodes]);
+  std::vector<int64_t> pos(off.begin(), off.end() - 1);
+  for (int64_t e = 0; e < n_tets; ++e)
+    for (int i = 0; i < 4; ++i) lst[pos[el[4 * e + i]]++] = (int32_t)e;
+}
+
+int check_elements(int64_t n_tets, const int32_t *el, int64_t n_nodes) {
+  HREQ(n_tets >= 0 && (n_tets == 0 || el), "elements missing");
+  for (int64_t k = 0; k < 4 * n_tets; ++k)
+    HREQ(el[k] >= 0 && el[k] < n_nodes, "element node id out of range");
+  return FP3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fp3d_host_greedy_coloring(int64_t n_tets, const int32_t *elements,
+                              int64_t n_nodes, int32_t *colors,
+                              int32_t *n_colors_out) {
+  int rc = check_elements(n_tets, elements, n_nodes);
+  if (rc) return rc;
+  HREQ(colors && n_colors_out, "colors output missing");
+  std::vector<int64_t> off;
+  std::vector<int32_t> lst;
+  node_incidence(n_tets, elements, n_nodes, off, lst);
+  const int kMax = 256;
+  int64_t mark[kMax];
+  for (int c = 0; c < kMax; ++c) mark[c] = -1;
+  for (int64_t e = 0; e < n_tets; ++e) colors[e] = -1;
+  int32_t ncol = 0;
+  for (int64_t e = 0; e < n_tets; ++e) {
+    for (int i = 0; i < 4; ++i) {
+      int32_t v = elements[4 * e + i];
+      for (int64_t k = off[v]; k < off[v + 1]; ++k) {
+        int32_t c = colors[lst[k]];
+        if (c >= 0) mark[c] = e;
+      }
+    }
+    int c = 0;
+    while (c < kMax && mark[c] == e) ++c;
+    if (c >= kMax) {
+      fp3d::set_error(
+          "greedy coloring exceeded 256 colors -- pathological mesh "
+          "connectivity (a node shared by >255 elements)");
+      return FP3D_ELIMIT;
+    }
+    colors[e] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  *n_colors_out = ncol;
+  return FP3D_OK;
+}
+
+int fp3d_host_csr_pattern(int64_t n_tets, const int32_t *elements,
+                          int64_t n_nodes, int32_t *indptr, int32_t *indices,
+                          int64_t *nnz_out) {
+  int rc = check_elements(n_tets, elements, n_nodes);
+  if (rc) return rc;
+  HREQ(indptr && nnz_out, "indptr/nnz output missing");
+  std::vector<int64_t> off;
+  std::vector<int32_t> lst;
+  node_incidence(n_tets, elements, n_nodes, off, lst);
+  std::vector<int32_t> row;
+  int64_t nnz = 0;
+  indptr[0] = 0;
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    row.clear();
+    for (int64_t k = off[v]; k < off[v + 1]; ++k)
+      for (int j = 0; j < 4; ++j) row.push_back(elements[4 * (int64_t)lst[k] + j]);
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    if (indices)
+      std::memcpy(indices + nnz, row.data(), row.size() * sizeof(int32_t));
+    nnz += (int64_t)row.size();
+    HREQ(nnz < INT32_MAX, "CSR pattern exceeds int32 nnz");
+    indptr[v + 1] = (int32_t)nnz;
+  }
+  *nnz_out = nnz;
+  return FP3D_OK;
+}
+
+int fp3d_host_elem_to_csr(int64_t n_tets, const int32_t *elements,
+                          const int32_t *indptr, const int32_t *indices,
+                          int32_t *out) {
+  HREQ(n_tets == 0 || (elements && indptr && indices && out),
+       "elem_to_csr: NULL pointer");
+  for (int64_t e = 0; e < n_tets; ++e)
+    for (int i = 0; i < 4; ++i) {
+      int32_t r = elements[4 * e + i];
+      const int32_t *lo = indices + indptr[r], *hi = indices + indptr[r + 1];
+      for (int j = 0; j < 4; ++j) {
+        int32_t c = elements[4 * e + j];
+        const int32_t *p = std::lower_bound(lo, hi, c);
+        HREQ(p != hi && *p == c, "elem_to_csr: entry missing from pattern");
+        out[16 * e + 4 * i + j] = (int32_t)(p - indices);
+      }
+    }
+  return FP3D_OK;
+}
+
+int fp3d_host_face_adjacency(int64_t n_tets, const int32_t *elements,
+                             int32_t *face_neighbors) {
+  HREQ(n_tets == 0 || (elements && face_neighbors), "face_adjacency: NULL");
+  static const int fd[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  struct Face {
+    int32_t a, b, c;
+    int64_t packed;  // e*4 + f, the order the reference visits faces in
+  };
+  std::vector<Face> faces((size_t)(4 * n_tets));
+  for (int64_t e = 0; e < n_tets; ++e)
+    for (int f = 0; f < 4; ++f) {
+      int32_t t[3] = {elements[4 * e + fd[f][0]], elements[4 * e + fd[f][1]],
+                      elements[4 * e + fd[f][2]]};
+      std::sort(t, t + 3);
+      faces[4 * e + f] = Face{t[0], t[1], t[2], 4 * e + f};
+      face_neighbors[4 * e + f] = -1;
+    }
+  std::sort(faces.begin(), faces.end(), [](const Face &x, const Face &y) {
+    if (x.a != y.a) return x.a < y.a;
+    if (x.b != y.b) return x.b < y.b;
+    if (x.c != y.c) return x.c < y.c;
+    return x.packed < y.packed;
+  });
+  // pyfp3d/mesh/metrics.py:248-276: the first owner waits in the map, the
+  // next occurrence links to it and clears the entry -> consecutive pairs
+  size_t i = 0;
+  while (i < faces.size()) {
+    size_t j = i;
+    while (j < faces.size() && faces[j].a == faces[i].a &&
+           faces[j].b == faces[i].b && faces[j].c == faces[i].c)
+      ++j;
+    for (size_t k = i; k + 1 < j; k += 2) {
+      int64_t p = faces[k].packed, q = faces[k + 1].packed;
+      face_neighbors[p] = (int32_t)(q / 4);
+      face_neighbors[q] = (int32_t)(p / 4);
+    }
+    i = j;
+  }
+  return FP3D_OK;
+}
+
+int fp3d_host_element_geometry(int64_t n_tets, const int32_t *elements,
+                               const double *nodes, double *B, double *V,
+                               int64_t *bad_elem) {
+  HREQ(n_tets == 0 || (elements && nodes && B && V), "geometry: NULL");
+  if (bad_elem) *bad_elem = -1;
+  for (int64_t e = 0; e < n_tets; ++e) {
+    const double *x0 = nodes + 3 * (int64_t)elements[4 * e];
+    double J[3][3];  // columns = edge vectors x_k - x_0
+    for (int k = 0; k < 3; ++k) {
+      const double *xk = nodes + 3 * (int64_t)elements[4 * e + k + 1];
+      for (int d = 0; d < 3; ++d) J[d][k] = xk[d] - x0[d];
+    }
+    // pyfp3d/mesh/metrics.py:58-66: det = v1 . (v2 x v3), V = |det| / 6
+    double c0 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+    double c1 = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+    double c2 = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+    double det = J[0][0] * c0 + J[1][0] * c1 + J[2][0] * c2;
+    double s1 = std::sqrt(J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0]);
+    double s2 = std::sqrt(J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1]);
+    double s3 = std::sqrt(J[0][2] * J[0][2] + J[1][2] * J[1][2] + J[2][2] * J[2][2]);
+    double scale = s1 * s2 * s3;
+    // pyfp3d/mesh/metrics.py:155-164 degenerate guard
+    if (scale == 0.0 || std::fabs(det) < 1e-12 * scale) {
+      if (bad_elem) *bad_elem = e;
+      fp3d::set_error(
+          "element_gradients: degenerate tetrahedron (|det J| < 1e-12 x "
+          "edge-length^3 scale) -- fix the mesh");
+      return FP3D_EINVAL;
+    }
+    V[e] = std::fabs(det) / 6.0;
+    // grad N_i = J^{-T} grad_ref N_i; J^{-1} = adj(J) / det.  Row k of
+    // J^{-1} (k = 0..2) is grad N_{k+1}; grad N_0 = -(sum of the others).
+    double inv[3][3];
+    inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    double *b = B + 12 * e;
+    for (int d = 0; d < 3; ++d) {
+      b[3 + d] = inv[0][d];
+      b[6 + d] = inv[1][d];
+      b[9 + d] = inv[2][d];
+      b[d] = -inv[0][d] - inv[1][d] - inv[2][d];
+    }
+  }
+  return FP3D_OK;
+}
+
+int fp3d_host_standard_aggregation(int64_t n, const int32_t *indptr,
+                                   const int32_t *indices, const double *data,
+                                   int32_t *agg, int64_t *n_agg) {
+  HREQ(n >= 0 && indptr && indices && agg && n_agg, "aggregation: NULL");
+  // strong connection = nonzero off-diagonal entry (symmetric strength at
+  // theta = 0); explicit zeros (e.g. Dirichlet-masked couplings) are weak.
+  auto strong = [&](int64_t q, int64_t i) {
+    return indices[q] != i && (!data || data[q] != 0.0);
+  };
+  std::vector<int64_t> x((size_t)n, 0);  // 0 = free, >0 agg+1, <0 pass-2
+  int64_t next = 1;
+  const int64_t kIsolated = INT64_MIN;
+  // pass 1: seed aggregates at nodes whose strong neighbours are all free
+  for (int64_t i = 0; i < n; ++i) {
+    if (x[i] != 0) continue;
+    bool has_nb = false, nb_aggregated = false;
+    for (int64_t q = indptr[i]; q < indptr[i + 1]; ++q) {
+      if (!strong(q, i)) continue;
+      has_nb = true;
+      if (x[indices[q]] != 0) {
+        nb_aggregated = true;
+        break;
+      }
+    }
+    if (!has_nb) {
+      x[i] = kIsolated;
+    } else if (!nb_aggregated) {
+      x[i] = next;
+      for (int64_t q = indptr[i]; q < indptr[i + 1]; ++q)
+        if (strong(q, i)) x[indices[q]] = next;
+      ++next;
+    }
+  }
+  // pass 2: attach leftovers to a neighbouring pass-1 aggregate
+  for (int64_t i = 0; i < n; ++i) {
+    if (x[i] != 0) continue;
+    for (int64_t q = indptr[i]; q < indptr[i + 1]; ++q) {
+      if (!strong(q, i)) continue;
+      int64_t xj = x[indices[q]];
+      if (xj > 0) {
+        x[i] = -xj;
+        break;
+      }
+    }
+  }
+  for (int64_t i = 0; i < n; ++i)
+    if (x[i] < 0 && x[i] != kIsolated) x[i] = -x[i];
+  // pass 3: remaining free nodes seed new aggregates with free neighbours
+  for (int64_t i = 0; i < n; ++i) {
+    if (x[i] != 0) continue;
+    x[i] = next;
+    for (int64_t q = indptr[i]; q < indptr[i + 1]; ++q)
+      if (strong(q, i) && x[indices[q]] == 0) x[indices[q]] = next;
+    ++next;
+  }
+  for (int64_t i = 0; i < n; ++i)
+    agg[i] = x[i] == kIsolated ? -1 : (int32_t)(x[i] - 1);
+  *n_agg = next - 1;
+  return FP3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,784 @@
+// Sparse and dense linear-algebra kernels of the device Krylov/AMG solve
+// (gfx950, wave64).  All HBM-bound:
+//   * CSR SpMV with G lanes per row (G = 4..64 picked from nnz/row), float64
+//     or float32 matrix values, float64 vectors and accumulation, with fused
+//     epilogues (residual, prolongation-add, Chebyshev step, Term 3 gather);
+//   * deterministic two-pass reductions (multi-dot, norms) for Arnoldi;
+//   * the Newton Term-3 operator built once per Newton step as a row-gather
+//     table (node -> active element incidences, in element order), so the
+//     upstream coupling is applied inside the SpMV without atomics and
+//     without rebuilding the CSR pattern.
+#include <cmath>
+
+#include "fp3d_common.h"
+
+namespace fp3d {
+
+// --------------------------------------------------------------- epilogues
+struct EpStore {  // y = s
+  double *y;
+  __device__ void operator()(int64_t r, double s) const { y[r] = s; }
+};
+struct EpResidual {  // r = b - s
+  const double *b;
+  double *r;
+  __device__ void operator()(int64_t i, double s) const { r[i] = b[i] - s; }
+};
+struct EpAdd {  // x += s
+  double *x;
+  __device__ void operator()(int64_t i, double s) const { x[i] += s; }
+};
+// Chebyshev smoothing (three-term recurrence on D^-1 A).  x is never read
+// by the SpMV of a kernel that writes it: each step adds the PREVIOUS
+// direction to x, and the last direction is added by k_add.
+// First step from a nonzero x: r = dinv (b - Ax); d = r/theta.
+struct EpChebFirst {
+  const double *b, *dinv;
+  double *r, *d;
+  double inv_theta;
+  __device__ void operator()(int64_t i, double s) const {
+    double ri = dinv[i] * (b[i] - s);
+    r[i] = ri;
+    d[i] = ri * inv_theta;
+  }
+};
+// Recurrence: x += d_old; r -= dinv A d_old; d_new = c1 d_old + c2 r
+struct EpChebIter {
+  const double *dinv, *d_old;
+  double *x, *r, *d_new;
+  double c1, c2;
+  __device__ void operator()(int64_t i, double s) const {
+    double dold = d_old[i];
+    double ri = r[i] - dinv[i] * s;
+    r[i] = ri;
+    d_new[i] = c1 * dold + c2 * ri;
+    x[i] += dold;
+  }
+};
+
+template <int G, typename T, typename Ep>
+__global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
+                                                 const int32_t *__restrict__ ptr,
+                                                 const int32_t *__restrict__ idx,
+                                                 const T *__restrict__ val,
+                                                 const double *__restrict__ x,
+                                                 Ep ep) {
+  constexpr int RPB = kBlock / G;
+  int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / G;
+  int lane = threadIdx.x % G;
+  if (row >= n_rows) return;
+  int32_t j0 = ptr[row], j1 = ptr[row + 1];
+  double s = 0.0;
+  for (int32_t j = j0 + lane; j < j1; j += G) s += (double)val[j] * x[idx[j]];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+  if (lane == 0) ep(row, s);
+}
+
+// SpMV + Term-3 gather + Dirichlet identity rows (free_mask nullable)
+template <int G, typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_t3(
+    int64_t n_rows, const int32_t *__restrict__ ptr,
+    const int32_t *__restrict__ idx, const T *__restrict__ val,
+    const double *__restrict__ x, const int32_t *__restrict__ t3ptr,
+    const int32_t *__restrict__ t3ent, const double *__restrict__ W,
+    const double *__restrict__ tval, const uint8_t *__restrict__ fmask,
+    double *__restrict__ y) {
+  constexpr int RPB = kBlock / G;
+  int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / G;
+  int lane = threadIdx.x % G;
+  if (row >= n_rows) return;
+  int32_t j0 = ptr[row], j1 = ptr[row + 1];
+  double s = 0.0;
+  for (int32_t j = j0 + lane; j < j1; j += G) s += (double)val[j] * x[idx[j]];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+  if (lane == 0) {
+    if (t3ptr) {
+      for (int32_t q = t3ptr[row]; q < t3ptr[row + 1]; ++q) {
+        int32_t ent = t3ent[q];
+        s += W[ent] * tval[ent >> 2];
+      }
+    }
+    if (fmask && !fmask[row]) s = x[row];
+    y[row] = s;
+  }
+}
+
+inline int lanes_for(int64_t n_rows, int64_t nnz) {
+  double avg = n_rows > 0 ? (double)nnz / (double)n_rows : 1.0;
+  if (avg <= 6) return 4;
+  if (avg <= 20) return 8;
+  if (avg <= 48) return 16;
+  if (avg <= 128) return 32;
+  return 64;
+}
+
+template <typename T, typename Ep>
+int launch_spmv(int64_t n_rows, int64_t nnz, const int32_t *ptr,
+                const int32_t *idx, const T *val, const double *x, Ep ep,
+                hipStream_t s) {
+  if (n_rows <= 0) return FP3D_OK;
+  int G = lanes_for(n_rows, nnz);
+  unsigned grid = grid_for(n_rows, kBlock / G);
+  switch (G) {
+    case 4: hipLaunchKernelGGL((k_spmv<4, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 8: hipLaunchKernelGGL((k_spmv<8, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 16: hipLaunchKernelGGL((k_spmv<16, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 32: hipLaunchKernelGGL((k_spmv<32, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    default: hipLaunchKernelGGL((k_spmv<64, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+  }
+  return check_launch("spmv");
+}
+
+template <typename Ep>
+int spmv_any(int64_t n_rows, int64_t nnz, const int32_t *ptr,
+             const int32_t *idx, const void *val, int is_f32, const double *x,
+             Ep ep, hipStream_t s) {
+  if (is_f32)
+    return launch_spmv<float, Ep>(n_rows, nnz, ptr, idx, (const float *)val, x,
+                                  ep, s);
+  return launch_spmv<double, Ep>(n_rows, nnz, ptr, idx, (const double *)val, x,
+                                 ep, s);
+}
+
+// ------------------------------------------------------------ BLAS-1 etc.
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x,
+                        double b, double *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+__global__ void k_mask_copy(int64_t n, const uint8_t *__restrict__ m,
+                            const double *__restrict__ x,
+                            double *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = m[i] ? x[i] : 0.0;
+}
+__global__ void k_to_f32(int64_t n, const double *__restrict__ x,
+                         float *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+__global__ void k_scale(int64_t n, double a, const double *__restrict__ x,
+                        double *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+// rows/cols of non-free dofs -> identity (principal submatrix embedding)
+__global__ __launch_bounds__(kBlock) void k_mask_dirichlet(
+    int64_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
+    const uint8_t *__restrict__ fm, double *__restrict__ data,
+    float *__restrict__ d32) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  bool rf = fm[r] != 0;
+  for (int32_t j = ptr[r]; j < ptr[r + 1]; ++j) {
+    int32_t c = idx[j];
+    double v = data[j];
+    if (!rf) v = (c == r) ? 1.0 : 0.0;
+    else if (!fm[c]) v = 0.0;
+    data[j] = v;
+    if (d32) d32[j] = (float)v;
+  }
+}
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wsum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kBlock / 64; ++k) t += sh[k];
+  return t;  // valid on thread 0
+}
+
+inline int rblocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  if (b > kReduceBlocks) b = kReduceBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// partial[k*nb + b] = sum over block b's rows of V_k . w
+__global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
+                                                   const double *__restrict__ V,
+                                                   int64_t ld,
+                                                   const double *__restrict__ w,
+                                                   double *__restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  int nb = gridDim.x;
+  for (int k = 0; k < nv; ++k) {
+    const double *v = V + (int64_t)k * ld;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)nb * blockDim.x)
+      s += v[i] * w[i];
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[(int64_t)k * nb + blockIdx.x] = t;
+  }
+}
+// one block per k, sums the nb partials in index order (wave-parallel then
+// fixed-order combine)
+__global__ __launch_bounds__(kBlock) void k_mdot_p2(int nb,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ out) {
+  __shared__ double sh[kBlock / 64];
+  int k = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(int64_t)k * nb + b];
+  double t = block_sum(s, sh);
+  if (threadIdx.x == 0) out[k] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_norms_p1(int64_t n,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ double shm[kBlock / 64];
+  double s = 0.0, m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = x[i];
+    s += v * v;
+    double a = fabs(v);
+    m = a > m ? a : m;
+  }
+  m = wmax(m);
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  double t = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    double mm = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) mm = shm[k] > mm ? shm[k] : mm;
+    part[2 * blockIdx.x] = t;
+    part[2 * blockIdx.x + 1] = mm;
+  }
+}
+__global__ void k_norms_p2(int nb, const double *__restrict__ part,
+                           double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0, m = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    s += part[2 * b];
+    m = part[2 * b + 1] > m ? part[2 * b + 1] : m;
+  }
+  out[0] = sqrt(s);
+  out[1] = m;
+}
+
+// w -= sum_k h[k] V_k   (h in LDS)
+__global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
+                                                 const double *__restrict__ V,
+                                                 int64_t ld,
+                                                 const double *__restrict__ h,
+                                                 double *__restrict__ w,
+                                                 double sign) {
+  __shared__ double hs[256];
+  for (int k = threadIdx.x; k < nv; k += blockDim.x) hs[k] = h[k];
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int k = 0; k < nv; ++k) acc += hs[k] * V[(int64_t)k * ld + i];
+  w[i] += sign * acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_mv(int64_t n,
+                                                    const double *__restrict__ M,
+                                                    const double *__restrict__ b,
+                                                    double *__restrict__ x) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t j = 0; j < n; ++j) s += M[i * n + j] * b[j];
+  x[i] = s;
+}
+
+// Chebyshev first step from x = 0: x = 0; r = dinv b; d = r/theta
+__global__ void k_cheb_zero(int64_t n, const double *__restrict__ b,
+                            const double *__restrict__ dinv, double inv_theta,
+                            double *__restrict__ x, double *__restrict__ r,
+                            double *__restrict__ d) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ri = dinv[i] * b[i];
+  r[i] = ri;
+  d[i] = ri * inv_theta;
+  x[i] = 0.0;
+}
+__global__ void k_add(int64_t n, const double *__restrict__ d,
+                      double *__restrict__ x) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+// ----------------------------------------------------------- Term 3 (device)
+// Workspace layout (byte offsets computed by t3_layout).
+struct T3Layout {
+  size_t flag, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, total;
+};
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+inline T3Layout t3_layout(int64_t nt, int64_t nn) {
+  T3Layout L;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = o;
+    o = align256(o + bytes);
+    return at;
+  };
+  int64_t nmax = nt > nn ? nt : nn;
+  int64_t nbs = (nmax + 1 + kBlock - 1) / kBlock + 1;
+  L.flag = take(sizeof(int32_t) * (nt + 1));
+  L.scan = take(sizeof(int32_t) * (nmax + 2));
+  L.bsum = take(sizeof(int32_t) * (nbs + 1));
+  L.act = take(sizeof(int32_t) * (nt + 1));
+  L.W = take(sizeof(double) * 4 * nt);
+  L.Cc = take(sizeof(double) * 4 * nt);
+  L.tval = take(sizeof(double) * nt);
+  L.cnt = take(sizeof(int32_t) * (nn + 1));
+  L.rowptr = take(sizeof(int32_t) * (nn + 2));
+  L.ent = take(sizeof(int32_t) * 4 * nt);
+  L.total = o + 256;  // header: n_active (int64) lives in the last 256 B
+  return L;
+}
+
+// per element: masked s_u, activity flag and the two coefficient sets
+__global__ __launch_bounds__(kBlock) void k_t3_coef(
+    int64_t nt, const int32_t *__restrict__ elements,
+    const double *__restrict__ B, const double *__restrict__ V,
+    const double *__restrict__ grad, const int32_t *__restrict__ up,
+    const double *__restrict__ s_u, const uint8_t *__restrict__ lim,
+    double inv_u2, int32_t *__restrict__ flag, double *__restrict__ W,
+    double *__restrict__ Cc) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  int64_t u = up[e];
+  double s = s_u[e];
+  if (lim) s = s * (lim[u] ? 1.0 : 0.0);
+  bool act = s != 0.0;
+  flag[e] = act ? 1 : 0;
+  if (!act) return;
+  double ge0 = grad[3 * e], ge1 = grad[3 * e + 1], ge2 = grad[3 * e + 2];
+  double gu0 = grad[3 * u], gu1 = grad[3 * u + 1], gu2 = grad[3 * u + 2];
+  double w = 2.0 * inv_u2 * s * V[e];
+  for (int i = 0; i < 4; ++i) {
+    double gi = ge0 * B[12 * e + 3 * i] + ge1 * B[12 * e + 3 * i + 1] +
+                ge2 * B[12 * e + 3 * i + 2];
+    W[4 * e + i] = w * gi;
+    Cc[4 * e + i] = gu0 * B[12 * u + 3 * i] + gu1 * B[12 * u + 3 * i + 1] +
+                    gu2 * B[12 * u + 3 * i + 2];
+  }
+}
+
+// exclusive scan, pass 1: per-block local scan + block totals
+__global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n,
+                                                   const int32_t *__restrict__ in,
+                                                   int32_t *__restrict__ out,
+                                                   int32_t *__restrict__ bsum) {
+  __shared__ int32_t sh[kBlock];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t v = i < n ? in[i] : 0;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int32_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = sh[threadIdx.x] - v;
+  if (threadIdx.x == kBlock - 1) bsum[blockIdx.x] = sh[kBlock - 1];
+}
+// pass 2: exclusive scan of the block totals in one workgroup (chunked)
+__global__ __launch_bounds__(kBlock) void k_scan_p2(int nb,
+                                                   int32_t *__restrict__ bsum,
+                                                   int32_t *__restrict__ total) {
+  __shared__ int32_t sh[kBlock];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += kBlock) {
+    int i = base + threadIdx.x;
+    int32_t v = i < nb ? bsum[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      int32_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += sh[kBlock - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+__global__ void k_scan_p3(int64_t n, int32_t *__restrict__ out,
+                          const int32_t *__restrict__ bsum) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += bsum[blockIdx.x];
+}
+
+int exclusive_scan(int64_t n, const int32_t *in, int32_t *out, int32_t *bsum,
+                   int32_t *total, hipStream_t s) {
+  int nb = (int)grid_for(n);
+  hipLaunchKernelGGL(k_scan_p1, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum);
+  hipLaunchKernelGGL(k_scan_p2, dim3(1), dim3(kBlock), 0, s, nb, bsum, total);
+  hipLaunchKernelGGL(k_scan_p3, dim3(nb), dim3(kBlock), 0, s, n, out, bsum);
+  return check_launch("exclusive_scan");
+}
+
+__global__ void k_t3_compact(int64_t nt, const int32_t *__restrict__ flag,
+                             const int32_t *__restrict__ pos,
+                             int32_t *__restrict__ act) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nt && flag[e]) act[pos[e]] = (int32_t)e;
+}
+// per node: number of active incident (element, local) entries
+__global__ void k_t3_count(int64_t nn, const int32_t *__restrict__ iptr,
+                           const int32_t *__restrict__ ient,
+                           const int32_t *__restrict__ flag,
+                           const uint8_t *__restrict__ fmask,
+                           int32_t *__restrict__ cnt) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  int32_t c = 0;
+  if (!fmask || fmask[v])
+    for (int32_t q = iptr[v]; q < iptr[v + 1]; ++q) c += flag[ient[q] >> 2];
+  cnt[v] = c;
+}
+__global__ void k_t3_fill(int64_t nn, const int32_t *__restrict__ iptr,
+                          const int32_t *__restrict__ ient,
+                          const int32_t *__restrict__ flag,
+                          const uint8_t *__restrict__ fmask,
+                          const int32_t *__restrict__ rowptr,
+                          int32_t *__restrict__ ent) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  if (fmask && !fmask[v]) return;
+  int32_t o = rowptr[v];
+  for (int32_t q = iptr[v]; q < iptr[v + 1]; ++q) {
+    int32_t en = ient[q];
+    if (flag[en >> 2]) ent[o++] = en;
+  }
+}
+// t_e = sum_k (grad_u . B_u,k) x[elements[u,k]] for the active elements
+__global__ void k_t3_tval(int64_t n_active, const int32_t *__restrict__ act,
+                          const int32_t *__restrict__ elements,
+                          const int32_t *__restrict__ up,
+                          const double *__restrict__ Cc,
+                          const double *__restrict__ x,
+                          double *__restrict__ tval) {
+  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_active) return;
+  int64_t e = act[a];
+  int64_t u = up[e];
+  const int4 nd = *reinterpret_cast<const int4 *>(elements + 4 * u);
+  tval[e] = Cc[4 * e] * x[nd.x] + Cc[4 * e + 1] * x[nd.y] +
+            Cc[4 * e + 2] * x[nd.z] + Cc[4 * e + 3] * x[nd.w];
+}
+
+}  // namespace fp3d
+
+using namespace fp3d;
+
+// ======================================================================
+// C-ABI
+// ======================================================================
+extern "C" {
+
+int fp3d_csr_spmv(int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                  const int32_t *indices, const double *data, const double *x,
+                  double *y, void *stream) {
+  FP3D_REQUIRE(n_rows == 0 || (indptr && indices && data && x && y),
+               "csr_spmv: NULL pointer");
+  FP3D_REQUIRE(x != y, "csr_spmv: x and y must not alias");
+  return launch_spmv<double>(n_rows, nnz, indptr, indices, data, x, EpStore{y},
+                             FP3D_STREAM(stream));
+}
+
+int fp3d_csr_spmv_f32(int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                      const int32_t *indices, const float *data,
+                      const double *x, double *y, void *stream) {
+  FP3D_REQUIRE(n_rows == 0 || (indptr && indices && data && x && y),
+               "csr_spmv_f32: NULL pointer");
+  FP3D_REQUIRE(x != y, "csr_spmv_f32: x and y must not alias");
+  return launch_spmv<float>(n_rows, nnz, indptr, indices, data, x, EpStore{y},
+                            FP3D_STREAM(stream));
+}
+
+int64_t fp3d_term3_workspace_bytes(int64_t n_tets, int64_t n_nodes) {
+  if (n_tets < 0 || n_nodes < 0) return -1;
+  return (int64_t)t3_layout(n_tets, n_nodes).total;
+}
+
+int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
+                     const int32_t *inc_ptr, const int32_t *inc_ent,
+                     const double *B, const double *V, const double *grad,
+                     const int32_t *upstream, const double *s_u,
+                     const uint8_t *lim, const uint8_t *free_mask,
+                     double inv_u2, void *t3, int64_t *n_active_out,
+                     void *stream) {
+  FP3D_REQUIRE(n_tets > 0 && n_nodes > 0, "term3_build: empty mesh");
+  FP3D_REQUIRE(elements && inc_ptr && inc_ent && B && V && grad && upstream &&
+                   s_u && t3 && n_active_out,
+               "term3_build: NULL pointer");
+  hipStream_t s = FP3D_STREAM(stream);
+  T3Layout L = t3_layout(n_tets, n_nodes);
+  char *base = (char *)t3;
+  int32_t *flag = (int32_t *)(base + L.flag);
+  int32_t *scan = (int32_t *)(base + L.scan);
+  int32_t *bsum = (int32_t *)(base + L.bsum);
+  int32_t *act = (int32_t *)(base + L.act);
+  double *W = (double *)(base + L.W);
+  double *Cc = (double *)(base + L.Cc);
+  int32_t *cnt = (int32_t *)(base + L.cnt);
+  int32_t *rowptr = (int32_t *)(base + L.rowptr);
+  int32_t *ent = (int32_t *)(base + L.ent);
+  int32_t *hdr = (int32_t *)(base + L.total - 256);
+  hipLaunchKernelGGL(k_t3_coef, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
+                     n_tets, elements, B, V, grad, upstream, s_u, lim, inv_u2,
+                     flag, W, Cc);
+  int rc = exclusive_scan(n_tets, flag, scan, bsum, hdr, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_t3_compact, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
+                     n_tets, flag, scan, act);
+  hipLaunchKernelGGL(k_t3_count, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s,
+                     n_nodes, inc_ptr, inc_ent, flag, free_mask, cnt);
+  rc = exclusive_scan(n_nodes, cnt, rowptr, bsum, rowptr + n_nodes, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_t3_fill, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s,
+                     n_nodes, inc_ptr, inc_ent, flag, free_mask, rowptr, ent);
+  rc = check_launch("term3_build");
+  if (rc) return rc;
+  int32_t na = 0;
+  FP3D_HIP(hipMemcpyAsync(&na, hdr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  FP3D_HIP(hipStreamSynchronize(s));
+  *n_active_out = na;
+  return FP3D_OK;
+}
+
+int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                        const int32_t *indices, const void *data, int is_f32,
+                        const int32_t *elements, const int32_t *upstream,
+                        const void *t3, int64_t n_active, int64_t n_tets,
+                        const uint8_t *free_mask, const double *x, double *y,
+                        void *stream) {
+  FP3D_REQUIRE(n_rows > 0 && indptr && indices && data && x && y,
+               "csr_spmv_term3: NULL pointer");
+  FP3D_REQUIRE(x != y, "csr_spmv_term3: x and y must not alias");
+  hipStream_t s = FP3D_STREAM(stream);
+  const int32_t *t3ptr = nullptr, *t3ent = nullptr;
+  const double *W = nullptr, *tval = nullptr;
+  if (t3 && n_active > 0) {
+    FP3D_REQUIRE(elements && upstream, "csr_spmv_term3: Term 3 needs mesh");
+    T3Layout L = t3_layout(n_tets, n_rows);
+    const char *base = (const char *)t3;
+    const int32_t *act = (const int32_t *)(base + L.act);
+    const double *Cc = (const double *)(base + L.Cc);
+    double *tv = (double *)(base + L.tval);
+    hipLaunchKernelGGL(k_t3_tval, dim3(grid_for(n_active)), dim3(kBlock), 0, s,
+                       n_active, act, elements, upstream, Cc, x, tv);
+    t3ptr = (const int32_t *)(base + L.rowptr);
+    t3ent = (const int32_t *)(base + L.ent);
+    W = (const double *)(base + L.W);
+    tval = tv;
+  }
+  int G = lanes_for(n_rows, nnz);
+  unsigned grid = grid_for(n_rows, kBlock / G);
+#define T3_LAUNCH(GG, T)                                                     \
+  hipLaunchKernelGGL((k_spmv_t3<GG, T>), dim3(grid), dim3(kBlock), 0, s,     \
+                     n_rows, indptr, indices, (const T *)data, x, t3ptr,     \
+                     t3ent, W, tval, free_mask, y)
+  if (is_f32) {
+    switch (G) {
+      case 4: T3_LAUNCH(4, float); break;
+      case 8: T3_LAUNCH(8, float); break;
+      case 16: T3_LAUNCH(16, float); break;
+      case 32: T3_LAUNCH(32, float); break;
+      default: T3_LAUNCH(64, float); break;
+    }
+  } else {
+    switch (G) {
+      case 4: T3_LAUNCH(4, double); break;
+      case 8: T3_LAUNCH(8, double); break;
+      case 16: T3_LAUNCH(16, double); break;
+      case 32: T3_LAUNCH(32, double); break;
+      default: T3_LAUNCH(64, double); break;
+    }
+  }
+#undef T3_LAUNCH
+  return check_launch("csr_spmv_term3");
+}
+
+int fp3d_csr_mask_dirichlet(int64_t n_rows, const int32_t *indptr,
+                            const int32_t *indices, const uint8_t *free_mask,
+                            double *data, float *data32, void *stream) {
+  FP3D_REQUIRE(n_rows == 0 || (indptr && indices && free_mask && data),
+               "csr_mask_dirichlet: NULL pointer");
+  if (n_rows == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_mask_dirichlet, dim3(grid_for(n_rows)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n_rows, indptr, indices, free_mask,
+                     data, data32);
+  return check_launch("csr_mask_dirichlet");
+}
+
+int fp3d_csr_to_f32(int64_t nnz, const double *data, float *out, void *stream) {
+  FP3D_REQUIRE(nnz == 0 || (data && out), "csr_to_f32: NULL pointer");
+  if (nnz == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), nnz, data, out);
+  return check_launch("csr_to_f32");
+}
+
+int fp3d_axpby(int64_t n, double a, const double *x, double b, double *y,
+               void *stream) {
+  FP3D_REQUIRE(n == 0 || (x && y), "axpby: NULL pointer");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, a, x, b, y);
+  return check_launch("axpby");
+}
+
+int fp3d_scale(int64_t n, double a, const double *x, double *y, void *stream) {
+  FP3D_REQUIRE(n == 0 || (x && y), "scale: NULL pointer");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, a, x, y);
+  return check_launch("scale");
+}
+
+int fp3d_mask_copy(int64_t n, const uint8_t *mask, const double *x, double *y,
+                   void *stream) {
+  FP3D_REQUIRE(n == 0 || (mask && x && y), "mask_copy: NULL pointer");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_mask_copy, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, mask, x, y);
+  return check_launch("mask_copy");
+}
+
+int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
+                   const double *w, double *work, double *out, void *stream) {
+  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && work && out && ld >= n,
+               "multi_dot: bad arguments");
+  hipStream_t s = FP3D_STREAM(stream);
+  int nb = rblocks(n);
+  hipLaunchKernelGGL(k_mdot_p1, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld, w,
+                     work);
+  hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
+  return check_launch("multi_dot");
+}
+
+int fp3d_multi_dot_work_doubles(int nv) { return nv * kReduceBlocks; }
+
+int fp3d_multi_axpy(int64_t n, int nv, const double *V, int64_t ld,
+                    const double *h, double sign, double *w, void *stream) {
+  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && h && w && ld >= n,
+               "multi_axpy: bad arguments");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_maxpy, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, nv, V, ld, h, w, sign);
+  return check_launch("multi_axpy");
+}
+
+int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
+               void *stream) {
+  FP3D_REQUIRE(x && work && out2, "norms: NULL pointer");
+  hipStream_t s = FP3D_STREAM(stream);
+  int nb = rblocks(n);
+  hipLaunchKernelGGL(k_norms_p1, dim3(nb), dim3(kBlock), 0, s, n, x, work);
+  hipLaunchKernelGGL(k_norms_p2, dim3(1), dim3(64), 0, s, nb, work, out2);
+  return check_launch("norms");
+}
+
+int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
+                         const int32_t *indices, const void *data, int is_f32,
+                         const double *dinv, const double *b, double *x,
+                         double *r, double *d0, double *d1, int degree,
+                         double lmin, double lmax, int zero_init,
+                         void *stream) {
+  FP3D_REQUIRE(n > 0 && indptr && indices && data && dinv && b && x && r &&
+                   d0 && d1 && degree >= 1 && lmax > lmin && lmin > 0.0,
+               "amg_cheb_smooth: bad arguments");
+  hipStream_t s = FP3D_STREAM(stream);
+  double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  double sigma = theta / delta;
+  double rho0 = 1.0 / sigma;
+  int rc;
+  if (zero_init) {
+    hipLaunchKernelGGL(k_cheb_zero, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                       b, dinv, 1.0 / theta, x, r, d0);
+    rc = check_launch("cheb_zero");
+  } else {
+    rc = spmv_any(n, nnz, indptr, indices, data, is_f32, x,
+                  EpChebFirst{b, dinv, r, d0, 1.0 / theta}, s);
+  }
+  if (rc) return rc;
+  double *dold = d0, *dnew = d1;
+  for (int k = 1; k < degree; ++k) {
+    double rho1 = 1.0 / (2.0 * sigma - rho0);
+    double c1 = rho1 * rho0, c2 = 2.0 * rho1 / delta;
+    rc = spmv_any(n, nnz, indptr, indices, data, is_f32, dold,
+                  EpChebIter{dinv, dold, x, r, dnew, c1, c2}, s);
+    if (rc) return rc;
+    rho0 = rho1;
+    double *t = dold;
+    dold = dnew;
+    dnew = t;
+  }
+  hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dold, x);
+  return check_launch("cheb_smooth");
+}
+
+int fp3d_amg_residual(int64_t n, int64_t nnz, const int32_t *indptr,
+                      const int32_t *indices, const void *data, int is_f32,
+                      const double *b, const double *x, double *r,
+                      void *stream) {
+  FP3D_REQUIRE(n > 0 && indptr && indices && data && b && x && r,
+               "amg_residual: bad arguments");
+  return spmv_any(n, nnz, indptr, indices, data, is_f32, x, EpResidual{b, r},
+                  FP3D_STREAM(stream));
+}
+
+int fp3d_amg_restrict(int64_t nc, int64_t nnz, const int32_t *indptr,
+                      const int32_t *indices, const void *data, int is_f32,
+                      const double *r, double *bc, void *stream) {
+  FP3D_REQUIRE(nc > 0 && indptr && indices && data && r && bc,
+               "amg_restrict: bad arguments");
+  return spmv_any(nc, nnz, indptr, indices, data, is_f32, r, EpStore{bc},
+                  FP3D_STREAM(stream));
+}
+
+int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
+                         const int32_t *indices, const void *data, int is_f32,
+                         const double *xc, double *x, void *stream) {
+  FP3D_REQUIRE(n > 0 && indptr && indices && data && xc && x,
+               "amg_prolong_add: bad arguments");
+  return spmv_any(n, nnz, indptr, indices, data, is_f32, xc, EpAdd{x},
+                  FP3D_STREAM(stream));
+}
+
+int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
+                      void *stream) {
+  FP3D_REQUIRE(n > 0 && M && b && x, "dense_matvec: bad arguments");
+  hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, M, b, x);
+  return check_launch("dense_matvec");
+}
+
+}  // extern "C"
