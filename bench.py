@@ -1,0 +1,307 @@
+"""Benchmark: Newton linearisation throughput (assembly Mtet/s) and Newton
+solve seconds (FP32-preconditioned vs all-FP64) of the full-potential solve
+on the M_inf = 0.84 bump channel, 192x64x64 hexes -> 4,718,592 tets,
+interior nodes jittered +-15 %, RCM-renumbered.
+
+A "step" is one Newton linearisation pass over the whole mesh with inputs
+resident in HBM: fused state sweep (grad, q2, limiter, density), upstream
+walk + nu switch + rho_tilde, coloured residual scatter, walk + exact
+sensitivities, coloured Newton Jacobian (Terms 1+2) into the fixed CSR and
+the Term-3 gather table.  `value` = n_tets / step time (Mtet/s).
+
+Usage: python bench.py [--steps K] [--warmup W] [--solve] [--no-cpu]
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+M_INF, UPWIND_C, M_CRIT, RHO_FLOOR, GAMMA = 0.84, 1.5, 0.95, 0.05, 1.4
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def constructed_phi(nodes):
+    """Multi-regime potential (sub/supersonic bands, accelerating and
+    shock-point branches) on the channel: 1.25 x + 0.35 sin(1.7 x)."""
+    x = nodes[:, 0]
+    return 1.25 * x + 0.35 * np.sin(1.7 * x)
+
+
+def algorithmic_bytes(n_tets, n_nodes, nnz):
+    """Compulsory HBM bytes of each phase (each input read once, each
+    output written once; float64 state, int32 connectivity)."""
+    return {
+        # elements 16, B 96, write grad 24 + q2n 8 + q2l 8 + rho 8 + lim 1;
+        # phi 8/node
+        "state": 161 * n_tets + 8 * n_nodes,
+        # fn 16, own centroid 24, elements 16, grad 24, q2 8, rho 8; write
+        # upstream 4, nu 8, rho_tilde 8; nodes 24/node
+        "upwind": 116 * n_tets + 24 * n_nodes,
+        # elements 16, B 96, V 8, rho_tilde 8, grad 24; R 8/node
+        "residual": 152 * n_tets + 8 * n_nodes,
+        # walk inputs as above (no nu/rt writes) + q2 8 + rho 8 + se/su 16
+        "sensitivities": 132 * n_tets + 24 * n_nodes,
+        # elem_to_csr 64, B 96, V 8, rho_tilde 8, grad 24, s_e 8, lim 1;
+        # CSR values written (zero fill + final) 16/nnz
+        "jacobian": 209 * n_tets + 16 * nnz,
+        # s_u 8, upstream 4, lim 1, flags 4 + incidence scan 16/tet + rows
+        "term3": 33 * n_tets + 12 * n_nodes,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--solve", action="store_true",
+                    help="also time the Newton solves (FP64 and FP32-precond)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-passes", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the "
+              "multi-GPU sharding of this path is a design note, not a run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from up3d_amd import _native
+    from up3d_amd.kernels.jacobian import PicardOperator
+    from up3d_amd.kernels.upwind import UpwindOperator
+    from up3d_amd.mesh.channel import config_mesh
+    from up3d_amd.physics.isentropic import q2_at_mach
+    from up3d_amd.solve.linear import NewtonOperator
+    from up3d_amd.device import to_dev
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    mesh = config_mesh(args.config)
+    log(f"[bench] mesh {args.config}: {mesh.n_tets} tets {mesh.n_nodes} nodes "
+        f"({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    op = PicardOperator(mesh.nodes, mesh.elements)
+    upw = UpwindOperator(mesh.nodes, mesh.elements)
+    st = op.store
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"[bench] set-up {setup_s:.1f}s: {st.n_colors} colours, nnz {st.nnz}")
+    nt, nn = st.n_tets, st.n_nodes
+    dev = st.device
+    phi = to_dev(constructed_phi(mesh.nodes))
+    grad = torch.empty((nt, 3), dtype=torch.float64, device=dev)
+    q2n = torch.empty(nt, dtype=torch.float64, device=dev)
+    q2l = torch.empty_like(q2n)
+    rho = torch.empty_like(q2n)
+    rt = torch.empty_like(q2n)
+    lim = torch.empty(nt, dtype=torch.uint8, device=dev)
+    fm = torch.ones(nn, dtype=torch.uint8, device=dev)
+    jac = NewtonOperator(st, fm)
+    q2cap = q2_at_mach(3.0, M_INF, GAMMA)
+
+    def ph_state():
+        op.state_dev(phi, 1.0, M_INF, GAMMA, q2cap, grad, q2n, q2l, rho, lim)
+
+    def ph_upwind():
+        upw.rho_tilde_dev(grad, q2l, rho, M_INF, UPWIND_C, M_CRIT, GAMMA,
+                          RHO_FLOOR, monitors=False)
+
+    def ph_residual():
+        op.residual_dev(upw.d_rt, grad)
+
+    def ph_sens():
+        upw.sensitivities_dev(grad, q2l, rho, M_INF, UPWIND_C, M_CRIT, GAMMA,
+                              RHO_FLOOR)
+
+    def ph_jac():
+        op.jacobian_dev(upw.d_rt, grad, upw.d_se, lim, 1.0, jac.data)
+
+    def ph_t3():
+        jac.build_term3(grad, upw.d_up, upw.d_su, lim, 1.0)
+
+    phases = [("state", ph_state), ("upwind", ph_upwind),
+              ("residual", ph_residual), ("sensitivities", ph_sens),
+              ("jacobian", ph_jac), ("term3", ph_t3)]
+
+    def step():
+        for _, f in phases:
+            f()
+        upw._monitors(upw.d_nu, upw.d_rt, RHO_FLOOR)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    value = nt / dt / 1e6
+    log(f"[bench] linearisation {dt * 1e3:.3f} ms/step -> {value:.1f} Mtet/s "
+        f"(n_term3_active {jac.n_active}, nu_max {upw.nu_max:.3f})")
+
+    # per-phase device time with HIP events on the launch stream
+    stream = torch.cuda.current_stream()
+    per = {}
+    for name, f in phases:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = max(3, args.steps)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        per[name] = e0.elapsed_time(e1) / reps       # ms
+    ab = algorithmic_bytes(nt, nn, st.nnz)
+    phase_rep = {k: {"ms": round(per[k], 4),
+                     "GBs": round(ab[k] / (per[k] * 1e-3) / 1e9, 1)}
+                 for k in per}
+    log("[bench] phases " + json.dumps(phase_rep))
+    dom = max((k for k in per if k != "term3"), key=lambda k: per[k])
+    achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(ab[dom])}
+    traffic_file = ROOT / "profiles" / "traffic.json"
+    if traffic_file.exists():
+        try:
+            roofline["traffic"] = json.loads(traffic_file.read_text()).get(dom)
+        except Exception:
+            pass
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(st, mesh, args.cpu_passes)
+        log(f"[bench] cpu baseline {cpu}")
+
+    solve = None
+    if args.solve:
+        solve = run_solves(mesh)
+
+    out = {
+        "metric": "Newton solve s + assembly Mtet/s: FP32-precond vs FP64 GPU vs CPU, M∞=0.84",
+        "value": round(value, 2), "unit": "Mtet/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "bump channel 192x64x64 hexes -> 4,718,592 tets, "
+                               "jitter 15%, RCM, M_inf=0.84, Newton linearisation",
+                   "n_tets": nt, "n_nodes": nn, "nnz": st.nnz,
+                   "n_colors": st.n_colors, "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "phases": phase_rep,
+        "setup_s": round(setup_s, 2), "solve": solve,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(st, mesh, passes):
+    """The oracle's C restatement (OpenMP, colour-serial like numba prange)
+    timed on the same linearisation pass over the same mesh."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    el = st.elements_host
+    B, V = st.B_host, st.V_host
+    off, ce = st.color_offsets, st.color_elems
+    e2c = st.e2c_host
+    fn, cen = st.face_neighbors_host, st.centroids_host
+    nodes = st.nodes_host
+    phi = constructed_phi(nodes)
+    q2cap = O.q2_at_mach(3.0, M_INF)
+
+    class _Op(O.OraclePicardOperator):
+        def __init__(self):
+            self.elements, self.B, self.V = el, B, V
+            self.color_offsets, self.color_elems = off, ce
+            self.n_colors = len(off) - 1
+            self.elem_to_csr = e2c
+            self.indices = st.indices_host
+            self.n_nodes, self.n_tets = st.n_nodes, st.n_tets
+
+    op = _Op()
+
+    def one():
+        g, q2 = O.element_velocity_q2(el, B, phi)
+        q2l = np.where(q2 < q2cap, q2, q2cap)
+        rho = O.density_field(q2l, M_INF)
+        up = O.upstream_elements(fn, cen, nodes, el, g)
+        nu, rt = O.rho_tilde_sweep(q2l, rho, up, M_INF, M_CRIT, UPWIND_C, GAMMA,
+                                   RHO_FLOOR)
+        op.assemble_residual(phi, rt, g)
+        up2 = O.upstream_elements(fn, cen, nodes, el, g)
+        se, su = O.rho_tilde_sensitivities(q2l, rho, up2, M_INF, M_CRIT,
+                                           UPWIND_C, GAMMA, RHO_FLOOR)
+        op.jacobian_terms12(g, rt, se, 1.0)
+        op.term3_coo(g, up2, su, 1.0)
+
+    one()                                   # first touch / warm caches
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        one()
+    dt = (time.perf_counter() - t0) / passes
+    return {"value": round(st.n_tets / dt / 1e6, 3), "unit": "Mtet/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{passes} full linearisation passes over the same "
+                      f"{st.n_tets}-tet mesh (oracle C, OpenMP {threads} threads)",
+            "s_per_pass": round(dt, 3)}
+
+
+def run_solves(mesh):
+    """Newton solve of the M_inf = 0.84 channel (Mach ramp) in both
+    precisions; seconds exclude mesh set-up, include AMG set-up."""
+    import torch
+
+    from up3d_amd.mesh.channel import boundary_nodes
+    from up3d_amd.solve.newton import solve_newton_transonic
+
+    dn = boundary_nodes(mesh, ["inflow", "outflow"])
+    dv = mesh.nodes[dn, 0].copy()
+    res = {}
+    for prec in ("fp32", "fp64"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = solve_newton_transonic(
+            mesh.nodes, mesh.elements, dn, dv, M_INF, m_start=0.6, dm=0.05,
+            newton_kw={"precision": prec, "n_picard_seed": 0,
+                       "phi_init": mesh.nodes[:, 0].copy(),
+                       "gmres_restart": 50, "n_newton_max": 40,
+                       "amg_rebuild_every": 4})
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res[prec] = {"seconds": round(wall, 2), "converged": bool(r["converged"]),
+                     "final_residual": float(r["residual_history"][-1]),
+                     "levels": [(round(m, 4), n, float(rr)) for m, n, rr in r["level_history"]],
+                     "timings_total": {k: round(v, 2) for k, v in r["timings_total"].items()}}
+        log(f"[bench] solve {prec}: {res[prec]}")
+        res[prec + "_phi"] = r["phi"]
+    if "fp64_phi" in res:
+        d = np.abs(res["fp32_phi"] - res["fp64_phi"]).max()
+        res["fp32_vs_fp64_phi_maxabs"] = float(d)
+    res.pop("fp32_phi", None)
+    res.pop("fp64_phi", None)
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,275 @@
+"""GPU parity of the HIP element kernels against the oracle (bit-exact for
+integer work and for the uncontracted FP64 kernels fed identical inputs;
+rho-dependent values within a few ulp because device pow() and glibc pow()
+may round differently)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+
+from .helpers import jittered_cube, transonic_field
+
+pytestmark = pytest.mark.gpu
+
+M, C, MC, FL, G = 0.8, 1.5, 0.95, 0.05, 1.4
+
+
+@pytest.fixture(scope="module")
+def mesh(gpu):
+    from up3d_amd.mesh.channel import bump_channel
+
+    m = bump_channel(18, 6, 4, jitter=0.15, seed=11, rcm=True)
+    print(f"[mesh] channel 18x6x4 n_tets={m.n_tets}", flush=True)
+    return m
+
+
+@pytest.fixture(scope="module")
+def ora(mesh):
+    op = O.OraclePicardOperator(mesh.nodes, mesh.elements)
+    upw = O.OracleUpwind(mesh.nodes, mesh.elements)
+    phi = transonic_field(mesh.nodes)
+    g, q2 = op.velocities(phi)
+    rho = O.density_field(q2, M)
+    up = upw.upstream(g)
+    return dict(op=op, upw=upw, phi=phi, g=g, q2=q2, rho=rho, up=up)
+
+
+def _ulp_close(a, b, n_ulp=8):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    tol = n_ulp * np.spacing(np.maximum(np.abs(a), np.abs(b)))
+    return np.all(np.abs(a - b) <= tol)
+
+
+def test_velocity_bitwise(mesh, ora):
+    from up3d_amd.kernels.gradient import element_velocity_q2
+
+    n = mesh.n_tets
+    g = np.empty((n, 3))
+    q2 = np.empty(n)
+    element_velocity_q2(mesh.elements, ora["op"].B, ora["phi"], g, q2)
+    assert np.array_equal(g, ora["g"]) and np.array_equal(q2, ora["q2"])
+
+
+def test_physics_fields(gpu):
+    from up3d_amd.physics import isentropic as P
+
+    q2 = np.linspace(0.0, 2.5, 1001)
+    for m in (0.0, 0.5, 0.84):
+        assert _ulp_close(P.density_field(q2, m), O.density_field(q2, m), 4)
+        assert np.array_equal(P.mach_squared_field(q2, m), O.mach_squared_field(q2, m))
+        assert np.array_equal(P.limit_q2_field(q2, m, 3.0), O.limit_q2_field(q2, m, 3.0))
+    assert np.all(P.density_field(q2, 0.0) == 1.0)
+
+
+def test_upstream_bitwise(mesh, ora):
+    from up3d_amd.kernels.upwind import upstream_elements
+
+    out = np.empty(mesh.n_tets, dtype=np.int64)
+    upstream_elements(ora["upw"].face_neighbors, ora["upw"].centroids, mesh.nodes,
+                      mesh.elements, ora["g"], out)
+    assert np.array_equal(out, ora["up"])
+    assert np.count_nonzero(out != np.arange(mesh.n_tets)) > mesh.n_tets // 2
+
+
+def test_sweeps(mesh, ora):
+    from up3d_amd.kernels import upwind as U
+
+    n = mesh.n_tets
+    q2, rho, up = ora["q2"], ora["rho"], ora["up"]
+    nu = np.empty(n)
+    rt = np.empty(n)
+    U.rho_tilde_sweep(q2, rho, up, M, MC, C, G, FL, nu, rt)
+    nu_o, rt_o = O.rho_tilde_sweep(q2, rho, up, M, MC, C, G, FL)
+    assert np.array_equal(nu, nu_o) and np.array_equal(rt, rt_o)
+    br = np.empty(n, dtype=np.int8)
+    U.classify_upwind_branches(q2, rho, up, M, MC, C, G, FL, br)
+    br_o = O.classify_upwind_branches(q2, rho, up, M, MC, C, G, FL)
+    assert np.array_equal(br, br_o)
+    assert set(np.unique(br)) >= {0, 1, 2}
+    U.rho_tilde_frozen_sweep(q2, rho, up, br, M, MC, C, G, FL, nu, rt)
+    assert np.array_equal(rt, O.rho_tilde_frozen_sweep(q2, rho, up, br, M, MC, C, G, FL)[1])
+    se = np.empty(n)
+    su = np.empty(n)
+    U.rho_tilde_sensitivities_sweep(q2, rho, up, M, MC, C, G, FL, se, su)
+    se_o, su_o = O.rho_tilde_sensitivities(q2, rho, up, M, MC, C, G, FL)
+    assert _ulp_close(se, se_o) and _ulp_close(su, su_o)
+    U.rho_tilde_frozen_sensitivities_sweep(q2, rho, up, br, M, MC, C, G, FL, se, su)
+    se_o, su_o = O.rho_tilde_frozen_sensitivities(q2, rho, up, br, M, MC, C, G)
+    assert _ulp_close(se, se_o) and _ulp_close(su, su_o)
+
+
+def test_colored_assembly_bitwise(mesh, ora):
+    from up3d_amd.kernels.jacobian import (assemble_jacobian_data_colored,
+                                           assemble_matrix_data_colored,
+                                           fill_term3_coo)
+    from up3d_amd.kernels.residual import assemble_residual_colored
+
+    op = ora["op"]
+    n = mesh.n_tets
+    rt = np.linspace(0.7, 1.3, n)
+    R = np.empty(mesh.n_nodes)
+    assemble_residual_colored(op.color_offsets, op.color_elems, mesh.elements,
+                              op.B, op.V, rt, ora["g"], R)
+    assert np.array_equal(R, op.assemble_residual(ora["phi"], rt, ora["g"]))
+    data = np.zeros(len(op.indices))
+    assemble_matrix_data_colored(op.color_offsets, op.color_elems, op.elem_to_csr,
+                                 op.B, op.V, rt, data)
+    assert np.array_equal(data, op.assemble_matrix(rt).data)
+    se = np.sin(np.arange(n)) * (np.arange(n) % 3 != 0)
+    data[:] = 0.0
+    assemble_jacobian_data_colored(op.color_offsets, op.color_elems,
+                                   op.elem_to_csr, op.B, op.V, rt, ora["g"], se,
+                                   1.0, data)
+    assert np.array_equal(data, op.jacobian_terms12(ora["g"], rt, se, 1.0))
+    su = np.cos(np.arange(n)) * (np.arange(n) % 4 == 0)
+    active, r_o, c_o, v_o = op.term3_coo(ora["g"], ora["up"], su, 1.0)
+    rows = np.empty(16 * len(active), dtype=np.int64)
+    cols = np.empty_like(rows)
+    vals = np.empty(16 * len(active))
+    fill_term3_coo(active, mesh.elements, op.B, op.V, ora["g"], ora["up"], su,
+                   1.0, rows, cols, vals)
+    assert np.array_equal(rows, r_o) and np.array_equal(cols, c_o)
+    assert np.array_equal(vals, v_o)
+
+
+def test_picard_operator_api(mesh, ora):
+    from up3d_amd.kernels.jacobian import PicardOperator
+
+    op = PicardOperator(mesh.nodes, mesh.elements)
+    oo = ora["op"]
+    assert op.n_colors == oo.n_colors
+    g, q2 = op.velocities(ora["phi"])
+    assert np.abs(g - ora["g"]).max() < 1e-12 * np.abs(ora["g"]).max()
+    rt = np.linspace(0.7, 1.3, mesh.n_tets)
+    A = op.assemble_matrix(rt)
+    Ao = oo.assemble_matrix(rt)
+    assert np.array_equal(A.indptr, Ao.indptr) and np.array_equal(A.indices, Ao.indices)
+    assert np.abs(A.data - Ao.data).max() < 1e-12 * np.abs(Ao.data).max()
+    R = op.assemble_residual(ora["phi"], rt)
+    Ro = oo.assemble_residual(ora["phi"], rt)
+    assert np.abs(R - Ro).max() < 1e-12 * np.abs(Ro).max()
+    # bit-determinism across calls
+    assert np.array_equal(op.assemble_matrix(rt).data, A.data)
+    assert np.array_equal(op.assemble_residual(ora["phi"], rt), R)
+    # Laplace limit: rho_tilde=None
+    assert np.abs(op.assemble_matrix().data - oo.assemble_matrix().data).max() < 1e-12
+
+
+def test_upwind_operator_and_jacobian(mesh, ora):
+    from up3d_amd.kernels.jacobian import PicardOperator
+    from up3d_amd.kernels.upwind import UpwindOperator
+
+    op = PicardOperator(mesh.nodes, mesh.elements)
+    upw = UpwindOperator(mesh.nodes, mesh.elements)
+    q2, rho, g = ora["q2"], ora["rho"], ora["g"]
+    rt = upw.rho_tilde(g, q2, rho, M, C, MC)
+    rt_o = ora["upw"].rho_tilde(g, q2, rho, M, C, MC)
+    assert _ulp_close(rt, rt_o)
+    assert upw.nu_max == ora["upw"].nu_max
+    assert upw.n_supersonic == ora["upw"].n_supersonic
+    se, su, up = upw.rho_tilde_sensitivities(g, q2, rho, M, C, MC)
+    se_o, su_o, up_o = ora["upw"].rho_tilde_sensitivities(g, q2, rho, M, C, MC)
+    assert np.array_equal(up, up_o)
+    assert _ulp_close(se, se_o) and _ulp_close(su, su_o)
+    upf, brf = upw.freeze_upwind_state(g, q2, rho, M, C, MC)
+    upf_o, brf_o = ora["upw"].freeze_upwind_state(g, q2, rho, M, C, MC)
+    assert np.array_equal(upf, upf_o) and np.array_equal(brf, brf_o)
+    J = op.assemble_newton_jacobian(ora["phi"], rt, se, su, up)
+    Jo = ora["op"].assemble_newton_jacobian(ora["phi"], rt_o, se_o, su_o, up_o)
+    assert op.n_term3_active == ora["op"].n_term3_active > 0
+    D = (J - Jo).tocsr()
+    assert abs(D).max() < 1e-12 * abs(Jo).max()
+
+
+def test_term3_device_operator(mesh, ora):
+    """Matrix-free Term 3 inside the SpMV == scipy (J + COO) on free rows."""
+    import torch
+
+    from up3d_amd.device import to_dev, to_host
+    from up3d_amd.kernels.jacobian import PicardOperator
+    from up3d_amd.kernels.upwind import UpwindOperator
+    from up3d_amd.solve.linear import NewtonOperator
+
+    op = PicardOperator(mesh.nodes, mesh.elements)
+    upw = UpwindOperator(mesh.nodes, mesh.elements)
+    st = op.store
+    q2, rho, g = ora["q2"], ora["rho"], ora["g"]
+    gs, q2s, rhos = st.to_storage(g.reshape(-1, 3)), st.to_storage(q2), st.to_storage(rho)
+    rts = upw.rho_tilde_dev(gs, q2s, rhos, M, C, MC).clone()
+    se, su, up = upw.sensitivities_dev(gs, q2s, rhos, M, C, MC)
+    lim = torch.ones(mesh.n_tets, dtype=torch.uint8, device=st.device)
+    dn = np.unique(mesh.boundary_faces["inflow"])
+    fm = np.ones(mesh.n_nodes, dtype=np.uint8)
+    fm[dn] = 0
+    jop = NewtonOperator(st, to_dev(fm, torch.uint8))
+    op.jacobian_dev(rts, gs, se, lim, 1.0, jop.data)
+    jop.finalize()
+    na = jop.build_term3(gs, up, su, lim, 1.0)
+    assert na > 0
+    # oracle Jacobian (original order) restricted to free dofs
+    rt_o = ora["upw"].rho_tilde(g, q2, rho, M, C, MC)
+    se_o, su_o, up_o = ora["upw"].rho_tilde_sensitivities(g, q2, rho, M, C, MC)
+    Jo = ora["op"].assemble_newton_jacobian(ora["phi"], rt_o, se_o, su_o, up_o)
+    free = np.nonzero(fm)[0]
+    x = np.random.default_rng(5).standard_normal(mesh.n_nodes)
+    x[dn] = 0.0
+    y_ref = np.zeros(mesh.n_nodes)
+    y_ref[free] = (Jo[free][:, free] @ x[free])
+    xd = to_dev(x)
+    yd = torch.empty_like(xd)
+    for f32 in (False, True):
+        jop.is_f32 = 0
+        if f32:
+            jop.finalize(f32=True)
+        jop.matvec(xd, yd)
+        y = to_host(yd)
+        tol = 1e-12 if not f32 else 1e-6
+        assert np.abs(y - y_ref).max() < tol * np.abs(y_ref).max()
+    # repeatability
+    y1 = to_host(jop.matvec(xd, yd)).copy()
+    assert np.array_equal(y1, to_host(jop.matvec(xd, yd)))
+
+
+def test_gmres_matches_scipy(mesh, ora, gpu):
+    import scipy.sparse.linalg as spla
+
+    from up3d_amd.solve.linear import build_amg_preconditioner, solve_gmres
+
+    rt = ora["upw"].rho_tilde(ora["g"], ora["q2"], ora["rho"], M, C, MC)
+    se, su, up = ora["upw"].rho_tilde_sensitivities(ora["g"], ora["q2"], ora["rho"], M, C, MC)
+    J = ora["op"].assemble_newton_jacobian(ora["phi"], rt, se, su, up)
+    A = ora["op"].assemble_matrix(rt)
+    dn = np.unique(mesh.boundary_faces["inflow"])
+    fm = np.ones(mesh.n_nodes, bool)
+    fm[dn] = False
+    free = np.nonzero(fm)[0]
+    Jf = J[free][:, free].tocsr()
+    Af = A[free][:, free].tocsr()
+    b = np.random.default_rng(3).standard_normal(len(free))
+    ml, Mp = build_amg_preconditioner(Af)
+    x, n_it, info = solve_gmres(Jf, b, M=Mp, rtol=1e-11, restart=30, maxiter=20)
+    assert info == 0
+    x_ref = spla.spsolve(Jf.tocsc(), b)
+    assert np.abs(x - x_ref).max() < 1e-8 * np.abs(x_ref).max()
+    assert np.linalg.norm(Jf @ x - b) <= 1.01e-11 * np.linalg.norm(b)
+
+
+def test_cg_amg_laplace(gpu):
+    import scipy.sparse.linalg as spla
+
+    from up3d_amd.solve.linear import solve_cg_amg
+
+    nodes, el = jittered_cube(10, seed=9)
+    op = O.OraclePicardOperator(nodes, el)
+    from .helpers import cube_boundary_mask
+
+    free = np.nonzero(~cube_boundary_mask(nodes))[0]
+    A = op.assemble_matrix()[free][:, free].tocsr()
+    b = np.ones(len(free))
+    x, n_it = solve_cg_amg(A, b, rtol=1e-12, maxiter=200)
+    assert n_it < 60
+    x_ref = spla.spsolve(A.tocsc(), b)
+    assert np.abs(x - x_ref).max() < 1e-9 * np.abs(x_ref).max()

@@ -1,0 +1,140 @@
+"""Host-side set-up of the product (native C++ in libfp3d_hip.so, runs on
+the CPU) against the oracle, the reference's fixtures and closed forms,
+plus the C-ABI export check.  No GPU needed."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from up3d_amd import _native
+from up3d_amd.kernels.store import csr_pattern, elem_to_csr
+from up3d_amd.mesh import channel
+from up3d_amd.mesh.coloring import color_partition_csr, greedy_coloring, validate_coloring
+from up3d_amd.mesh.metrics import build_face_adjacency, precompute_element_geometry
+
+from .helpers import jittered_cube, structured_cube_mesh
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = (ROOT / "include" / "fp3d.h").read_text()
+    decl = set(re.findall(r"\b(fp3d_[a-z0-9_]+)\s*\(", hdr))
+    lib = ctypes.CDLL(str(_native.LIB_PATH))
+    missing = [s for s in sorted(decl) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(_native.EXPORTED) <= decl
+
+
+def test_library_loads_without_gpu():
+    assert _native.lib().fp3d_version() == 1
+    assert _native.device_count() >= 0
+
+
+@pytest.mark.parametrize("mesh", ["kuhn8", "jcube", "channel"])
+def test_coloring_bit_identical_to_oracle(mesh):
+    if mesh == "kuhn8":
+        _, el = structured_cube_mesh(8)
+    elif mesh == "jcube":
+        _, el = jittered_cube(6, seed=2)
+    else:
+        el = channel.bump_channel(12, 5, 4, jitter=0.15, seed=4, rcm=True).elements
+    c1, n1 = greedy_coloring(el)
+    c2, n2 = O.greedy_coloring(el)
+    assert n1 == n2 and np.array_equal(c1, c2)
+    o1 = color_partition_csr(el)
+    o2 = O.color_partition_csr(el)
+    assert np.array_equal(o1[0], o2[0]) and np.array_equal(o1[1], o2[1])
+    assert validate_coloring(el, c1)
+
+
+def test_coloring_golden_kuhn8_product():
+    _, el = structured_cube_mesh(8)
+    c, n = greedy_coloring(el)
+    cnt = np.bincount(c)
+    assert (n, cnt.min(), cnt.max()) == (31, 3, 153)
+
+
+def test_coloring_limit_raises():
+    # a node shared by > 255 elements forces > 256 colours
+    k = 300
+    el = np.stack([np.zeros(k), 1 + 3 * np.arange(k), 2 + 3 * np.arange(k),
+                   3 + 3 * np.arange(k)], axis=1).astype(np.int32)
+    with pytest.raises(ValueError, match="256"):
+        greedy_coloring(el)
+
+
+def test_csr_pattern_and_elem_to_csr_bit_identical():
+    nodes, el = jittered_cube(5, seed=1)
+    ip, ix = csr_pattern(el, len(nodes))
+    pat = O.build_csr_pattern(el, len(nodes))
+    assert np.array_equal(ip, pat.indptr) and np.array_equal(ix, pat.indices)
+    assert np.array_equal(elem_to_csr(el, ip, ix),
+                          O.build_elem_to_csr(el, pat.indptr, pat.indices))
+
+
+def test_face_adjacency_identical_to_oracle():
+    _, el = jittered_cube(5, seed=1)
+    fn, _ = build_face_adjacency(el)
+    assert np.array_equal(fn, O.build_face_adjacency(el))
+
+
+def test_geometry_matches_oracle():
+    nodes, el = jittered_cube(5, seed=1)
+    B, V = precompute_element_geometry(nodes, el)
+    Bo, Vo = O.precompute_element_geometry(nodes, el)
+    assert np.abs(B - Bo).max() < 1e-13 * np.abs(Bo).max()
+    assert np.abs(V - Vo).max() < 1e-15 * np.abs(Vo).max()
+    bad = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], dtype=float)
+    with pytest.raises(ValueError, match="degenerate"):
+        precompute_element_geometry(bad, np.array([[0, 1, 2, 3]]))
+
+
+@pytest.mark.parametrize("shape", [(48, 16, 16), (6, 4, 3)])
+def test_channel_sizes_closed_form(shape):
+    nx, ny, nz = shape
+    if nx == 48:
+        ijk, el = channel.kuhn_box(nx, ny, nz)
+        assert len(el) == 73728 and len(ijk) == 14161
+        return
+    m = channel.bump_channel(nx, ny, nz, jitter=0.15, seed=3, rcm=True)
+    assert m.n_tets == 6 * nx * ny * nz
+    assert m.n_nodes == (nx + 1) * (ny + 1) * (nz + 1)
+    ip, ix = csr_pattern(m.elements, m.n_nodes)
+    N = m.n_nodes
+    s = sum((nx + 1 - a) * (ny + 1 - b) * (nz + 1 - c)
+            for a, b, c in [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (0, 1, 1),
+                            (1, 0, 1), (1, 1, 1)])
+    assert len(ix) == N + 2 * s
+    _, V = precompute_element_geometry(m.nodes, m.elements)
+    assert V.min() > 0.0
+    # flat boundaries: inflow/outflow areas are 1, side walls area 3 - bump
+    for tag, area in (("inflow", 1.0), ("outflow", 1.0), ("ceiling", 3.0)):
+        f = m.boundary_faces[tag]
+        p = m.nodes[f]
+        a = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+        assert a.sum() == pytest.approx(area, rel=1e-12)
+    b = channel.neumann_flux_rhs(m)
+    assert abs(b.sum()) < 1e-14 and b.min() < 0 < b.max()
+
+
+def test_bump_profile():
+    x = np.array([0.5, 1.0, 1.5, 2.0, 2.5])
+    y = channel.bump_profile(x, 0.1)
+    assert y[0] == 0 and y[-1] == 0 and abs(y[1]) < 1e-15 and abs(y[3]) < 1e-15
+    assert y[2] == pytest.approx(0.1, rel=1e-14)
+
+
+def test_mach_schedule_and_ew():
+    from up3d_amd.solve.continuation import mach_schedule
+    from up3d_amd.solve.newton import _ew_forcing
+
+    assert mach_schedule(0.84, 0.7, 0.05) == pytest.approx([0.7, 0.75, 0.8, 0.84])
+    assert mach_schedule(0.6) == [0.6]
+    assert _ew_forcing(1.0, None, None) == 1e-2
+    assert _ew_forcing(1e-3, 1e-1, 1e-2) == pytest.approx(0.9e-4)
+    assert _ew_forcing(1e-12, 1.0, 1e-2) == 1e-10

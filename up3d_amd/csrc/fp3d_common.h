@@ -46,3 +46,6 @@ constexpr int kReduceBlocks = 1024;
   } while (0)
 
 #define FP3D_STREAM(s) ((hipStream_t)(s))
+
+// element connectivity rows are loaded as one int4 (16 bytes)
+#define FP3D_ALIGNED16(p) ((((uintptr_t)(p)) & 15u) == 0)

@@ -645,6 +645,7 @@ int fp3d_element_velocity_q2(int64_t n_tets, const int32_t *elements,
                              double *grad_out, double *q2_out, void *stream) {
   FP3D_REQUIRE(n_tets == 0 || (elements && B && phi && grad_out && q2_out),
                "element_velocity_q2: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements), "elements must be 16-byte aligned");
   ELEMWISE(k_velocity_q2, n_tets, elements, B, phi, grad_out, q2_out);
 }
 
@@ -656,6 +657,7 @@ int fp3d_element_state(int64_t n_tets, const int32_t *elements, const double *B,
   FP3D_REQUIRE(n_tets == 0 || (elements && B && phi && grad_out && q2n_out &&
                                q2l_out && rho_out),
                "element_state: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements), "elements must be 16-byte aligned");
   FP3D_REQUIRE(u2 > 0.0, "element_state: u_inf^2 must be > 0");
   ELEMWISE(k_element_state, n_tets, elements, B, phi, u2, m_inf, gamma, q2cap,
            grad_out, q2n_out, q2l_out, rho_out, lim_out);
@@ -788,6 +790,9 @@ int fp3d_assemble_residual_colored(int n_colors, const int64_t *color_offsets,
   int rc = check_offsets(n_colors, color_offsets);
   if (rc) return rc;
   FP3D_REQUIRE(R && n_nodes >= 0, "assemble_residual: R missing");
+  FP3D_REQUIRE(elements && B && V && rho_tilde && grad,
+               "assemble_residual: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements), "elements must be 16-byte aligned");
   if (n_nodes > 0)
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n_nodes)), dim3(kBlock), 0,
                        FP3D_STREAM(stream), n_nodes, 0.0, R);

@@ -143,14 +143,13 @@ int spmv_any(int64_t n_rows, int64_t nnz, const int32_t *ptr,
 }
 
 // ------------------------------------------------------------ BLAS-1 etc.
-__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x,
-                        double b, double *__restrict__ y) {
+__global__ void k_axpby(int64_t n, double a, const double *x, double b,
+                        double *y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i] + b * y[i];
 }
 __global__ void k_mask_copy(int64_t n, const uint8_t *__restrict__ m,
-                            const double *__restrict__ x,
-                            double *__restrict__ y) {
+                            const double *x, double *y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = m[i] ? x[i] : 0.0;
 }
@@ -159,8 +158,7 @@ __global__ void k_to_f32(int64_t n, const double *__restrict__ x,
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (float)x[i];
 }
-__global__ void k_scale(int64_t n, double a, const double *__restrict__ x,
-                        double *__restrict__ y) {
+__global__ void k_scale(int64_t n, double a, const double *x, double *y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i];
 }
@@ -587,6 +585,7 @@ int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
   const double *W = nullptr, *tval = nullptr;
   if (t3 && n_active > 0) {
     FP3D_REQUIRE(elements && upstream, "csr_spmv_term3: Term 3 needs mesh");
+    FP3D_REQUIRE(FP3D_ALIGNED16(elements), "elements must be 16-byte aligned");
     T3Layout L = t3_layout(n_tets, n_rows);
     const char *base = (const char *)t3;
     const int32_t *act = (const int32_t *)(base + L.act);

@@ -1,0 +1,1 @@
+"""Device element kernels (pyfp3d/kernels)."""

@@ -1,0 +1,401 @@
+"""Device Krylov solvers and the reference's linear-algebra entry points
+(pyfp3d/solve/linear.py).
+
+* gmres_dev: restarted GMRES with left preconditioning and the same
+  restart / inner-tolerance logic as the SciPy routine the reference calls
+  (scipy.sparse.linalg.gmres, pyfp3d/solve/linear.py:134-201): convergence
+  is declared on the TRUE residual ||b - A x|| <= max(atol, rtol ||b||).
+  The Arnoldi step uses classical Gram-Schmidt with one re-orthogonalisation
+  (CGS2) so each orthogonalisation is two fused multi-dot / multi-axpy
+  passes instead of k dependent dot products; dots and norms are
+  deterministic device reductions in float64, Givens rotations run on the
+  host in float64 (one device->host copy per Krylov step).
+* pcg_dev: preconditioned CG (scipy.sparse.linalg.cg semantics) for the SPD
+  Picard systems.
+* NewtonOperator: J_ff x with J's Terms 1+2 in CSR (float64 or float32
+  values) and Term 3 applied matrix-free, Dirichlet rows as identity.
+"""
+
+from __future__ import annotations
+
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+from scipy.linalg import get_lapack_funcs
+
+from .. import _native
+from ..device import empty, ptr, require_gpu, stream_handle, to_dev, to_host, zeros
+from .amg import AMGHierarchy, DeviceCSR
+
+_lartg = get_lapack_funcs("lartg", dtype=np.float64)
+
+
+class _Red:
+    """Device scratch for the deterministic reductions."""
+
+    def __init__(self, max_nv: int):
+        L = _native.lib()
+        self.work = empty(max(L.fp3d_multi_dot_work_doubles(max_nv),
+                              L.fp3d_reduce_work_doubles()))
+        self.out = empty(2 * max_nv + 8)
+
+    def norm(self, x: torch.Tensor, at: int = 0) -> None:
+        _native.call("fp3d_norms", x.numel(), ptr(x), ptr(self.work),
+                     ptr(self.out[at:]), stream_handle())
+
+    def dots(self, V: torch.Tensor, nv: int, w: torch.Tensor, at: int) -> None:
+        n = w.numel()
+        _native.call("fp3d_multi_dot", n, nv, ptr(V), V.stride(0), ptr(w),
+                     ptr(self.work), ptr(self.out[at:]), stream_handle())
+
+
+def norm2(x: torch.Tensor, red: Optional[_Red] = None) -> float:
+    red = red or _Red(1)
+    red.norm(x)
+    return float(red.out[0].item())
+
+
+def dot(a: torch.Tensor, b: torch.Tensor, red: Optional[_Red] = None) -> float:
+    red = red or _Red(1)
+    _native.call("fp3d_multi_dot", a.numel(), 1, ptr(a), a.numel(), ptr(b),
+                 ptr(red.work), ptr(red.out), stream_handle())
+    return float(red.out[0].item())
+
+
+def axpby(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
+    _native.call("fp3d_axpby", x.numel(), float(a), ptr(x), float(b), ptr(y),
+                 stream_handle())
+    return y
+
+
+def scale(a: float, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+    _native.call("fp3d_scale", x.numel(), float(a), ptr(x), ptr(y),
+                 stream_handle())
+    return y
+
+
+def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
+              x0: Optional[torch.Tensor] = None, rtol: float = 1e-5,
+              atol: float = 0.0, restart: int = 20, maxiter: Optional[int] = None,
+              callback: Optional[Callable] = None):
+    """Solve A x = b on the device.  matvec(x, y) / psolve(x, y) write into y.
+    Returns (x, info, n_inner) with SciPy's info convention."""
+    n = b.numel()
+    if maxiter is None:
+        maxiter = 10 * n
+    restart = int(min(restart, n))
+    red = _Red(restart + 1)
+    tmp = empty(n)
+    r = empty(n)
+
+    def prec(src, dst):
+        if psolve is None:
+            dst.copy_(src)
+        else:
+            psolve(src, dst)
+        return dst
+
+    bnrm2 = norm2(b, red)
+    atol = max(float(atol), float(rtol) * bnrm2)
+    x = zeros(n) if x0 is None else x0.clone()
+    if bnrm2 == 0.0:
+        return zeros(n), 0, 0
+    eps = np.finfo(np.float64).eps
+    Mb_nrm2 = norm2(prec(b, tmp), red)
+    ptol_max_factor = 1.0
+    ptol = Mb_nrm2 * min(ptol_max_factor, atol / bnrm2)
+    presid = 0.0
+    V = empty((restart + 1, n))
+    h = np.zeros((restart, restart + 1))
+    givens = np.zeros((restart, 2))
+    inner = 0
+    rnorm = np.inf
+    x_nonzero = x0 is not None and bool(torch.any(x != 0).item())
+    for iteration in range(maxiter):
+        if iteration == 0:
+            if x_nonzero:
+                matvec(x, tmp)
+                r.copy_(b)
+                axpby(-1.0, tmp, 1.0, r)
+            else:
+                r.copy_(b)
+            if norm2(r, red) < atol:
+                return x, 0, 0
+        prec(r, V[0])
+        t0 = norm2(V[0], red)
+        scale(1.0 / t0, V[0], V[0])
+        S = np.zeros(restart + 1)
+        S[0] = t0
+        breakdown = False
+        col = 0
+        for col in range(restart):
+            matvec(V[col], tmp)
+            w = V[col + 1]
+            prec(tmp, w)
+            k = col + 1
+            # slots: out[0] = ||w|| before, out[2:2+k] = first-pass h,
+            # out[2+k:2+2k] = re-orthogonalisation h, out[2+2k] = ||w|| after
+            sh = stream_handle()
+            red.norm(w, 0)
+            red.dots(V, k, w, 2)
+            _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
+                         ptr(red.out[2:]), -1.0, ptr(w), sh)
+            red.dots(V, k, w, 2 + k)
+            _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
+                         ptr(red.out[2 + k:]), -1.0, ptr(w), sh)
+            red.norm(w, 2 + 2 * k)
+            host = to_host(red.out[: 4 + 2 * k])
+            h0 = host[0]
+            hv = host[2:2 + k] + host[2 + k:2 + 2 * k]
+            h1 = host[2 + 2 * k]
+            h[col, :k] = hv
+            h[col, col + 1] = h1
+            if h1 <= eps * h0:
+                h[col, col + 1] = 0.0
+                breakdown = True
+            else:
+                scale(1.0 / h1, w, w)
+            for j in range(col):
+                c, s_ = givens[j, 0], givens[j, 1]
+                n0, n1 = h[col, j], h[col, j + 1]
+                h[col, j], h[col, j + 1] = c * n0 + s_ * n1, -s_ * n0 + c * n1
+            c, s_, mag = _lartg(h[col, col], h[col, col + 1])
+            givens[col, 0], givens[col, 1] = c, s_
+            h[col, col], h[col, col + 1] = mag, 0.0
+            t = -s_ * S[col]
+            S[col], S[col + 1] = c * S[col], t
+            presid = abs(t)
+            inner += 1
+            if callback is not None:
+                callback(presid / bnrm2)
+            if presid <= ptol or breakdown:
+                break
+        if h[col, col] == 0.0:
+            S[col] = 0.0
+        y = np.zeros(col + 1)
+        y[:] = S[:col + 1]
+        for k2 in range(col, 0, -1):
+            if y[k2] != 0.0:
+                y[k2] /= h[k2, k2]
+                y[:k2] -= y[k2] * h[k2, :k2]
+        if y[0] != 0.0:
+            y[0] /= h[0, 0]
+        yd = to_dev(y, torch.float64)
+        _native.call("fp3d_multi_axpy", n, col + 1, ptr(V), V.stride(0), ptr(yd),
+                     1.0, ptr(x), stream_handle())
+        matvec(x, tmp)
+        r.copy_(b)
+        axpby(-1.0, tmp, 1.0, r)
+        rnorm = norm2(r, red)
+        if rnorm <= atol:
+            break
+        elif breakdown:
+            break
+        elif presid <= ptol:
+            ptol_max_factor = max(eps, 0.25 * ptol_max_factor)
+        else:
+            ptol_max_factor = min(1.0, 1.5 * ptol_max_factor)
+        ptol = presid * min(ptol_max_factor, atol / rnorm)
+    info = 0 if rnorm <= atol else maxiter
+    return x, info, inner
+
+
+def pcg_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
+            x0: Optional[torch.Tensor] = None, rtol: float = 1e-5,
+            atol: float = 0.0, maxiter: Optional[int] = None):
+    """Preconditioned CG (scipy.sparse.linalg.cg semantics: stop when
+    ||r|| <= max(atol, rtol ||b||)).  Returns (x, info, n_iter)."""
+    n = b.numel()
+    maxiter = 10 * n if maxiter is None else maxiter
+    red = _Red(1)
+    bnrm2 = norm2(b, red)
+    if bnrm2 == 0.0:
+        return zeros(n), 0, 0
+    atol = max(float(atol), float(rtol) * bnrm2)
+    x = zeros(n) if x0 is None else x0.clone()
+    r = b.clone()
+    q = empty(n)
+    if x0 is not None:
+        matvec(x, q)
+        axpby(-1.0, q, 1.0, r)
+    z = empty(n)
+    p = empty(n)
+    rho_prev = None
+    for it in range(maxiter):
+        if norm2(r, red) < atol:
+            return x, 0, it
+        if psolve is None:
+            z.copy_(r)
+        else:
+            psolve(r, z)
+        rho_cur = dot(r, z, red)
+        if rho_prev is None:
+            p.copy_(z)
+        else:
+            axpby(1.0, z, rho_cur / rho_prev, p)
+        matvec(p, q)
+        alpha = rho_cur / dot(p, q, red)
+        axpby(alpha, p, 1.0, x)
+        axpby(-alpha, q, 1.0, r)
+        rho_prev = rho_cur
+    return x, (0 if norm2(r, red) < atol else maxiter), maxiter
+
+
+# ----------------------------------------------------- Newton Jacobian operator
+class NewtonOperator:
+    """x -> J_ff x on full-length vectors with zero Dirichlet entries: the
+    Terms 1+2 CSR (Dirichlet rows/cols masked to identity, float64 or
+    float32 values) plus the matrix-free Term 3 of the active elements."""
+
+    def __init__(self, store, free_mask: torch.Tensor):
+        self.store = store
+        self.free_mask = free_mask
+        self.n = store.n_nodes
+        self.data = empty(store.nnz)
+        self.data32 = None
+        self.is_f32 = 0
+        nbytes = _native.lib().fp3d_term3_workspace_bytes(store.n_tets,
+                                                          store.n_nodes)
+        self.t3 = torch.empty(int(nbytes), dtype=torch.uint8,
+                              device=store.device)
+        self.n_active = 0
+        self.upstream = None
+
+    def finalize(self, f32: bool = False) -> None:
+        """Mask Dirichlet rows/cols in place; optionally keep a float32 copy
+        that the matvec then uses."""
+        st = self.store
+        if f32 and self.data32 is None:
+            self.data32 = torch.empty(st.nnz, dtype=torch.float32,
+                                      device=st.device)
+        _native.call("fp3d_csr_mask_dirichlet", st.n_nodes, ptr(st.indptr),
+                     ptr(st.indices), ptr(self.free_mask), ptr(self.data),
+                     ptr(self.data32) if f32 else None, stream_handle())
+        self.is_f32 = 1 if f32 else 0
+
+    def build_term3(self, grad, upstream, s_u, lim, inv_u2) -> int:
+        import ctypes as C
+
+        st = self.store
+        na = C.c_int64(0)
+        _native.check(_native.lib().fp3d_term3_build(
+            st.n_tets, st.n_nodes, ptr(st.el), ptr(st.inc_ptr), ptr(st.inc_ent),
+            ptr(st.B), ptr(st.V), ptr(grad), ptr(upstream), ptr(s_u), ptr(lim),
+            ptr(self.free_mask), float(inv_u2), ptr(self.t3), C.byref(na),
+            stream_handle()), "term3_build")
+        self.n_active = int(na.value)
+        self.upstream = upstream
+        return self.n_active
+
+    def matvec(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+        st = self.store
+        vals = self.data32 if self.is_f32 else self.data
+        use_t3 = self.n_active > 0
+        _native.call("fp3d_csr_spmv_term3", st.n_nodes, st.nnz, ptr(st.indptr),
+                     ptr(st.indices), ptr(vals), self.is_f32, ptr(st.el),
+                     ptr(self.upstream) if use_t3 else None,
+                     ptr(self.t3) if use_t3 else None, self.n_active,
+                     st.n_tets, ptr(self.free_mask), ptr(x), ptr(y),
+                     stream_handle())
+        return y
+
+
+# ------------------------------------------- reference-compatible API
+def apply_dirichlet(A: sp.spmatrix, b: np.ndarray, dirichlet_nodes: np.ndarray,
+                    dirichlet_values: np.ndarray):
+    """Free-dof principal-submatrix reduction (pyfp3d/solve/linear.py:20-58);
+    host bookkeeping, returns (A_free, b_free, free_idx, phi_dirichlet)."""
+    n = A.shape[0]
+    is_dir = np.zeros(n, dtype=bool)
+    is_dir[dirichlet_nodes] = True
+    free = np.where(~is_dir)[0]
+    phi_d = np.zeros(n)
+    phi_d[dirichlet_nodes] = dirichlet_values
+    A = A.tocsr()
+    b_free = b[free] - A[free][:, dirichlet_nodes] @ dirichlet_values
+    return A[free][:, free].tocsr(), b_free, free, phi_d
+
+
+class _HostOperatorOnDevice:
+    """Wrap a scipy matrix (or an object with .matvec) as a device matvec."""
+
+    def __init__(self, A, f32: bool = False):
+        if sp.issparse(A):
+            self.csr = DeviceCSR(A, f32=f32)
+            self.mv = self.csr.matvec
+            self.shape = A.shape
+        elif hasattr(A, "matvec") and hasattr(A, "shape"):
+            self.csr = None
+            self.shape = A.shape
+            op = A
+
+            def mv(x, y):
+                y.copy_(to_dev(np.asarray(op.matvec(to_host(x)), dtype=np.float64)))
+                return y
+
+            self.mv = mv
+        else:
+            raise TypeError(f"unsupported operator {type(A)!r}")
+
+
+def build_amg_preconditioner(A: sp.spmatrix, precision: str = "f64"):
+    """(ml, M): SA hierarchy (host set-up, seed pinned) and its device
+    preconditioner (pyfp3d/solve/linear.py:61-81)."""
+    ml = AMGHierarchy(sp.csr_matrix(A), precision=precision, seed=0)
+    return ml, ml
+
+
+def _psolve_of(M):
+    if M is None:
+        return None
+    if isinstance(M, AMGHierarchy):
+        return M.apply
+    if hasattr(M, "matvec"):
+        def ps(x, y):
+            y.copy_(to_dev(np.asarray(M.matvec(to_host(x)), dtype=np.float64)))
+            return y
+        return ps
+    raise TypeError(f"unsupported preconditioner {type(M)!r}")
+
+
+def solve_gmres(A, b, M=None, rtol: float = 1e-6, atol: float = 0.0,
+                x0=None, restart: int = 60, maxiter: int = 50,
+                on_fail: str = "retry") -> Tuple[np.ndarray, int, int]:
+    """Restarted preconditioned GMRES on the device with the reference's
+    retry-at-2x-restart policy (pyfp3d/solve/linear.py:134-201).
+    A: scipy sparse matrix, an object with .matvec, or a device callable
+    matvec(x, y).  Returns (x, n_inner_iterations, info); RuntimeError under
+    on_fail='retry' when the retry also fails."""
+    host = not isinstance(b, torch.Tensor)
+    bd = to_dev(b, torch.float64) if host else b
+    mv = A if callable(A) and not sp.issparse(A) and not hasattr(A, "matvec") \
+        else _HostOperatorOnDevice(A).mv
+    ps = _psolve_of(M)
+    x0d = None if x0 is None else (to_dev(x0, torch.float64)
+                                   if not isinstance(x0, torch.Tensor) else x0)
+    x, info, n_it = gmres_dev(mv, bd, ps, x0d, rtol, atol, restart, maxiter)
+    if info != 0 and on_fail == "retry":
+        n_first = n_it
+        x, info, n_it = gmres_dev(mv, bd, ps, x, rtol, atol, 2 * restart, maxiter)
+        if info != 0:
+            raise RuntimeError(
+                f"GMRES did not converge (info={info}) after {n_first} + {n_it} "
+                f"inner iterations (restart {restart} then {2 * restart}, "
+                f"rtol={rtol:g})")
+        n_it += n_first
+    return (to_host(x) if host else x), n_it, info
+
+
+def solve_cg_amg(A: sp.spmatrix, b, rtol: float = 1e-10, maxiter: int = 500):
+    """CG preconditioned by SA-AMG on the device (pyfp3d/solve/linear.py:204-239).
+    Returns (x, n_iterations); RuntimeError when CG does not converge."""
+    host = not isinstance(b, torch.Tensor)
+    _, M = build_amg_preconditioner(A)
+    Ad = DeviceCSR(sp.csr_matrix(A))
+    bd = to_dev(b, torch.float64) if host else b
+    x, info, n_it = pcg_dev(Ad.matvec, bd, M.apply, rtol=rtol, maxiter=maxiter)
+    if info != 0:
+        raise RuntimeError(f"CG did not converge (info={info}) after {n_it} iterations")
+    return (to_host(x) if host else x), n_it

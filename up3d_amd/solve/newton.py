@@ -1,0 +1,570 @@
+"""Newton solve of the non-lifting full-potential problem on the device
+(pyfp3d/solve/newton.py:97-1307, specialised to a case without wake
+stations: the Gamma unknowns, B, K and F of the coupled system vanish and
+the eliminated step is J_ff dphi = -R_free).  The reference's conforming
+driver needs a WakeCut and cannot run a non-lifting case as shipped, so
+this is the same chain with n_st = 0, with identical keyword names,
+defaults, control logic and result keys.
+
+Per Newton step (all device-resident, storage element order):
+  eval_residual: fused grad/q2/limiter/density sweep -> walk + nu switch +
+      rho_tilde (live) or frozen sweep -> coloured residual scatter -> minus
+      the external (boundary-flux) RHS -> ||R_free||
+  assemble: sensitivities (s_e, s_u) -> Terms 1+2 coloured into the fixed
+      CSR -> Dirichlet embedding (+ float32 copy) -> Term 3 gather table
+  precond: SA-AMG of the Picard matrix A(rho_tilde) every amg_rebuild_every
+      steps (host set-up, device V-cycle)
+  linsolve: device GMRES(restart) with Eisenstat-Walker forcing
+  line search: safety-only backtracking on ||R||^2, best-of on failure.
+
+precision="fp64" is the all-FP64 solve; precision="fp32" keeps the
+residual, norms, Arnoldi/Givens and the coarsest AMG solve in float64 and
+moves the GMRES Jacobian values and every AMG matrix to float32; the outer
+Newton iteration on the float64 residual acts as iterative refinement.
+"""
+
+from __future__ import annotations
+
+import time
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from .. import _native
+from ..device import empty, ptr, stream_handle, to_dev, to_host
+from ..kernels.jacobian import PicardOperator
+from ..kernels.upwind import UpwindOperator
+from ..physics.isentropic import q2_at_mach
+from .common import (Reducer, amg_from_device, dirichlet_setup, mask_dirichlet,
+                     masked)
+from .continuation import mach_schedule
+from .linear import NewtonOperator, axpby, gmres_dev, scale
+from .timing import finalize, new_timings, snapshot, step_delta, sum_timings
+
+
+class _State:
+    """Device buffers of one evaluated state (phi, element state, R)."""
+
+    def __init__(self, n, nt):
+        self.phi = empty(n)
+        self.grad = empty((nt, 3))
+        self.q2n = empty(nt)
+        self.q2l = empty(nt)
+        self.rho = empty(nt)
+        self.rt = empty(nt)
+        self.lim = empty(nt, torch.uint8)
+        self.R = empty(n)          # free-masked residual
+        self.n_limited = 0
+        self.n_floored = 0
+        self.nu_max = 0.0
+        self.n_nu_active = 0
+        self.r_norm = 0.0
+        self.merit = 0.0
+
+
+class NewtonWorkspace:
+    """Per-case device workspace (pyfp3d/solve/newton.py:97-501 for
+    n_st = 0): operators, the free/Dirichlet split, the external RHS, state
+    buffers, the Jacobian operator and the preconditioner."""
+
+    def __init__(self, nodes, elements, dirichlet_nodes, dirichlet_values,
+                 u_inf: float = 1.0, gamma_air: float = 1.4,
+                 external_rhs: Optional[np.ndarray] = None,
+                 precision: str = "fp64"):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"unknown precision {precision!r}")
+        self.precision = precision
+        self.op = PicardOperator(nodes, elements)
+        self.upw = UpwindOperator(nodes, elements, weighted=False)
+        st = self.st = self.op.store
+        self.n = n = st.n_nodes
+        self.nt = nt = st.n_tets
+        self.u_inf = float(u_inf)
+        self.gamma_air = float(gamma_air)
+        self.fm, self.dn, self.dv, self.phi_d = dirichlet_setup(
+            n, dirichlet_nodes, dirichlet_values, st.device)
+        self.n_free = int(n - len(np.unique(np.asarray(dirichlet_nodes))))
+        self.b_ext = None
+        if external_rhs is not None:
+            e = np.asarray(external_rhs, dtype=np.float64)
+            if e.shape != (n,):
+                raise ValueError(f"external_rhs must be ({n},), got {e.shape}")
+            self.b_ext = to_dev(e)
+        self.states = [_State(n, nt) for _ in range(3)]
+        self.jac = NewtonOperator(st, self.fm)
+        self.A_pic = empty(st.nnz)
+        self.red = Reducer()
+        self.work = empty(_native.lib().fp3d_reduce_work_doubles())
+        self.cnt = empty(1)
+        self.m_inf = None
+        self.q2cap = np.inf
+        self.precond = None
+
+    def set_mach(self, m_inf: float, m_cap: float = 3.0) -> None:
+        if not 0.0 <= m_inf < 1.0:
+            raise ValueError(f"Newton driver needs 0 <= M_inf < 1, got {m_inf}")
+        self.m_inf = float(m_inf)
+        self.q2cap = float(q2_at_mach(m_cap, m_inf, self.gamma_air))
+
+    def free_state(self, *busy):
+        for s in self.states:
+            if all(s is not b for b in busy):
+                return s
+        raise RuntimeError("no free state buffer")
+
+    # ------------------------------------------------------- residual path
+    def eval_residual(self, s: _State, upwind_c, m_crit, rho_floor,
+                      frozen=None) -> _State:
+        """The single state-reconstruction + residual path; s.phi must hold
+        the full potential (Dirichlet entries set)."""
+        op, upw = self.op, self.upw
+        op.state_dev(s.phi, self.u_inf ** 2, self.m_inf, self.gamma_air,
+                     self.q2cap, s.grad, s.q2n, s.q2l, s.rho, s.lim)
+        _native.call("fp3d_count_limited", self.nt, ptr(s.lim), ptr(self.work),
+                     ptr(self.cnt), stream_handle())
+        s.n_limited = int(self.cnt.item())
+        if self.m_inf > 0.0 and frozen is not None:
+            upw.frozen_dev(s.q2l, s.rho, frozen[0], frozen[1], self.m_inf,
+                           upwind_c, m_crit, self.gamma_air, rho_floor)
+            s.rt.copy_(upw.d_rt)
+        elif self.m_inf > 0.0:
+            upw.rho_tilde_dev(s.grad, s.q2l, s.rho, self.m_inf, upwind_c, m_crit,
+                              self.gamma_air, rho_floor)
+            s.rt.copy_(upw.d_rt)
+        else:
+            s.rt.copy_(s.rho)
+            upw.nu_max, upw.n_supersonic, upw.n_floored = 0.0, 0, 0
+        s.n_floored = upw.n_floored
+        s.nu_max = upw.nu_max
+        s.n_nu_active = upw.n_supersonic
+        R = op.residual_dev(s.rt, s.grad)
+        if self.b_ext is not None:
+            axpby(-1.0, self.b_ext, 1.0, R)
+        masked(self.fm, R, s.R)
+        l2, linf = self.red.norms(s.R)
+        s.r_norm = linf
+        s.merit = l2 * l2
+        return s
+
+    # ------------------------------------------------------- linearisation
+    def assemble_coupled(self, s: _State, upwind_c, m_crit, rho_floor,
+                         frozen=None):
+        """Exact Jacobian at state s into the NewtonOperator (Terms 1+2 in
+        CSR, Dirichlet-embedded, Term 3 as a gather table)."""
+        upw = self.upw
+        inv_u2 = 1.0 / (self.u_inf * self.u_inf)
+        if self.m_inf > 0.0 and frozen is not None:
+            upstream = frozen[0]
+            se, su = upw.frozen_sensitivities_dev(
+                s.q2l, s.rho, frozen[0], frozen[1], self.m_inf, upwind_c,
+                m_crit, self.gamma_air, rho_floor)
+        elif self.m_inf > 0.0:
+            se, su, upstream = upw.sensitivities_dev(
+                s.grad, s.q2l, s.rho, self.m_inf, upwind_c, m_crit,
+                self.gamma_air, rho_floor)
+        else:
+            se = upw.d_se.zero_()
+            su = upw.d_su.zero_()
+            upstream = upw.walk_dev(s.grad)
+        self.op.jacobian_dev(s.rt, s.grad, se, s.lim, inv_u2, self.jac.data)
+        self.jac.finalize(f32=self.precision == "fp32")
+        if self.m_inf > 0.0:
+            self.jac.build_term3(s.grad, upstream, su, s.lim, inv_u2)
+        else:
+            self.jac.n_active = 0
+        return self.jac
+
+    def build_preconditioner(self, s: _State):
+        self.op.matrix_dev(s.rt, self.A_pic)
+        mask_dirichlet(self.st, self.fm, self.A_pic)
+        self.precond = amg_from_device(
+            self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64")
+        return self.precond
+
+    def freeze(self, s: _State, upwind_c, m_crit, rho_floor):
+        return self.upw.freeze_dev(s.grad, s.q2l, s.rho, self.m_inf, upwind_c,
+                                   m_crit, self.gamma_air, rho_floor)
+
+
+def _ew_forcing(r_norm, r_norm_prev, eta_prev, eta0=1e-2, gamma_ew=0.9,
+                eta_max=1e-2, eta_floor=1e-10):
+    """Eisenstat-Walker choice 2 with the standard safeguard
+    (pyfp3d/solve/newton.py:504-513)."""
+    if r_norm_prev is None:
+        return eta0
+    eta = gamma_ew * (r_norm / r_norm_prev) ** 2
+    safeguard = gamma_ew * eta_prev ** 2
+    if safeguard > 0.1:
+        eta = max(eta, safeguard)
+    return float(min(eta_max, max(eta, eta_floor)))
+
+
+def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
+                 m_inf: float, u_inf: float = 1.0, gamma_air: float = 1.4,
+                 upwind_c: float = 1.5, m_crit: float = 0.95,
+                 m_cap: float = 3.0, rho_floor: float = 0.05,
+                 phi_init: Optional[np.ndarray] = None, n_picard_seed: int = 5,
+                 n_newton_max: int = 30, tol_residual: float = 1e-10,
+                 ew_eta0: float = 1e-2, ew_gamma: float = 0.9,
+                 ew_eta_max: float = 1e-2, amg_rebuild_every: int = 2,
+                 gmres_restart: int = 60, gmres_maxiter: int = 10,
+                 line_search: bool = True, rtol_seed: float = 1e-7,
+                 tol_residual_loose: Optional[float] = None,
+                 tol_residual_rel: Optional[float] = None,
+                 accept_on_stall: bool = False,
+                 freeze_tol: Optional[float] = None,
+                 freeze_refresh_max: int = 2, freeze_max_reverts: int = 3,
+                 workspace: Optional[NewtonWorkspace] = None,
+                 external_rhs: Optional[np.ndarray] = None,
+                 precision: str = "fp64", verbose: bool = False
+                 ) -> Dict[str, object]:
+    """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
+    n_st = 0; precond is always the device AMG).  Converged when
+    ||R_free||_inf < tol_residual with 0 limited and 0 floored elements."""
+    if m_inf > 0.0 and upwind_c <= 0.0:
+        raise ValueError("the Newton driver runs with the walk upwind machinery "
+                         "active for m_inf > 0; pass upwind_c > 0")
+    ws = workspace
+    if ws is None:
+        ws = NewtonWorkspace(nodes, elements, dirichlet_nodes, dirichlet_values,
+                             u_inf, gamma_air, external_rhs, precision)
+    elif ws.precision != precision:
+        raise ValueError(f"workspace precision {ws.precision!r} != {precision!r}")
+    ws.set_mach(m_inf, m_cap)
+    t_wall0 = time.perf_counter()
+    timings = new_timings()
+    step_records = []
+
+    def sync():
+        torch.cuda.synchronize()
+
+    # ---- initial state
+    t0 = time.perf_counter()
+    cur = ws.states[0]
+    if phi_init is not None:
+        cur.phi.copy_(to_dev(phi_init, torch.float64)
+                      if not isinstance(phi_init, torch.Tensor) else phi_init)
+        cur.phi.index_copy_(0, ws.dn, ws.dv)
+    elif n_picard_seed > 0:
+        from .picard import solve_subsonic
+
+        seed = solve_subsonic(nodes, elements, dirichlet_nodes, dirichlet_values,
+                              m_inf=m_inf, gamma_air=gamma_air, u_inf=u_inf,
+                              body_source_rhs=external_rhs,
+                              n_picard_max=n_picard_seed, tol_rho=1e-3,
+                              rtol=rtol_seed)
+        cur.phi.copy_(to_dev(seed["phi"]))
+    else:
+        from ..constraints.dirichlet import freestream_phi
+
+        cur.phi.copy_(to_dev(freestream_phi(ws.st.nodes_host, 0.0, u_inf)))
+        cur.phi.index_copy_(0, ws.dn, ws.dv)
+    sync()
+    timings["seed"] = time.perf_counter() - t0
+
+    residual_history, clamp_history, eta_history, newton_orders = [], [], [], []
+    n_gmres_total = 0
+    n_gmres_stalled = 0
+    converged = False
+    accept_reason = None
+    eta_prev = None
+    t0 = time.perf_counter()
+    ws.eval_residual(cur, upwind_c, m_crit, rho_floor)
+    sync()
+    timings["residual"] += time.perf_counter() - t0
+
+    frozen = None
+    n_freeze_refresh = 0
+    residual_unfrozen = None
+    n_assignment_stale = 0
+    assignment_cycle = False
+    prev_live_residual = None
+    freeze_point = None            # (phi copy, r_norm)
+    freeze_cooldown = 0
+    n_freeze_reverts = 0
+    r_level_best = np.inf
+    freeze_armed = freeze_tol is not None
+    prev_snap = snapshot(timings)
+    prev_gm = [0]
+    dphi = empty(ws.n)
+    rhs = empty(ws.n)
+    M_pre = None
+
+    def close_step(rec):
+        nonlocal prev_snap
+        rec.update(step_delta(timings, prev_snap))
+        rec["n_lin_iters"] = n_gmres_total - prev_gm[0]
+        rec["wall_cum_s"] = time.perf_counter() - t_wall0
+        prev_snap = snapshot(timings)
+        prev_gm[0] = n_gmres_total
+
+    for it in range(n_newton_max):
+        r_norm = cur.r_norm
+        residual_history.append(r_norm)
+        clamp_history.append((cur.n_limited, cur.n_floored))
+        if step_records:
+            close_step(step_records[-1])
+        rec = {"i": it, "residual": r_norm, "n_limited": cur.n_limited,
+               "n_floored": cur.n_floored, "frozen": frozen is not None,
+               "n_lin_solves": 0, "eta": None, "lam": None}
+        step_records.append(rec)
+        if len(residual_history) >= 3:
+            r2, r1, r0 = residual_history[-1], residual_history[-2], residual_history[-3]
+            if r2 > 0.0 and r1 > 0.0 and r0 > 0.0 and r1 != r0:
+                newton_orders.append(float(np.log(r2 / r1) / np.log(r1 / r0)))
+        if verbose:
+            print(f"  newton {it:2d}: |R|={r_norm:.3e} lim={cur.n_limited} "
+                  f"flr={cur.n_floored}" + (" [frozen]" if frozen is not None else ""),
+                  flush=True)
+        r_level_best = min(r_level_best, r_norm)
+
+        # frozen-phase safety net: revert to the freeze point when diverging
+        if frozen is not None and freeze_point is not None:
+            if (r_norm > 10.0 * freeze_point[1] or cur.n_limited > 0
+                    or cur.n_floored > 0):
+                cur.phi.copy_(freeze_point[0])
+                frozen = None
+                freeze_point = None
+                freeze_cooldown = 5
+                n_freeze_reverts += 1
+                r_level_best = np.inf
+                if n_freeze_reverts >= freeze_max_reverts:
+                    freeze_armed = False
+                t0 = time.perf_counter()
+                ws.eval_residual(cur, upwind_c, m_crit, rho_floor)
+                sync()
+                timings["residual"] += time.perf_counter() - t0
+                continue
+        if freeze_cooldown > 0:
+            freeze_cooldown -= 1
+        # level fail-fast
+        if r_norm > 100.0 * r_level_best and r_norm > 1e-3:
+            break
+        live_stalled = (len(residual_history) >= 7 and r_norm < 1e-3
+                        and r_norm > 0.25 * min(residual_history[-7:-1])
+                        and r_norm < 2.0 * min(residual_history[-7:-1]))
+        if (frozen is None and freeze_armed and freeze_tol is not None
+                and freeze_cooldown == 0 and m_inf > 0.0
+                and (r_norm < freeze_tol or live_stalled)
+                and cur.n_limited == 0 and cur.n_floored == 0):
+            frozen = ws.freeze(cur, upwind_c, m_crit, rho_floor)
+            freeze_point = (cur.phi.clone(), r_norm)
+            r_level_best = np.inf
+        accept = None
+        if cur.n_limited == 0 and cur.n_floored == 0:
+            if r_norm < tol_residual:
+                accept = "tol"
+            elif frozen is None and it > 0:
+                if tol_residual_loose is not None and r_norm < tol_residual_loose:
+                    accept = "loose_tol"
+                elif (tol_residual_rel is not None
+                      and r_norm <= tol_residual_rel * residual_history[0]):
+                    accept = "rel_drop"
+                elif accept_on_stall and live_stalled:
+                    accept = "stall"
+        if accept is not None:
+            if frozen is None:
+                converged = True
+                accept_reason = accept
+                break
+            # honesty check: the live residual at the frozen solution
+            live = ws.free_state(cur)
+            live.phi.copy_(cur.phi)
+            ws.eval_residual(live, upwind_c, m_crit, rho_floor)
+            residual_unfrozen = live.r_norm
+            up_new, br_new = ws.freeze(live, upwind_c, m_crit, rho_floor)
+            n_assignment_stale = int(
+                (up_new != frozen[0]).sum().item()
+                + ((br_new != frozen[1]) & (up_new == frozen[0])).sum().item())
+            if verbose:
+                print(f"  frozen system converged; live |R|={residual_unfrozen:.3e}"
+                      f", stale assignments={n_assignment_stale}", flush=True)
+            if residual_unfrozen < tol_residual:
+                converged = True
+                cur = live
+                break
+            if (prev_live_residual is not None
+                    and residual_unfrozen > 0.5 * prev_live_residual):
+                assignment_cycle = True
+                converged = True
+                break
+            if n_freeze_refresh >= freeze_refresh_max:
+                converged = True
+                break
+            prev_live_residual = residual_unfrozen
+            n_freeze_refresh += 1
+            frozen = (up_new, br_new)
+            freeze_point = (live.phi.clone(), residual_unfrozen)
+            r_level_best = np.inf
+            cur = live
+            continue
+
+        # ---- linearisation
+        t0 = time.perf_counter()
+        ws.assemble_coupled(cur, upwind_c, m_crit, rho_floor, frozen)
+        sync()
+        timings["assembly"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        if M_pre is None or it % amg_rebuild_every == 0:
+            M_pre = ws.build_preconditioner(cur)
+        sync()
+        timings["precond"] += time.perf_counter() - t0
+        eta = _ew_forcing(r_norm, residual_history[-2] if it > 0 else None,
+                          eta_prev, ew_eta0, ew_gamma, ew_eta_max)
+        eta_prev = eta
+        eta_history.append(eta)
+        rec["eta"] = eta
+        rec["n_lin_solves"] = 1
+        t0 = time.perf_counter()
+        scale(-1.0, cur.R, rhs)
+        x, info, n_it = gmres_dev(ws.jac.matvec, rhs, M_pre.apply, rtol=eta,
+                                  restart=gmres_restart, maxiter=gmres_maxiter)
+        dphi.copy_(x)
+        sync()
+        timings["linsolve"] += time.perf_counter() - t0
+        n_gmres_total += n_it
+        if info != 0:
+            n_gmres_stalled += 1
+
+        # ---- safety-only backtracking, least-bad on failure
+        lam = 1.0
+        best = None
+        accepted = False
+        trial = None
+        t0 = time.perf_counter()
+        for _ in range(5):
+            trial = ws.free_state(cur, best[1] if best else None)
+            trial.phi.copy_(cur.phi)
+            axpby(lam, dphi, 1.0, trial.phi)
+            ws.eval_residual(trial, upwind_c, m_crit, rho_floor, frozen)
+            mt = trial.merit
+            if np.isfinite(mt) and (not line_search or mt <= cur.merit):
+                accepted = True
+                break
+            if np.isfinite(mt) and (best is None or mt < best[0]):
+                best = (mt, trial, lam)
+            lam *= 0.5
+        sync()
+        timings["residual"] += time.perf_counter() - t0
+        if not accepted:
+            if best is None:
+                raise RuntimeError(f"Newton line search failed at iteration {it}")
+            _, trial, lam = best
+        rec["lam"] = float(lam)
+        cur = trial
+
+    if step_records:
+        close_step(step_records[-1])
+    if converged and accept_reason is None:
+        accept_reason = "tol"
+    m2 = empty(ws.nt)
+    _native.call("fp3d_mach_squared_field", ws.nt, ptr(cur.q2l), float(m_inf),
+                 float(gamma_air), ptr(m2), stream_handle())
+    mach2_max = ws.red.norms(m2)[1]
+    timings["jacobian"] = timings["assembly"]
+    timings["amg_setup"] = timings["precond"]
+    timings["gmres"] = timings["linsolve"]
+    finalize(timings, time.perf_counter() - t_wall0)
+    return {
+        "phi": to_host(cur.phi), "phi_dev": cur.phi.clone(),
+        "gamma": np.zeros(0), "converged": converged,
+        "accept_reason": accept_reason,
+        "n_newton": len(residual_history) - (1 if converged else 0),
+        "residual_history": residual_history,
+        "F_history": [0.0] * len(residual_history),
+        "clamp_history": clamp_history, "newton_orders": newton_orders,
+        "eta_history": eta_history, "n_gmres_total": n_gmres_total,
+        "n_gmres_stalled": n_gmres_stalled, "froze": frozen is not None,
+        "n_freeze_refresh": n_freeze_refresh,
+        "residual_unfrozen": residual_unfrozen,
+        "n_assignment_stale": n_assignment_stale,
+        "assignment_cycle": assignment_cycle,
+        "n_freeze_reverts": n_freeze_reverts, "mach2_max": mach2_max,
+        "nu_max": cur.nu_max, "n_nu_active": cur.n_nu_active,
+        "n_limited": cur.n_limited, "n_floored": cur.n_floored,
+        "jacobian_nnz": ws.st.nnz, "n_term3_active": ws.jac.n_active,
+        "n_refactor": 0, "step_records": step_records, "timings": timings,
+        "precision": ws.precision, "workspace": ws,
+    }
+
+
+def solve_newton_transonic(nodes, elements, dirichlet_nodes, dirichlet_values,
+                           m_inf: float, m_start: float = 0.70, dm: float = 0.05,
+                           dm_min: float = 0.01, upwind_c: float = 1.5,
+                           m_crit: float = 0.95, upwind_c_post=None,
+                           freeze_tol: Optional[float] = 1e-6,
+                           intermediate_tol: Optional[float] = None,
+                           newton_kw: Optional[dict] = None,
+                           verbose: bool = False) -> Dict[str, object]:
+    """Upward Mach continuation of solve_newton
+    (pyfp3d/solve/newton.py:1144-1307): warm starts from the last CONVERGED
+    level, Mach step halved on failure down to dm_min, optional loose
+    intermediate levels and post-ramp upwind_c stages."""
+    if m_inf < m_start:
+        raise ValueError(f"upward ramp only: m_inf {m_inf} < m_start {m_start}")
+    kw = dict(newton_kw or {})
+    kw.setdefault("upwind_c", upwind_c)
+    kw.setdefault("m_crit", m_crit)
+    kw.setdefault("freeze_tol", freeze_tol)
+    ws = None
+    result = None
+    last_good = None
+    level_history, level_results = [], []
+    levels = mach_schedule(m_inf, m_start, dm)
+    loose = [True] * (len(levels) - 1) + [False]
+
+    def run_level(m, lvl_kw):
+        t_lvl = time.perf_counter()
+        res = solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
+                           m_inf=m, workspace=ws, verbose=verbose, **lvl_kw)
+        wall = time.perf_counter() - t_lvl
+        level_history.append((m, res["n_newton"], res["residual_history"][-1]))
+        level_results.append({
+            "m": m, "upwind_c": lvl_kw["upwind_c"],
+            "converged": res["converged"], "accept_reason": res["accept_reason"],
+            "n_newton": res["n_newton"],
+            "residual_history": res["residual_history"],
+            "step_records": res["step_records"],
+            "n_lin_iters": res["n_gmres_total"], "wall_s": wall,
+            "timings": res["timings"]})
+        return res
+
+    i = 0
+    while i < len(levels):
+        m = levels[i]
+        lvl_kw = dict(kw)
+        if intermediate_tol is not None and loose[i]:
+            lvl_kw.update(tol_residual_loose=intermediate_tol,
+                          tol_residual_rel=1e-3, accept_on_stall=True,
+                          freeze_tol=None)
+        if last_good is not None:
+            lvl_kw.update(phi_init=last_good["phi_dev"], n_picard_seed=0)
+        if verbose:
+            print(f"newton continuation level M = {m:.4f}", flush=True)
+        result = run_level(m, lvl_kw)
+        ws = result["workspace"]
+        if not result["converged"]:
+            m_prev = m_start if i == 0 else levels[i - 1]
+            dm_new = 0.5 * (m - m_prev)
+            if i == 0 or dm_new < dm_min:
+                break
+            levels.insert(i, m_prev + dm_new)
+            loose.insert(i, False)
+            continue
+        last_good = result
+        i += 1
+    if result["converged"] and upwind_c_post:
+        for c_post in upwind_c_post:
+            lvl_kw = dict(kw)
+            lvl_kw.update(upwind_c=c_post, phi_init=result["phi_dev"],
+                          n_picard_seed=0)
+            result = run_level(m_inf, lvl_kw)
+            if not result["converged"]:
+                break
+    result = dict(result)
+    result["level_history"] = level_history
+    result["level_results"] = level_results
+    result["timings_total"] = sum_timings([lr["timings"] for lr in level_results])
+    result.pop("workspace", None)
+    return result
