@@ -684,7 +684,11 @@ class OracleNewton:
 
     def solve(self, m_inf, phi_init, upwind_c=1.5, m_crit=0.95, m_cap=3.0,
               rho_floor=0.05, n_newton_max=30, tol_residual=1e-10,
-              freeze_tol=None):
+              freeze_tol=None, freeze_refresh_max=2):
+        """pyfp3d/solve/newton.py:781-1087 control flow for n_st = 0: freeze
+        on r < freeze_tol or a live stall (:866-881), honesty re-evaluation
+        and refresh of the freeze (:893-947), safety line search
+        (:1059-1087); direct sparse linear step."""
         phi_free = _f64(phi_init)[self.free].copy()
         R, st = self.eval_residual(phi_free, m_inf, upwind_c, m_crit, m_cap,
                                    rho_floor)
@@ -692,18 +696,43 @@ class OracleNewton:
         hist = []
         frozen = None
         converged = False
+        n_refresh = 0
+        prev_live = None
         for _ in range(n_newton_max):
             r = float(np.max(np.abs(R)))
             hist.append(r)
+            stalled = (len(hist) >= 7 and r < 1e-3
+                       and r > 0.25 * min(hist[-7:-1])
+                       and r < 2.0 * min(hist[-7:-1]))
             if (frozen is None and freeze_tol is not None and m_inf > 0.0
-                    and r < freeze_tol and st["n_limited"] == 0
+                    and (r < freeze_tol or stalled) and st["n_limited"] == 0
                     and st["n_floored"] == 0):
                 frozen = self.upw.freeze_upwind_state(
                     st["grad"], st["q2l"], st["rho"], m_inf, upwind_c, m_crit,
                     self.gamma_air, rho_floor)
             if r < tol_residual and st["n_limited"] == 0 and st["n_floored"] == 0:
-                converged = True
-                break
+                if frozen is None:
+                    converged = True
+                    break
+                R_live, st_live = self.eval_residual(phi_free, m_inf, upwind_c,
+                                                     m_crit, m_cap, rho_floor)
+                r_live = float(np.max(np.abs(R_live)))
+                if r_live < tol_residual:
+                    converged = True
+                    st = st_live
+                    break
+                if (prev_live is not None and r_live > 0.5 * prev_live) \
+                        or n_refresh >= freeze_refresh_max:
+                    converged = True
+                    break
+                prev_live = r_live
+                n_refresh += 1
+                frozen = self.upw.freeze_upwind_state(
+                    st_live["grad"], st_live["q2l"], st_live["rho"], m_inf,
+                    upwind_c, m_crit, self.gamma_air, rho_floor)
+                R, st = R_live, st_live
+                merit = float(R @ R)
+                continue
             J = self.jacobian(st, m_inf, upwind_c, m_crit, rho_floor, frozen)
             dphi = spla.splu(J).solve(-R)
             lam = 1.0
