@@ -54,8 +54,9 @@ def algorithmic_bytes(n_tets, n_nodes, nnz):
         "upwind": 116 * n_tets + 24 * n_nodes,
         # elements 16, B 96, V 8, rho_tilde 8, grad 24; R 8/node
         "residual": 152 * n_tets + 8 * n_nodes,
-        # walk inputs as above (no nu/rt writes) + q2 8 + rho 8 + se/su 16
-        "sensitivities": 132 * n_tets + 24 * n_nodes,
+        # upstream 4, q2 8, rho 8, write s_e 8, s_u 8 (+ q2/rho of u(e),
+        # non-local, counted once above)
+        "sensitivities": 36 * n_tets,
         # elem_to_csr 64, B 96, V 8, rho_tilde 8, grad 24, s_e 8, lim 1;
         # CSR values written (zero fill + final) 16/nnz
         "jacobian": 209 * n_tets + 16 * nnz,
@@ -127,8 +128,9 @@ def main():
         op.residual_dev(upw.d_rt, grad)
 
     def ph_sens():
-        upw.sensitivities_dev(grad, q2l, rho, M_INF, UPWIND_C, M_CRIT, GAMMA,
-                              RHO_FLOOR)
+        # the Newton driver reuses the residual's walk (same grad -> same u)
+        upw.sensitivities_at_dev(q2l, rho, upw.d_up, M_INF, UPWIND_C, M_CRIT,
+                                 GAMMA, RHO_FLOOR)
 
     def ph_jac():
         op.jacobian_dev(upw.d_rt, grad, upw.d_se, lim, 1.0, jac.data)

@@ -82,6 +82,23 @@ int fp3d_upstream_elements(int64_t n_tets, const int32_t *face_neighbors,
                            const double *centroids, const double *nodes,
                            const int32_t *elements, const double *grad,
                            int max_hops, int32_t *out, void *stream);
+/* The same walk reading a per-face centroid displacement table
+ * face_disp[e][f][:] = centroid(face_neighbors[e][f]) - centroid(e)
+ * (float64 (n,4,3), any value where the neighbour is -1): two contiguous
+ * rows per hop instead of five scattered centroid gathers; identical u(e).
+ * elements, face_neighbors and face_disp must be 16-byte aligned. */
+int fp3d_upstream_elements_d(int64_t n_tets, const int32_t *face_neighbors,
+                             const double *face_disp, const double *nodes,
+                             const int32_t *elements, const double *grad,
+                             int max_hops, int32_t *out, void *stream);
+int fp3d_upwind_rho_tilde_d(int64_t n_tets, const int32_t *face_neighbors,
+                            const double *face_disp, const double *nodes,
+                            const int32_t *elements, const double *grad,
+                            const double *q2, const double *rho, int max_hops,
+                            double m_inf, double m_crit, double C, double gamma,
+                            double rho_floor, int32_t *upstream_out,
+                            double *nu_out, double *rho_tilde_out,
+                            void *stream);
 /* pyfp3d/kernels/upwind.py:456-500 rho_tilde_sweep. */
 int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,
                          const int32_t *upstream, double m_inf, double m_crit,

@@ -177,6 +177,68 @@ __device__ __forceinline__ int32_t walk_upstream(
   return cur;
 }
 
+// Same walk reading the per-face displacement table dcen[k][f] =
+// centroid(nb) - centroid(k) (0 on the boundary, precomputed once in
+// float64 with the same subtraction): one 16 B neighbour row and one 96 B
+// displacement row per hop instead of five scattered centroid gathers.
+// The dot products are the same IEEE operations, so u(e) is identical.
+__device__ __forceinline__ int32_t walk_upstream_d(
+    int64_t e, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    double gx, double gy, double gz, int max_hops) {
+  double pmin = 1e300, pmax = -1e300;
+  const int4 nd4 = *reinterpret_cast<const int4 *>(elements + 4 * e);
+  const int32_t nds[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int64_t nd = nds[k];
+    double p = nodes[3 * nd] * gx + nodes[3 * nd + 1] * gy + nodes[3 * nd + 2] * gz;
+    if (p < pmin) pmin = p;
+    if (p > pmax) pmax = p;
+  }
+  double target = 0.8 * (pmax - pmin);
+  int32_t cur = (int32_t)e;
+  double reach = 0.0;
+  int hops = 0;
+  while (hops < max_hops && reach < target) {
+    const int4 nb4 = *reinterpret_cast<const int4 *>(fn + 4 * (int64_t)cur);
+    const double2 *dp = reinterpret_cast<const double2 *>(dcen + 12 * (int64_t)cur);
+    double2 d01 = dp[0], d23 = dp[1], d45 = dp[2], d67 = dp[3], d89 = dp[4],
+            dab = dp[5];
+    const int32_t nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const double dx[4] = {d01.x, d23.y, d67.x, d89.y};
+    const double dy[4] = {d01.y, d45.x, d67.y, dab.x};
+    const double dz[4] = {d23.x, d45.y, d89.x, dab.y};
+    int32_t best = -1;
+    double best_disp = 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      if (nbs[f] >= 0) {
+        double d = dx[f] * gx + dy[f] * gy + dz[f] * gz;
+        if (d < best_disp) {
+          best_disp = d;
+          best = nbs[f];
+        }
+      }
+    }
+    if (best < 0) break;
+    cur = best;
+    reach += -best_disp;
+    hops += 1;
+  }
+  return cur;
+}
+
+__global__ __launch_bounds__(kBlock) void k_upstream_d(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, int max_hops, int32_t *__restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  out[e] = walk_upstream_d(e, fn, dcen, nodes, elements, grad[3 * e],
+                           grad[3 * e + 1], grad[3 * e + 2], max_hops);
+}
+
 __global__ __launch_bounds__(kBlock) void k_upstream(
     int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ cen,
     const double *__restrict__ nodes, const int32_t *__restrict__ elements,
@@ -226,6 +288,24 @@ __global__ __launch_bounds__(kBlock) void k_upwind_fused(
   if (e >= n) return;
   int32_t u = walk_upstream(e, fn, cen, nodes, elements, grad[3 * e],
                             grad[3 * e + 1], grad[3 * e + 2], max_hops);
+  if (up_out) up_out[e] = u;
+  double nu, rt;
+  sweep_one(e, u, q2, rho, m, mc2, C, g, floor_, nu, rt);
+  nu_out[e] = nu;
+  rt_out[e] = rt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_upwind_fused_d(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, const double *__restrict__ q2,
+    const double *__restrict__ rho, int max_hops, double m, double mc2,
+    double C, double g, double floor_, int32_t *__restrict__ up_out,
+    double *__restrict__ nu_out, double *__restrict__ rt_out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int32_t u = walk_upstream_d(e, fn, dcen, nodes, elements, grad[3 * e],
+                              grad[3 * e + 1], grad[3 * e + 2], max_hops);
   if (up_out) up_out[e] = u;
   double nu, rt;
   sweep_one(e, u, q2, rho, m, mc2, C, g, floor_, nu, rt);
@@ -423,18 +503,38 @@ __global__ __launch_bounds__(kBlock) void k_monitors_p1(
     work[3 * blockIdx.x + 2] = c;
   }
 }
-__global__ void k_monitors_p2(int nb, const double *__restrict__ work,
-                              double *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+// second pass: 256 lanes each fold a fixed strided subset, then a fixed
+// shuffle tree and a fixed-order combine of the 4 wavefront results
+__global__ __launch_bounds__(kBlock) void k_monitors_p2(
+    int nb, const double *__restrict__ work, double *__restrict__ out) {
+  __shared__ double s[3][kBlock / 64];
   double a = -INFINITY, b = 0.0, c = 0.0;
-  for (int k = 0; k < nb; ++k) {
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
     a = work[3 * k] > a ? work[3 * k] : a;
     b += work[3 * k + 1];
     c += work[3 * k + 2];
   }
-  out[0] = a;
-  out[1] = b;
-  out[2] = c;
+  a = wave_max(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    s[0][w] = a;
+    s[1][w] = b;
+    s[2][w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; ++k) {
+      a = s[0][0] > s[0][k] ? s[0][0] : s[0][k];
+      s[0][0] = a;
+      s[1][0] += s[1][k];
+      s[2][0] += s[2][k];
+    }
+    out[0] = s[0][0];
+    out[1] = s[1][0];
+    out[2] = s[2][0];
+  }
 }
 __global__ __launch_bounds__(kBlock) void k_count_zero_p1(
     int64_t n, const uint8_t *__restrict__ m, double *__restrict__ work) {
@@ -452,12 +552,19 @@ __global__ __launch_bounds__(kBlock) void k_count_zero_p1(
     work[blockIdx.x] = t;
   }
 }
-__global__ void k_sum_p2(int nb, const double *__restrict__ work,
-                         double *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(kBlock) void k_sum_p2(
+    int nb, const double *__restrict__ work, double *__restrict__ out) {
+  __shared__ double s[kBlock / 64];
   double t = 0.0;
-  for (int k = 0; k < nb; ++k) t += work[k];
-  out[0] = t;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) t += work[k];
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) r += s[k];
+    out[0] = r;
+  }
 }
 
 inline int reduce_blocks(int64_t n) {
@@ -675,6 +782,40 @@ int fp3d_upstream_elements(int64_t n_tets, const int32_t *face_neighbors,
            max_hops, out);
 }
 
+int fp3d_upstream_elements_d(int64_t n_tets, const int32_t *face_neighbors,
+                             const double *face_disp, const double *nodes,
+                             const int32_t *elements, const double *grad,
+                             int max_hops, int32_t *out, void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (face_neighbors && face_disp && nodes &&
+                               elements && grad && out),
+               "upstream_elements_d: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements) && FP3D_ALIGNED16(face_neighbors) &&
+                   FP3D_ALIGNED16(face_disp),
+               "upstream_elements_d: arrays must be 16-byte aligned");
+  ELEMWISE(k_upstream_d, n_tets, face_neighbors, face_disp, nodes, elements,
+           grad, max_hops, out);
+}
+
+int fp3d_upwind_rho_tilde_d(int64_t n_tets, const int32_t *face_neighbors,
+                            const double *face_disp, const double *nodes,
+                            const int32_t *elements, const double *grad,
+                            const double *q2, const double *rho, int max_hops,
+                            double m_inf, double m_crit, double C, double gamma,
+                            double rho_floor, int32_t *upstream_out,
+                            double *nu_out, double *rho_tilde_out,
+                            void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (face_neighbors && face_disp && nodes &&
+                               elements && grad && q2 && rho && nu_out &&
+                               rho_tilde_out),
+               "upwind_rho_tilde_d: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements) && FP3D_ALIGNED16(face_neighbors) &&
+                   FP3D_ALIGNED16(face_disp),
+               "upwind_rho_tilde_d: arrays must be 16-byte aligned");
+  ELEMWISE(k_upwind_fused_d, n_tets, face_neighbors, face_disp, nodes, elements,
+           grad, q2, rho, max_hops, m_inf, m_crit * m_crit, C, gamma,
+           rho_floor, upstream_out, nu_out, rho_tilde_out);
+}
+
 int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,
                          const int32_t *upstream, double m_inf, double m_crit,
                          double C, double gamma, double rho_floor,
@@ -756,8 +897,8 @@ int fp3d_upwind_monitors(int64_t n, const double *nu, const double *rho_tilde,
   int nb = reduce_blocks(n);
   hipLaunchKernelGGL(k_monitors_p1, dim3(nb), dim3(kBlock), 0,
                      FP3D_STREAM(stream), n, nu, rho_tilde, rho_floor, work);
-  hipLaunchKernelGGL(k_monitors_p2, dim3(1), dim3(64), 0, FP3D_STREAM(stream),
-                     nb, work, out3);
+  hipLaunchKernelGGL(k_monitors_p2, dim3(1), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), nb, work, out3);
   return check_launch("upwind_monitors");
 }
 
@@ -767,8 +908,8 @@ int fp3d_count_limited(int64_t n, const uint8_t *lim, double *work,
   int nb = reduce_blocks(n);
   hipLaunchKernelGGL(k_count_zero_p1, dim3(nb), dim3(kBlock), 0,
                      FP3D_STREAM(stream), n, lim, work);
-  hipLaunchKernelGGL(k_sum_p2, dim3(1), dim3(64), 0, FP3D_STREAM(stream), nb,
-                     work, out1);
+  hipLaunchKernelGGL(k_sum_p2, dim3(1), dim3(kBlock), 0, FP3D_STREAM(stream),
+                     nb, work, out1);
   return check_launch("count_limited");
 }
 

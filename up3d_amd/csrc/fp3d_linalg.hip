@@ -265,16 +265,24 @@ __global__ __launch_bounds__(kBlock) void k_norms_p1(int64_t n,
     part[2 * blockIdx.x + 1] = mm;
   }
 }
-__global__ void k_norms_p2(int nb, const double *__restrict__ part,
-                           double *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(kBlock) void k_norms_p2(
+    int nb, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ double shm[kBlock / 64];
   double s = 0.0, m = 0.0;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     s += part[2 * b];
     m = part[2 * b + 1] > m ? part[2 * b + 1] : m;
   }
-  out[0] = sqrt(s);
-  out[1] = m;
+  m = wmax(m);
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  double t = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    double mm = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) mm = shm[k] > mm ? shm[k] : mm;
+    out[0] = sqrt(t);
+    out[1] = mm;
+  }
 }
 
 // w -= sum_k h[k] V_k   (h in LDS)
@@ -400,31 +408,33 @@ __global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n,
   if (i < n) out[i] = sh[threadIdx.x] - v;
   if (threadIdx.x == kBlock - 1) bsum[blockIdx.x] = sh[kBlock - 1];
 }
-// pass 2: exclusive scan of the block totals in one workgroup (chunked)
+// pass 2: exclusive scan of the block totals in one workgroup: each lane
+// owns a contiguous segment, the 256 segment totals are scanned in LDS,
+// then every lane rewrites its segment (integer sums: order-independent)
 __global__ __launch_bounds__(kBlock) void k_scan_p2(int nb,
                                                    int32_t *__restrict__ bsum,
                                                    int32_t *__restrict__ total) {
   __shared__ int32_t sh[kBlock];
-  __shared__ int32_t carry;
-  if (threadIdx.x == 0) carry = 0;
+  int seg = (nb + kBlock - 1) / kBlock;
+  int lo = threadIdx.x * seg;
+  int hi = lo + seg < nb ? lo + seg : nb;
+  int32_t t = 0;
+  for (int i = lo; i < hi; ++i) t += bsum[i];
+  sh[threadIdx.x] = t;
   __syncthreads();
-  for (int base = 0; base < nb; base += kBlock) {
-    int i = base + threadIdx.x;
-    int32_t v = i < nb ? bsum[i] : 0;
-    sh[threadIdx.x] = v;
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int32_t v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
     __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-      int32_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += sh[kBlock - 1];
+    sh[threadIdx.x] += v;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  int32_t run = sh[threadIdx.x] - t;
+  for (int i = lo; i < hi; ++i) {
+    int32_t v = bsum[i];
+    bsum[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == kBlock - 1) *total = sh[kBlock - 1];
 }
 __global__ void k_scan_p3(int64_t n, int32_t *__restrict__ out,
                           const int32_t *__restrict__ bsum) {
@@ -701,7 +711,7 @@ int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
   hipStream_t s = FP3D_STREAM(stream);
   int nb = rblocks(n);
   hipLaunchKernelGGL(k_norms_p1, dim3(nb), dim3(kBlock), 0, s, n, x, work);
-  hipLaunchKernelGGL(k_norms_p2, dim3(1), dim3(64), 0, s, nb, work, out2);
+  hipLaunchKernelGGL(k_norms_p2, dim3(1), dim3(kBlock), 0, s, nb, work, out2);
   return check_launch("norms");
 }
 

@@ -19,6 +19,7 @@ Device layout (storage order, k = storage slot):
   e2c      int32 (n_tets, 16)    CSR slot of (elements[k,i], elements[k,j])
   fn       int32 (n_tets, 4)     face neighbours as storage slots, -1 boundary
   cen      f64   (n_tets, 3)     centroids
+  dcen     f64   (n_tets, 12)    centroid(fn[k,f]) - centroid(k), per face
   nodes    f64   (n_nodes, 3)
   indptr   int32 (n_nodes+1,), indices int32 (nnz,)
   inc_ptr  int32 (n_nodes+1,), inc_ent int32 (4 n_tets,)  node -> 4k+i
@@ -134,6 +135,12 @@ class MeshStore:
         self.e2c = to_dev(e2c[perm].reshape(self.n_tets, 16), torch.int32)
         self.fn = to_dev(fn_s, torch.int32)
         self.cen = to_dev(cen[perm], torch.float64)
+        # per-face centroid displacement, storage order (0 on the boundary)
+        cs = cen[perm]
+        dcen = np.where((fn_s >= 0)[:, :, None],
+                        cs[np.maximum(fn_s, 0)] - cs[:, None, :], 0.0)
+        self.dcen = to_dev(np.ascontiguousarray(dcen).reshape(self.n_tets, 12),
+                           torch.float64)
         self.nodes = to_dev(nodes, torch.float64)
         self.indptr = to_dev(indptr, torch.int32)
         self.indices = to_dev(indices, torch.int32)

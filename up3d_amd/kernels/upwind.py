@@ -146,24 +146,35 @@ class UpwindOperator:
     def walk_dev(self, grad: torch.Tensor, out=None) -> torch.Tensor:
         st = self.store
         out = self.d_up if out is None else out
-        _native.call("fp3d_upstream_elements", st.n_tets, ptr(st.fn),
-                     ptr(st.cen), ptr(st.nodes), ptr(st.el), ptr(grad), 4,
+        _native.call("fp3d_upstream_elements_d", st.n_tets, ptr(st.fn),
+                     ptr(st.dcen), ptr(st.nodes), ptr(st.el), ptr(grad), 4,
                      ptr(out), stream_handle())
         return out
 
     def rho_tilde_dev(self, grad, q2, rho, m_inf, C, m_crit, gamma=GAMMA,
-                      rho_floor=0.05, monitors=True):
+                      rho_floor=0.05, monitors=True, up_out=None, rt_out=None):
         """Fused walk + switch + rho_tilde; monitors (nu_max, n_supersonic,
         n_floored) refreshed by a deterministic device reduction."""
         st = self.store
-        _native.call("fp3d_upwind_rho_tilde", st.n_tets, ptr(st.fn),
-                     ptr(st.cen), ptr(st.nodes), ptr(st.el), ptr(grad), ptr(q2),
+        up = self.d_up if up_out is None else up_out
+        rt = self.d_rt if rt_out is None else rt_out
+        _native.call("fp3d_upwind_rho_tilde_d", st.n_tets, ptr(st.fn),
+                     ptr(st.dcen), ptr(st.nodes), ptr(st.el), ptr(grad), ptr(q2),
                      ptr(rho), 4, float(m_inf), float(m_crit), float(C),
-                     float(gamma), float(rho_floor), ptr(self.d_up),
-                     ptr(self.d_nu), ptr(self.d_rt), stream_handle())
+                     float(gamma), float(rho_floor), ptr(up), ptr(self.d_nu),
+                     ptr(rt), stream_handle())
         if monitors:
-            self._monitors(self.d_nu, self.d_rt, rho_floor)
-        return self.d_rt
+            self._monitors(self.d_nu, rt, rho_floor)
+        return rt
+
+    def sensitivities_at_dev(self, q2, rho, upstream, m_inf, C, m_crit,
+                             gamma=GAMMA, rho_floor=0.05):
+        """(s_e, s_u) for a given upstream selection (no walk)."""
+        _native.call("fp3d_rho_tilde_sensitivities", self.store.n_tets, ptr(q2),
+                     ptr(rho), ptr(upstream), float(m_inf), float(m_crit),
+                     float(C), float(gamma), float(rho_floor), ptr(self.d_se),
+                     ptr(self.d_su), stream_handle())
+        return self.d_se, self.d_su
 
     def _monitors(self, nu, rt, rho_floor):
         _native.call("fp3d_upwind_monitors", self.store.n_tets, ptr(nu), ptr(rt),

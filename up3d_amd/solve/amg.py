@@ -163,7 +163,7 @@ class AMGHierarchy:
 
     def __init__(self, A: sp.csr_matrix, precision: str = "f64",
                  degree: int = 2, eig_ratio: float = 20.0, max_levels: int = 10,
-                 max_coarse: int = 10, seed: int = 0):
+                 max_coarse: int = 10, seed: int = 0, use_graph: bool = True):
         if precision not in ("f64", "f32"):
             raise ValueError(f"unknown AMG precision {precision!r}")
         host, coarse = sa_setup(A, max_levels=max_levels, max_coarse=max_coarse,
@@ -174,6 +174,13 @@ class AMGHierarchy:
         self.coarse = to_dev(np.ascontiguousarray(coarse), torch.float64)
         self.shapes = [lv.n for lv in self.levels]
         self.nnzs = [lv.A.nnz for lv in self.levels]
+        # HIP graph of the whole V-cycle on fixed buffers (captured lazily on
+        # the first apply, after one eager V-cycle has loaded the kernels)
+        self.use_graph = use_graph
+        self._graph = None
+        self._gb = empty(self.levels[0].n)
+        self._gx = empty(self.levels[0].n)
+        self._eager_calls = 0
 
     @property
     def operator_complexity(self) -> float:
@@ -209,9 +216,30 @@ class AMGHierarchy:
                      ptr(b), ptr(x), ptr(L.r), ptr(L.d0), ptr(L.d1), L.degree,
                      L.lmin, L.lmax, 0, s)
 
-    def apply(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
-        """x = M b (one V-cycle from a zero initial guess)."""
+    def apply_eager(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        """x = M b with one launch per kernel (no graph)."""
         self._vcycle(0, b, x)
+        return x
+
+    def _capture(self):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._vcycle(0, self._gb, self._gx)
+        self._graph = g
+
+    def apply(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        """x = M b (one V-cycle from a zero initial guess).  After the first
+        call the V-cycle is replayed as one HIP graph on fixed buffers."""
+        if not self.use_graph:
+            return self.apply_eager(b, x)
+        if self._graph is None:
+            if self._eager_calls == 0:
+                self._eager_calls += 1
+                return self.apply_eager(b, x)
+            self._capture()
+        self._gb.copy_(b)
+        self._graph.replay()
+        x.copy_(self._gx)
         return x
 
     def aspreconditioner(self):

@@ -39,7 +39,7 @@ class _Red:
         L = _native.lib()
         self.work = empty(max(L.fp3d_multi_dot_work_doubles(max_nv),
                               L.fp3d_reduce_work_doubles()))
-        self.out = empty(2 * max_nv + 8)
+        self.out = empty(2 * max_nv + 10)
 
     def norm(self, x: torch.Tensor, at: int = 0) -> None:
         _native.call("fp3d_norms", x.numel(), ptr(x), ptr(self.work),
@@ -135,21 +135,21 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
             w = V[col + 1]
             prec(tmp, w)
             k = col + 1
-            # slots: out[0] = ||w|| before, out[2:2+k] = first-pass h,
-            # out[2+k:2+2k] = re-orthogonalisation h, out[2+2k] = ||w|| after
+            # one fused pass gives V_j . w for j < k AND w . w (V[k] is w):
+            # out[2:2+k] = h, out[2+k] = ||w||^2 before; re-orthogonalisation
+            # h2 -> out[3+k:3+2k]; ||w|| after -> out[0]
             sh = stream_handle()
-            red.norm(w, 0)
-            red.dots(V, k, w, 2)
+            red.dots(V, k + 1, w, 2)
             _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
                          ptr(red.out[2:]), -1.0, ptr(w), sh)
-            red.dots(V, k, w, 2 + k)
+            red.dots(V, k, w, 3 + k)
             _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
-                         ptr(red.out[2 + k:]), -1.0, ptr(w), sh)
-            red.norm(w, 2 + 2 * k)
-            host = to_host(red.out[: 4 + 2 * k])
-            h0 = host[0]
-            hv = host[2:2 + k] + host[2 + k:2 + 2 * k]
-            h1 = host[2 + 2 * k]
+                         ptr(red.out[3 + k:]), -1.0, ptr(w), sh)
+            red.norm(w, 0)
+            host = to_host(red.out[: 3 + 2 * k])
+            h0 = float(np.sqrt(max(host[2 + k], 0.0)))
+            h1 = host[0]
+            hv = host[2:2 + k] + host[3 + k:3 + 2 * k]
             h[col, :k] = hv
             h[col, col + 1] = h1
             if h1 <= eps * h0:

@@ -54,6 +54,7 @@ class _State:
         self.rho = empty(nt)
         self.rt = empty(nt)
         self.lim = empty(nt, torch.uint8)
+        self.up = empty(nt, torch.int32)   # upstream of the live walk
         self.R = empty(n)          # free-masked residual
         self.n_limited = 0
         self.n_floored = 0
@@ -130,8 +131,8 @@ class NewtonWorkspace:
             s.rt.copy_(upw.d_rt)
         elif self.m_inf > 0.0:
             upw.rho_tilde_dev(s.grad, s.q2l, s.rho, self.m_inf, upwind_c, m_crit,
-                              self.gamma_air, rho_floor)
-            s.rt.copy_(upw.d_rt)
+                              self.gamma_air, rho_floor, up_out=s.up,
+                              rt_out=s.rt)
         else:
             s.rt.copy_(s.rho)
             upw.nu_max, upw.n_supersonic, upw.n_floored = 0.0, 0, 0
@@ -160,8 +161,11 @@ class NewtonWorkspace:
                 s.q2l, s.rho, frozen[0], frozen[1], self.m_inf, upwind_c,
                 m_crit, self.gamma_air, rho_floor)
         elif self.m_inf > 0.0:
-            se, su, upstream = upw.sensitivities_dev(
-                s.grad, s.q2l, s.rho, self.m_inf, upwind_c, m_crit,
+            # the live walk on this state's grad already ran in eval_residual
+            # (pyfp3d re-walks the same grad, which yields the same u(e))
+            upstream = s.up
+            se, su = upw.sensitivities_at_dev(
+                s.q2l, s.rho, upstream, self.m_inf, upwind_c, m_crit,
                 self.gamma_air, rho_floor)
         else:
             se = upw.d_se.zero_()
@@ -221,7 +225,11 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                  ) -> Dict[str, object]:
     """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
     n_st = 0; precond is always the device AMG).  Converged when
-    ||R_free||_inf < tol_residual with 0 limited and 0 floored elements."""
+    ||R_free||_inf < tol_residual with 0 limited and 0 floored elements.
+    amg_rebuild_every = k > 0 rebuilds the AMG hierarchy every k Newton
+    steps (the reference's rule); k <= 0 selects the adaptive rule (keep
+    the hierarchy, also across Mach levels of a shared workspace, until a
+    Krylov solve stalls or its iteration count doubles)."""
     if m_inf > 0.0 and upwind_c <= 0.0:
         raise ValueError("the Newton driver runs with the walk upwind machinery "
                          "active for m_inf > 0; pass upwind_c > 0")
@@ -289,7 +297,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     prev_gm = [0]
     dphi = empty(ws.n)
     rhs = empty(ws.n)
-    M_pre = None
+    M_pre = ws.precond if amg_rebuild_every <= 0 else None
+    last_lin = [0, 0, 0]      # (info, n_iter) of the last solve, n_iter after rebuild
 
     def close_step(rec):
         nonlocal prev_snap
@@ -406,8 +415,17 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
         sync()
         timings["assembly"] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        if M_pre is None or it % amg_rebuild_every == 0:
+        if amg_rebuild_every > 0:
+            rebuild = M_pre is None or it % amg_rebuild_every == 0
+        else:
+            # adaptive: keep the hierarchy while the Krylov solves stay cheap;
+            # rebuild after a stalled solve or when the iteration count has
+            # doubled since the last rebuild
+            rebuild = (M_pre is None or last_lin[0] != 0
+                       or last_lin[1] > 2 * max(last_lin[2], 10))
+        if rebuild:
             M_pre = ws.build_preconditioner(cur)
+            last_lin[2] = -1
         sync()
         timings["precond"] += time.perf_counter() - t0
         eta = _ew_forcing(r_norm, residual_history[-2] if it > 0 else None,
@@ -426,6 +444,9 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
         n_gmres_total += n_it
         if info != 0:
             n_gmres_stalled += 1
+        last_lin[0], last_lin[1] = info, n_it
+        if last_lin[2] < 0:
+            last_lin[2] = n_it
 
         # ---- safety-only backtracking, least-bad on failure
         lam = 1.0
