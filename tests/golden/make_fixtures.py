@@ -1,0 +1,55 @@
+"""Writes tests/golden/ fixtures from the reference's committed data files
+(run here, where /root/reference is mounted; the GPU box only reads the
+fixtures).  Only data is stored: mesh arrays read from the reference's
+.msh files and the expected numbers its result CSVs hold.
+
+  naca0012_coarse.npz  nodes/elements of cases/meshes/naca0012_2.5d/coarse.msh
+  golden.json          expected values with their reference sources
+"""
+
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+REF = Path("/root/reference")
+sys.path.insert(0, str(ROOT))
+
+from up3d_amd.mesh.reader import read_mesh  # noqa: E402
+
+
+def main():
+    m = read_mesh(REF / "cases/meshes/naca0012_2.5d/coarse.msh")
+    np.savez_compressed(HERE / "naca0012_coarse.npz", nodes=m.nodes,
+                        elements=m.elements)
+    g73 = {}
+    for ln in (REF / "cases/demo/p7_diff_flux/results/g73_constructed.csv").read_text().splitlines()[1:]:
+        k, v = ln.split(",")
+        g73[k] = float(v)
+    stats = {}
+    for ln in (REF / "cases/meshes/naca0012_2.5d/coarse_stats.csv").read_text().splitlines()[1:]:
+        k, v = ln.split(",")
+        stats[k] = float(v)
+    col = {}
+    for ln in (REF / "cases/demo/p0_infrastructure/results/g03_coloring.csv").read_text().splitlines()[1:]:
+        p = ln.split(",")
+        col[p[0]] = {"n_elements": int(p[1]), "n_colors": int(p[2]),
+                     "min_class": int(p[4]), "max_class": int(p[5])}
+    out = {
+        "g73_constructed": {"source": "cases/demo/p7_diff_flux/results/g73_constructed.csv",
+                            "values": g73},
+        "naca_coarse_stats": {"source": "cases/meshes/naca0012_2.5d/coarse_stats.csv",
+                              "values": stats},
+        "g03_coloring": {"source": "cases/demo/p0_infrastructure/results/g03_coloring.csv",
+                         "values": col},
+        "naca_medium_colors": {"source": "cases/demo/p3_subsonic/results/assembly_debt.csv:6",
+                               "values": {"n_colors": 27}},
+    }
+    (HERE / "golden.json").write_text(json.dumps(out, indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -588,10 +588,16 @@ __global__ __launch_bounds__(kBlock) void k_residual_color(
   double gx = grad[3 * e], gy = grad[3 * e + 1], gz = grad[3 * e + 2];
   const int4 nd = *reinterpret_cast<const int4 *>(elements + 4 * e);
   const double *b = B + 12 * e;
-  R[nd.x] += w * (gx * b[0] + gy * b[1] + gz * b[2]);
-  R[nd.y] += w * (gx * b[3] + gy * b[4] + gz * b[5]);
-  R[nd.z] += w * (gx * b[6] + gy * b[7] + gz * b[8]);
-  R[nd.w] += w * (gx * b[9] + gy * b[10] + gz * b[11]);
+  // the 4 nodes of one element are distinct: issue all loads, then stores
+  double r0 = R[nd.x], r1 = R[nd.y], r2 = R[nd.z], r3 = R[nd.w];
+  r0 += w * (gx * b[0] + gy * b[1] + gz * b[2]);
+  r1 += w * (gx * b[3] + gy * b[4] + gz * b[5]);
+  r2 += w * (gx * b[6] + gy * b[7] + gz * b[8]);
+  r3 += w * (gx * b[9] + gy * b[10] + gz * b[11]);
+  R[nd.x] = r0;
+  R[nd.y] = r1;
+  R[nd.z] = r2;
+  R[nd.w] = r3;
 }
 
 // pyfp3d/kernels/jacobian.py:117-129
@@ -607,15 +613,27 @@ __global__ __launch_bounds__(kBlock) void k_matrix_color(
   double b[12];
 #pragma unroll
   for (int t = 0; t < 12; ++t) b[t] = B[12 * e + t];
-  const int32_t *slot = e2c + 16 * e;
+  // the 16 CSR slots of one element are distinct: load all, add, store all
+  int32_t q[16];
+  const int4 *s4 = reinterpret_cast<const int4 *>(e2c + 16 * e);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    int4 v = s4[t];
+    q[4 * t] = v.x; q[4 * t + 1] = v.y; q[4 * t + 2] = v.z; q[4 * t + 3] = v.w;
+  }
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = data[q[t]];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     double bi0 = b[3 * i], bi1 = b[3 * i + 1], bi2 = b[3 * i + 2];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      data[slot[4 * i + j]] +=
+      acc[4 * i + j] +=
           w * (bi0 * b[3 * j] + bi1 * b[3 * j + 1] + bi2 * b[3 * j + 2]);
   }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) data[q[t]] = acc[t];
 }
 
 // pyfp3d/kernels/jacobian.py:146-180 (Term 1 kept as its own expression so
@@ -638,7 +656,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
   double b[12];
 #pragma unroll
   for (int t = 0; t < 12; ++t) b[t] = B[12 * e + t];
-  const int32_t *slot = e2c + 16 * e;
+  int32_t q[16];
+  const int4 *s4 = reinterpret_cast<const int4 *>(e2c + 16 * e);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    int4 v = s4[t];
+    q[4 * t] = v.x; q[4 * t + 1] = v.y; q[4 * t + 2] = v.z; q[4 * t + 3] = v.w;
+  }
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = data[q[t]];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     double bi0 = b[3 * i], bi1 = b[3 * i + 1], bi2 = b[3 * i + 2];
@@ -648,9 +675,11 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
       double bj0 = b[3 * j], bj1 = b[3 * j + 1], bj2 = b[3 * j + 2];
       double val = w1 * (bi0 * bj0 + bi1 * bj1 + bi2 * bj2);
       if (w2 != 0.0) val += w2 * gi * (g0 * bj0 + g1 * bj1 + g2 * bj2);
-      data[slot[4 * i + j]] += val;
+      acc[4 * i + j] += val;
     }
   }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) data[q[t]] = acc[t];
 }
 
 // pyfp3d/kernels/jacobian.py:197-228 (fixed slot slice [16a, 16a+16))
@@ -956,6 +985,7 @@ int fp3d_assemble_matrix_colored(int n_colors, const int64_t *color_offsets,
   if (rc) return rc;
   FP3D_REQUIRE(elem_to_csr && B && V && rho_tilde && data,
                "assemble_matrix: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elem_to_csr), "elem_to_csr must be 16-byte aligned");
   for (int c = 0; c < n_colors; ++c) {
     int64_t k0 = color_offsets[c], k1 = color_offsets[c + 1];
     if (k1 == k0) continue;
@@ -977,6 +1007,7 @@ int fp3d_assemble_jacobian_colored(int n_colors, const int64_t *color_offsets,
   if (rc) return rc;
   FP3D_REQUIRE(elem_to_csr && B && V && rho_tilde && grad && s_e && data,
                "assemble_jacobian: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elem_to_csr), "elem_to_csr must be 16-byte aligned");
   for (int c = 0; c < n_colors; ++c) {
     int64_t k0 = color_offsets[c], k1 = color_offsets[c + 1];
     if (k1 == k0) continue;

@@ -442,6 +442,12 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
         sync()
         timings["linsolve"] += time.perf_counter() - t0
         n_gmres_total += n_it
+        rec["gmres_iters"] = n_it
+        rec["gmres_info"] = info
+        if verbose:
+            print(f"      gmres eta={eta:.1e} its={n_it} info={info} "
+                  f"t3_active={ws.jac.n_active}"
+                  + (" [amg rebuilt]" if rebuild else ""), flush=True)
         if info != 0:
             n_gmres_stalled += 1
         last_lin[0], last_lin[1] = info, n_it
