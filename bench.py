@@ -75,6 +75,8 @@ def main():
                     help="also time the Newton solves (FP64 and FP32-precond)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the "
@@ -147,6 +149,14 @@ def main():
             f()
         upw._monitors(upw.d_nu, upw.d_rt, RHO_FLOOR)
 
+    if args.calib:
+        # plain copy of known size: 1 GiB read + 1 GiB written, so rocprof's
+        # FETCH_SIZE / WRITE_SIZE for this dispatch give the counter scale
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -180,6 +180,20 @@ int fp3d_assemble_jacobian_colored(int n_colors, const int64_t *color_offsets,
                                    const double *grad, const double *s_e,
                                    const uint8_t *lim, double inv_u2,
                                    double *data, void *stream);
+/* SPD preconditioner matrix (no reference counterpart): Terms 1+2 with the
+ * Term-2 weight clamped from below at -clamp * rho_tilde V / q2, so every
+ * element matrix stays positive semi-definite (0 < clamp < 1).  Used to
+ * build the AMG hierarchy that preconditions the Newton GMRES. */
+int fp3d_assemble_jacobian_spd_colored(int n_colors,
+                                       const int64_t *color_offsets,
+                                       const int32_t *elem_index,
+                                       const int32_t *elem_to_csr,
+                                       const double *B, const double *V,
+                                       const double *rho_tilde,
+                                       const double *grad, const double *s_e,
+                                       const uint8_t *lim, double inv_u2,
+                                       double clamp, double *data,
+                                       void *stream);
 /* pyfp3d/kernels/jacobian.py:183-228 fill_term3_coo (rows/cols int32). */
 int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
                         const int32_t *elements, const double *B,
@@ -282,6 +296,28 @@ int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
 /* x = M b, M dense row-major n x n (the coarsest-level pseudo-inverse) */
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
                       void *stream);
+
+/* ---------------------------------------------------- discrete adjoint
+ * (new functionality: the reference has no adjoint.)  At a frozen upstream
+ * selection and branch (pyfp3d/kernels/upwind.py:544-643):
+ * out = d rho_tilde / d M_inf per element (branch-wise, at fixed q2). */
+int fp3d_drho_tilde_dmach(int64_t n, const double *q2, const double *rho,
+                          const int32_t *upstream, const int8_t *branch,
+                          double m_inf, double m_crit, double C, double gamma,
+                          double *out, void *stream);
+/* out (n_nodes, 3) = d(lambda^T R)/d(node coordinates) for the residual
+ * R_i = sum_e rho_tilde_e V_e (g_e . B_e,i) with the rho_tilde chain
+ * through (s_e, s_u); down_ptr/down_list: for each element the elements
+ * whose upstream it is (ascending); inc_ptr/inc_ent: node -> (4e+i).
+ * work: fp3d_adjoint_dx_work_doubles(n_tets) doubles. */
+int fp3d_adjoint_dx(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
+                    const double *B, const double *V, const double *grad,
+                    const double *rho_tilde, const double *s_e,
+                    const double *s_u, const int32_t *down_ptr,
+                    const int32_t *down_list, const int32_t *inc_ptr,
+                    const int32_t *inc_ent, const double *lam, double *work,
+                    double *out, void *stream);
+int64_t fp3d_adjoint_dx_work_doubles(int64_t n_tets);
 
 /* --------------------------------------------------- host-side set-up
  * (plain host pointers; run once per mesh, as in the reference) */

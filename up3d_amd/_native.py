@@ -70,6 +70,9 @@ _SIGS = {
     "fp3d_assemble_jacobian_colored": (_int, [_int, _pi64, _vp, _vp, _vp, _vp,
                                               _vp, _vp, _vp, _vp, _dbl, _vp,
                                               _vp]),
+    "fp3d_assemble_jacobian_spd_colored": (_int, [_int, _pi64, _vp, _vp, _vp,
+                                                  _vp, _vp, _vp, _vp, _vp, _dbl,
+                                                  _dbl, _vp, _vp]),
     "fp3d_fill_term3_coo": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                    _dbl, _vp, _vp, _vp, _vp]),
     "fp3d_term3_workspace_bytes": (_i64, [_i64, _i64]),
@@ -99,6 +102,11 @@ _SIGS = {
     "fp3d_amg_prolong_add": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
                                     _vp]),
     "fp3d_dense_matvec": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "fp3d_drho_tilde_dmach": (_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl,
+                                     _dbl, _vp, _vp]),
+    "fp3d_adjoint_dx": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_adjoint_dx_work_doubles": (_i64, [_i64]),
     "fp3d_host_greedy_coloring": (_int, [_i64, _vp, _i64, _vp, _pi32]),
     "fp3d_host_csr_pattern": (_int, [_i64, _vp, _i64, _vp, _vp, _pi64]),
     "fp3d_host_elem_to_csr": (_int, [_i64, _vp, _vp, _vp, _vp]),

@@ -643,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
     const int32_t *__restrict__ e2c, const double *__restrict__ B,
     const double *__restrict__ V, const double *__restrict__ rt,
     const double *__restrict__ grad, const double *__restrict__ s_e,
-    const uint8_t *__restrict__ lim, double inv_u2,
+    const uint8_t *__restrict__ lim, double inv_u2, double clamp,
     double *__restrict__ data) {
   int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= k1) return;
@@ -653,6 +653,13 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
   double se = s_e[e];
   if (lim) se = se * (lim[e] ? 1.0 : 0.0);
   double w2 = 2.0 * inv_u2 * se * V[e];
+  if (clamp > 0.0) {
+    // SPD preconditioner variant: the streamwise softening never removes
+    // more than a fraction `clamp` of the element's stiffness along g
+    double q2 = g0 * g0 + g1 * g1 + g2 * g2;
+    double lo = q2 > 0.0 ? -clamp * w1 / q2 : 0.0;
+    w2 = w2 > lo ? w2 : lo;
+  }
   double b[12];
 #pragma unroll
   for (int t = 0; t < 12; ++t) b[t] = B[12 * e + t];
@@ -1013,9 +1020,35 @@ int fp3d_assemble_jacobian_colored(int n_colors, const int64_t *color_offsets,
     if (k1 == k0) continue;
     hipLaunchKernelGGL(k_jacobian_color, dim3(grid_for(k1 - k0)), dim3(kBlock),
                        0, FP3D_STREAM(stream), k0, k1, elem_index, elem_to_csr,
-                       B, V, rho_tilde, grad, s_e, lim, inv_u2, data);
+                       B, V, rho_tilde, grad, s_e, lim, inv_u2, 0.0, data);
   }
   return check_launch("assemble_jacobian_colored");
+}
+
+int fp3d_assemble_jacobian_spd_colored(int n_colors,
+                                       const int64_t *color_offsets,
+                                       const int32_t *elem_index,
+                                       const int32_t *elem_to_csr,
+                                       const double *B, const double *V,
+                                       const double *rho_tilde,
+                                       const double *grad, const double *s_e,
+                                       const uint8_t *lim, double inv_u2,
+                                       double clamp, double *data,
+                                       void *stream) {
+  int rc = check_offsets(n_colors, color_offsets);
+  if (rc) return rc;
+  FP3D_REQUIRE(elem_to_csr && B && V && rho_tilde && grad && s_e && data,
+               "assemble_jacobian_spd: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elem_to_csr), "elem_to_csr must be 16-byte aligned");
+  FP3D_REQUIRE(clamp > 0.0 && clamp < 1.0, "clamp must lie in (0, 1)");
+  for (int c = 0; c < n_colors; ++c) {
+    int64_t k0 = color_offsets[c], k1 = color_offsets[c + 1];
+    if (k1 == k0) continue;
+    hipLaunchKernelGGL(k_jacobian_color, dim3(grid_for(k1 - k0)), dim3(kBlock),
+                       0, FP3D_STREAM(stream), k0, k1, elem_index, elem_to_csr,
+                       B, V, rho_tilde, grad, s_e, lim, inv_u2, clamp, data);
+  }
+  return check_launch("assemble_jacobian_spd_colored");
 }
 
 int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,

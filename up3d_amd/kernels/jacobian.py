@@ -187,6 +187,17 @@ class PicardOperator:
                      float(inv_u2), ptr(out), stream_handle())
         return out
 
+    def jacobian_spd_dev(self, rho_tilde, grad, s_e, lim, inv_u2, clamp, out):
+        """SPD preconditioner matrix: Terms 1+2 with the Term-2 weight
+        clamped at -clamp * rho_tilde V / q2."""
+        st = self.store
+        _native.call("fp3d_fill", self.nnz, 0.0, ptr(out), stream_handle())
+        _native.call("fp3d_assemble_jacobian_spd_colored", self.n_colors,
+                     st.offsets_ct, None, ptr(st.e2c), ptr(st.B), ptr(st.V),
+                     ptr(rho_tilde), ptr(grad), ptr(s_e), ptr(lim),
+                     float(inv_u2), float(clamp), ptr(out), stream_handle())
+        return out
+
     # ---------------------------------------------- reference-compatible API
     def _csr(self, data: np.ndarray) -> sp.csr_matrix:
         return sp.csr_matrix((data, self._pattern_indices, self._pattern_indptr),

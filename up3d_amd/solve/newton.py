@@ -179,8 +179,18 @@ class NewtonWorkspace:
             self.jac.n_active = 0
         return self.jac
 
-    def build_preconditioner(self, s: _State):
-        self.op.matrix_dev(s.rt, self.A_pic)
+    def build_preconditioner(self, s: _State, precond_matrix: str = "picard",
+                             clamp: float = 0.9, se=None):
+        """AMG of the Picard matrix A(rho_tilde) (the reference's choice) or
+        of the SPD clamped Jacobian (precond_matrix="jacobian_spd")."""
+        if precond_matrix == "jacobian_spd" and se is not None:
+            self.op.jacobian_spd_dev(s.rt, s.grad, se, s.lim,
+                                     1.0 / (self.u_inf * self.u_inf), clamp,
+                                     self.A_pic)
+        elif precond_matrix in ("picard", "jacobian_spd"):
+            self.op.matrix_dev(s.rt, self.A_pic)
+        else:
+            raise ValueError(f"unknown precond_matrix {precond_matrix!r}")
         mask_dirichlet(self.st, self.fm, self.A_pic)
         self.precond = amg_from_device(
             self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64")
@@ -221,7 +231,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                  freeze_refresh_max: int = 2, freeze_max_reverts: int = 3,
                  workspace: Optional[NewtonWorkspace] = None,
                  external_rhs: Optional[np.ndarray] = None,
-                 precision: str = "fp64", verbose: bool = False
+                 precision: str = "fp64", precond_matrix: str = "picard",
+                 precond_clamp: float = 0.9, verbose: bool = False
                  ) -> Dict[str, object]:
     """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
     n_st = 0; precond is always the device AMG).  Converged when
@@ -229,7 +240,11 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     amg_rebuild_every = k > 0 rebuilds the AMG hierarchy every k Newton
     steps (the reference's rule); k <= 0 selects the adaptive rule (keep
     the hierarchy, also across Mach levels of a shared workspace, until a
-    Krylov solve stalls or its iteration count doubles)."""
+    Krylov solve stalls or its iteration count doubles).  precond_matrix
+    "picard" builds the AMG on A(rho_tilde) as the reference does;
+    "jacobian_spd" builds it on the Newton Terms 1+2 with the streamwise
+    (Term 2) weight clamped at -precond_clamp * rho_tilde V / q2 (SPD, keeps
+    the near-sonic anisotropy the Picard matrix lacks)."""
     if m_inf > 0.0 and upwind_c <= 0.0:
         raise ValueError("the Newton driver runs with the walk upwind machinery "
                          "active for m_inf > 0; pass upwind_c > 0")
@@ -424,7 +439,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
             rebuild = (M_pre is None or last_lin[0] != 0
                        or last_lin[1] > 2 * max(last_lin[2], 10))
         if rebuild:
-            M_pre = ws.build_preconditioner(cur)
+            M_pre = ws.build_preconditioner(cur, precond_matrix, precond_clamp,
+                                            se=ws.upw.d_se)
             last_lin[2] = -1
         sync()
         timings["precond"] += time.perf_counter() - t0
