@@ -217,15 +217,18 @@ class ChannelForceProblem:
         st = L["st"]
         n, nt = st.n_nodes, st.n_tets
         fm = L["fm"]
-        # functional derivatives (host autograd)
-        nodes_t = torch.tensor(sol["nodes"], dtype=torch.float64, requires_grad=True)
-        phi_t = torch.tensor(sol["phi"], dtype=torch.float64, requires_grad=True)
-        m_t = torch.tensor(sol["m_inf"], dtype=torch.float64, requires_grad=True)
-        a_t = torch.tensor(sol["alpha_deg"], dtype=torch.float64, requires_grad=True)
-        val, cf, cl = _wall_functional(nodes_t, phi_t, m_t, a_t,
-                                       torch.from_numpy(self.wall_faces),
-                                       torch.tensor(self.flip), weights[0], weights[1])
-        dFx, dFphi, dFm, dFa = torch.autograd.grad(val, (nodes_t, phi_t, m_t, a_t))
+        # functional derivatives (host autograd; also valid inside backward)
+        with torch.enable_grad():
+            nodes_t = torch.tensor(sol["nodes"], dtype=torch.float64, requires_grad=True)
+            phi_t = torch.tensor(sol["phi"], dtype=torch.float64, requires_grad=True)
+            m_t = torch.tensor(sol["m_inf"], dtype=torch.float64, requires_grad=True)
+            a_t = torch.tensor(sol["alpha_deg"], dtype=torch.float64, requires_grad=True)
+            val, cf, cl = _wall_functional(nodes_t, phi_t, m_t, a_t,
+                                           torch.from_numpy(self.wall_faces),
+                                           torch.tensor(self.flip), weights[0],
+                                           weights[1])
+            dFx, dFphi, dFm, dFa = torch.autograd.grad(val, (nodes_t, phi_t, m_t, a_t))
+        val, cf, cl = val.detach(), cf.detach(), cl.detach()
         dFx, dFphi = dFx.numpy(), dFphi.numpy()
         # adjoint solve J_ff^T lam = (dF/dphi)_free
         b = to_dev(dFphi)
@@ -284,7 +287,7 @@ class ChannelForceProblem:
         grad_x[self.dn, 1] += coef * np.sin(a)
         return {"d_y_floor": grad_x[self.floor_nodes, 1].copy(),
                 "d_nodes": grad_x, "d_m_inf": dF_dm, "d_alpha_deg": dF_da,
-                "value": float(val), "cf": cf.detach().numpy(), "cl": float(cl),
+                "value": float(val), "cf": cf.numpy(), "cl": float(cl),
                 "adjoint_iters": n_it, "adjoint_info": info,
                 "lambda": to_host(lam)}
 
@@ -327,12 +330,13 @@ class ChannelForceProblem:
         dphi, info, _ = gmres_dev(mv, rhs, ml.apply, rtol=rtol, restart=60, maxiter=40)
         dphi_h = to_host(dphi) + vd
         t = lambda v, g=False: torch.tensor(v, dtype=torch.float64, requires_grad=g)  # noqa: E731
-        m_t, a_t = t(sol["m_inf"], True), t(sol["alpha_deg"], True)
-        phi_t = t(sol["phi"], True)
-        val, _, _ = _wall_functional(t(nodes), phi_t, m_t, a_t,
-                                     torch.from_numpy(self.wall_faces),
-                                     t(self.flip), weights[0], weights[1])
-        gphi, gm, ga = torch.autograd.grad(val, (phi_t, m_t, a_t))
+        with torch.enable_grad():
+            m_t, a_t = t(sol["m_inf"], True), t(sol["alpha_deg"], True)
+            phi_t = t(sol["phi"], True)
+            val, _, _ = _wall_functional(t(nodes), phi_t, m_t, a_t,
+                                         torch.from_numpy(self.wall_faces),
+                                         t(self.flip), weights[0], weights[1])
+            gphi, gm, ga = torch.autograd.grad(val, (phi_t, m_t, a_t))
         return float(gm) * dm + float(ga) * dalpha_deg + float(gphi.numpy() @ dphi_h)
 
 

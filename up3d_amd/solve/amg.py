@@ -46,7 +46,7 @@ def standard_aggregation(A: sp.csr_matrix):
     return agg, int(na.value)
 
 
-def spectral_radius_dinv_a(A: sp.csr_matrix, dinv: np.ndarray, iters: int = 20,
+def spectral_radius_dinv_a(A: sp.csr_matrix, dinv: np.ndarray, iters: int = 12,
                            seed: int = 0) -> float:
     """Power-iteration estimate of rho(D^-1 A) from a seeded start vector."""
     rng = np.random.default_rng(seed)

@@ -240,7 +240,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     amg_rebuild_every = k > 0 rebuilds the AMG hierarchy every k Newton
     steps (the reference's rule); k <= 0 selects the adaptive rule (keep
     the hierarchy, also across Mach levels of a shared workspace, until a
-    Krylov solve stalls or its iteration count doubles).  precond_matrix
+    Krylov solve stalls or its iteration count doubles, at most every 3
+    Newton steps).  precond_matrix
     "picard" builds the AMG on A(rho_tilde) as the reference does;
     "jacobian_spd" builds it on the Newton Terms 1+2 with the streamwise
     (Term 2) weight clamped at -precond_clamp * rho_tilde V / q2 (SPD, keeps
@@ -313,7 +314,9 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     dphi = empty(ws.n)
     rhs = empty(ws.n)
     M_pre = ws.precond if amg_rebuild_every <= 0 else None
-    last_lin = [0, 0, 0]      # (info, n_iter) of the last solve, n_iter after rebuild
+    # (info, n_iter) of the last solve, n_iter right after the last rebuild,
+    # Newton step of the last rebuild
+    last_lin = [0, 0, 0, -10]
 
     def close_step(rec):
         nonlocal prev_snap
@@ -436,12 +439,15 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
             # adaptive: keep the hierarchy while the Krylov solves stay cheap;
             # rebuild after a stalled solve or when the iteration count has
             # doubled since the last rebuild
-            rebuild = (M_pre is None or last_lin[0] != 0
-                       or last_lin[1] > 2 * max(last_lin[2], 10))
+            rebuild = (M_pre is None
+                       or ((last_lin[0] != 0
+                            or last_lin[1] > 2 * max(last_lin[2], 10))
+                           and it - last_lin[3] >= 3))
         if rebuild:
             M_pre = ws.build_preconditioner(cur, precond_matrix, precond_clamp,
                                             se=ws.upw.d_se)
             last_lin[2] = -1
+            last_lin[3] = it
         sync()
         timings["precond"] += time.perf_counter() - t0
         eta = _ew_forcing(r_norm, residual_history[-2] if it > 0 else None,
