@@ -248,6 +248,18 @@ int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
 int fp3d_csr_mask_dirichlet(int64_t n_rows, const int32_t *indptr,
                             const int32_t *indices, const uint8_t *free_mask,
                             double *data, float *data32, void *stream);
+/* Lumped nodal volume m_v = sum_e V_e / 4 over the node -> (4k+i)
+ * incidence (fixed order).  No reference counterpart: the pseudo-transient
+ * Newton globalisation is an addition of this library. */
+int fp3d_lumped_volume(int64_t n_nodes, const int32_t *inc_ptr,
+                       const int32_t *inc_ent, const double *V, double *out,
+                       void *stream);
+/* data[diag_pos[r]] += scale * w[r] on free rows (pseudo-transient term
+ * m_v / dt on the Jacobian or preconditioner diagonal).  No reference
+ * counterpart. */
+int fp3d_csr_diag_shift(int64_t n_rows, const int32_t *diag_pos,
+                        const double *w, double scale,
+                        const uint8_t *free_mask, double *data, void *stream);
 int fp3d_csr_to_f32(int64_t nnz, const double *data, float *out, void *stream);
 int fp3d_fill(int64_t n, double value, double *x, void *stream);
 /* y = a x + b y */

@@ -84,6 +84,8 @@ _SIGS = {
                                    _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
     "fp3d_csr_mask_dirichlet": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_to_f32": (_int, [_i64, _vp, _vp, _vp]),
+    "fp3d_lumped_volume": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_csr_diag_shift": (_int, [_i64, _vp, _vp, _dbl, _vp, _vp, _vp]),
     "fp3d_fill": (_int, [_i64, _dbl, _vp, _vp]),
     "fp3d_axpby": (_int, [_i64, _dbl, _vp, _dbl, _vp, _vp]),
     "fp3d_scale": (_int, [_i64, _dbl, _vp, _vp, _vp]),

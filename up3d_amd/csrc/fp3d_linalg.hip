@@ -181,6 +181,28 @@ __global__ __launch_bounds__(kBlock) void k_mask_dirichlet(
   }
 }
 
+// lumped nodal volume m_v = sum over incident (element, local) entries of
+// V_e / 4, summed in the incidence order (ascending storage element)
+__global__ __launch_bounds__(kBlock) void k_lumped_volume(
+    int64_t nn, const int32_t *__restrict__ iptr,
+    const int32_t *__restrict__ ient, const double *__restrict__ V,
+    double *__restrict__ out) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  double m = 0.0;
+  for (int32_t q = iptr[v]; q < iptr[v + 1]; ++q) m += 0.25 * V[ient[q] >> 2];
+  out[v] = m;
+}
+
+// data[diag[r]] += s * w[r] on free rows (pseudo-transient diagonal)
+__global__ __launch_bounds__(kBlock) void k_diag_shift(
+    int64_t n, const int32_t *__restrict__ diag, const double *__restrict__ w,
+    double s, const uint8_t *__restrict__ fm, double *__restrict__ data) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || !fm[r]) return;
+  data[diag[r]] += s * w[r];
+}
+
 __device__ __forceinline__ double wsum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
@@ -645,6 +667,29 @@ int fp3d_csr_mask_dirichlet(int64_t n_rows, const int32_t *indptr,
                      FP3D_STREAM(stream), n_rows, indptr, indices, free_mask,
                      data, data32);
   return check_launch("csr_mask_dirichlet");
+}
+
+int fp3d_lumped_volume(int64_t n_nodes, const int32_t *inc_ptr,
+                       const int32_t *inc_ent, const double *V, double *out,
+                       void *stream) {
+  FP3D_REQUIRE(n_nodes == 0 || (inc_ptr && inc_ent && V && out),
+               "lumped_volume: NULL pointer");
+  if (n_nodes == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_lumped_volume, dim3(grid_for(n_nodes)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n_nodes, inc_ptr, inc_ent, V, out);
+  return check_launch("lumped_volume");
+}
+
+int fp3d_csr_diag_shift(int64_t n_rows, const int32_t *diag_pos,
+                        const double *w, double scale,
+                        const uint8_t *free_mask, double *data, void *stream) {
+  FP3D_REQUIRE(n_rows == 0 || (diag_pos && w && free_mask && data),
+               "csr_diag_shift: NULL pointer");
+  if (n_rows == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_diag_shift, dim3(grid_for(n_rows)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n_rows, diag_pos, w, scale, free_mask,
+                     data);
+  return check_launch("csr_diag_shift");
 }
 
 int fp3d_csr_to_f32(int64_t nnz, const double *data, float *out, void *stream) {

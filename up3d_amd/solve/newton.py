@@ -101,6 +101,28 @@ class NewtonWorkspace:
         self.m_inf = None
         self.q2cap = np.inf
         self.precond = None
+        self._ptc = None
+
+    def ptc_weights(self):
+        """(lumped nodal volume, CSR diagonal positions) on the device."""
+        if self._ptc is None:
+            st = self.st
+            m = empty(self.n)
+            _native.call("fp3d_lumped_volume", self.n, ptr(st.inc_ptr),
+                         ptr(st.inc_ent), ptr(st.V), ptr(m), stream_handle())
+            ip, ix = st.indptr_host, st.indices_host
+            rows = np.repeat(np.arange(self.n), np.diff(ip))
+            diag = np.flatnonzero(ix == rows)
+            if len(diag) != self.n:
+                raise RuntimeError("CSR pattern lacks a diagonal entry")
+            self._ptc = (m, to_dev(diag.astype(np.int32), torch.int32))
+        return self._ptc
+
+    def _diag_shift(self, data, scale):
+        if scale > 0.0:
+            m, diag = self.ptc_weights()
+            _native.call("fp3d_csr_diag_shift", self.n, ptr(diag), ptr(m),
+                         float(scale), ptr(self.fm), ptr(data), stream_handle())
 
     def set_mach(self, m_inf: float, m_cap: float = 3.0) -> None:
         if not 0.0 <= m_inf < 1.0:
@@ -150,9 +172,10 @@ class NewtonWorkspace:
 
     # ------------------------------------------------------- linearisation
     def assemble_coupled(self, s: _State, upwind_c, m_crit, rho_floor,
-                         frozen=None):
+                         frozen=None, ptc_scale: float = 0.0):
         """Exact Jacobian at state s into the NewtonOperator (Terms 1+2 in
-        CSR, Dirichlet-embedded, Term 3 as a gather table)."""
+        CSR, Dirichlet-embedded, Term 3 as a gather table); ptc_scale > 0
+        adds ptc_scale * (lumped nodal volume) to the free diagonal."""
         upw = self.upw
         inv_u2 = 1.0 / (self.u_inf * self.u_inf)
         if self.m_inf > 0.0 and frozen is not None:
@@ -172,6 +195,7 @@ class NewtonWorkspace:
             su = upw.d_su.zero_()
             upstream = upw.walk_dev(s.grad)
         self.op.jacobian_dev(s.rt, s.grad, se, s.lim, inv_u2, self.jac.data)
+        self._diag_shift(self.jac.data, ptc_scale)
         self.jac.finalize(f32=self.precision == "fp32")
         if self.m_inf > 0.0:
             self.jac.build_term3(s.grad, upstream, su, s.lim, inv_u2)
@@ -180,7 +204,7 @@ class NewtonWorkspace:
         return self.jac
 
     def build_preconditioner(self, s: _State, precond_matrix: str = "picard",
-                             clamp: float = 0.9, se=None):
+                             clamp: float = 0.9, se=None, ptc_scale: float = 0.0):
         """AMG of the Picard matrix A(rho_tilde) (the reference's choice) or
         of the SPD clamped Jacobian (precond_matrix="jacobian_spd")."""
         if precond_matrix == "jacobian_spd" and se is not None:
@@ -191,6 +215,7 @@ class NewtonWorkspace:
             self.op.matrix_dev(s.rt, self.A_pic)
         else:
             raise ValueError(f"unknown precond_matrix {precond_matrix!r}")
+        self._diag_shift(self.A_pic, ptc_scale)
         mask_dirichlet(self.st, self.fm, self.A_pic)
         self.precond = amg_from_device(
             self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64")
@@ -232,7 +257,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                  workspace: Optional[NewtonWorkspace] = None,
                  external_rhs: Optional[np.ndarray] = None,
                  precision: str = "fp64", precond_matrix: str = "picard",
-                 precond_clamp: float = 0.9, verbose: bool = False
+                 precond_clamp: float = 0.9, ptc_dt0: Optional[float] = None,
+                 ptc_dt_max: float = 1e6, verbose: bool = False
                  ) -> Dict[str, object]:
     """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
     n_st = 0; precond is always the device AMG).  Converged when
@@ -245,7 +271,14 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     "picard" builds the AMG on A(rho_tilde) as the reference does;
     "jacobian_spd" builds it on the Newton Terms 1+2 with the streamwise
     (Term 2) weight clamped at -precond_clamp * rho_tilde V / q2 (SPD, keeps
-    the near-sonic anisotropy the Picard matrix lacks)."""
+    the near-sonic anisotropy the Picard matrix lacks).
+
+    ptc_dt0 (off by default; an addition without a reference counterpart)
+    turns on pseudo-transient continuation: each linear step solves
+    (J + M_lumped / dt) dphi = -R with the switched-evolution-relaxation
+    rule dt = ptc_dt0 * ||R_0||_2 / ||R||_2 (R_0 the level's first
+    residual); once dt reaches ptc_dt_max the steps are plain Newton.  The
+    converged state is the same discrete solution (R = 0)."""
     if m_inf > 0.0 and upwind_c <= 0.0:
         raise ValueError("the Newton driver runs with the walk upwind machinery "
                          "active for m_inf > 0; pass upwind_c > 0")
@@ -307,6 +340,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     freeze_point = None            # (phi copy, r_norm)
     freeze_cooldown = 0
     n_freeze_reverts = 0
+    ptc_r2_ref = None
     r_level_best = np.inf
     freeze_armed = freeze_tol is not None
     prev_snap = snapshot(timings)
@@ -332,7 +366,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
         clamp_history.append((cur.n_limited, cur.n_floored))
         if step_records:
             close_step(step_records[-1])
-        rec = {"i": it, "residual": r_norm, "n_limited": cur.n_limited,
+        rec = {"i": it, "residual": r_norm, "residual_l2": float(np.sqrt(cur.merit)),
+               "n_limited": cur.n_limited,
                "n_floored": cur.n_floored, "frozen": frozen is not None,
                "n_lin_solves": 0, "eta": None, "lam": None}
         step_records.append(rec)
@@ -341,7 +376,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
             if r2 > 0.0 and r1 > 0.0 and r0 > 0.0 and r1 != r0:
                 newton_orders.append(float(np.log(r2 / r1) / np.log(r1 / r0)))
         if verbose:
-            print(f"  newton {it:2d}: |R|={r_norm:.3e} lim={cur.n_limited} "
+            print(f"  newton {it:2d}: |R|={r_norm:.3e} |R|2={np.sqrt(cur.merit):.3e} "
+                  f"lim={cur.n_limited} "
                   f"flr={cur.n_floored}" + (" [frozen]" if frozen is not None else ""),
                   flush=True)
         r_level_best = min(r_level_best, r_norm)
@@ -428,8 +464,17 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
             continue
 
         # ---- linearisation
+        ptc_scale = 0.0
+        if ptc_dt0 is not None:
+            r2 = float(np.sqrt(cur.merit))
+            if ptc_r2_ref is None:
+                ptc_r2_ref = r2
+            dt = ptc_dt0 * ptc_r2_ref / max(r2, 1e-300)
+            ptc_scale = 0.0 if dt >= ptc_dt_max else 1.0 / dt
+            rec["ptc_dt"] = dt
         t0 = time.perf_counter()
-        ws.assemble_coupled(cur, upwind_c, m_crit, rho_floor, frozen)
+        ws.assemble_coupled(cur, upwind_c, m_crit, rho_floor, frozen,
+                            ptc_scale=ptc_scale)
         sync()
         timings["assembly"] += time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -445,7 +490,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                            and it - last_lin[3] >= 3))
         if rebuild:
             M_pre = ws.build_preconditioner(cur, precond_matrix, precond_clamp,
-                                            se=ws.upw.d_se)
+                                            se=ws.upw.d_se, ptc_scale=ptc_scale)
             last_lin[2] = -1
             last_lin[3] = it
         sync()
@@ -501,6 +546,9 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                 raise RuntimeError(f"Newton line search failed at iteration {it}")
             _, trial, lam = best
         rec["lam"] = float(lam)
+        if verbose:
+            print(f"      step lam={lam:g}" + (f" dt={rec['ptc_dt']:.2e}"
+                                               if "ptc_dt" in rec else ""), flush=True)
         cur = trial
 
     if step_records:
