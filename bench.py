@@ -57,14 +57,16 @@ def algorithmic_bytes(n_tets, n_nodes, nnz):
         # fn 16, own centroid 24, elements 16, grad 24, q2 8, rho 8; write
         # upstream 4, nu 8, rho_tilde 8; nodes 24/node
         "upwind": 116 * n_tets + 24 * n_nodes,
-        # elements 16, B 96, V 8, rho_tilde 8, grad 24; R 8/node
-        "residual": 152 * n_tets + 8 * n_nodes,
+        # node gather: incidence 16, B 96, V 8, rho_tilde 8, grad 24;
+        # inc_ptr 4 + R 8 per node
+        "residual": 152 * n_tets + 12 * n_nodes,
         # upstream 4, q2 8, rho 8, write s_e 8, s_u 8 (+ q2/rho of u(e),
         # non-local, counted once above)
         "sensitivities": 36 * n_tets,
-        # elem_to_csr 64, B 96, V 8, rho_tilde 8, grad 24, s_e 8, lim 1;
-        # CSR values written (zero fill + final) 16/nnz
-        "jacobian": 209 * n_tets + 16 * nnz,
+        # node gather: incidence 16, elem_to_csr 64, B 96, V 8, rho_tilde 8,
+        # grad 24, s_e 8, lim 1; indptr + inc_ptr 8 per node; CSR values
+        # written once 8/nnz
+        "jacobian": 225 * n_tets + 8 * n_nodes + 8 * nnz,
         # s_u 8, upstream 4, lim 1, flags 4 + incidence scan 16/tet + rows
         "term3": 33 * n_tets + 12 * n_nodes,
     }
@@ -249,6 +251,7 @@ def cpu_baseline(st, mesh, passes):
 
     class _Op(O.OraclePicardOperator):
         def __init__(self):
+            self.order = "colored"
             self.elements, self.B, self.V = el, B, V
             self.color_offsets, self.color_elems = off, ce
             self.n_colors = len(off) - 1

@@ -194,6 +194,43 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
                                        const uint8_t *lim, double inv_u2,
                                        double clamp, double *data,
                                        void *stream);
+/* ------------------------------------------------- node-gather assembly
+ * The coloured sums above computed row by row (no colour passes, no
+ * atomics): node v adds its incident (element, local) entries in ascending
+ * element order, the accumulation order of the reference's serial assembly
+ * (pyfp3d/kernels/residual.py:144-189 assemble_residual; the Picard/Newton
+ * matrices of pyfp3d/kernels/jacobian.py:105-180 summed as the serial
+ * triplets of pyfp3d/kernels/residual.py:106-129 + COO->CSR).  inc_ptr /
+ * inc_ent: node -> (4 e + i) incidence sorted by e.  Used by the operator
+ * classes (pyfp3d/kernels/jacobian.py:231-375 PicardOperator).
+ *
+ * Two stages: the element pass writes terms[4e + a] = rho_tilde V
+ * (grad . B_a), or the 16 matrix values K[16e + 4a + b]; the node pass sums
+ * them per node / per CSR slot.  Matrix terms: s_e == NULL gives the Picard
+ * matrix A = sum rho_tilde V B_i.B_j (grad unused); else Terms 1+2 with s_e
+ * masked by lim (lim may be NULL); clamp in (0, 1) gives the SPD
+ * preconditioner variant (as fp3d_assemble_jacobian_spd_colored), 0 the
+ * exact Terms 1+2.  gather_csr_terms writes every CSR value (no zero fill);
+ * max_seg >= max over row blocks of indptr[r0 + rows] - indptr[r0] (rows =
+ * fp3d_gather_rows_per_block()), at most 8192.  terms and K 32-byte
+ * aligned. */
+int fp3d_gather_rows_per_block(void);
+int fp3d_element_residual_terms(int64_t n_tets, const double *B,
+                                const double *V, const double *rho_tilde,
+                                const double *grad, double *terms,
+                                void *stream);
+int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
+                           const int32_t *inc_ent, const double *terms,
+                           double *R, void *stream);
+int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
+                              const double *rho_tilde, const double *grad,
+                              const double *s_e, const uint8_t *lim,
+                              double inv_u2, double clamp, double *K,
+                              void *stream);
+int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
+                          const int32_t *inc_ptr, const int32_t *inc_ent,
+                          const int32_t *elem_to_csr, const double *K,
+                          double *data, void *stream);
 /* pyfp3d/kernels/jacobian.py:183-228 fill_term3_coo (rows/cols int32). */
 int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
                         const int32_t *elements, const double *B,

@@ -104,6 +104,66 @@ void or_assemble_residual_colored(int n_colors, const int64_t *off,
   }
 }
 
+/* ---- pyfp3d/kernels/residual.py:144-189 assemble_residual (serial) ----
+ * The coloured residual's per-element arithmetic, accumulated in ascending
+ * element order as the reference's serial assembly does. */
+void or_assemble_residual_serial(int64_t n, const int32_t *el, const double *B,
+                                 const double *V, const double *rt,
+                                 const double *grad, int64_t n_nodes,
+                                 double *R) {
+  for (int64_t i = 0; i < n_nodes; ++i) R[i] = 0.0;
+  for (int64_t e = 0; e < n; ++e) {
+    double w = rt[e] * V[e];
+    double gx = grad[3 * e], gy = grad[3 * e + 1], gz = grad[3 * e + 2];
+    for (int i = 0; i < 4; ++i)
+      R[el[4 * e + i]] += w * (gx * B[12 * e + 3 * i] + gy * B[12 * e + 3 * i + 1] +
+                               gz * B[12 * e + 3 * i + 2]);
+  }
+}
+
+/* ---- Picard matrix / Newton Terms 1+2 in serial element order ----------
+ * pyfp3d/kernels/jacobian.py:105-180 per-element values, summed per CSR slot
+ * in ascending element order (the serial triplet assembly of
+ * pyfp3d/kernels/residual.py:106-129 followed by COO->CSR).  s_e == NULL
+ * gives the Picard matrix; clamp > 0 the clamped (SPD) variant. */
+void or_assemble_jacobian_serial(int64_t n, const int32_t *emap,
+                                 const double *B, const double *V,
+                                 const double *rt, const double *grad,
+                                 const double *s_e, const uint8_t *lim,
+                                 double inv_u2, double clamp, int64_t nnz,
+                                 double *data) {
+  for (int64_t k = 0; k < nnz; ++k) data[k] = 0.0;
+  for (int64_t e = 0; e < n; ++e) {
+    double w1 = rt[e] * V[e];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, w2 = 0.0;
+    if (s_e) {
+      g0 = grad[3 * e];
+      g1 = grad[3 * e + 1];
+      g2 = grad[3 * e + 2];
+      double se = s_e[e];
+      if (lim) se = se * (lim[e] ? 1.0 : 0.0);
+      w2 = 2.0 * inv_u2 * se * V[e];
+      if (clamp > 0.0) {
+        double q2 = g0 * g0 + g1 * g1 + g2 * g2;
+        double lo = q2 > 0.0 ? -clamp * w1 / q2 : 0.0;
+        w2 = w2 > lo ? w2 : lo;
+      }
+    }
+    for (int i = 0; i < 4; ++i) {
+      double bi0 = B[12 * e + 3 * i], bi1 = B[12 * e + 3 * i + 1],
+             bi2 = B[12 * e + 3 * i + 2];
+      double gi = g0 * bi0 + g1 * bi1 + g2 * bi2;
+      for (int j = 0; j < 4; ++j) {
+        double bj0 = B[12 * e + 3 * j], bj1 = B[12 * e + 3 * j + 1],
+               bj2 = B[12 * e + 3 * j + 2];
+        double val = w1 * (bi0 * bj0 + bi1 * bj1 + bi2 * bj2);
+        if (w2 != 0.0) val += w2 * gi * (g0 * bj0 + g1 * bj1 + g2 * bj2);
+        data[emap[16 * e + 4 * i + j]] += val;
+      }
+    }
+  }
+}
+
 /* ---- pyfp3d/kernels/jacobian.py:77-102 build_elem_to_csr -------------- */
 void or_build_elem_to_csr(int64_t n, const int32_t *el, const int32_t *indptr,
                           const int32_t *indices, int32_t *emap) {

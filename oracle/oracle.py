@@ -363,9 +363,16 @@ def rho_tilde_sensitivities(q2, rho, upstream, m, m_crit, Cc, g, rho_floor):
 
 class OraclePicardOperator:
     """pyfp3d/kernels/jacobian.py:231-375 PicardOperator restated on the CPU
-    (same set-up order: geometry, colouring, CSR pattern, elem_to_csr)."""
+    (same set-up order: geometry, colouring, CSR pattern, elem_to_csr).
+    order="serial" sums every residual entry and CSR slot in ascending
+    element order (the reference's serial assembly,
+    pyfp3d/kernels/residual.py:106-189); order="colored" in colour-class
+    order (the reference's coloured kernels)."""
 
-    def __init__(self, nodes, elements):
+    def __init__(self, nodes, elements, order="serial"):
+        if order not in ("serial", "colored"):
+            raise ValueError(f"unknown order {order!r}")
+        self.order = order
         self.nodes = _f64(nodes)
         self.elements = _i32a(elements)
         self.n_nodes = len(nodes)
@@ -387,8 +394,29 @@ class OraclePicardOperator:
         return sp.csr_matrix((data, self.pattern.indices, self.pattern.indptr),
                              shape=self.pattern.shape)
 
+    def _serial(self, rt, grad, s_e, lim, inv_u2, clamp):
+        data = np.empty(len(self.indices))
+        rt = _f64(rt)
+        g = None if grad is None else _f64(grad)
+        se = None if s_e is None else _f64(s_e)
+        lm = None if lim is None else np.ascontiguousarray(lim, dtype=np.uint8)
+        _lib().or_assemble_jacobian_serial(
+            C.c_int64(self.n_tets), _p(self.elem_to_csr, _i32), _p(self.B, _d),
+            _p(self.V, _d), _p(rt, _d), None if g is None else _p(g, _d),
+            None if se is None else _p(se, _d),
+            None if lm is None else lm.ctypes.data_as(C.POINTER(C.c_uint8)),
+            C.c_double(inv_u2), C.c_double(clamp), C.c_int64(len(data)),
+            _p(data, _d))
+        return data
+
+    def jacobian_spd_terms12(self, grad, rho_tilde, s_e, inv_u2, clamp, lim=None):
+        """The clamped (SPD) Terms 1+2 preconditioner matrix values."""
+        return self._serial(rho_tilde, grad, s_e, lim, inv_u2, clamp)
+
     def assemble_matrix(self, rho_tilde=None):
         rt = self.rho_ones if rho_tilde is None else _f64(rho_tilde)
+        if self.order == "serial":
+            return self._csr(self._serial(rt, None, None, None, 0.0, 0.0))
         data = np.zeros(len(self.indices))
         _lib().or_assemble_matrix_colored(
             C.c_int(self.n_colors), _p(self.color_offsets, _i64),
@@ -402,6 +430,12 @@ class OraclePicardOperator:
             grad, _ = self.velocities(phi)
         grad = _f64(grad)
         R = np.empty(self.n_nodes)
+        if self.order == "serial":
+            _lib().or_assemble_residual_serial(
+                C.c_int64(self.n_tets), _p(self.elements, _i32), _p(self.B, _d),
+                _p(self.V, _d), _p(_f64(rt), _d), _p(grad, _d),
+                C.c_int64(self.n_nodes), _p(R, _d))
+            return R
         _lib().or_assemble_residual_colored(
             C.c_int(self.n_colors), _p(self.color_offsets, _i64),
             _p(self.color_elems, _i64), _p(self.elements, _i32), _p(self.B, _d),
@@ -410,6 +444,8 @@ class OraclePicardOperator:
         return R
 
     def jacobian_terms12(self, grad, rho_tilde, s_e, inv_u2):
+        if self.order == "serial":
+            return self._serial(rho_tilde, grad, s_e, None, inv_u2, 0.0)
         data = np.zeros(len(self.indices))
         g = _f64(grad)
         rt = _f64(rho_tilde)

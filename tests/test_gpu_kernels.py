@@ -33,7 +33,8 @@ def ora(mesh):
     g, q2 = op.velocities(phi)
     rho = O.density_field(q2, M)
     up = upw.upstream(g)
-    return dict(op=op, upw=upw, phi=phi, g=g, q2=q2, rho=rho, up=up)
+    opc = O.OraclePicardOperator(mesh.nodes, mesh.elements, order="colored")
+    return dict(op=op, opc=opc, upw=upw, phi=phi, g=g, q2=q2, rho=rho, up=up)
 
 
 def _ulp_close(a, b, n_ulp=8):
@@ -107,7 +108,7 @@ def test_colored_assembly_bitwise(mesh, ora):
                                            fill_term3_coo)
     from up3d_amd.kernels.residual import assemble_residual_colored
 
-    op = ora["op"]
+    op = ora["opc"]
     n = mesh.n_tets
     rt = np.linspace(0.7, 1.3, n)
     R = np.empty(mesh.n_nodes)
@@ -133,6 +134,47 @@ def test_colored_assembly_bitwise(mesh, ora):
                    1.0, rows, cols, vals)
     assert np.array_equal(rows, r_o) and np.array_equal(cols, c_o)
     assert np.array_equal(vals, v_o)
+
+
+def test_gather_assembly_bitwise(mesh, ora):
+    """Operator hot path (node gather, one launch) == the oracle's serial
+    element-order assembly, bit for bit: residual, Picard matrix, Terms 1+2
+    with a limiter mask, and the clamped SPD variant."""
+    import torch
+
+    from up3d_amd.device import to_dev, to_host
+    from up3d_amd.kernels.jacobian import PicardOperator
+
+    op = PicardOperator(mesh.nodes, mesh.elements)
+    # the oracle's assembly fed the device's own geometry (the host set-up's
+    # cofactor B_e agrees with the oracle's to ~1e-15, checked elsewhere)
+    oo = O.OraclePicardOperator(mesh.nodes, mesh.elements)
+    oo.B = np.ascontiguousarray(op.store.B_host.reshape(-1, 4, 3))
+    oo.V = np.ascontiguousarray(op.store.V_host)
+    n = mesh.n_tets
+    rt = np.linspace(0.7, 1.3, n)
+    g = ora["g"]
+    se = np.sin(np.arange(n)) * (np.arange(n) % 3 != 0)
+    lim = (np.arange(n) % 5 != 0).astype(np.uint8)
+    rtd, gd, sed = to_dev(rt), to_dev(g.reshape(-1, 3)), to_dev(se)
+    limd = to_dev(lim, torch.uint8)
+    R = to_host(op.residual_dev(rtd, gd, torch.empty(mesh.n_nodes, dtype=torch.float64,
+                                                     device=rtd.device)))
+    assert np.array_equal(R, oo.assemble_residual(ora["phi"], rt, g))
+    out = torch.full((op.nnz,), np.nan, dtype=torch.float64, device=rtd.device)
+    assert np.array_equal(to_host(op.matrix_dev(rtd, out)), oo.assemble_matrix(rt).data)
+    out.fill_(np.nan)
+    J = to_host(op.jacobian_dev(rtd, gd, sed, limd, 1.0, out))
+    assert np.array_equal(J, oo._serial(rt, g, se, lim, 1.0, 0.0))
+    assert np.array_equal(to_host(op.jacobian_dev(rtd, gd, sed, None, 0.5, out)),
+                          oo.jacobian_terms12(g, rt, se, 0.5))
+    Js = to_host(op.jacobian_spd_dev(rtd, gd, -(sed.abs() * 40.0), limd, 1.0, 0.9, out))
+    assert np.array_equal(Js, oo.jacobian_spd_terms12(g, rt, -np.abs(se) * 40.0, 1.0,
+                                                      0.9, lim))
+    # serial order vs the reference's colour order: same sums to 1e-13
+    Jc = ora["opc"].jacobian_terms12(g, rt, se, 1.0)
+    assert np.abs(to_host(op.jacobian_dev(rtd, gd, sed, None, 1.0, out)) - Jc).max() \
+        <= 1e-13 * np.abs(Jc).max()
 
 
 def test_picard_operator_api(mesh, ora):

@@ -150,9 +150,12 @@ def test_freestream_residual_machine_zero(jcube):
         assert np.abs(R[~cube_boundary_mask(nodes)]).max() < 1e-12
 
 
-def test_colored_vs_serial_reference(jcube):
+@pytest.mark.parametrize("order", ["serial", "colored"])
+def test_colored_vs_serial_reference(jcube, order):
+    # tests/test_p3_assembly.py:42-69: fast assembly vs the serial
+    # reference (geometry recomputed, scipy COO->CSR) to 1e-13
     nodes, el = jcube
-    op = O.OraclePicardOperator(nodes, el)
+    op = O.OraclePicardOperator(nodes, el, order=order)
     A = op.assemble_matrix()
     Aref = O.stiffness_matrix_reference(nodes, el)
     assert np.array_equal(A.indptr, Aref.indptr) and np.array_equal(A.indices, Aref.indices)
@@ -163,6 +166,36 @@ def test_colored_vs_serial_reference(jcube):
     assert np.abs(R - Rr).max() < 1e-13 * np.abs(Rr).max()
     # A phi == R for the Laplace residual (linearity)
     assert np.abs(A @ phi - R).max() < 1e-12 * np.abs(R).max()
+
+
+def test_serial_order_is_element_order(jcube):
+    """The serial oracle sums each CSR slot in ascending element order:
+    recompute slot sums with an explicit Python loop over elements for a
+    few rows (brute force) and the clamped variant's PSD property."""
+    nodes, el = jcube
+    op = O.OraclePicardOperator(nodes, el)
+    n = len(el)
+    rng = np.random.default_rng(2)
+    rt = rng.uniform(0.5, 1.5, n)
+    g = rng.standard_normal((n, 3))
+    se = -np.abs(rng.standard_normal(n))
+    J = op.jacobian_terms12(g, rt, se, 1.0)
+    ref = np.zeros_like(J)
+    for e in range(n):
+        B = op.B[e]
+        w1 = rt[e] * op.V[e]
+        w2 = 2.0 * se[e] * op.V[e]
+        for i in range(4):
+            gi = g[e] @ B[i]
+            for j in range(4):
+                val = w1 * (B[i, 0] * B[j, 0] + B[i, 1] * B[j, 1] + B[i, 2] * B[j, 2])
+                val += w2 * gi * (g[e, 0] * B[j, 0] + g[e, 1] * B[j, 1] + g[e, 2] * B[j, 2])
+                ref[op.elem_to_csr[e, i, j]] += val
+    assert np.abs(J - ref).max() <= 1e-15 * np.abs(ref).max()
+    Js = op.jacobian_spd_terms12(g, rt, se * 50.0, 1.0, 0.9)
+    A = op._csr(Js).toarray()
+    assert np.allclose(A, A.T)
+    assert np.linalg.eigvalsh(A).min() > -1e-12 * np.abs(A).max()
 
 
 def test_laplace_harmonic_quadratic_exact(kuhn8):

@@ -84,6 +84,12 @@ _SIGS = {
                                    _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
     "fp3d_csr_mask_dirichlet": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_to_f32": (_int, [_i64, _vp, _vp, _vp]),
+    "fp3d_gather_rows_per_block": (_int, []),
+    "fp3d_element_residual_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_gather_node_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_element_matrix_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl,
+                                         _vp, _vp]),
+    "fp3d_gather_csr_terms": (_int, [_i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_lumped_volume": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_diag_shift": (_int, [_i64, _vp, _vp, _dbl, _vp, _vp, _vp]),
     "fp3d_fill": (_int, [_i64, _dbl, _vp, _vp]),

@@ -689,6 +689,158 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
   for (int t = 0; t < 16; ++t) data[q[t]] = acc[t];
 }
 
+// ---------------------------------------------------- node-gather assembly
+// The same sums as the coloured scatters, computed row by row: node v
+// gathers the contributions of its incident (element, local index) entries
+// in ascending element order -- the accumulation order of the reference's
+// serial assembly (pyfp3d/kernels/residual.py:144-189,
+// pyfp3d/kernels/residual.py:106-129 + COO->CSR) -- with no colour passes,
+// no atomics, and each output written exactly once.
+//
+// Workgroups are dealt to the 8 XCDs round-robin; xcd_block() maps them so
+// that each XCD walks one contiguous range of row blocks, keeping the
+// elements shared by neighbouring rows in that XCD's L2.  The grid is padded
+// to a multiple of 8 so the mapping is a bijection.
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nb_pad) {
+  int64_t per = nb_pad >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+// Two stages: an element pass writes each element's contributions once,
+// coalesced (4 residual terms, or the 16 matrix values row-major [4a + b]);
+// the node pass then reads one short record per incidence entry instead of
+// the element's geometry and state (reading those per entry costs about
+// 1.6x more on the 4.7M-tet channel).
+__global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
+    int64_t n, const double *__restrict__ B, const double *__restrict__ V,
+    const double *__restrict__ rt, const double *__restrict__ grad,
+    double *__restrict__ c) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double w = rt[e] * V[e];
+  double gx = grad[3 * e], gy = grad[3 * e + 1], gz = grad[3 * e + 2];
+  const double *b = B + 12 * e;
+  double4 o;
+  o.x = w * (gx * b[0] + gy * b[1] + gz * b[2]);
+  o.y = w * (gx * b[3] + gy * b[4] + gz * b[5]);
+  o.z = w * (gx * b[6] + gy * b[7] + gz * b[8]);
+  o.w = w * (gx * b[9] + gy * b[10] + gz * b[11]);
+  *reinterpret_cast<double4 *>(c + 4 * e) = o;
+}
+
+// The loads of kUnroll consecutive entries are issued together (they are
+// independent), then added one by one in entry order, so the sum keeps the
+// ascending-element order while kUnroll gathers are in flight per lane.
+constexpr int kUnroll = 8;
+
+__global__ __launch_bounds__(kBlock) void k_gather_terms(
+    int64_t nn, int64_t nb_pad, const int32_t *__restrict__ iptr,
+    const int32_t *__restrict__ ient, const double *__restrict__ c,
+    double *__restrict__ R) {
+  int64_t v = xcd_block(blockIdx.x, nb_pad) * kBlock + threadIdx.x;
+  if (v >= nn) return;
+  double r = 0.0;
+  const int32_t q1 = iptr[v + 1];
+  for (int32_t q = iptr[v]; q < q1; q += kUnroll) {
+    double t[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k)
+      t[k] = (q + k < q1) ? c[ient[q + k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k)
+      if (q + k < q1) r += t[k];
+  }
+  R[v] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
+    int64_t n, const double *__restrict__ B, const double *__restrict__ V,
+    const double *__restrict__ rt, const double *__restrict__ grad,
+    const double *__restrict__ s_e, const uint8_t *__restrict__ lim,
+    double inv_u2, double clamp, double *__restrict__ K) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double w1 = rt[e] * V[e];
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, w2 = 0.0;
+  if (s_e) {
+    g0 = grad[3 * e];
+    g1 = grad[3 * e + 1];
+    g2 = grad[3 * e + 2];
+    double se = s_e[e];
+    if (lim) se = se * (lim[e] ? 1.0 : 0.0);
+    w2 = 2.0 * inv_u2 * se * V[e];
+    if (clamp > 0.0) {
+      double q2 = g0 * g0 + g1 * g1 + g2 * g2;
+      double lo = q2 > 0.0 ? -clamp * w1 / q2 : 0.0;
+      w2 = w2 > lo ? w2 : lo;
+    }
+  }
+  double b[12];
+#pragma unroll
+  for (int t = 0; t < 12; ++t) b[t] = B[12 * e + t];
+  double k[16];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double bi0 = b[3 * i], bi1 = b[3 * i + 1], bi2 = b[3 * i + 2];
+    double gi = g0 * bi0 + g1 * bi1 + g2 * bi2;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double bj0 = b[3 * j], bj1 = b[3 * j + 1], bj2 = b[3 * j + 2];
+      double val = w1 * (bi0 * bj0 + bi1 * bj1 + bi2 * bj2);
+      if (w2 != 0.0) val += w2 * gi * (g0 * bj0 + g1 * bj1 + g2 * bj2);
+      k[4 * i + j] = val;
+    }
+  }
+  double4 *o = reinterpret_cast<double4 *>(K + 16 * e);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    o[i] = make_double4(k[4 * i], k[4 * i + 1], k[4 * i + 2], k[4 * i + 3]);
+}
+
+__global__ __launch_bounds__(kGatherRows) void k_gather_csr(
+    int64_t nn, int64_t nb_pad, int max_seg, const int32_t *__restrict__ indptr,
+    const int32_t *__restrict__ iptr, const int32_t *__restrict__ ient,
+    const int32_t *__restrict__ e2c, const double *__restrict__ K,
+    double *__restrict__ data) {
+  extern __shared__ double seg[];
+  const int64_t r0 = xcd_block(blockIdx.x, nb_pad) * kGatherRows;
+  if (r0 >= nn) return;
+  const int64_t r1 = r0 + kGatherRows < nn ? r0 + kGatherRows : nn;
+  const int32_t base = indptr[r0];
+  const int32_t len = indptr[r1] - base;
+  if (len > max_seg) return;
+  for (int t = threadIdx.x; t < len; t += kGatherRows) seg[t] = 0.0;
+  __syncthreads();
+  const int64_t v = r0 + threadIdx.x;
+  if (v < r1) {
+    constexpr int U = 4;
+    const int32_t q1 = iptr[v + 1];
+    for (int32_t q = iptr[v]; q < q1; q += U) {
+      int4 sl[U];
+      double4 kv[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        if (q + k < q1) {
+          const int64_t ent = ient[q + k];   // 4 e + a: row a of element e
+          sl[k] = *reinterpret_cast<const int4 *>(e2c + 4 * ent);
+          kv[k] = *reinterpret_cast<const double4 *>(K + 4 * ent);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        if (q + k < q1) {
+          seg[sl[k].x - base] += kv[k].x;
+          seg[sl[k].y - base] += kv[k].y;
+          seg[sl[k].z - base] += kv[k].z;
+          seg[sl[k].w - base] += kv[k].w;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < len; t += kGatherRows) data[base + t] = seg[t];
+}
+
 // pyfp3d/kernels/jacobian.py:197-228 (fixed slot slice [16a, 16a+16))
 __global__ __launch_bounds__(kBlock) void k_term3_coo(
     int64_t n_active, const int32_t *__restrict__ active,
@@ -1049,6 +1201,81 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
                        B, V, rho_tilde, grad, s_e, lim, inv_u2, clamp, data);
   }
   return check_launch("assemble_jacobian_spd_colored");
+}
+
+static inline unsigned xcd_grid(int64_t n_blocks) {
+  return (unsigned)(((n_blocks + 7) / 8) * 8);
+}
+
+int fp3d_gather_rows_per_block(void) { return kGatherRows; }
+
+int fp3d_element_residual_terms(int64_t n_tets, const double *B,
+                                const double *V, const double *rho_tilde,
+                                const double *grad, double *terms,
+                                void *stream) {
+  FP3D_REQUIRE(n_tets >= 0, "element_residual_terms: n_tets < 0");
+  if (n_tets == 0) return FP3D_OK;
+  FP3D_REQUIRE(B && V && rho_tilde && grad && terms,
+               "element_residual_terms: NULL pointer");
+  FP3D_REQUIRE(((uintptr_t)terms & 31) == 0, "terms must be 32-byte aligned");
+  hipLaunchKernelGGL(k_elem_residual_terms, dim3(grid_for(n_tets)),
+                     dim3(kBlock), 0, FP3D_STREAM(stream), n_tets, B, V,
+                     rho_tilde, grad, terms);
+  return check_launch("element_residual_terms");
+}
+
+int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
+                           const int32_t *inc_ent, const double *terms,
+                           double *R, void *stream) {
+  FP3D_REQUIRE(n_nodes >= 0, "gather_node_terms: n_nodes < 0");
+  if (n_nodes == 0) return FP3D_OK;
+  FP3D_REQUIRE(inc_ptr && inc_ent && terms && R,
+               "gather_node_terms: NULL pointer");
+  int64_t nb = (n_nodes + kBlock - 1) / kBlock;
+  unsigned g = xcd_grid(nb);
+  hipLaunchKernelGGL(k_gather_terms, dim3(g), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n_nodes, (int64_t)g, inc_ptr,
+                     inc_ent, terms, R);
+  return check_launch("gather_node_terms");
+}
+
+int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
+                              const double *rho_tilde, const double *grad,
+                              const double *s_e, const uint8_t *lim,
+                              double inv_u2, double clamp, double *K,
+                              void *stream) {
+  FP3D_REQUIRE(n_tets >= 0, "element_matrix_terms: n_tets < 0");
+  if (n_tets == 0) return FP3D_OK;
+  FP3D_REQUIRE(B && V && rho_tilde && K, "element_matrix_terms: NULL pointer");
+  FP3D_REQUIRE(!s_e || grad, "element_matrix_terms: s_e needs grad");
+  FP3D_REQUIRE(((uintptr_t)K & 31) == 0, "K must be 32-byte aligned");
+  FP3D_REQUIRE(clamp == 0.0 || (clamp > 0.0 && clamp < 1.0 && s_e),
+               "element_matrix_terms: clamp must be 0 or in (0, 1) with s_e");
+  hipLaunchKernelGGL(k_elem_matrix_terms, dim3(grid_for(n_tets)), dim3(kBlock),
+                     0, FP3D_STREAM(stream), n_tets, B, V, rho_tilde,
+                     s_e ? grad : nullptr, s_e, lim, inv_u2, clamp, K);
+  return check_launch("element_matrix_terms");
+}
+
+int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
+                          const int32_t *inc_ptr, const int32_t *inc_ent,
+                          const int32_t *elem_to_csr, const double *K,
+                          double *data, void *stream) {
+  FP3D_REQUIRE(n_nodes >= 0, "gather_csr_terms: n_nodes < 0");
+  if (n_nodes == 0) return FP3D_OK;
+  FP3D_REQUIRE(indptr && inc_ptr && inc_ent && elem_to_csr && K && data,
+               "gather_csr_terms: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elem_to_csr), "elem_to_csr must be 16-byte aligned");
+  FP3D_REQUIRE(((uintptr_t)K & 31) == 0, "K must be 32-byte aligned");
+  FP3D_REQUIRE(max_seg > 0 && (int64_t)max_seg * 8 <= 65536,
+               "gather_csr_terms: max_seg must lie in (0, 8192]");
+  int64_t nb = (n_nodes + kGatherRows - 1) / kGatherRows;
+  unsigned g = xcd_grid(nb);
+  hipLaunchKernelGGL(k_gather_csr, dim3(g), dim3(kGatherRows),
+                     (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
+                     n_nodes, (int64_t)g, max_seg, indptr, inc_ptr, inc_ent,
+                     elem_to_csr, K, data);
+  return check_launch("gather_csr_terms");
 }
 
 int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,

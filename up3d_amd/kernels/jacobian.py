@@ -114,13 +114,14 @@ def fill_term3_coo(active, elements, B, V, grad, upstream, s_u, inv_u2, rows,
 # ------------------------------------------------------------ the operator
 class PicardOperator:
     """Per-mesh assembly workspace (pyfp3d/kernels/jacobian.py:231-375),
-    resident in HBM in colour-major element order (kernels/store.py).
+    resident in HBM (kernels/store.py).
 
-    Reference-compatible methods take/return numpy arrays in the original
-    element order (velocities, assemble_matrix, assemble_residual,
+    Reference-compatible methods take/return numpy arrays
+    (velocities, assemble_matrix, assemble_residual,
     assemble_newton_jacobian).  The *_dev methods are the device hot path
-    used by the solvers: storage-order device tensors in and out, nothing
-    allocated per call."""
+    used by the solvers: device tensors in and out, nothing allocated per
+    call; the residual and matrices are node-row gathers (one launch each,
+    ascending-element accumulation, no colour passes)."""
 
     def __init__(self, nodes: np.ndarray, elements: np.ndarray):
         self.store = st = MeshStore.get(nodes, elements)
@@ -136,11 +137,13 @@ class PicardOperator:
         self._pattern_indptr = st.indptr_host
         self._pattern_indices = st.indices_host
         n = self.n_tets
-        # preallocated hot-loop buffers (storage order)
+        # preallocated hot-loop buffers
         self.d_grad = empty((n, 3))
         self.d_q2 = empty(n)
         self.d_R = empty(self.n_nodes)
         self.d_ones = torch.ones(n, dtype=torch.float64, device=st.device)
+        self.d_terms = empty(4 * n)      # element residual terms
+        self.d_K = None                  # element matrices, on first use
 
     # --------------------------------------------------------- device path
     def velocities_dev(self, phi: torch.Tensor, grad=None, q2=None):
@@ -152,7 +155,7 @@ class PicardOperator:
         return grad, q2
 
     def state_dev(self, phi, u2, m_inf, gamma, q2cap, grad, q2n, q2l, rho, lim):
-        """Fused grad / q2 / limiter / density sweep (storage order)."""
+        """Fused grad / q2 / limiter / density sweep."""
         st = self.store
         _native.call("fp3d_element_state", self.n_tets, ptr(st.el), ptr(st.B),
                      ptr(phi), float(u2), float(m_inf), float(gamma),
@@ -161,42 +164,45 @@ class PicardOperator:
 
     def residual_dev(self, rho_tilde: torch.Tensor, grad: torch.Tensor,
                      out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """R_v = sum over incident (e, i) of rho_tilde_e V_e (grad_e . B_e,i):
+        element terms, then the node gather in ascending element order."""
         st = self.store
         R = self.d_R if out is None else out
-        _native.call("fp3d_assemble_residual_colored", self.n_colors,
-                     st.offsets_ct, None, ptr(st.el), ptr(st.B), ptr(st.V),
-                     ptr(rho_tilde), ptr(grad), self.n_nodes, ptr(R),
+        _native.call("fp3d_element_residual_terms", self.n_tets, ptr(st.B),
+                     ptr(st.V), ptr(rho_tilde), ptr(grad), ptr(self.d_terms),
                      stream_handle())
+        _native.call("fp3d_gather_node_terms", self.n_nodes, ptr(st.inc_ptr),
+                     ptr(st.inc_ent), ptr(self.d_terms), ptr(R), stream_handle())
         return R
 
-    def matrix_dev(self, rho_tilde: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+    def _gather_matrix(self, rho_tilde, grad, s_e, lim, inv_u2, clamp, out):
         st = self.store
-        _native.call("fp3d_fill", self.nnz, 0.0, ptr(out), stream_handle())
-        _native.call("fp3d_assemble_matrix_colored", self.n_colors,
-                     st.offsets_ct, None, ptr(st.e2c), ptr(st.B), ptr(st.V),
-                     ptr(rho_tilde), ptr(out), stream_handle())
+        if self.d_K is None:
+            self.d_K = empty(16 * self.n_tets)
+        _native.call("fp3d_element_matrix_terms", self.n_tets, ptr(st.B),
+                     ptr(st.V), ptr(rho_tilde),
+                     ptr(grad) if grad is not None else None,
+                     ptr(s_e) if s_e is not None else None,
+                     ptr(lim) if lim is not None else None, float(inv_u2),
+                     float(clamp), ptr(self.d_K), stream_handle())
+        _native.call("fp3d_gather_csr_terms", self.n_nodes, st.gather_max_seg,
+                     ptr(st.indptr), ptr(st.inc_ptr), ptr(st.inc_ent),
+                     ptr(st.e2c), ptr(self.d_K), ptr(out), stream_handle())
         return out
+
+    def matrix_dev(self, rho_tilde: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """Picard matrix values A = sum rho_tilde V B_i.B_j (every CSR value
+        written)."""
+        return self._gather_matrix(rho_tilde, None, None, None, 0.0, 0.0, out)
 
     def jacobian_dev(self, rho_tilde, grad, s_e, lim, inv_u2, out):
         """Terms 1+2 (s_e masked by lim when lim is not None)."""
-        st = self.store
-        _native.call("fp3d_fill", self.nnz, 0.0, ptr(out), stream_handle())
-        _native.call("fp3d_assemble_jacobian_colored", self.n_colors,
-                     st.offsets_ct, None, ptr(st.e2c), ptr(st.B), ptr(st.V),
-                     ptr(rho_tilde), ptr(grad), ptr(s_e), ptr(lim),
-                     float(inv_u2), ptr(out), stream_handle())
-        return out
+        return self._gather_matrix(rho_tilde, grad, s_e, lim, inv_u2, 0.0, out)
 
     def jacobian_spd_dev(self, rho_tilde, grad, s_e, lim, inv_u2, clamp, out):
         """SPD preconditioner matrix: Terms 1+2 with the Term-2 weight
         clamped at -clamp * rho_tilde V / q2."""
-        st = self.store
-        _native.call("fp3d_fill", self.nnz, 0.0, ptr(out), stream_handle())
-        _native.call("fp3d_assemble_jacobian_spd_colored", self.n_colors,
-                     st.offsets_ct, None, ptr(st.e2c), ptr(st.B), ptr(st.V),
-                     ptr(rho_tilde), ptr(grad), ptr(s_e), ptr(lim),
-                     float(inv_u2), float(clamp), ptr(out), stream_handle())
-        return out
+        return self._gather_matrix(rho_tilde, grad, s_e, lim, inv_u2, clamp, out)
 
     # ---------------------------------------------- reference-compatible API
     def _csr(self, data: np.ndarray) -> sp.csr_matrix:
@@ -229,7 +235,7 @@ class PicardOperator:
     def assemble_newton_jacobian(self, phi, rho_tilde, s_e, s_u, upstream,
                                  u_inf: float = 1.0, lim_mask=None) -> sp.csr_matrix:
         """Exact Newton Jacobian at frozen upstream selection
-        (pyfp3d/kernels/jacobian.py:288-359): Terms 1+2 coloured into the CSR
+        (pyfp3d/kernels/jacobian.py:288-359): Terms 1+2 gathered into the CSR
         on the device, Term 3 as device COO added with scipy (the
         reference's own merge).  Records newton_nnz and n_term3_active."""
         st = self.store
@@ -252,14 +258,14 @@ class PicardOperator:
         active = np.nonzero(s_u)[0].astype(np.int64)
         self.n_term3_active = int(len(active))
         if len(active):
-            # Term 3 on the device from the resident storage-order arrays
+            # Term 3 on the device from the resident arrays
             # (rows/cols are node ids, identical in both numberings)
             n3 = 16 * len(active)
             rows = empty(n3, torch.int32)
             cols = empty(n3, torch.int32)
             vals = empty(n3)
-            active_s = np.sort(st.inv_perm_host[active]).astype(np.int32)
-            fill_term3_coo(to_dev(active_s, torch.int32), st.el, st.B, st.V, g,
+            fill_term3_coo(to_dev(active.astype(np.int32), torch.int32),
+                           st.el, st.B, st.V, g,
                            st.upstream_to_storage(up_o), st.to_storage(s_u),
                            inv_u2, rows, cols, vals)
             coo = sp.coo_matrix((to_host(vals), (to_host(rows).astype(np.int64),

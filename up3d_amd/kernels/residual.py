@@ -1,10 +1,12 @@
 """Galerkin P1 residual assembly on the device
 (pyfp3d/kernels/residual.py:43-103).
 
-R_i = sum_e rho_tilde_e V_e (grad(phi)_e . grad(N_i)): colour classes are
-launched in order and elements within a class in parallel; a class shares
-no node, so each R_i update is a plain read-modify-write and the per-node
-accumulation order is the colour sequence -- bit-deterministic, no atomics.
+R_i = sum_e rho_tilde_e V_e (grad(phi)_e . grad(N_i)).  The drop-in
+assemble_residual_colored launches colour classes in order and elements
+within a class in parallel; a class shares no node, so each R_i update is a
+plain read-modify-write and the per-node accumulation order is the colour
+sequence -- bit-deterministic, no atomics.  The operators' hot path
+(PicardOperator.residual_dev) is the node gather in ascending element order.
 """
 
 from __future__ import annotations
@@ -40,8 +42,8 @@ def assemble_residual_colored(color_offsets, color_elems, elements, B, V,
 
 
 def assemble_stiffness_matrix(nodes, elements):
-    """SPD Laplace stiffness matrix (rho == 1) through the coloured device
-    path, as a scipy CSR (pyfp3d/kernels/residual.py:79-99)."""
+    """SPD Laplace stiffness matrix (rho == 1) through the device assembly,
+    as a scipy CSR (pyfp3d/kernels/residual.py:79-99)."""
     from .jacobian import PicardOperator
 
     return PicardOperator(nodes, elements).assemble_matrix()

@@ -1,28 +1,28 @@
-"""Per-mesh device store: all element data resident in HBM in COLOUR-MAJOR
-order.
+"""Per-mesh device store: all element data resident in HBM in the mesh's
+own element order.
 
 Set-up (host, once per mesh, as pyfp3d/kernels/jacobian.py:246-262 does):
 geometry B_e/V_e, greedy colouring + stable colour partition, canonical CSR
-pattern, elem_to_csr, face adjacency, centroids.  The elements are then
-renumbered so that colour class c is the contiguous range
-[color_offsets[c], color_offsets[c+1]) and, inside it, ascending original
-element id -- exactly the reference's colour_elems order.  Every element
-kernel therefore reads coalesced rows, the coloured scatters keep the
-reference's accumulation order (colour by colour, one element per node
-per colour), and element-indexed results convert to the original order
-only at the API boundary (`perm`: storage slot -> original id).
+pattern, elem_to_csr, face adjacency, centroids.  Element arrays stay in
+the caller's element order (a mesh whose elements are sorted by node ids
+after RCM, as the reference's meshes are, keeps face neighbours and the
+elements around a node close in memory), so element ids are the same in
+the API and on the device.  The node-gather assembly walks the
+node -> (4 e + i) incidence in ascending e; the coloured drop-in kernels
+visit colour class c through the index list color_elems.
 
-Device layout (storage order, k = storage slot):
+Device layout (element e, node v):
   el       int32 (n_tets, 4)     node ids
   B        f64   (n_tets, 12)    shape gradients, row-major (i, d)
   V        f64   (n_tets,)
-  e2c      int32 (n_tets, 16)    CSR slot of (elements[k,i], elements[k,j])
-  fn       int32 (n_tets, 4)     face neighbours as storage slots, -1 boundary
+  e2c      int32 (n_tets, 16)    CSR slot of (elements[e,i], elements[e,j])
+  fn       int32 (n_tets, 4)     face neighbours, -1 on the boundary
   cen      f64   (n_tets, 3)     centroids
-  dcen     f64   (n_tets, 12)    centroid(fn[k,f]) - centroid(k), per face
+  dcen     f64   (n_tets, 12)    centroid(fn[e,f]) - centroid(e), per face
+  celems   int32 (n_tets,)       colour-partitioned element ids
   nodes    f64   (n_nodes, 3)
   indptr   int32 (n_nodes+1,), indices int32 (nnz,)
-  inc_ptr  int32 (n_nodes+1,), inc_ent int32 (4 n_tets,)  node -> 4k+i
+  inc_ptr  int32 (n_nodes+1,), inc_ent int32 (4 n_tets,)  node -> 4e+i
 """
 
 from __future__ import annotations
@@ -71,9 +71,10 @@ def elem_to_csr(elements: np.ndarray, indptr: np.ndarray, indices: np.ndarray):
 
 
 class MeshStore:
-    """Device-resident mesh in colour-major order (module docstring)."""
+    """Device-resident mesh (module docstring)."""
 
     _cache: Dict[str, "MeshStore"] = {}
+    max_resident = 4      # meshes kept set up on the device (LRU)
 
     @classmethod
     def get(cls, nodes: np.ndarray, elements: np.ndarray) -> "MeshStore":
@@ -84,11 +85,12 @@ class MeshStore:
         h.update(nd.tobytes())
         h.update(el.tobytes())
         key = h.hexdigest()
-        st = cls._cache.get(key)
+        st = cls._cache.pop(key, None)
         if st is None:
             st = cls(nd, el)
-            cls._cache.clear()   # keep one mesh resident at a time
-            cls._cache[key] = st
+        cls._cache[key] = st          # most recently used last
+        while len(cls._cache) > cls.max_resident:
+            cls._cache.pop(next(iter(cls._cache)))
         return st
 
     def __init__(self, nodes: np.ndarray, elements: np.ndarray):
@@ -115,60 +117,59 @@ class MeshStore:
         self.face_neighbors_host = fn
         self.centroids_host = cen
         self.e2c_host = e2c
-        # ---- colour-major renumbering
-        perm = celems.astype(np.int64)                 # slot -> original
-        inv = np.empty_like(perm)
-        inv[perm] = np.arange(self.n_tets)
-        self.perm_host, self.inv_perm_host = perm, inv
-        el_s = el[perm]
-        fn_s = fn[perm]
-        fn_s = np.where(fn_s >= 0, inv[np.maximum(fn_s, 0)], -1).astype(np.int32)
-        # node -> (4 k + i) incidence in storage order (ascending k)
-        flat = el_s.reshape(-1).astype(np.int64)
+        # ---- node -> (4 e + i) incidence, ascending e
+        flat = el.reshape(-1).astype(np.int64)
         order = np.argsort(flat, kind="stable")
         inc_ptr = np.zeros(self.n_nodes + 1, dtype=np.int64)
         np.cumsum(np.bincount(flat, minlength=self.n_nodes), out=inc_ptr[1:])
+        # LDS bound of the node-gather matrix assembly: the longest CSR slice
+        # owned by one block of consecutive rows
+        rows = _native.lib().fp3d_gather_rows_per_block()
+        starts = np.arange(0, self.n_nodes, rows)
+        ends = np.minimum(starts + rows, self.n_nodes)
+        self.gather_max_seg = int((indptr[ends].astype(np.int64)
+                                   - indptr[starts]).max()) if self.n_nodes else 1
+        if self.gather_max_seg * 8 > 65536:
+            raise ValueError(f"node valence too high for the gather assembly "
+                             f"({self.gather_max_seg} CSR entries per "
+                             f"{rows}-row block)")
         # ---- upload
-        self.el = to_dev(el_s, torch.int32)
-        self.B = to_dev(B[perm].reshape(self.n_tets, 12), torch.float64)
-        self.V = to_dev(V[perm], torch.float64)
-        self.e2c = to_dev(e2c[perm].reshape(self.n_tets, 16), torch.int32)
-        self.fn = to_dev(fn_s, torch.int32)
-        self.cen = to_dev(cen[perm], torch.float64)
-        # per-face centroid displacement, storage order (0 on the boundary)
-        cs = cen[perm]
-        dcen = np.where((fn_s >= 0)[:, :, None],
-                        cs[np.maximum(fn_s, 0)] - cs[:, None, :], 0.0)
+        self.el = to_dev(el, torch.int32)
+        self.B = to_dev(B.reshape(self.n_tets, 12), torch.float64)
+        self.V = to_dev(V, torch.float64)
+        self.e2c = to_dev(e2c.reshape(self.n_tets, 16), torch.int32)
+        self.fn = to_dev(fn.astype(np.int32), torch.int32)
+        self.cen = to_dev(cen, torch.float64)
+        # per-face centroid displacement (0 on the boundary)
+        dcen = np.where((fn >= 0)[:, :, None],
+                        cen[np.maximum(fn, 0)] - cen[:, None, :], 0.0)
         self.dcen = to_dev(np.ascontiguousarray(dcen).reshape(self.n_tets, 12),
                            torch.float64)
+        self.celems = to_dev(celems.astype(np.int32), torch.int32)
         self.nodes = to_dev(nodes, torch.float64)
         self.indptr = to_dev(indptr, torch.int32)
         self.indices = to_dev(indices, torch.int32)
         self.inc_ptr = to_dev(inc_ptr.astype(np.int32), torch.int32)
         self.inc_ent = to_dev(order.astype(np.int32), torch.int32)
-        self.perm = to_dev(perm, torch.int64)
-        self.inv_perm = to_dev(inv, torch.int64)
         self.offsets_ct = self.color_offsets.ctypes.data_as(
             C.POINTER(C.c_int64))
 
-    # element arrays: original order <-> storage order
+    # element arrays: API order == device order
     def to_storage(self, a, dtype=torch.float64) -> torch.Tensor:
-        t = to_dev(a, dtype) if not (isinstance(a, torch.Tensor) and a.is_cuda) \
-            else a.to(dtype)
-        return t.index_select(0, self.perm).contiguous()
+        if isinstance(a, torch.Tensor) and a.is_cuda:
+            return a.to(dtype).contiguous()
+        return to_dev(a, dtype)
 
     def to_original(self, t: torch.Tensor) -> torch.Tensor:
-        return t.index_select(0, self.inv_perm).contiguous()
+        return t
 
     def upstream_to_original(self, up_s: torch.Tensor) -> torch.Tensor:
-        """storage-slot upstream ids -> original element ids, original order"""
-        return self.perm.index_select(0, up_s.long()).index_select(0, self.inv_perm)
+        return up_s.long()
 
     def upstream_to_storage(self, up_o) -> torch.Tensor:
         t = up_o if isinstance(up_o, torch.Tensor) and up_o.is_cuda \
             else to_dev(np.asarray(up_o), torch.int64)
-        return self.inv_perm.index_select(0, t.long().index_select(0, self.perm)
-                                          ).to(torch.int32).contiguous()
+        return t.to(torch.int32).contiguous()
 
     @property
     def device(self):

@@ -112,7 +112,7 @@ def rho_tilde_sensitivities_sweep(q2, rho, upstream, m_inf, m_crit, C, gamma,
 class UpwindOperator:
     """Per-mesh upwinding workspace (pyfp3d/kernels/upwind.py:733-920), walk
     mode.  Reference-compatible methods take numpy in the original element
-    order; *_dev methods work on storage-order device tensors.
+    order; *_dev methods work on device tensors.
 
     weighted=True (the reference's blend / streamline-kernel fluxes) is not
     part of the Newton path (it runs weighted=False,

@@ -6,12 +6,13 @@ driver needs a WakeCut and cannot run a non-lifting case as shipped, so
 this is the same chain with n_st = 0, with identical keyword names,
 defaults, control logic and result keys.
 
-Per Newton step (all device-resident, storage element order):
+Per Newton step (all device-resident):
   eval_residual: fused grad/q2/limiter/density sweep -> walk + nu switch +
-      rho_tilde (live) or frozen sweep -> coloured residual scatter -> minus
+      rho_tilde (live) or frozen sweep -> node-gather residual -> minus
       the external (boundary-flux) RHS -> ||R_free||
-  assemble: sensitivities (s_e, s_u) -> Terms 1+2 coloured into the fixed
-      CSR -> Dirichlet embedding (+ float32 copy) -> Term 3 gather table
+  assemble: sensitivities (s_e, s_u) -> Terms 1+2 node-gathered into the
+      fixed CSR -> Dirichlet embedding (+ float32 copy) -> Term 3 gather
+      table
   precond: SA-AMG of the Picard matrix A(rho_tilde) every amg_rebuild_every
       steps (host set-up, device V-cycle)
   linsolve: device GMRES(restart) with Eisenstat-Walker forcing
