@@ -313,6 +313,15 @@ int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
                    const double *w, double *work, double *out, void *stream);
 int fp3d_multi_dot_work_doubles(int nv);
 /* w += sign * sum_k h[k] v_k  (h on device, nv <= 256) */
+/* Re-orthogonalisation pass of CGS2 (no reference counterpart; the
+ * reference's GMRES is scipy's): out[k] = V_k . w only when the first pass
+ * cancelled, judged on the device by the DGKS test
+ * ||w||^2 - sum h_k^2 < eta2 ||w||^2 with h = cond[0..nv), ||w||^2 =
+ * cond[nv] (the first pass's outputs); otherwise out[0..nv) = 0 exactly and
+ * fp3d_multi_axpy with those coefficients returns without touching w. */
+int fp3d_multi_dot_reorth(int64_t n, int nv, const double *V, int64_t ld,
+                          const double *w, const double *cond, double eta2,
+                          double *work, double *out, void *stream);
 int fp3d_multi_axpy(int64_t n, int nv, const double *V, int64_t ld,
                     const double *h, double sign, double *w, void *stream);
 /* out2 = {||x||_2, ||x||_inf} on device */

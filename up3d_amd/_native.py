@@ -90,6 +90,7 @@ _SIGS = {
     "fp3d_element_matrix_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl,
                                          _vp, _vp]),
     "fp3d_gather_csr_terms": (_int, [_i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_multi_dot_reorth": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _dbl, _vp, _vp, _vp]),
     "fp3d_lumped_volume": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_diag_shift": (_int, [_i64, _vp, _vp, _dbl, _vp, _vp, _vp]),
     "fp3d_fill": (_int, [_i64, _dbl, _vp, _vp]),

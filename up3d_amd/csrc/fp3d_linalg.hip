@@ -91,15 +91,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t3(
   int32_t j0 = ptr[row], j1 = ptr[row + 1];
   double s = 0.0;
   for (int32_t j = j0 + lane; j < j1; j += G) s += (double)val[j] * x[idx[j]];
+  // the row's Term-3 entries are strided over the same G lanes (fixed lane
+  // assignment and reduction tree: deterministic)
+  if (t3ptr) {
+    const int32_t q1 = t3ptr[row + 1];
+    for (int32_t q = t3ptr[row] + lane; q < q1; q += G) {
+      int32_t ent = t3ent[q];
+      s += W[ent] * tval[ent >> 2];
+    }
+  }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
   if (lane == 0) {
-    if (t3ptr) {
-      for (int32_t q = t3ptr[row]; q < t3ptr[row + 1]; ++q) {
-        int32_t ent = t3ent[q];
-        s += W[ent] * tval[ent >> 2];
-      }
-    }
     if (fmask && !fmask[row]) s = x[row];
     y[row] = s;
   }
@@ -233,22 +236,57 @@ inline int rblocks(int64_t n) {
   return (int)b;
 }
 
-// partial[k*nb + b] = sum over block b's rows of V_k . w
+// partial[k*nb + b] = sum over block b's rows of V_k . w.  Vectors are
+// taken kMdotChunk at a time so each thread keeps that many independent
+// loads in flight; per chunk: wave shuffles, one LDS exchange, fixed order.
+// cond != NULL makes the call a CGS2 re-orthogonalisation pass that runs
+// only when the first pass cancelled (DGKS: ||w'||^2 = ||w||^2 - ||h||^2 <
+// eta2 ||w||^2 with h = cond[0..nv), ||w||^2 = cond[nv]); otherwise the
+// partials are written as exact zeros.
+constexpr int kMdotChunk = 8;
 __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
                                                    const double *__restrict__ V,
                                                    int64_t ld,
                                                    const double *__restrict__ w,
-                                                   double *__restrict__ part) {
-  __shared__ double sh[kBlock / 64];
-  int nb = gridDim.x;
-  for (int k = 0; k < nv; ++k) {
-    const double *v = V + (int64_t)k * ld;
-    double s = 0.0;
+                                                   double *__restrict__ part,
+                                                   const double *__restrict__ cond,
+                                                   double eta2) {
+  __shared__ double sh[kMdotChunk][kBlock / 64];
+  const int nb = gridDim.x;
+  if (cond) {
+    double h2 = 0.0;
+    for (int k = 0; k < nv; ++k) h2 += cond[k] * cond[k];
+    double w2 = cond[nv];
+    if (!(w2 - h2 < eta2 * w2)) {
+      for (int k = threadIdx.x; k < nv; k += blockDim.x)
+        part[(int64_t)k * nb + blockIdx.x] = 0.0;
+      return;
+    }
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k0 = 0; k0 < nv; k0 += kMdotChunk) {
+    double acc[kMdotChunk];
+#pragma unroll
+    for (int c = 0; c < kMdotChunk; ++c) acc[c] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)nb * blockDim.x)
-      s += v[i] * w[i];
-    double t = block_sum(s, sh);
-    if (threadIdx.x == 0) part[(int64_t)k * nb + blockIdx.x] = t;
+         i += (int64_t)nb * blockDim.x) {
+      const double wi = w[i];
+#pragma unroll
+      for (int c = 0; c < kMdotChunk; ++c)
+        if (k0 + c < nv) acc[c] += V[(int64_t)(k0 + c) * ld + i] * wi;
+    }
+#pragma unroll
+    for (int c = 0; c < kMdotChunk; ++c) {
+      double v = wsum(acc[c]);
+      if (lane == 0) sh[c][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMdotChunk && k0 + (int)threadIdx.x < nv) {
+      double t = 0.0;
+      for (int q = 0; q < kBlock / 64; ++q) t += sh[threadIdx.x][q];
+      part[(int64_t)(k0 + threadIdx.x) * nb + blockIdx.x] = t;
+    }
+    __syncthreads();
   }
 }
 // one block per k, sums the nb partials in index order (wave-parallel then
@@ -315,8 +353,16 @@ __global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
                                                  double *__restrict__ w,
                                                  double sign) {
   __shared__ double hs[256];
-  for (int k = threadIdx.x; k < nv; k += blockDim.x) hs[k] = h[k];
+  __shared__ int any_nz;
+  if (threadIdx.x == 0) any_nz = 0;
   __syncthreads();
+  for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+    hs[k] = h[k];
+    if (hs[k] != 0.0) any_nz = 1;
+  }
+  __syncthreads();
+  // all-zero coefficients (a skipped re-orthogonalisation): w unchanged
+  if (!any_nz) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
@@ -356,7 +402,7 @@ __global__ void k_add(int64_t n, const double *__restrict__ d,
 // ----------------------------------------------------------- Term 3 (device)
 // Workspace layout (byte offsets computed by t3_layout).
 struct T3Layout {
-  size_t flag, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, total;
+  size_t flag, flag8, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, total;
 };
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 inline T3Layout t3_layout(int64_t nt, int64_t nn) {
@@ -370,6 +416,7 @@ inline T3Layout t3_layout(int64_t nt, int64_t nn) {
   int64_t nmax = nt > nn ? nt : nn;
   int64_t nbs = (nmax + 1 + kBlock - 1) / kBlock + 1;
   L.flag = take(sizeof(int32_t) * (nt + 1));
+  L.flag8 = take(nt + 1);
   L.scan = take(sizeof(int32_t) * (nmax + 2));
   L.bsum = take(sizeof(int32_t) * (nbs + 1));
   L.act = take(sizeof(int32_t) * (nt + 1));
@@ -389,8 +436,8 @@ __global__ __launch_bounds__(kBlock) void k_t3_coef(
     const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ grad, const int32_t *__restrict__ up,
     const double *__restrict__ s_u, const uint8_t *__restrict__ lim,
-    double inv_u2, int32_t *__restrict__ flag, double *__restrict__ W,
-    double *__restrict__ Cc) {
+    double inv_u2, int32_t *__restrict__ flag, uint8_t *__restrict__ flag8,
+    double *__restrict__ W, double *__restrict__ Cc) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
   int64_t u = up[e];
@@ -398,6 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_t3_coef(
   if (lim) s = s * (lim[u] ? 1.0 : 0.0);
   bool act = s != 0.0;
   flag[e] = act ? 1 : 0;
+  flag8[e] = act ? 1 : 0;
   if (!act) return;
   double ge0 = grad[3 * e], ge1 = grad[3 * e + 1], ge2 = grad[3 * e + 2];
   double gu0 = grad[3 * u], gu1 = grad[3 * u + 1], gu2 = grad[3 * u + 2];
@@ -479,22 +527,34 @@ __global__ void k_t3_compact(int64_t nt, const int32_t *__restrict__ flag,
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < nt && flag[e]) act[pos[e]] = (int32_t)e;
 }
-// per node: number of active incident (element, local) entries
+// per node: number of active incident (element, local) entries.  The
+// byte flags of kT3Unroll consecutive entries are fetched together (the
+// node loops are bound by these independent gathers, not by arithmetic).
+constexpr int kT3Unroll = 8;
 __global__ void k_t3_count(int64_t nn, const int32_t *__restrict__ iptr,
                            const int32_t *__restrict__ ient,
-                           const int32_t *__restrict__ flag,
+                           const uint8_t *__restrict__ flag8,
                            const uint8_t *__restrict__ fmask,
                            int32_t *__restrict__ cnt) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   int32_t c = 0;
-  if (!fmask || fmask[v])
-    for (int32_t q = iptr[v]; q < iptr[v + 1]; ++q) c += flag[ient[q] >> 2];
+  if (!fmask || fmask[v]) {
+    const int32_t q1 = iptr[v + 1];
+    for (int32_t q = iptr[v]; q < q1; q += kT3Unroll) {
+      uint8_t f[kT3Unroll];
+#pragma unroll
+      for (int k = 0; k < kT3Unroll; ++k)
+        f[k] = (q + k < q1) ? flag8[ient[q + k] >> 2] : (uint8_t)0;
+#pragma unroll
+      for (int k = 0; k < kT3Unroll; ++k) c += f[k];
+    }
+  }
   cnt[v] = c;
 }
 __global__ void k_t3_fill(int64_t nn, const int32_t *__restrict__ iptr,
                           const int32_t *__restrict__ ient,
-                          const int32_t *__restrict__ flag,
+                          const uint8_t *__restrict__ flag8,
                           const uint8_t *__restrict__ fmask,
                           const int32_t *__restrict__ rowptr,
                           int32_t *__restrict__ ent) {
@@ -502,9 +562,18 @@ __global__ void k_t3_fill(int64_t nn, const int32_t *__restrict__ iptr,
   if (v >= nn) return;
   if (fmask && !fmask[v]) return;
   int32_t o = rowptr[v];
-  for (int32_t q = iptr[v]; q < iptr[v + 1]; ++q) {
-    int32_t en = ient[q];
-    if (flag[en >> 2]) ent[o++] = en;
+  const int32_t q1 = iptr[v + 1];
+  for (int32_t q = iptr[v]; q < q1; q += kT3Unroll) {
+    int32_t en[kT3Unroll];
+    uint8_t f[kT3Unroll];
+#pragma unroll
+    for (int k = 0; k < kT3Unroll; ++k) {
+      en[k] = (q + k < q1) ? ient[q + k] : 0;
+      f[k] = (q + k < q1) ? flag8[en[k] >> 2] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int k = 0; k < kT3Unroll; ++k)
+      if (f[k]) ent[o++] = en[k];
   }
 }
 // t_e = sum_k (grad_u . B_u,k) x[elements[u,k]] for the active elements
@@ -572,6 +641,7 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
   T3Layout L = t3_layout(n_tets, n_nodes);
   char *base = (char *)t3;
   int32_t *flag = (int32_t *)(base + L.flag);
+  uint8_t *flag8 = (uint8_t *)(base + L.flag8);
   int32_t *scan = (int32_t *)(base + L.scan);
   int32_t *bsum = (int32_t *)(base + L.bsum);
   int32_t *act = (int32_t *)(base + L.act);
@@ -583,17 +653,17 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
   int32_t *hdr = (int32_t *)(base + L.total - 256);
   hipLaunchKernelGGL(k_t3_coef, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
                      n_tets, elements, B, V, grad, upstream, s_u, lim, inv_u2,
-                     flag, W, Cc);
+                     flag, flag8, W, Cc);
   int rc = exclusive_scan(n_tets, flag, scan, bsum, hdr, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_t3_compact, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
                      n_tets, flag, scan, act);
   hipLaunchKernelGGL(k_t3_count, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s,
-                     n_nodes, inc_ptr, inc_ent, flag, free_mask, cnt);
+                     n_nodes, inc_ptr, inc_ent, flag8, free_mask, cnt);
   rc = exclusive_scan(n_nodes, cnt, rowptr, bsum, rowptr + n_nodes, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_t3_fill, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s,
-                     n_nodes, inc_ptr, inc_ent, flag, free_mask, rowptr, ent);
+                     n_nodes, inc_ptr, inc_ent, flag8, free_mask, rowptr, ent);
   rc = check_launch("term3_build");
   if (rc) return rc;
   int32_t na = 0;
@@ -733,9 +803,23 @@ int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
   hipStream_t s = FP3D_STREAM(stream);
   int nb = rblocks(n);
   hipLaunchKernelGGL(k_mdot_p1, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld, w,
-                     work);
+                     work, (const double *)nullptr, 0.0);
   hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
   return check_launch("multi_dot");
+}
+
+int fp3d_multi_dot_reorth(int64_t n, int nv, const double *V, int64_t ld,
+                          const double *w, const double *cond, double eta2,
+                          double *work, double *out, void *stream) {
+  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && cond && work && out && ld >= n,
+               "multi_dot_reorth: bad arguments");
+  FP3D_REQUIRE(eta2 > 0.0 && eta2 <= 1.0, "multi_dot_reorth: eta2 in (0, 1]");
+  hipStream_t s = FP3D_STREAM(stream);
+  int nb = rblocks(n);
+  hipLaunchKernelGGL(k_mdot_p1, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld, w,
+                     work, cond, eta2);
+  hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
+  return check_launch("multi_dot_reorth");
 }
 
 int fp3d_multi_dot_work_doubles(int nv) { return nv * kReduceBlocks; }

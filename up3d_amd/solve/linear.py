@@ -5,9 +5,10 @@
   restart / inner-tolerance logic as the SciPy routine the reference calls
   (scipy.sparse.linalg.gmres, pyfp3d/solve/linear.py:134-201): convergence
   is declared on the TRUE residual ||b - A x|| <= max(atol, rtol ||b||).
-  The Arnoldi step uses classical Gram-Schmidt with one re-orthogonalisation
-  (CGS2) so each orthogonalisation is two fused multi-dot / multi-axpy
-  passes instead of k dependent dot products; dots and norms are
+  The Arnoldi step uses classical Gram-Schmidt with a re-orthogonalisation
+  pass where the first one cancelled (CGS2 with the DGKS test, decided on
+  the device), so each orthogonalisation is one or two fused multi-dot /
+  multi-axpy passes instead of k dependent dot products; dots and norms are
   deterministic device reductions in float64, Givens rotations run on the
   host in float64 (one device->host copy per Krylov step).
 * pcg_dev: preconditioned CG (scipy.sparse.linalg.cg semantics) for the SPD
@@ -142,7 +143,12 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
             red.dots(V, k + 1, w, 2)
             _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
                          ptr(red.out[2:]), -1.0, ptr(w), sh)
-            red.dots(V, k, w, 3 + k)
+            # second CGS pass only where the first one cancelled (device-side
+            # DGKS test on out[2:2+k] = h and out[2+k] = ||w||^2; skipped
+            # passes leave exact zeros and an untouched w)
+            _native.call("fp3d_multi_dot_reorth", n, k, ptr(V), V.stride(0),
+                         ptr(w), ptr(red.out[2:]), 0.5, ptr(red.work),
+                         ptr(red.out[3 + k:]), sh)
             _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
                          ptr(red.out[3 + k:]), -1.0, ptr(w), sh)
             red.norm(w, 0)
