@@ -10,9 +10,10 @@ sensitivities, coloured Newton Jacobian (Terms 1+2) into the fixed CSR and
 the Term-3 gather table.  `value` = n_tets / step time (Mtet/s).
 
 The `solve` object holds the Newton solve seconds to ||R_free||_inf < 1e-10
-at M_inf = 0.84 in both precisions (grid-sequenced from the 96x32x32
-channel), the FP32-vs-FP64 differences of phi, wall Cp and x-force, and the
-per-mesh Mach/upwind_c level histories.
+at M_inf = 0.84 (grid-sequenced 48x16x16 -> 96x32x32 -> 192x64x64): FP64
+cold (including the per-mesh host set-up), then FP32-preconditioned and
+FP64 warm, the FP32-vs-FP64 differences of phi, wall Cp and x-force, and
+the per-mesh Mach/upwind_c level histories.
 
 Usage: python bench.py [--steps K] [--warmup W] [--no-solve] [--no-cpu]
 Prints ONE JSON line on stdout; progress goes to stderr.
@@ -290,17 +291,24 @@ def cpu_baseline(st, mesh, passes):
 def run_solves(mesh, config):
     """Newton solve of the M_inf = 0.84 channel to ||R_free||_inf < 1e-10 in
     both precisions.  For the 192x64x64 workload the solve is grid-sequenced
-    from the 96x32x32 channel (Mach ramp at upwind_c 3 there, then the fine
-    mesh at M 0.84 with upwind_c 3 -> 2 -> 1.5); seconds run from the start
-    of the coarse solve to the fine-mesh acceptance and include every AMG
-    set-up and the mesh-to-mesh interpolation, not mesh generation."""
+    48x16x16 -> 96x32x32 -> 192x64x64 (jittered, RCM): Mach ramp at
+    upwind_c 3 on the coarsest mesh, upwind_c 3 -> 2 -> 1.5 on the middle
+    one, and the fine mesh solved at M 0.84, upwind_c 1.5 from the
+    interpolated solution.  Seconds run from the start of the coarsest solve
+    to the fine-mesh acceptance and include every AMG set-up and transfer;
+    the first (cold) call also includes the per-mesh host set-up
+    (colouring, CSR, geometry, upload), later (warm) calls reuse it."""
     import torch
 
-    from up3d_amd.mesh.channel import boundary_nodes, config_mesh
+    from up3d_amd.mesh.channel import boundary_nodes, bump_channel, config_mesh
     from up3d_amd.post.surface import wall_force_coefficients
     from up3d_amd.solve.sequencing import solve_newton_sequenced
 
-    chain = [config_mesh("c3"), mesh] if config == "c2" else [mesh]
+    if config == "c2":
+        chain = [bump_channel(48, 16, 16, jitter=0.15, seed=20260103, rcm=True),
+                 config_mesh("c3"), mesh]
+    else:
+        chain = [mesh]
     defs = []
     for m in chain:
         dn = boundary_nodes(m, ["inflow", "outflow"])
@@ -308,37 +316,45 @@ def run_solves(mesh, config):
                      "dirichlet_nodes": dn, "dirichlet_values": m.nodes[dn, 0].copy()})
     floor = mesh.boundary_faces["floor"]
     res = {"method": ("grid-sequenced " + " -> ".join(f"{len(m.elements)} tets" for m in chain)
-                      + "; Newton + GMRES(200) + SA-AMG of the clamped Jacobian; "
-                      "upwind_c 3 ramp, post 2.0, 1.5")}
+                      + "; Newton + GMRES(200) + SA-AMG of the clamped Jacobian, adaptive "
+                      "AMG rebuild; upwind_c 3 Mach ramp, 3 -> 2 -> 1.5 on the middle mesh, "
+                      "fine mesh at 1.5")}
     phis, cps = {}, {}
-    for prec in ("fp32", "fp64"):
+
+    def one(prec):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r = solve_newton_sequenced(
             defs, M_INF, m_start=0.6, dm=0.05, upwind_c_ramp=3.0,
-            upwind_c_post=(2.0, 1.5),
+            upwind_c_post=(2.0, 1.5), post_stages_on="coarser",
             newton_kw={"precision": prec, "gmres_restart": 200, "gmres_maxiter": 5,
-                       "n_newton_max": 30, "amg_rebuild_every": 2,
+                       "n_newton_max": 30, "amg_rebuild_every": 0,
                        "precond_matrix": "jacobian_spd"})
         torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        wf = wall_force_coefficients(mesh.nodes, mesh.elements, floor, r["phi"],
-                                     m_inf=M_INF)
-        cf = wf["cf"]
-        cps[prec] = wf["cp_tri"]
-        res[prec] = {"seconds": round(wall, 2), "converged": bool(r["converged"]),
-                     "residual_inf": float(r["residual_history"][-1]),
-                     "residual_inf_live": r.get("residual_unfrozen"),
-                     "n_gmres_total": int(sum(mr["levels"][i][5] for mr in r["mesh_results"]
-                                              for i in range(len(mr["levels"])))),
-                     "meshes": [{"n_tets": mr["n_tets"], "wall_s": round(mr["wall_s"], 2),
-                                 "levels": [(round(a, 4), c, n, float("%.3e" % rr), ok)
-                                            for a, c, n, rr, ok, _ in mr["levels"]]}
-                                for mr in r["mesh_results"]],
-                     "timings_total": {k: round(v, 2) for k, v in r["timings_total"].items()},
-                     "wall_cf": [float(x) for x in cf]}
-        log(f"[bench] solve {prec}: {json.dumps(res[prec])}")
-        phis[prec] = r["phi"]
+        return r, time.perf_counter() - t0
+
+    runs = [("fp64", "cold"), ("fp32", "warm"), ("fp64", "warm")]
+    for prec, kind in runs:
+        r, wall = one(prec)
+        key = prec if kind == "warm" else prec + "_cold"
+        entry = {"seconds": round(wall, 2), "converged": bool(r["converged"]),
+                 "residual_inf": float(r["residual_history"][-1]),
+                 "residual_inf_live": r.get("residual_unfrozen"),
+                 "n_gmres_total": int(sum(lv[5] for mr in r["mesh_results"]
+                                          for lv in mr["levels"])),
+                 "meshes": [{"n_tets": mr["n_tets"], "wall_s": round(mr["wall_s"], 2),
+                             "levels": [(round(a, 4), c, n, float("%.3e" % rr), ok, it)
+                                        for a, c, n, rr, ok, it in mr["levels"]]}
+                            for mr in r["mesh_results"]],
+                 "timings_total": {k: round(v, 2) for k, v in r["timings_total"].items()}}
+        if kind == "warm":
+            wf = wall_force_coefficients(mesh.nodes, mesh.elements, floor, r["phi"],
+                                         m_inf=M_INF)
+            entry["wall_cf"] = [float(x) for x in wf["cf"]]
+            cps[prec] = wf["cp_tri"]
+            phis[prec] = r["phi"]
+        res[key] = entry
+        log(f"[bench] solve {prec} ({kind}): {json.dumps(entry)}")
     res["fp32_vs_fp64_phi_maxabs"] = float(np.abs(phis["fp32"] - phis["fp64"]).max())
     res["fp32_vs_fp64_cp_maxabs"] = float(np.abs(cps["fp32"] - cps["fp64"]).max())
     res["fp32_vs_fp64_cf_x_abs"] = abs(res["fp32"]["wall_cf"][0] - res["fp64"]["wall_cf"][0])

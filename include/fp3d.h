@@ -411,6 +411,16 @@ int fp3d_host_standard_aggregation(int64_t n, const int32_t *indptr,
                                    const int32_t *indices, const double *data,
                                    int32_t *agg, int64_t *n_agg);
 
+/* Point location for mesh-to-mesh P1 transfer (no reference counterpart;
+ * grid sequencing): for each point, the containing tet (visibility walk over
+ * face_neighbors, face f opposite local node f) and its 4 barycentric
+ * coordinates; a point outside the mesh gets the boundary tet where the
+ * walk stopped (coordinates then extrapolate). */
+int fp3d_host_locate_points(int64_t n_tets, const int32_t *elements,
+                            const double *nodes, const int32_t *face_neighbors,
+                            int64_t n_pts, const double *pts,
+                            int32_t *out_elem, double *out_bary);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,3 +138,26 @@ def test_mach_schedule_and_ew():
     assert _ew_forcing(1.0, None, None) == 1e-2
     assert _ew_forcing(1e-3, 1e-1, 1e-2) == pytest.approx(0.9e-4)
     assert _ew_forcing(1e-12, 1.0, 1e-2) == 1e-10
+
+
+def test_p1_transfer_exact_for_linear_fields():
+    """Mesh-to-mesh P1 transfer (grid sequencing): linear fields are
+    reproduced to rounding on a finer jittered mesh of the same domain,
+    coarse-node values are returned exactly at coinciding points, and the
+    located barycentric coordinates sum to one."""
+    from up3d_amd.mesh.channel import bump_channel
+    from up3d_amd.mesh.transfer import interpolate_p1, locate_points
+
+    coarse = bump_channel(12, 4, 4, jitter=0.15, seed=1, rcm=True)
+    fine = bump_channel(24, 8, 8, jitter=0.15, seed=2, rcm=True)
+    f = lambda p: 1.3 * p[:, 0] - 0.7 * p[:, 1] + 0.2 * p[:, 2] + 0.05  # noqa: E731
+    v = interpolate_p1(coarse.nodes, coarse.elements, f(coarse.nodes), fine.nodes)
+    assert np.abs(v - f(fine.nodes)).max() < 1e-13
+    g = np.sin(coarse.nodes[:, 0]) * np.cos(2 * coarse.nodes[:, 1])
+    assert np.abs(interpolate_p1(coarse.nodes, coarse.elements, g, coarse.nodes)
+                  - g).max() < 1e-13
+    elem, bary = locate_points(coarse.nodes, coarse.elements, fine.nodes)
+    assert np.abs(bary.sum(axis=1) - 1.0).max() < 1e-13
+    # interior fine nodes lie inside their located coarse tet
+    inside = bary.min(axis=1) >= -1e-10
+    assert inside.mean() > 0.95

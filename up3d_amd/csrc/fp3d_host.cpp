@@ -291,4 +291,92 @@ int fp3d_host_standard_aggregation(int64_t n, const int32_t *indptr,
   return FP3D_OK;
 }
 
+
+// Point location for P1 transfer between meshes (no reference counterpart):
+// a visibility walk through the face graph, starting from the previous
+// point's element (queries in mesh order are spatially coherent), moving
+// across the face opposite the most negative barycentric coordinate.  A walk
+// that reaches the boundary while still outside keeps the last element (a
+// point just outside the polyhedral hull extrapolates linearly from it).
+int fp3d_host_locate_points(int64_t n_tets, const int32_t *elements,
+                            const double *nodes, const int32_t *face_neighbors,
+                            int64_t n_pts, const double *pts,
+                            int32_t *out_elem, double *out_bary) {
+  HREQ(n_tets > 0 && elements && nodes && face_neighbors,
+       "locate_points: mesh missing");
+  HREQ(n_pts >= 0 && (n_pts == 0 || (pts && out_elem && out_bary)),
+       "locate_points: points missing");
+  auto bary = [&](int64_t e, const double *p, double *l) {
+    const double *a = nodes + 3 * (int64_t)elements[4 * e];
+    double m[3][3];
+    for (int j = 0; j < 3; ++j) {
+      const double *b = nodes + 3 * (int64_t)elements[4 * e + 1 + j];
+      for (int d = 0; d < 3; ++d) m[d][j] = b[d] - a[d];
+    }
+    double r[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                 m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                 m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    double x[3];
+    for (int j = 0; j < 3; ++j) {
+      double c[3][3];
+      for (int d = 0; d < 3; ++d)
+        for (int k = 0; k < 3; ++k) c[d][k] = (k == j) ? r[d] : m[d][k];
+      x[j] = (c[0][0] * (c[1][1] * c[2][2] - c[1][2] * c[2][1]) -
+              c[0][1] * (c[1][0] * c[2][2] - c[1][2] * c[2][0]) +
+              c[0][2] * (c[1][0] * c[2][1] - c[1][1] * c[2][0])) / det;
+    }
+    l[1] = x[0];
+    l[2] = x[1];
+    l[3] = x[2];
+    l[0] = 1.0 - x[0] - x[1] - x[2];
+  };
+  int64_t cur = 0;
+  const int64_t max_steps = 4096;
+  for (int64_t q = 0; q < n_pts; ++q) {
+    const double *p = pts + 3 * q;
+    double l[4];
+    bool lost = false;
+    for (int64_t step = 0;; ++step) {
+      bary(cur, p, l);
+      int worst = 0;
+      for (int i = 1; i < 4; ++i)
+        if (l[i] < l[worst]) worst = i;
+      if (l[worst] >= -1e-12) break;
+      if (step >= max_steps) {
+        lost = true;   // a walk that cycles: fall back to a full scan
+        break;
+      }
+      // face f of the reference's adjacency is opposite local node f
+      int32_t nb = face_neighbors[4 * cur + worst];
+      if (nb < 0) {
+        // at the boundary: try the other negative faces before stopping
+        int alt = -1;
+        for (int i = 0; i < 4; ++i)
+          if (i != worst && l[i] < -1e-12 && face_neighbors[4 * cur + i] >= 0 &&
+              (alt < 0 || l[i] < l[alt]))
+            alt = i;
+        if (alt < 0) break;
+        nb = face_neighbors[4 * cur + alt];
+      }
+      cur = nb;
+    }
+    if (lost) {
+      double best = -1e300, lb[4];
+      for (int64_t e = 0; e < n_tets; ++e) {
+        bary(e, p, lb);
+        double mn = std::min(std::min(lb[0], lb[1]), std::min(lb[2], lb[3]));
+        if (mn > best) {
+          best = mn;
+          cur = e;
+          for (int i = 0; i < 4; ++i) l[i] = lb[i];
+        }
+      }
+    }
+    out_elem[q] = (int32_t)cur;
+    for (int i = 0; i < 4; ++i) out_bary[4 * q + i] = l[i];
+  }
+  return FP3D_OK;
+}
+
 }  // extern "C"

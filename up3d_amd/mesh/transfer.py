@@ -8,41 +8,35 @@ linearly onto the nodes of a finer mesh, seeds the fine-mesh Newton solve.
 from __future__ import annotations
 
 import numpy as np
-from scipy.spatial import cKDTree
 
 
-def _barycentric(nodes, elements, cand, pts):
-    """Barycentric coordinates of pts (m, 3) in candidate tets cand (m, k)."""
-    p0 = nodes[elements[cand, 0]]                       # (m, k, 3)
-    T = np.stack([nodes[elements[cand, i]] - p0 for i in (1, 2, 3)], axis=-1)
-    rhs = pts[:, None, :] - p0
-    lam123 = np.linalg.solve(T, rhs[..., None])[..., 0]  # (m, k, 3)
-    lam0 = 1.0 - lam123.sum(-1, keepdims=True)
-    return np.concatenate([lam0, lam123], axis=-1)      # (m, k, 4)
+def locate_points(nodes, elements, points, face_neighbors=None):
+    """(element id (m,), barycentric coordinates (m, 4)) of each point in
+    the tet mesh (nodes, elements): a native visibility walk over the face
+    graph from the previous point's element; points outside the mesh get
+    the boundary tet where the walk stopped."""
+    from .. import _native
+    from .metrics import build_face_adjacency
+
+    nd = np.ascontiguousarray(nodes, dtype=np.float64)
+    el = np.ascontiguousarray(elements, dtype=np.int32)
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    if face_neighbors is None:
+        face_neighbors, _ = build_face_adjacency(el)
+    fn = np.ascontiguousarray(face_neighbors, dtype=np.int32)
+    elem = np.empty(len(pts), dtype=np.int32)
+    bary = np.empty((len(pts), 4))
+    _native.check(_native.lib().fp3d_host_locate_points(
+        len(el), el.ctypes.data, nd.ctypes.data, fn.ctypes.data, len(pts),
+        pts.ctypes.data, elem.ctypes.data, bary.ctypes.data), "locate_points")
+    return elem, bary
 
 
-def interpolate_p1(nodes, elements, values, points, k: int = 16,
-                   chunk: int = 65536) -> np.ndarray:
+def interpolate_p1(nodes, elements, values, points) -> np.ndarray:
     """Piecewise-linear interpolant of the nodal `values` on the tet mesh
-    (nodes, elements) evaluated at `points`.  Each point is located among
-    the k tets with the nearest centroids; a point outside all of them
-    (mesh-boundary jitter) takes the candidate whose smallest barycentric
-    coordinate is largest, i.e. a short linear extrapolation."""
-    nodes = np.asarray(nodes, dtype=np.float64)
-    elements = np.asarray(elements, dtype=np.int64)
-    values = np.asarray(values, dtype=np.float64)
-    points = np.asarray(points, dtype=np.float64)
-    cen = nodes[elements].mean(axis=1)
-    tree = cKDTree(cen)
-    out = np.empty(len(points))
-    for s in range(0, len(points), chunk):
-        pts = points[s:s + chunk]
-        _, cand = tree.query(pts, k=min(k, len(elements)))
-        cand = cand.reshape(len(pts), -1)
-        lam = _barycentric(nodes, elements, cand, pts)
-        best = np.argmax(lam.min(axis=-1), axis=1)
-        rows = np.arange(len(pts))
-        e = cand[rows, best]
-        lb = lam[rows, best]
-        out[s:s + chunk] = np.einsum("mi,mi->m", lb, values[elements[e]])
-    return out
+    (nodes, elements) evaluated at `points` (points just outside the mesh
+    extrapolate linearly from the nearest boundary tet on the walk)."""
+    elem, bary = locate_points(nodes, elements, points)
+    el = np.asarray(elements, dtype=np.int64)
+    v = np.asarray(values, dtype=np.float64)
+    return np.einsum("mi,mi->m", bary, v[el[elem]])

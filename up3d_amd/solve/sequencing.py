@@ -32,6 +32,7 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
                            dm_min: float = 0.005, upwind_c_ramp: float = 3.0,
                            upwind_c_post: Sequence[float] = (2.0, 1.5),
                            m_crit: float = 0.95,
+                           post_stages_on: str = "finest",
                            newton_kw: Optional[dict] = None,
                            verbose: bool = False) -> Dict[str, object]:
     """meshes: coarse-to-fine list of dicts with keys nodes, elements,
@@ -39,7 +40,15 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
     dirichlet_fn(nodes) -> values.  Returns the finest mesh's result dict
     (solve_newton_transonic keys) plus `mesh_results` (per-mesh summary),
     `timings_total` (all meshes; the interpolations count as `seed`) and
-    `wall_s`."""
+    `wall_s`.
+
+    post_stages_on="finest" runs the upwind_c_post stages on the finest mesh
+    only; "coarser" runs them on every mesh after the first and lets the
+    finest mesh start directly at the last (production) upwind_c from the
+    interpolated production-C solution, so the expensive mesh runs one
+    Newton level instead of len(upwind_c_post) + 1."""
+    if post_stages_on not in ("finest", "coarser"):
+        raise ValueError(f"unknown post_stages_on {post_stages_on!r}")
     if len(meshes) < 1:
         raise ValueError("need at least one mesh")
     kw = dict(newton_kw or {})
@@ -69,12 +78,19 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
             t_seed = time.perf_counter() - t0
         finest = level == len(meshes) - 1
         lvl_kw = dict(kw, phi_init=phi0, n_picard_seed=0)
+        c_level, post = upwind_c_ramp, None
+        if post_stages_on == "finest":
+            post = list(upwind_c_post) if finest else None
+        elif prev is not None:
+            if finest and len(upwind_c_post):
+                c_level = upwind_c_post[-1]
+            else:
+                post = list(upwind_c_post)
         t0 = time.perf_counter()
         result = solve_newton_transonic(
             nodes, elements, dn, dv, m_inf,
             m_start=m_start if prev is None else m_inf, dm=dm, dm_min=dm_min,
-            upwind_c=upwind_c_ramp, m_crit=m_crit,
-            upwind_c_post=list(upwind_c_post) if finest else None,
+            upwind_c=c_level, m_crit=m_crit, upwind_c_post=post,
             newton_kw=lvl_kw, verbose=verbose)
         if torch.cuda.is_available():
             torch.cuda.synchronize()
