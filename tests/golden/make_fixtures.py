@@ -4,6 +4,8 @@ fixtures).  Only data is stored: mesh arrays read from the reference's
 .msh files and the expected numbers its result CSVs hold.
 
   naca0012_coarse.npz  nodes/elements of cases/meshes/naca0012_2.5d/coarse.msh
+  naca0012_{coarse,medium}_bc.npz  nodes, elements and the wall / farfield
+                       boundary faces of the same meshes (level-set wake)
   golden.json          expected values with their reference sources
 """
 
@@ -25,6 +27,12 @@ def main():
     m = read_mesh(REF / "cases/meshes/naca0012_2.5d/coarse.msh")
     np.savez_compressed(HERE / "naca0012_coarse.npz", nodes=m.nodes,
                         elements=m.elements)
+    for lv in ("coarse", "medium"):
+        m = read_mesh(REF / f"cases/meshes/naca0012_2.5d/{lv}.msh")
+        np.savez_compressed(HERE / f"naca0012_{lv}_bc.npz", nodes=m.nodes,
+                            elements=m.elements,
+                            wall=m.boundary_faces["wall"],
+                            farfield=m.boundary_faces["farfield"])
     g73 = {}
     for ln in (REF / "cases/demo/p7_diff_flux/results/g73_constructed.csv").read_text().splitlines()[1:]:
         k, v = ln.split(",")
@@ -47,6 +55,14 @@ def main():
                          "values": col},
         "naca_medium_colors": {"source": "cases/demo/p3_subsonic/results/assembly_debt.csv:6",
                                "values": {"n_colors": 27}},
+        # level-set wake, NACA0012 2.5D, alpha 2 deg, incompressible, vortex
+        # far field, wall-adjacent pressure-equality Kutta (the reference's
+        # measured table, coarse / medium / fine)
+        "ls_gamma_naca_a2": {"source": "tests/test_b4_te_control_volume.py:186-189",
+                             "values": {"levelset": [0.1177, 0.1191, 0.1197],
+                                        "conforming": [0.1175, 0.1200, 0.1202],
+                                        "b3_mass_row": [0.2074, 0.1760, 0.1704],
+                                        "levels": ["coarse", "medium", "fine"]}},
     }
     (HERE / "golden.json").write_text(json.dumps(out, indent=1))
 

@@ -1,0 +1,336 @@
+"""Multivalued (extended-DOF) operator of a level-set wake
+(pyfp3d/wake/multivalued.py:43-907 and pyfp3d/kernels/cut_assembly.py,
+restated).
+
+The extended system has n_total = n_main + n_ext unknowns: the main DOF of
+a node holds the potential on the node's own side of the sheet, the aux
+DOF of a cut-element node the other side's.  Its mass-conservation part is
+
+  * plain elements (neither cut nor in the below-TE fan): the ordinary
+    single-valued P1 assembly on main DOFs, weighted by the element density
+    of the main field (a mixed-side plain element reads the main field too,
+    pyfp3d/wake/multivalued.py:543-628);
+  * cut elements: the same element matrix twice, on dofs_upper with the
+    upper density and on dofs_lower with the lower density;
+  * below-TE fan elements: once, on main DOFs with the TE node's aux DOF,
+    lower density.
+
+Design: the plain part is the device Picard-matrix assembly
+(PicardOperator.matrix_dev) with the special (cut + fan) elements given a
+zero weight; everything that involves a special element or an aux row is a
+small host COO correction C (O(sheet area) entries).  The Krylov operator
+is y = [A_plain x_main ; 0] + C x on the device.
+
+Closures (aux rows):
+  * "continuity": aux_k - main_j = 0 (the weld; the extended solve then
+    equals the single-valued one), cut-element entries across the sheet
+    redirected to aux columns;
+  * "wake_ls": the non-TE aux rows carry the wake least-squares condition
+    V_e [(n.B_a)(n.B_c) + (u.B_a)(u.B_c)] (delta_c) with delta_c =
+    x[up_c] - x[lo_c]; with te_kutta="pressure" the TE aux rows carry the
+    linearised pressure-equality Kutta s.(q_u - q_l) = 0 (s the frozen mean
+    q_u + q_l over the wall-adjacent TE control volumes) and their lower
+    mass conservation is re-routed onto the TE main row; te_kutta="mass"
+    keeps lower-side mass conservation there.
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from ..device import empty, to_dev, to_host
+from ..kernels.jacobian import PicardOperator
+
+
+def _elem_matrices(B, V, w):
+    """(m, 4, 4) w_e V_e (B_a . B_b)."""
+    return np.einsum("e,ead,ebd->eab", V * w, B, B)
+
+
+def isentropic_density(q2, m_inf, gamma_air=1.4):
+    """rho(q2) = [1 + (g-1)/2 M^2 (1 - q2)]^(1/(g-1)), base floored at 1e-10
+    (pyfp3d/physics/isentropic.py:29-58); host form for the O(sheet)
+    special elements."""
+    base = 1.0 + 0.5 * (gamma_air - 1.0) * m_inf * m_inf * (1.0 - q2)
+    return np.maximum(base, 1e-10) ** (1.0 / (gamma_air - 1.0))
+
+
+class MultivaluedOperator:
+    def __init__(self, nodes, elements, cm, levelset=None):
+        self.op = PicardOperator(nodes, elements)
+        st = self.op.store
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self.el = np.asarray(st.elements_host, dtype=np.int64)
+        self.B = st.B_host.reshape(-1, 4, 3)
+        self.V = st.V_host
+        self.cm = cm
+        self.levelset = levelset
+        self.n_main = cm.n_main
+        self.n_ext = cm.n_ext_dofs
+        self.n_total = cm.n_total_dofs
+        if st.n_nodes != self.n_main:
+            raise ValueError("CutElementMap was built on a different mesh")
+        nt = len(self.el)
+        self.is_cut = np.zeros(nt, dtype=bool)
+        self.is_cut[cm.cut_elems] = True
+        self.is_tel = np.zeros(nt, dtype=bool)
+        self.is_tel[cm.te_lower_elems] = True
+        self.special = np.flatnonzero(self.is_cut | self.is_tel)
+        self.is_te = np.zeros(self.n_main, dtype=bool)
+        self.is_te[cm.te_nodes] = True
+        self.cut_nodes = np.flatnonzero(cm.ext_dof_of_node >= 0)
+        self.cut_aux = cm.ext_dof_of_node[self.cut_nodes]
+        # lower-copy DOF vector of the fan elements (TE node -> its aux)
+        tel = cm.te_lower_elems
+        self.tel_dof = np.where(self.is_te[self.el[tel]],
+                                cm.ext_dof_of_node[self.el[tel]], self.el[tel])
+        # device weight mask: 0 on special elements
+        keep = np.ones(nt)
+        keep[self.special] = 0.0
+        self.d_plain_mask = to_dev(keep)
+        self._nonte_rows = cm.ext_dof_of_node[
+            self.cut_nodes[~self.is_te[self.cut_nodes]]]
+        self._ls = None
+        if levelset is not None and len(cm.cut_elems):
+            cen = self.nodes[self.el[cm.cut_elems]].mean(axis=1)
+            self._ls = self._wake_ls_coo(levelset.surface_normals(cen),
+                                         levelset.direction)
+        self._te_cv = self._te_control_volumes()
+
+    # ------------------------------------------------------- COO pieces
+    def _wake_ls_coo(self, n_hat, u_hat):
+        """pyfp3d/kernels/cut_assembly.py:230-318 wake_ls_coo."""
+        cm = self.cm
+        cut = cm.cut_elems
+        B, V = self.B[cut], self.V[cut]
+        u = np.asarray(u_hat, dtype=np.float64)
+        u = u / np.linalg.norm(u)
+        Bn = np.einsum("ead,ed->ea", B, n_hat)
+        Bu = np.einsum("ead,d->ea", B, u)
+        P = V[:, None, None] * (Bn[:, :, None] * Bn[:, None, :]
+                                + Bu[:, :, None] * Bu[:, None, :])
+        conn = self.el[cut]
+        ok = ~self.is_te[conn]
+        e_i, a_i = np.nonzero(ok)
+        rows = np.repeat(cm.ext_dof_of_node[conn[e_i, a_i]], 4)
+        k = P[e_i, a_i, :].reshape(-1)
+        cu = cm.dofs_upper[e_i].reshape(-1)
+        cl = cm.dofs_lower[e_i].reshape(-1)
+        return (np.concatenate([rows, rows]), np.concatenate([cu, cl]),
+                np.concatenate([k, -k]))
+
+    def _scatter(self, ids, dofs, w):
+        K = _elem_matrices(self.B[ids], self.V[ids], w)
+        return (np.repeat(dofs, 4, axis=1).reshape(-1),
+                np.tile(dofs, (1, 4)).reshape(-1), K.reshape(-1))
+
+    def _te_control_volumes(self):
+        """pyfp3d/wake/multivalued.py:178-248: per TE node, the wall-adjacent
+        (>= 3 wall nodes) elements of its fan, split into the copies that
+        reference the TE through its main DOF (upper) and its aux DOF
+        (lower)."""
+        cm = self.cm
+        el = self.el
+        cut_index = np.full(len(el), -1, dtype=np.int64)
+        cut_index[cm.cut_elems] = np.arange(len(cm.cut_elems))
+        tel_index = np.full(len(el), -1, dtype=np.int64)
+        tel_index[cm.te_lower_elems] = np.arange(len(cm.te_lower_elems))
+        if cm.wall_nodes is None:
+            on_wall = np.ones(len(el), dtype=bool)
+        else:
+            w = np.zeros(self.n_main, dtype=bool)
+            w[cm.wall_nodes] = True
+            on_wall = w[el].sum(axis=1) >= 3
+        cvs = []
+        for t in cm.te_nodes:
+            fan = np.flatnonzero((el == t).any(axis=1) & on_wall)
+            ue, ud, le, ld = [], [], [], []
+            for e in fan:
+                k = cut_index[e]
+                if k >= 0:
+                    ue.append(e); ud.append(cm.dofs_upper[k])
+                    le.append(e); ld.append(cm.dofs_lower[k])
+                elif tel_index[e] >= 0:
+                    le.append(e); ld.append(self.tel_dof[tel_index[e]])
+                else:
+                    ue.append(e); ud.append(el[e])
+            cvs.append({"upper_elems": np.array(ue, dtype=np.int64),
+                        "upper_dofs": np.array(ud, dtype=np.int64).reshape(-1, 4),
+                        "lower_elems": np.array(le, dtype=np.int64),
+                        "lower_dofs": np.array(ld, dtype=np.int64).reshape(-1, 4)})
+        return cvs
+
+    def te_velocities(self, phi_ext):
+        """(q_upper, q_lower) (n_te, 3): volume-weighted gradients over each
+        TE node's upper / lower wall-adjacent control volume."""
+        x = np.asarray(phi_ext, dtype=np.float64)
+        qu = np.zeros((len(self._te_cv), 3))
+        ql = np.zeros((len(self._te_cv), 3))
+        for i, cv in enumerate(self._te_cv):
+            for key, out in (("upper", qu), ("lower", ql)):
+                e, d = cv[key + "_elems"], cv[key + "_dofs"]
+                if len(e):
+                    g = np.einsum("ead,ea->ed", self.B[e], x[d])
+                    out[i] = (self.V[e][:, None] * g).sum(0) / self.V[e].sum()
+        return qu, ql
+
+    def _te_kutta_coo(self, phi_ext):
+        """pyfp3d/kernels/cut_assembly.py:321-398 te_kutta_coo: s.(q_u - q_l)
+        with s = q_u + q_l frozen at phi_ext, on the TE aux rows."""
+        qu, ql = self.te_velocities(phi_ext)
+        s = qu + ql
+        rows, cols, data = [], [], []
+        for i, t in enumerate(self.cm.te_nodes):
+            r = self.cm.ext_dof_of_node[t]
+            for key, sign in (("upper", 1.0), ("lower", -1.0)):
+                cv = self._te_cv[i]
+                e, d = cv[key + "_elems"], cv[key + "_dofs"]
+                if not len(e):
+                    continue
+                w = self.V[e] / self.V[e].sum()
+                c = sign * w[:, None] * np.einsum("ead,d->ea", self.B[e], s[i])
+                rows.append(np.full(d.size, r, dtype=np.int64))
+                cols.append(d.reshape(-1))
+                data.append(c.reshape(-1))
+        if not rows:
+            z = np.empty(0, dtype=np.int64)
+            return z, z, np.empty(0)
+        return np.concatenate(rows), np.concatenate(cols), np.concatenate(data)
+
+    # ------------------------------------------------------- fields
+    def side_potentials(self, phi_ext):
+        """(phi_upper, phi_lower) nodal fields (main except at cut nodes of
+        the opposite side, which read their aux DOF)."""
+        x = np.asarray(phi_ext, dtype=np.float64)
+        up = x[: self.n_main].copy()
+        lo = x[: self.n_main].copy()
+        plus = self.cm.node_side[self.cut_nodes] == 1
+        lo[self.cut_nodes[plus]] = x[self.cut_aux[plus]]
+        up[self.cut_nodes[~plus]] = x[self.cut_aux[~plus]]
+        return up, lo
+
+    def node_jump(self, phi_ext, node_ids):
+        x = np.asarray(phi_ext, dtype=np.float64)
+        ids = np.asarray(node_ids, dtype=np.int64)
+        aux = self.cm.ext_dof_of_node[ids]
+        jump = np.zeros(len(ids))
+        h = aux >= 0
+        jump[h] = self.cm.node_side[ids[h]] * (x[ids[h]] - x[aux[h]])
+        return jump
+
+    def te_jump(self, phi_ext):
+        return self.node_jump(phi_ext, self.cm.te_nodes)
+
+    def main_potential(self, phi_ext):
+        return np.asarray(phi_ext)[: self.n_main]
+
+    def special_densities(self, phi_ext, m_inf, gamma_air=1.4, u_inf=1.0):
+        """(rho_up_cut, rho_lo_cut, rho_lo_tel): isentropic densities of the
+        cut elements' two copies and of the fan elements' lower copy."""
+        x = np.asarray(phi_ext, dtype=np.float64)
+        cm = self.cm
+
+        def rho(ids, dofs):
+            g = np.einsum("ead,ea->ed", self.B[ids], x[dofs])
+            return isentropic_density((g * g).sum(1) / u_inf ** 2, m_inf,
+                                      gamma_air)
+        return (rho(cm.cut_elems, cm.dofs_upper), rho(cm.cut_elems, cm.dofs_lower),
+                rho(cm.te_lower_elems, self.tel_dof))
+
+    # ------------------------------------------------------- assembly
+    def plain_weight_dev(self, rho_main_dev: Optional[torch.Tensor]):
+        """Device element weights of the plain part (special elements 0)."""
+        if rho_main_dev is None:
+            return self.d_plain_mask.clone()
+        return rho_main_dev * self.d_plain_mask
+
+    def correction(self, special_rho=None, closure: str = "wake_ls",
+                   te_kutta: str = "pressure", phi_ext=None) -> sp.csr_matrix:
+        """Host (n_total x n_total) correction C: special-element mass
+        conservation, aux rows (closure) and the TE Kutta rows."""
+        cm = self.cm
+        nT = self.n_total
+        if special_rho is None:
+            r_uc = np.ones(len(cm.cut_elems))
+            r_lc = np.ones(len(cm.cut_elems))
+            r_lt = np.ones(len(cm.te_lower_elems))
+        else:
+            r_uc, r_lc, r_lt = special_rho
+        parts = []
+        if closure == "continuity":
+            # single-valued special elements on main DOFs with the upper
+            # weight, then cross-sheet entries of cut elements moved to aux
+            # columns (pyfp3d/kernels/cut_assembly.py:56-116), aux weld rows
+            conn_c = self.el[cm.cut_elems]
+            parts.append(self._scatter(cm.cut_elems, conn_c, r_uc))
+            parts.append(self._scatter(cm.te_lower_elems, self.el[cm.te_lower_elems], r_lt))
+            K = _elem_matrices(self.B[cm.cut_elems], self.V[cm.cut_elems], r_uc)
+            side = cm.node_side[conn_c]
+            aux = cm.ext_dof_of_node[conn_c]
+            e_i, a_i, b_i = np.nonzero(side[:, :, None] != side[:, None, :])
+            k = K[e_i, a_i, b_i]
+            row = conn_c[e_i, a_i]
+            parts.append((np.concatenate([row, row]),
+                          np.concatenate([conn_c[e_i, b_i], aux[e_i, b_i]]),
+                          np.concatenate([-k, k])))
+            ones = np.ones(len(self.cut_aux))
+            parts.append((np.concatenate([self.cut_aux, self.cut_aux]),
+                          np.concatenate([self.cut_aux, self.cut_nodes]),
+                          np.concatenate([ones, -ones])))
+        elif closure == "wake_ls":
+            if self._ls is None:
+                raise ValueError("wake_ls closure needs a level set and cut elements")
+            m_parts = [self._scatter(cm.cut_elems, cm.dofs_upper, r_uc),
+                       self._scatter(cm.cut_elems, cm.dofs_lower, r_lc),
+                       self._scatter(cm.te_lower_elems, self.tel_dof, r_lt)]
+            r = np.concatenate([p[0] for p in m_parts])
+            c = np.concatenate([p[1] for p in m_parts])
+            d = np.concatenate([p[2] for p in m_parts])
+            drop = np.zeros(nT, dtype=bool)
+            drop[self._nonte_rows] = True
+            keep = ~drop[r]
+            r, c, d = r[keep], c[keep], d[keep]
+            if te_kutta == "pressure":
+                if phi_ext is None:
+                    raise ValueError("te_kutta='pressure' needs phi_ext")
+                reroute = np.full(nT, -1, dtype=np.int64)
+                reroute[cm.ext_dof_of_node[cm.te_nodes]] = cm.te_nodes
+                hit = reroute[r] >= 0
+                r = r.copy()
+                r[hit] = reroute[r[hit]]
+                parts.append((r, c, d))
+                parts.append(self._ls)
+                parts.append(self._te_kutta_coo(phi_ext))
+            elif te_kutta == "mass":
+                parts.append((r, c, d))
+                parts.append(self._ls)
+            else:
+                raise ValueError(f"te_kutta={te_kutta!r} unknown")
+        else:
+            raise ValueError(f"closure={closure!r} unknown")
+        rows = np.concatenate([p[0] for p in parts])
+        cols = np.concatenate([p[1] for p in parts])
+        data = np.concatenate([p[2] for p in parts])
+        C = sp.coo_matrix((data, (rows, cols)), shape=(nT, nT)).tocsr()
+        C.sum_duplicates()
+        return C
+
+    def assemble_matrix(self, rho_main=None, special_rho=None,
+                        closure: str = "wake_ls", te_kutta: str = "pressure",
+                        phi_ext=None) -> sp.csr_matrix:
+        """The whole extended matrix as a host scipy CSR (plain part from the
+        device assembly) -- for tests and small cases; the solver keeps the
+        plain part on the device."""
+        op = self.op
+        w = self.plain_weight_dev(None if rho_main is None else to_dev(rho_main))
+        vals = empty(op.nnz)
+        op.matrix_dev(w, vals)
+        A = sp.csr_matrix((to_host(vals), op.store.indices_host,
+                           op.store.indptr_host), shape=(self.n_main, self.n_main))
+        E = sp.bmat([[A, None], [None, sp.csr_matrix((self.n_ext, self.n_ext))]],
+                    format="csr")
+        return (E + self.correction(special_rho, closure, te_kutta, phi_ext)).tocsr()
