@@ -120,3 +120,50 @@ def test_compressible_lifting_gamma(gpu):
     assert r["converged"]
     # quoted to 6 digits; measured here 0.087174
     assert abs(r["gamma"] - 0.087173) <= 2e-6
+
+
+def test_upwind_subcritical_is_noop(gpu):
+    """upwind_c > 0 on a subcritical case: nu == 0 everywhere, so the
+    per-side artificial density equals the plain density and the circulation
+    is unchanged (tests/test_b6_transonic.py:84-100)."""
+    from up3d_amd.solve.picard_ls import solve_multivalued_lifting
+
+    mesh = _mesh("coarse")
+    _, mv = _setup(mesh)
+    r_off = solve_multivalued_lifting(mv, mesh, 0.5, alpha_deg=1.25)
+    r_on = solve_multivalued_lifting(mv, mesh, 0.5, alpha_deg=1.25, upwind_c=1.5)
+    assert r_on["nu_max"] == 0.0 and r_on["n_nu_active"] == 0
+    assert r_on["n_limited"] == 0 and r_on["n_floored"] == 0
+    assert r_on["converged"] and abs(r_on["gamma"] - r_off["gamma"]) < 1e-7
+
+
+def test_side_graphs_do_not_cross_the_wake(gpu):
+    from up3d_amd.device import to_host
+
+    mesh = _mesh("coarse")
+    _, mv = _setup(mesh)
+    for (fn_d, _), keep in zip(mv._side_graphs(), (mv.in_upper, mv.in_lower)):
+        fn = to_host(fn_d)
+        nb = fn[keep][fn[keep] >= 0]
+        assert keep[nb].all(), "upwind graph leaks across the wake"
+        assert (fn[~keep] == -1).all()
+
+
+def test_supercritical_pocket_m070(gpu):
+    """M0.70 alpha 1.25: a supersonic pocket (M_max in (1, 1.3)) converges
+    with per-side upwinding and supersonic-zone damping, 0 limited / floored,
+    and sits on the wing, not on the wake strip
+    (tests/test_b6_transonic.py:155-180)."""
+    from up3d_amd.solve.picard_ls import solve_multivalued_transonic
+
+    mesh = _mesh("coarse")
+    _, mv = _setup(mesh)
+    r = solve_multivalued_transonic(mv, mesh, 0.70, alpha_deg=1.25, n_outer_level=400)
+    L = r["levels"][-1]
+    print(f"[wake] M0.70 levels={[(lv['m_inf'], lv['n_outer'], lv['converged'], round(lv['mach_max'], 4)) for lv in r['levels']]} "
+          f"gamma={r['gamma']:.6f}", flush=True)
+    assert L["converged"]
+    assert 1.0 < L["mach_max"] < 1.3
+    assert L["n_limited"] == 0 and L["n_floored"] == 0
+    act = mv.nu_active_elements()
+    assert len(act) > 0 and not mv.is_cut[act].any()

@@ -44,7 +44,7 @@ class ExtendedSystem:
 
     def __init__(self, mvop, rho_main_dev, special_rho, phi_ext, dir_dofs,
                  dir_vals, closure="wake_ls", te_kutta="pressure",
-                 amg=None):
+                 amg=None, damping_theta=None, damp_rows=None, x_prev=None):
         op = mvop.op
         st = op.store
         self.mvop = mvop
@@ -63,6 +63,23 @@ class ExtendedSystem:
         self.vals = empty(op.nnz)
         op.matrix_dev(mvop.plain_weight_dev(rho_main_dev), self.vals)
         C = mvop.correction(special_rho, closure, te_kutta, phi_ext)
+        d_damp = None
+        if damping_theta is not None and damp_rows is not None:
+            # theta * diag(E) on the damped free main rows (supersonic-zone
+            # damping, pyfp3d/solve/picard_ls.py:626-640): added to C's
+            # diagonal, with theta diag x_prev on the right-hand side
+            ip, ix = st.indptr_host, st.indices_host
+            if not hasattr(mvop, "_diag_pos"):
+                rows_ = np.repeat(np.arange(nm), np.diff(ip))
+                mvop._diag_pos = np.flatnonzero(ix == rows_)
+            dA = to_host(self.vals)[mvop._diag_pos] + C.diagonal()[:nm]
+            d = np.zeros(nT)
+            d[:nm] = damping_theta * dA
+            d[~damp_rows] = 0.0
+            d[dir_dofs] = 0.0
+            np.clip(d, 0.0, None, out=d)
+            C = (C + sp.diags(d)).tocsr()
+            d_damp = d
         # b = -E x_D on free rows, x_D on Dirichlet rows
         y = empty(nT)
         xDd = to_dev(xD)
@@ -71,6 +88,9 @@ class ExtendedSystem:
         y[nm:].zero_()
         yh = to_host(y) + C @ xD
         b = -yh
+        if d_damp is not None:
+            # (E + D) x = b + D x_prev; D vanishes on the Dirichlet rows
+            b += d_damp * np.asarray(x_prev)
         b[dir_dofs] = dir_vals
         self.b = to_dev(b)
         # Dirichlet embedding: identity rows, zero columns
@@ -136,17 +156,26 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
                               omega_gamma: float = 0.5, omega_rho: float = 0.5,
                               tol_gamma: float = 1e-6, tol_rho: float = 1e-6,
                               n_outer_max: int = 80, gamma_init: float = 0.0,
+                              gamma_farfield_fixed: Optional[float] = None,
                               te_kutta: str = "pressure",
                               farfield: str = "vortex",
+                              upwind_c: float = 0.0, m_crit: float = 0.95,
+                              m_cap: float = 3.0, rho_floor: float = 0.05,
+                              damping_theta: Optional[float] = None,
+                              tol_residual: Optional[float] = None,
                               phi_init: Optional[np.ndarray] = None,
                               linear_rtol: float = 1e-10,
                               gmres_restart: int = 60, gmres_maxiter: int = 50,
                               amg_rebuild_every: int = 5,
                               verbose: bool = False) -> Dict[str, object]:
-    """`mesh` needs .nodes and .boundary_faces["farfield"].  Returns the
-    reference's result keys: phi_ext, phi, gamma, cl_kj, te_jump, n_outer,
-    converged, gamma_history, drho_history, residual_history,
-    residual_norm, timings, mach2_max, n_gmres_total, n_gmres_stalled."""
+    """`mesh` needs .nodes and .boundary_faces["farfield"].  upwind_c > 0
+    (with m_inf > 0) runs the per-side artificial density with side-
+    restricted walks, and damping_theta the theta*diag damping on the main
+    rows of nu > 0 elements (the reference's supersonic-zone scope).
+    Returns the reference's result keys: phi_ext, phi, gamma, cl_kj,
+    te_jump, n_outer, converged, gamma_history, drho_history,
+    residual_history, residual_norm, timings, mach2_max, nu_max,
+    n_nu_active, n_limited, n_floored, n_gmres_total, n_gmres_stalled."""
     if farfield != "vortex":
         raise NotImplementedError("only the vortex far field is built")
     if not 0.0 <= m_inf < 1.0:
@@ -158,6 +187,7 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
     beta = float(np.sqrt(1.0 - m_inf ** 2))
     op = mvop.op
     st = op.store
+    use_upwind = upwind_c > 0.0 and m_inf > 0.0
     nodes = np.asarray(mesh.nodes, dtype=np.float64)
     ffaces = mesh.boundary_faces["farfield"]
     phi_ext = np.zeros(mvop.n_total)
@@ -168,25 +198,37 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
         phi_ext[mvop.cut_aux] = phi_ext[mvop.cut_nodes]
     gamma = float(gamma_init)
     gamma_history, drho_history, residual_history = [gamma], [], []
-    rho_main = None        # device (n_tets,) densities of the main field
+    w_dev = None           # device element weights (plain part / AMG)
     special = None         # host densities of the special copies
+    rt_pair = None         # (upper, lower) device densities (upwind path)
+    # the first outer damps every main row; later outers only the rows of
+    # nu > 0 elements (pyfp3d/solve/picard_ls.py:562-566)
+    damp_rows = None
+    if damping_theta is not None:
+        damp_rows = np.zeros(mvop.n_total, dtype=bool)
+        damp_rows[: mvop.n_main] = True
     amg = None
     n_gmres_total = n_gmres_stalled = 0
     converged = False
     residual = np.inf
-    q2cap = np.inf
-    grad = empty((st.n_tets, 3))
-    q2n = empty(st.n_tets)
-    q2l = empty(st.n_tets)
-    rho_new = empty(st.n_tets)
-    lim = empty(st.n_tets, torch.uint8)
+    dgamma = drho = np.inf
+    nt = st.n_tets
+    grad = empty((nt, 3))
+    q2n = empty(nt)
+    q2l = empty(nt)
+    rho_new = empty(nt)
+    lim = empty(nt, torch.uint8)
+    own_plus = torch.tensor(mvop.in_upper, device=st.device)
     for outer in range(n_outer_max):
         with phase(timings, "assembly"):
-            ff, vals = _farfield_main(nodes, ffaces, alpha_deg, gamma, u_inf,
+            g_ff = gamma if gamma_farfield_fixed is None else gamma_farfield_fixed
+            ff, vals = _farfield_main(nodes, ffaces, alpha_deg, g_ff, u_inf,
                                       vortex_center, beta)
             reuse = amg if (amg is not None and outer % amg_rebuild_every) else None
-            sysm = ExtendedSystem(mvop, rho_main, special, phi_ext, ff, vals,
-                                  "wake_ls", te_kutta, amg=reuse)
+            sysm = ExtendedSystem(mvop, w_dev, special, phi_ext, ff, vals,
+                                  "wake_ls", te_kutta, amg=reuse,
+                                  damping_theta=damping_theta,
+                                  damp_rows=damp_rows, x_prev=phi_ext)
             amg = sysm.amg
             x_prev = to_dev(np.where(sysm.free, phi_ext, 0.0))
             x_prev[to_dev(ff, torch.int64)] = to_dev(vals)
@@ -208,32 +250,49 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
             gamma = (1.0 - omega_gamma) * gamma + omega_gamma * g_new
             gamma_history.append(gamma)
         with phase(timings, "residual"):
-            phid = to_dev(phi_ext[: mvop.n_main])
-            op.state_dev(phid, u_inf ** 2, m_inf, gamma_air, q2cap, grad, q2n,
-                         q2l, rho_new, lim)
-            sp_new = mvop.special_densities(phi_ext, m_inf, gamma_air, u_inf)
-            if rho_main is None:
-                drho = 0.0
-                rho_main = rho_new.clone()
-                special = sp_new
+            if use_upwind:
+                ru, rl = mvop.side_rho_tilde_dev(phi_ext, m_inf, upwind_c, m_crit,
+                                                 gamma_air, u_inf, m_cap, rho_floor)
+                if damping_theta is not None:
+                    act = mvop.nu_active_elements()
+                    damp_rows = np.zeros(mvop.n_total, dtype=bool)
+                    damp_rows[np.unique(mvop.el[act])] = True
+                if rt_pair is None:
+                    drho = 0.0
+                    rt_pair = (ru, rl)
+                else:
+                    new_own = torch.where(own_plus, ru, rl)
+                    old_own = torch.where(own_plus, *rt_pair)
+                    drho = float((new_own - old_own).abs().max().item())
+                    rt_pair = tuple(o + omega_rho * (n_ - o)
+                                    for o, n_ in zip(rt_pair, (ru, rl)))
+                w_dev, special = mvop.side_weights(*rt_pair)
             else:
-                d_plain = float(((rho_new - rho_main) * mvop.d_plain_mask).abs().max().item())
-                d_sp = max([float(np.abs(a - b).max()) if len(a) else 0.0
-                            for a, b in zip(sp_new, special)] + [0.0])
-                drho = max(d_plain, d_sp)
-                rho_main = rho_main + omega_rho * (rho_new - rho_main)
-                special = tuple(o + omega_rho * (n - o) for o, n in zip(special, sp_new))
+                phid = to_dev(phi_ext[: mvop.n_main])
+                op.state_dev(phid, u_inf ** 2, m_inf, gamma_air, np.inf, grad, q2n,
+                             q2l, rho_new, lim)
+                sp_new = mvop.special_densities(phi_ext, m_inf, gamma_air, u_inf)
+                if w_dev is None:
+                    drho = 0.0
+                    w_dev = rho_new.clone()
+                    special = sp_new
+                else:
+                    d_plain = float(((rho_new - w_dev) * mvop.d_plain_mask).abs().max().item())
+                    d_sp = max([float(np.abs(a_ - b_).max()) if len(a_) else 0.0
+                                for a_, b_ in zip(sp_new, special)] + [0.0])
+                    drho = max(d_plain, d_sp)
+                    w_dev = w_dev + omega_rho * (rho_new - w_dev)
+                    special = tuple(o + omega_rho * (n_ - o) for o, n_ in zip(special, sp_new))
             drho_history.append(drho)
         if verbose:
-            print(f"  ls-picard {outer:2d}: |R|={residual:.3e} gamma={gamma:.6f} "
-                  f"dgamma={dgamma:.2e} drho={drho:.2e} gmres={n_it} info={info}",
-                  flush=True)
-        if outer > 0 and dgamma < tol_gamma and drho < tol_rho:
+            print(f"  ls-picard {outer:3d}: |R|={residual:.3e} gamma={gamma:.6f} "
+                  f"dgamma={dgamma:.2e} drho={drho:.2e} gmres={n_it} info={info}"
+                  + (f" nu_max={mvop.nu_max:.3f}" if use_upwind else ""), flush=True)
+        res_ok = tol_residual is not None and residual < tol_residual
+        if outer > 0 and ((dgamma < tol_gamma and drho < tol_rho) or res_ok):
             converged = True
             break
-    m2 = empty(st.n_tets)
-    _native.call("fp3d_mach_squared_field", st.n_tets, ptr(q2n), float(m_inf),
-                 float(gamma_air), ptr(m2), stream_handle())
+    mach2_max = mvop.own_side_mach2_max(phi_ext, m_inf, gamma_air, u_inf)
     finalize(timings, time.perf_counter() - t_wall0)
     return {
         "phi_ext": phi_ext, "phi": mvop.main_potential(phi_ext),
@@ -242,6 +301,65 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
         "converged": converged, "gamma_history": gamma_history,
         "drho_history": drho_history, "residual_history": residual_history,
         "residual_norm": residual, "timings": timings,
-        "mach2_max": float(m2.max().item()), "farfield": farfield,
+        "mach2_max": mach2_max, "farfield": farfield,
+        "nu_max": mvop.nu_max if use_upwind else 0.0,
+        "n_nu_active": mvop.n_nu_active if use_upwind else 0,
+        "n_limited": mvop.n_limited if use_upwind else 0,
+        "n_floored": mvop.n_floored if use_upwind else 0,
         "n_gmres_total": n_gmres_total, "n_gmres_stalled": n_gmres_stalled,
     }
+
+
+def solve_multivalued_transonic(mvop, mesh, m_target: float,
+                                alpha_deg: float = 1.25, u_inf: float = 1.0,
+                                gamma_air: float = 1.4, m_start: float = 0.60,
+                                dm: float = 0.05, upwind_c: float = 1.5,
+                                m_crit: float = 0.95, damping_theta: float = 0.2,
+                                omega_rho: float = 0.5, n_outer_seed: int = 120,
+                                n_outer_level: int = 400, tol_rho: float = 1e-6,
+                                tol_gamma: float = 1e-6,
+                                tol_residual: Optional[float] = 1e-7,
+                                farfield_lag: str = "live",
+                                **kwargs) -> Dict[str, object]:
+    """Mach continuation of the level-set Picard (pyfp3d/solve/picard_ls.py:
+    919-1044): each level is a warm-started solve_multivalued_lifting from
+    the previous level's full extended state and circulation;
+    farfield_lag="level" freezes the far-field vortex at the previous
+    level's circulation.  Returns the final level's dict plus `levels` and
+    `mach_schedule`."""
+    from .continuation import mach_schedule
+    from .timing import sum_timings
+
+    if farfield_lag not in ("live", "level"):
+        raise ValueError(f"farfield_lag={farfield_lag!r} unknown")
+    schedule = mach_schedule(m_target, m_start=m_start, dm=dm)
+    phi_ext, gamma, levels, res = None, 0.0, [], None
+    t0 = time.perf_counter()
+    for k, m in enumerate(schedule):
+        tl = time.perf_counter()
+        res = solve_multivalued_lifting(
+            mvop, mesh, m, alpha_deg=alpha_deg, u_inf=u_inf, gamma_air=gamma_air,
+            upwind_c=upwind_c, m_crit=m_crit, damping_theta=damping_theta,
+            omega_rho=omega_rho, n_outer_max=n_outer_seed if k == 0 else n_outer_level,
+            tol_rho=tol_rho, tol_gamma=tol_gamma, tol_residual=tol_residual,
+            gamma_init=gamma, phi_init=phi_ext,
+            gamma_farfield_fixed=gamma if (farfield_lag == "level" and k > 0) else None,
+            **kwargs)
+        phi_ext, gamma = res["phi_ext"], res["gamma"]
+        levels.append({"m_inf": float(m), "gamma": float(gamma),
+                       "cl_kj": float(res["cl_kj"]),
+                       "mach_max": float(np.sqrt(res["mach2_max"])),
+                       "n_outer": int(res["n_outer"]),
+                       "converged": bool(res["converged"]),
+                       "n_limited": int(res["n_limited"]),
+                       "n_floored": int(res["n_floored"]),
+                       "residual_norm": float(res["residual_norm"]),
+                       "wall_s": time.perf_counter() - tl,
+                       "timings": res["timings"],
+                       "n_lin_iters": int(res["n_gmres_total"])})
+    res = dict(res)
+    res["levels"] = levels
+    res["mach_schedule"] = [float(m) for m in schedule]
+    res["timings_total"] = sum_timings([lv["timings"] for lv in levels])
+    res["wall_s"] = time.perf_counter() - t0
+    return res

@@ -42,8 +42,10 @@ import numpy as np
 import scipy.sparse as sp
 import torch
 
-from ..device import empty, to_dev, to_host
+from .. import _native
+from ..device import empty, ptr, stream_handle, to_dev, to_host
 from ..kernels.jacobian import PicardOperator
+from ..physics.isentropic import q2_at_mach
 
 
 def _elem_matrices(B, V, w):
@@ -100,6 +102,19 @@ class MultivaluedOperator:
             self._ls = self._wake_ls_coo(levelset.surface_normals(cen),
                                          levelset.direction)
         self._te_cv = self._te_control_volumes()
+        # per-side element sets (pyfp3d/wake/multivalued.py:582-606): plain
+        # elements with any "+" node and cut elements carry an upper copy;
+        # plain all-"-", cut and fan elements a lower copy
+        plain = ~self.is_cut & ~self.is_tel
+        self.plain_plus = plain & (cm.node_side[self.el].max(axis=1) == 1)
+        self.in_upper = self.plain_plus | self.is_cut
+        self.in_lower = (plain & ~self.plain_plus) | self.is_cut | self.is_tel
+        self._graphs = None
+        self.nu_max = 0.0
+        self.n_nu_active = 0
+        self.n_floored = 0
+        self.n_limited = 0
+        self._nu_active = None
 
     # ------------------------------------------------------- COO pieces
     def _wake_ls_coo(self, n_hat, u_hat):
@@ -240,6 +255,134 @@ class MultivaluedOperator:
                                       gamma_air)
         return (rho(cm.cut_elems, cm.dofs_upper), rho(cm.cut_elems, cm.dofs_lower),
                 rho(cm.te_lower_elems, self.tel_dof))
+
+    # ------------------------------------------------- per-side upwinding
+    def _side_graphs(self):
+        """Face graphs restricted to each side's element set (the wake is a
+        slip line: the upwind walk never crosses it), as device
+        (face_neighbors, face displacement) pairs
+        (pyfp3d/wake/multivalued.py:608-640)."""
+        if self._graphs is None:
+            st = self.op.store
+            fn = np.asarray(st.face_neighbors_host, dtype=np.int64)
+            cen = np.asarray(st.centroids_host, dtype=np.float64)
+            out = []
+            for keep in (self.in_upper, self.in_lower):
+                ok = (fn >= 0) & keep[:, None] & keep[np.maximum(fn, 0)]
+                f = np.where(ok, fn, -1)
+                disp = np.where(ok[:, :, None], cen[np.maximum(f, 0)] - cen[:, None, :], 0.0)
+                out.append((to_dev(f.astype(np.int32), torch.int32),
+                            to_dev(np.ascontiguousarray(disp).reshape(len(fn), 12))))
+            self._graphs = tuple(out)
+        return self._graphs
+
+    def side_rho_tilde_dev(self, phi_ext, m_inf, upwind_c, m_crit,
+                           gamma_air=1.4, u_inf=1.0, m_cap=3.0,
+                           rho_floor=0.05):
+        """(rho_tilde_upper, rho_tilde_lower) device (n_tets,) arrays: each
+        side's artificial density from its own field (main field on plain
+        elements, the side copy on cut / fan elements), speed-limited, walked
+        on that side's restricted face graph
+        (pyfp3d/wake/multivalued.py:642-716).  Own-side monitors refreshed."""
+        op, st = self.op, self.op.store
+        cm = self.cm
+        nt = st.n_tets
+        x = np.asarray(phi_ext, dtype=np.float64)
+        q2cap = float(q2_at_mach(m_cap, m_inf, gamma_air))
+        grad = empty((nt, 3))
+        q2n = empty(nt)
+        q2l = empty(nt)
+        rho = empty(nt)
+        lim = empty(nt, torch.uint8)
+        op.state_dev(to_dev(x[: self.n_main]), u_inf ** 2, m_inf, gamma_air,
+                     q2cap, grad, q2n, q2l, rho, lim)
+        lim_h = to_host(lim).astype(bool)
+        (fu, du), (fl, dl) = self._side_graphs()
+        sides = ((cm.cut_elems, cm.dofs_upper, fu, du, self.in_upper),
+                 (np.concatenate([cm.cut_elems, cm.te_lower_elems]),
+                  np.concatenate([cm.dofs_lower, self.tel_dof]), fl, dl,
+                  self.in_lower))
+        out, nus = [], []
+        limited = 0
+        floored = 0
+        for ids, dofs, f, d, keep in sides:
+            g_s, q2l_s, rho_s = grad.clone(), q2l.clone(), rho.clone()
+            if len(ids):
+                g = np.einsum("ead,ea->ed", self.B[ids], x[dofs])
+                q2 = (g * g).sum(1) / u_inf ** 2
+                ql = np.minimum(q2, q2cap)
+                idd = to_dev(ids, torch.int64)
+                g_s[idd] = to_dev(g)
+                q2l_s[idd] = to_dev(ql)
+                rho_s[idd] = to_dev(isentropic_density(ql, m_inf, gamma_air))
+                special_lim = np.zeros(nt, dtype=bool)
+                special_lim[ids] = q2 > q2cap
+                lim_side = lim_h.copy()
+                lim_side[ids] = special_lim[ids]
+            else:
+                lim_side = lim_h
+            limited += int(np.count_nonzero(lim_side & keep))
+            up = empty(nt, torch.int32)
+            nu = empty(nt)
+            rt = empty(nt)
+            _native.call("fp3d_upwind_rho_tilde_d", nt, ptr(f), ptr(d),
+                         ptr(st.nodes), ptr(st.el), ptr(g_s), ptr(q2l_s),
+                         ptr(rho_s), 4, float(m_inf), float(m_crit),
+                         float(upwind_c), float(gamma_air), float(rho_floor),
+                         ptr(up), ptr(nu), ptr(rt), stream_handle())
+            kd = to_dev(keep.astype(np.uint8), torch.uint8).bool()
+            floored += int(((rt == rho_floor) & kd).sum().item())
+            nus.append(torch.where(kd, nu, torch.zeros_like(nu)))
+            out.append(rt)
+        self.nu_max = float(max(n.max().item() for n in nus))
+        act = (nus[0] > 0.0) | (nus[1] > 0.0)
+        self.n_nu_active = int(((nus[0] > 0.0).sum() + (nus[1] > 0.0).sum()).item())
+        self.n_floored = floored
+        self.n_limited = limited
+        self._nu_active = act
+        return out[0], out[1]
+
+    def nu_active_elements(self) -> np.ndarray:
+        if self._nu_active is None:
+            return np.zeros(0, dtype=np.int64)
+        return np.flatnonzero(to_host(self._nu_active))
+
+    def side_weights(self, rt_up_dev, rt_lo_dev):
+        """(element weights on the device for the plain part / the AMG,
+        host special-copy weights)."""
+        pp = getattr(self, "_d_plain_plus", None)
+        if pp is None:
+            pp = self._d_plain_plus = to_dev(self.plain_plus.astype(np.uint8),
+                                             torch.uint8).bool()
+        w = torch.where(pp, rt_up_dev, rt_lo_dev)
+        cm = self.cm
+        ru, rl = to_host(rt_up_dev), to_host(rt_lo_dev)
+        return w, (ru[cm.cut_elems], rl[cm.cut_elems], rl[cm.te_lower_elems])
+
+    def own_side_mach2_max(self, phi_ext, m_inf, gamma_air=1.4, u_inf=1.0):
+        """Largest own-side element M^2 (main field on plain elements, both
+        copies on cut elements, the lower copy on fan elements)."""
+        op, st = self.op, self.op.store
+        nt = st.n_tets
+        x = np.asarray(phi_ext, dtype=np.float64)
+        grad, q2 = op.velocities_dev(to_dev(x[: self.n_main]), empty((nt, 3)), empty(nt))
+        m2 = empty(nt)
+        q2n = q2 / u_inf ** 2
+        _native.call("fp3d_mach_squared_field", nt, ptr(q2n),
+                     float(m_inf), float(gamma_air), ptr(m2), stream_handle())
+        mask = torch.ones(nt, dtype=torch.bool, device=m2.device)
+        if len(self.special):
+            mask[to_dev(self.special, torch.int64)] = False
+        best = float(m2[mask].max().item()) if bool(mask.any()) else 0.0
+        cm = self.cm
+        for ids, dofs in ((cm.cut_elems, cm.dofs_upper), (cm.cut_elems, cm.dofs_lower),
+                          (cm.te_lower_elems, self.tel_dof)):
+            if len(ids):
+                g = np.einsum("ead,ea->ed", self.B[ids], x[dofs])
+                q = (g * g).sum(1) / u_inf ** 2
+                den = 1.0 + 0.5 * (gamma_air - 1.0) * m_inf ** 2 * (1.0 - q)
+                best = max(best, float((q * m_inf ** 2 / den).max()))
+        return best
 
     # ------------------------------------------------------- assembly
     def plain_weight_dev(self, rho_main_dev: Optional[torch.Tensor]):
