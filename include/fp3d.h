@@ -227,10 +227,12 @@ int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
                               const double *s_e, const uint8_t *lim,
                               double inv_u2, double clamp, double *K,
                               void *stream);
+/* rel8[16e + 4a + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]]
+ * (uint8: every row is at most 256 long), 4-byte aligned. */
 int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                           const int32_t *inc_ptr, const int32_t *inc_ent,
-                          const int32_t *elem_to_csr, const double *K,
-                          double *data, void *stream);
+                          const uint8_t *rel8, const double *K, double *data,
+                          void *stream);
 /* pyfp3d/kernels/jacobian.py:183-228 fill_term3_coo (rows/cols int32). */
 int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
                         const int32_t *elements, const double *B,

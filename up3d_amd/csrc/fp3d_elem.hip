@@ -797,10 +797,13 @@ __global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
     o[i] = make_double4(k[4 * i], k[4 * i + 1], k[4 * i + 2], k[4 * i + 3]);
 }
 
+// rel8[16e + 4a + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]]: the
+// slot's offset inside its row (rows are at most 256 long), read as one
+// 32-bit word per incidence entry -- a quarter of the elem_to_csr bytes.
 __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
     int64_t nn, int64_t nb_pad, int max_seg, const int32_t *__restrict__ indptr,
     const int32_t *__restrict__ iptr, const int32_t *__restrict__ ient,
-    const int32_t *__restrict__ e2c, const double *__restrict__ K,
+    const uint32_t *__restrict__ rel8, const double *__restrict__ K,
     double *__restrict__ data) {
   extern __shared__ double seg[];
   const int64_t r0 = xcd_block(blockIdx.x, nb_pad) * kGatherRows;
@@ -814,25 +817,26 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
   const int64_t v = r0 + threadIdx.x;
   if (v < r1) {
     constexpr int U = 4;
+    const int32_t rb = indptr[v] - base;
     const int32_t q1 = iptr[v + 1];
     for (int32_t q = iptr[v]; q < q1; q += U) {
-      int4 sl[U];
+      uint32_t of[U];
       double4 kv[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         if (q + k < q1) {
           const int64_t ent = ient[q + k];   // 4 e + a: row a of element e
-          sl[k] = *reinterpret_cast<const int4 *>(e2c + 4 * ent);
+          of[k] = rel8[ent];
           kv[k] = *reinterpret_cast<const double4 *>(K + 4 * ent);
         }
       }
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         if (q + k < q1) {
-          seg[sl[k].x - base] += kv[k].x;
-          seg[sl[k].y - base] += kv[k].y;
-          seg[sl[k].z - base] += kv[k].z;
-          seg[sl[k].w - base] += kv[k].w;
+          seg[rb + (int)(of[k] & 0xffu)] += kv[k].x;
+          seg[rb + (int)((of[k] >> 8) & 0xffu)] += kv[k].y;
+          seg[rb + (int)((of[k] >> 16) & 0xffu)] += kv[k].z;
+          seg[rb + (int)(of[k] >> 24)] += kv[k].w;
         }
       }
     }
@@ -1259,13 +1263,13 @@ int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
 
 int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                           const int32_t *inc_ptr, const int32_t *inc_ent,
-                          const int32_t *elem_to_csr, const double *K,
-                          double *data, void *stream) {
+                          const uint8_t *rel8, const double *K, double *data,
+                          void *stream) {
   FP3D_REQUIRE(n_nodes >= 0, "gather_csr_terms: n_nodes < 0");
   if (n_nodes == 0) return FP3D_OK;
-  FP3D_REQUIRE(indptr && inc_ptr && inc_ent && elem_to_csr && K && data,
+  FP3D_REQUIRE(indptr && inc_ptr && inc_ent && rel8 && K && data,
                "gather_csr_terms: NULL pointer");
-  FP3D_REQUIRE(FP3D_ALIGNED16(elem_to_csr), "elem_to_csr must be 16-byte aligned");
+  FP3D_REQUIRE(((uintptr_t)rel8 & 3) == 0, "rel8 must be 4-byte aligned");
   FP3D_REQUIRE(((uintptr_t)K & 31) == 0, "K must be 32-byte aligned");
   FP3D_REQUIRE(max_seg > 0 && (int64_t)max_seg * 8 <= 65536,
                "gather_csr_terms: max_seg must lie in (0, 8192]");
@@ -1274,7 +1278,7 @@ int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
   hipLaunchKernelGGL(k_gather_csr, dim3(g), dim3(kGatherRows),
                      (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
                      n_nodes, (int64_t)g, max_seg, indptr, inc_ptr, inc_ent,
-                     elem_to_csr, K, data);
+                     (const uint32_t *)rel8, K, data);
   return check_launch("gather_csr_terms");
 }
 

@@ -187,7 +187,7 @@ class PicardOperator:
                      float(clamp), ptr(self.d_K), stream_handle())
         _native.call("fp3d_gather_csr_terms", self.n_nodes, st.gather_max_seg,
                      ptr(st.indptr), ptr(st.inc_ptr), ptr(st.inc_ent),
-                     ptr(st.e2c), ptr(self.d_K), ptr(out), stream_handle())
+                     ptr(st.rel8), ptr(self.d_K), ptr(out), stream_handle())
         return out
 
     def matrix_dev(self, rho_tilde: torch.Tensor, out: torch.Tensor) -> torch.Tensor:

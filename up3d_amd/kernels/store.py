@@ -16,6 +16,7 @@ Device layout (element e, node v):
   B        f64   (n_tets, 12)    shape gradients, row-major (i, d)
   V        f64   (n_tets,)
   e2c      int32 (n_tets, 16)    CSR slot of (elements[e,i], elements[e,j])
+  rel8     uint8 (n_tets, 16)    that slot's offset inside row elements[e,i]
   fn       int32 (n_tets, 4)     face neighbours, -1 on the boundary
   cen      f64   (n_tets, 3)     centroids
   dcen     f64   (n_tets, 12)    centroid(fn[e,f]) - centroid(e), per face
@@ -138,6 +139,12 @@ class MeshStore:
         self.B = to_dev(B.reshape(self.n_tets, 12), torch.float64)
         self.V = to_dev(V, torch.float64)
         self.e2c = to_dev(e2c.reshape(self.n_tets, 16), torch.int32)
+        # slot offset inside its row, one byte per (e, a, b)
+        rel = e2c.reshape(self.n_tets, 4, 4) - indptr[el][:, :, None]
+        if self.n_tets and (rel.min() < 0 or rel.max() > 255):
+            raise ValueError("a CSR row is longer than 256 entries")
+        self.rel8 = to_dev(np.ascontiguousarray(rel.astype(np.uint8)).reshape(self.n_tets, 16),
+                           torch.uint8)
         self.fn = to_dev(fn.astype(np.int32), torch.int32)
         self.cen = to_dev(cen, torch.float64)
         # per-face centroid displacement (0 on the boundary)

@@ -296,7 +296,8 @@ class MultivaluedOperator:
         lim = empty(nt, torch.uint8)
         op.state_dev(to_dev(x[: self.n_main]), u_inf ** 2, m_inf, gamma_air,
                      q2cap, grad, q2n, q2l, rho, lim)
-        lim_h = to_host(lim).astype(bool)
+        # the state sweep's lim is 1 where the speed limiter is INACTIVE
+        limited_h = ~to_host(lim).astype(bool)
         (fu, du), (fl, dl) = self._side_graphs()
         sides = ((cm.cut_elems, cm.dofs_upper, fu, du, self.in_upper),
                  (np.concatenate([cm.cut_elems, cm.te_lower_elems]),
@@ -315,12 +316,10 @@ class MultivaluedOperator:
                 g_s[idd] = to_dev(g)
                 q2l_s[idd] = to_dev(ql)
                 rho_s[idd] = to_dev(isentropic_density(ql, m_inf, gamma_air))
-                special_lim = np.zeros(nt, dtype=bool)
-                special_lim[ids] = q2 > q2cap
-                lim_side = lim_h.copy()
-                lim_side[ids] = special_lim[ids]
+                lim_side = limited_h.copy()
+                lim_side[ids] = q2 > q2cap
             else:
-                lim_side = lim_h
+                lim_side = limited_h
             limited += int(np.count_nonzero(lim_side & keep))
             up = empty(nt, torch.int32)
             nu = empty(nt)
