@@ -63,6 +63,14 @@ def main():
                                         "conforming": [0.1175, 0.1200, 0.1202],
                                         "b3_mass_row": [0.2074, 0.1760, 0.1704],
                                         "levels": ["coarse", "medium", "fine"]}},
+        # level-set wake, NACA0012 2.5D, alpha 1.25 deg, per-side upwinding
+        # C = 1.5, vortex far field, converged circulation (the reference's
+        # measured table: M0.60 upwind on/off, then the M0.65-0.75 coarse and
+        # M0.70 medium LS column)
+        "ls_gamma_naca_a125": {"source": "docs/design_track_b.md:799,873-878",
+                               "values": {"coarse": {"0.60": 0.0968, "0.65": 0.1043,
+                                                     "0.70": 0.1158, "0.75": 0.1391},
+                                          "medium": {"0.70": 0.1193}}},
     }
     (HERE / "golden.json").write_text(json.dumps(out, indent=1))
 

@@ -276,14 +276,16 @@ class MultivaluedOperator:
             self._graphs = tuple(out)
         return self._graphs
 
-    def side_rho_tilde_dev(self, phi_ext, m_inf, upwind_c, m_crit,
-                           gamma_air=1.4, u_inf=1.0, m_cap=3.0,
-                           rho_floor=0.05):
-        """(rho_tilde_upper, rho_tilde_lower) device (n_tets,) arrays: each
-        side's artificial density from its own field (main field on plain
-        elements, the side copy on cut / fan elements), speed-limited, walked
-        on that side's restricted face graph
-        (pyfp3d/wake/multivalued.py:642-716).  Own-side monitors refreshed."""
+    def side_state_dev(self, phi_ext, m_inf, upwind_c, m_crit,
+                       gamma_air=1.4, u_inf=1.0, m_cap=3.0, rho_floor=0.05,
+                       frozen=None, sensitivities=False):
+        """Per-side upwinded state (pyfp3d/wake/multivalued.py:642-780):
+        for each side a dict of device (n_tets,) arrays -- grad, q2l, rho,
+        lim (1 = speed limiter inactive), upstream, rho_tilde and, with
+        sensitivities=True, s_e / s_u -- from that side's field (main field
+        on plain elements, the side copy on cut / fan elements), walked on
+        the side's restricted face graph, or at a frozen per-side (upstream,
+        branch) selection.  Own-side monitors refreshed."""
         op, st = self.op, self.op.store
         cm = self.cm
         nt = st.n_tets
@@ -296,18 +298,15 @@ class MultivaluedOperator:
         lim = empty(nt, torch.uint8)
         op.state_dev(to_dev(x[: self.n_main]), u_inf ** 2, m_inf, gamma_air,
                      q2cap, grad, q2n, q2l, rho, lim)
-        # the state sweep's lim is 1 where the speed limiter is INACTIVE
-        limited_h = ~to_host(lim).astype(bool)
         (fu, du), (fl, dl) = self._side_graphs()
         sides = ((cm.cut_elems, cm.dofs_upper, fu, du, self.in_upper),
                  (np.concatenate([cm.cut_elems, cm.te_lower_elems]),
                   np.concatenate([cm.dofs_lower, self.tel_dof]), fl, dl,
                   self.in_lower))
-        out, nus = [], []
-        limited = 0
-        floored = 0
-        for ids, dofs, f, d, keep in sides:
-            g_s, q2l_s, rho_s = grad.clone(), q2l.clone(), rho.clone()
+        out = []
+        limited = floored = 0
+        for k, (ids, dofs, f, d, keep) in enumerate(sides):
+            g_s, q2l_s, rho_s, lim_s = grad.clone(), q2l.clone(), rho.clone(), lim.clone()
             if len(ids):
                 g = np.einsum("ead,ea->ed", self.B[ids], x[dofs])
                 q2 = (g * g).sum(1) / u_inf ** 2
@@ -316,30 +315,89 @@ class MultivaluedOperator:
                 g_s[idd] = to_dev(g)
                 q2l_s[idd] = to_dev(ql)
                 rho_s[idd] = to_dev(isentropic_density(ql, m_inf, gamma_air))
-                lim_side = limited_h.copy()
-                lim_side[ids] = q2 > q2cap
-            else:
-                lim_side = limited_h
-            limited += int(np.count_nonzero(lim_side & keep))
-            up = empty(nt, torch.int32)
+                lim_s[idd] = to_dev((q2 <= q2cap).astype(np.uint8), torch.uint8)
+            kd = to_dev(keep.astype(np.uint8), torch.uint8).bool()
+            limited += int(((lim_s == 0) & kd).sum().item())
             nu = empty(nt)
             rt = empty(nt)
-            _native.call("fp3d_upwind_rho_tilde_d", nt, ptr(f), ptr(d),
-                         ptr(st.nodes), ptr(st.el), ptr(g_s), ptr(q2l_s),
-                         ptr(rho_s), 4, float(m_inf), float(m_crit),
-                         float(upwind_c), float(gamma_air), float(rho_floor),
-                         ptr(up), ptr(nu), ptr(rt), stream_handle())
-            kd = to_dev(keep.astype(np.uint8), torch.uint8).bool()
-            floored += int(((rt == rho_floor) & kd).sum().item())
-            nus.append(torch.where(kd, nu, torch.zeros_like(nu)))
-            out.append(rt)
+            if frozen is None:
+                up = empty(nt, torch.int32)
+                _native.call("fp3d_upwind_rho_tilde_d", nt, ptr(f), ptr(d),
+                             ptr(st.nodes), ptr(st.el), ptr(g_s), ptr(q2l_s),
+                             ptr(rho_s), 4, float(m_inf), float(m_crit),
+                             float(upwind_c), float(gamma_air), float(rho_floor),
+                             ptr(up), ptr(nu), ptr(rt), stream_handle())
+                floored += int(((rt == rho_floor) & kd).sum().item())
+            else:
+                up, br = frozen[k]
+                _native.call("fp3d_rho_tilde_frozen_sweep", nt, ptr(q2l_s),
+                             ptr(rho_s), ptr(up), ptr(br), float(m_inf),
+                             float(m_crit), float(upwind_c), float(gamma_air),
+                             float(rho_floor), ptr(nu), ptr(rt), stream_handle())
+                floored += int(((br == 3) & kd).sum().item())
+            side = {"grad": g_s, "q2l": q2l_s, "rho": rho_s, "lim": lim_s,
+                    "upstream": up, "rho_tilde": rt, "nu": nu, "keep": kd}
+            if sensitivities:
+                se, su = empty(nt), empty(nt)
+                if frozen is None:
+                    _native.call("fp3d_rho_tilde_sensitivities", nt, ptr(q2l_s),
+                                 ptr(rho_s), ptr(up), float(m_inf), float(m_crit),
+                                 float(upwind_c), float(gamma_air), float(rho_floor),
+                                 ptr(se), ptr(su), stream_handle())
+                else:
+                    _native.call("fp3d_rho_tilde_frozen_sensitivities", nt,
+                                 ptr(q2l_s), ptr(rho_s), ptr(up), ptr(frozen[k][1]),
+                                 float(m_inf), float(m_crit), float(upwind_c),
+                                 float(gamma_air), float(rho_floor), ptr(se),
+                                 ptr(su), stream_handle())
+                side["s_e"], side["s_u"] = se, su
+            out.append(side)
+        nus = [torch.where(sd["keep"], sd["nu"], torch.zeros_like(sd["nu"])) for sd in out]
         self.nu_max = float(max(n.max().item() for n in nus))
-        act = (nus[0] > 0.0) | (nus[1] > 0.0)
         self.n_nu_active = int(((nus[0] > 0.0).sum() + (nus[1] > 0.0).sum()).item())
         self.n_floored = floored
         self.n_limited = limited
-        self._nu_active = act
-        return out[0], out[1]
+        self._nu_active = (nus[0] > 0.0) | (nus[1] > 0.0)
+        return out
+
+    def freeze_side_state(self, phi_ext, m_inf, upwind_c, m_crit,
+                          gamma_air=1.4, u_inf=1.0, m_cap=3.0, rho_floor=0.05):
+        """Per-side (upstream, branch) at phi_ext (device), for a frozen-
+        selection Newton finish (pyfp3d/wake/multivalued.py:781-796)."""
+        sides = self.side_state_dev(phi_ext, m_inf, upwind_c, m_crit,
+                                    gamma_air, u_inf, m_cap, rho_floor)
+        out = []
+        for sd in sides:
+            br = empty(self.op.store.n_tets, torch.int8)
+            _native.call("fp3d_classify_upwind_branches", self.op.store.n_tets,
+                         ptr(sd["q2l"]), ptr(sd["rho"]), ptr(sd["upstream"]),
+                         float(m_inf), float(m_crit), float(upwind_c),
+                         float(gamma_air), float(rho_floor), ptr(br),
+                         stream_handle())
+            out.append((sd["upstream"].clone(), br))
+        return tuple(out)
+
+    def side_rho_tilde_dev(self, phi_ext, m_inf, upwind_c, m_crit,
+                           gamma_air=1.4, u_inf=1.0, m_cap=3.0,
+                           rho_floor=0.05):
+        """(rho_tilde_upper, rho_tilde_lower) device arrays (live walk)."""
+        sides = self.side_state_dev(phi_ext, m_inf, upwind_c, m_crit, gamma_air,
+                                    u_inf, m_cap, rho_floor)
+        return sides[0]["rho_tilde"], sides[1]["rho_tilde"]
+
+    def read_dofs(self):
+        """(upper, lower) (n_tets, 4) DOF vectors each side's field reads:
+        main DOFs on plain elements, the side copy on cut elements, the fan's
+        lower copy (pyfp3d/wake/multivalued.py:606-660)."""
+        if getattr(self, "_read", None) is None:
+            cm = self.cm
+            ru = self.el.copy()
+            ru[cm.cut_elems] = cm.dofs_upper
+            rl = self.el.copy()
+            rl[cm.cut_elems] = cm.dofs_lower
+            rl[cm.te_lower_elems] = self.tel_dof
+            self._read = (ru, rl)
+        return self._read
 
     def nu_active_elements(self) -> np.ndarray:
         if self._nu_active is None:
