@@ -126,3 +126,49 @@ def test_ls_newton_medium_m070(gpu):
     assert r["target_reached"] and r["n_limited"] == 0 and r["n_floored"] == 0
     ref = _gold()["medium"]["0.70"]
     assert abs(r["gamma"] - ref) <= 0.01 * ref
+
+
+def test_schur_reduced_system_solves_fused_matrix(gpu):
+    """SchurReducedSystem on the free-reduced fused level-set matrix equals a
+    direct solve; the aux rows hold exactly by back-substitution
+    (tests/test_b14_schur_ls.py's honesty identity)."""
+    import scipy.sparse.linalg as spla
+
+    from up3d_amd.constraints.dirichlet import freestream_phi
+    from up3d_amd.solve.schur_ls import SchurReducedSystem, main_block_preconditioner
+
+    mesh = _mesh("coarse")
+    _, mv = _setup(mesh)
+    phi = np.zeros(mv.n_total)
+    phi[: mv.n_main] = freestream_phi(mesh.nodes, 2.0)
+    phi[mv.cut_aux] = phi[mv.cut_nodes] - 0.1 * mv.cm.node_side[mv.cut_nodes]
+    E = mv.assemble_matrix(closure="wake_ls", te_kutta="pressure", phi_ext=phi)
+    ff = np.unique(mesh.boundary_faces["farfield"])
+    free = np.setdiff1d(np.arange(mv.n_total), ff)
+    A = E[free][:, free].tocsr()
+    b = np.random.default_rng(3).standard_normal(len(free))
+    S = SchurReducedSystem(A, free, mv.n_main, n_aux_expected=mv.n_ext)
+    M = main_block_preconditioner(mv, None, S.main_free)
+    x, n_it, info = S.solve(b, M=M, rtol=1e-12, restart=60, maxiter=40)
+    xd = spla.spsolve(A.tocsc(), b)
+    print(f"[wake-newton] schur: {n_it} GMRES iterations, |x - direct| = "
+          f"{np.abs(x - xd).max():.2e}", flush=True)
+    assert info == 0
+    assert np.abs(x - xd).max() <= 1e-8 * np.abs(xd).max()
+    r = A @ x - b
+    assert np.abs(r[S.n_mf:]).max() <= 1e-10 * np.abs(b).max()
+    with pytest.raises(ValueError):
+        SchurReducedSystem(A, free, mv.n_main, n_aux_expected=mv.n_ext + 1)
+
+
+def test_ls_newton_schur_mode(gpu):
+    from up3d_amd.solve.newton_ls import solve_multivalued_newton
+
+    mesh = _mesh("coarse")
+    _, mv = _setup(mesh)
+    r = solve_multivalued_newton(mv, mesh, 0.70, n_seed=40, freeze_tol=1e-6,
+                                 linear="schur")
+    print(f"[wake-newton] schur M0.70: gamma={r['gamma']:.8f} n={r['n_newton']} "
+          f"gmres={r['n_gmres_total']}", flush=True)
+    assert r["converged"] and r["n_gmres_stalled"] == 0
+    assert abs(r["gamma"] - _gold()["coarse"]["0.70"]) <= 0.5e-4

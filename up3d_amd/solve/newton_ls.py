@@ -212,10 +212,12 @@ class LSNewtonSystem:
         Dm = sp.diags(fm.astype(np.float64))
         Jc = (Dm @ Jc @ Dm).tocsr()
         Jc.eliminate_zeros()
+        self.Jc_host = Jc
         self.Jc = DeviceCSR(Jc)
         self.Jc_am = DeviceCSR(Jc[nm:, :nm].tocsr()) if nT > nm else None
         self.aux_lu = spla.splu(Jc[nm:, nm:].tocsc()) if nT > nm else None
         self._t_aux = empty(max(nT - nm, 1))
+        self._schur = None
         # AMG of the single-valued Picard matrix at the current densities
         if self._amg is None or self._amg_age >= 3:
             A_sv = empty(op.nnz)
@@ -236,6 +238,66 @@ class LSNewtonSystem:
                      ptr(J.indices), ptr(J.data), 0, ptr(x), ptr(y),
                      stream_handle())
         return y
+
+    # ---------------------------------------------------- linear step
+    def _schur_parts(self):
+        if self._schur is None:
+            nm = self.nm
+            Jc = self.Jc_host
+            self._schur = (DeviceCSR(Jc[:nm, :nm].tocsr()), DeviceCSR(Jc[:nm, nm:].tocsr()),
+                           empty(nm))
+        return self._schur
+
+    def schur_matvec(self, x, y):
+        """K x = J_mm x - J_ma J_aa^{-1} J_am x on main-length vectors
+        (pyfp3d/solve/schur_ls.py:95-103)."""
+        nm = self.nm
+        cmm, cma, t = self._schur_parts()
+        self.nop.matvec(x, y)
+        _native.call("fp3d_amg_prolong_add", cmm.n, cmm.nnz, ptr(cmm.indptr),
+                     ptr(cmm.indices), ptr(cmm.data), 0, ptr(x), ptr(y),
+                     stream_handle())
+        if self.nT > nm:
+            C = self.Jc_am
+            _native.call("fp3d_csr_spmv", C.n, C.nnz, ptr(C.indptr), ptr(C.indices),
+                         ptr(C.data), ptr(x), ptr(self._t_aux), stream_handle())
+            v = to_dev(self.aux_lu.solve(to_host(self._t_aux)))
+            _native.call("fp3d_csr_spmv", cma.n, cma.nnz, ptr(cma.indptr),
+                         ptr(cma.indices), ptr(cma.data), ptr(v), ptr(t),
+                         stream_handle())
+            y.sub_(t)
+        return y
+
+    def solve_step(self, rhs, rtol, restart, maxiter, mode="block"):
+        """(d host (n_total,), info, n_inner) for J d = rhs.  mode 'block':
+        GMRES on the whole system with the block preconditioner; 'schur':
+        GMRES on the aux-eliminated main system, aux by back-substitution."""
+        nm = self.nm
+        if mode == "block":
+            d, info, n_it = gmres_dev(self.matvec, to_dev(rhs), self.psolve,
+                                      rtol=rtol, restart=restart, maxiter=maxiter)
+            return to_host(d), info, n_it
+        if mode != "schur":
+            raise ValueError(f"linear mode {mode!r} unknown")
+        b_a = rhs[nm:]
+        r = rhs[:nm].copy()
+        cmm, cma, t = self._schur_parts()
+        if self.nT > nm:
+            v = to_dev(self.aux_lu.solve(b_a))
+            _native.call("fp3d_csr_spmv", cma.n, cma.nnz, ptr(cma.indptr),
+                         ptr(cma.indices), ptr(cma.data), ptr(v), ptr(t),
+                         stream_handle())
+            r -= to_host(t)
+
+        def prec(a, b):
+            return self._amg.apply(a.contiguous(), b)
+        x_m, info, n_it = gmres_dev(self.schur_matvec, to_dev(r), prec, rtol=rtol,
+                                    restart=restart, maxiter=maxiter)
+        x_m = to_host(x_m)
+        x_a = np.zeros(self.nT - nm)
+        if self.nT > nm:
+            x_a = self.aux_lu.solve(b_a - self.Jc_host[nm:, :nm] @ x_m)
+        return np.concatenate([x_m, x_a]), info, n_it
 
     def psolve(self, r, z):
         nm = self.nm
@@ -264,12 +326,14 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
                              freeze_refresh_max: int = 8,
                              freeze_max_reverts: int = 3, stall_window: int = 8,
                              linear_rtol: float = 1e-10, gmres_restart: int = 60,
-                             gmres_maxiter: int = 50,
+                             gmres_maxiter: int = 50, linear: str = "block",
                              verbose: bool = False) -> Dict[str, object]:
     """Level-set lifting Newton with a Picard seed, safety line search on
     ||R_free||_inf (halving down to lam_min, best-of on failure) and the
     freeze / honesty / refresh logic of the reference driver; the far-field
-    vortex takes the current circulation at every step.  Returns the
+    vortex takes the current circulation at every step.  linear='block'
+    solves each step on the whole extended system (AMG main block + exact
+    aux block), 'schur' on the aux-eliminated main system.  Returns the
     reference's result keys (phi_ext, phi, gamma, cl_kj, te_jump, converged,
     residual_history, n_newton, n_limited, n_floored, nu_max, mach2_max,
     n_gmres_total, n_gmres_stalled, accept_reason, froze, n_freeze_refresh,
@@ -377,12 +441,10 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
             sysm.linearize(phi, sides, C, w, ff)
         with phase(timings, "linsolve"):
             rhs = np.where(free, -R, 0.0)
-            d, info, n_it = gmres_dev(sysm.matvec, to_dev(rhs), sysm.psolve,
-                                      rtol=linear_rtol, restart=gmres_restart,
-                                      maxiter=gmres_maxiter)
+            d, info, n_it = sysm.solve_step(rhs, linear_rtol, gmres_restart,
+                                            gmres_maxiter, linear)
             n_gmres_total += n_it
             n_gmres_stalled += int(info != 0)
-            d = to_host(d)
         best = None
         lam = 1.0
         with phase(timings, "residual"):
