@@ -44,11 +44,12 @@ from .timing import finalize, new_timings, phase
 
 class LSNewtonSystem:
     def __init__(self, mvop, m_inf, upwind_c=1.5, m_crit=0.95, gamma_air=1.4,
-                 u_inf=1.0, m_cap=3.0, rho_floor=0.05):
+                 u_inf=1.0, m_cap=3.0, rho_floor=0.05, external_rhs=None):
         self.mv = mvop
         self.m_inf, self.upwind_c, self.m_crit = float(m_inf), float(upwind_c), float(m_crit)
         self.gamma_air, self.u_inf = float(gamma_air), float(u_inf)
         self.m_cap, self.rho_floor = float(m_cap), float(rho_floor)
+        self.external_rhs = external_rhs
         self.nm, self.nT = mvop.n_main, mvop.n_total
         cm = mvop.cm
         self.te_aux = cm.ext_dof_of_node[cm.te_nodes].astype(np.int64)
@@ -95,6 +96,8 @@ class LSNewtonSystem:
         R = np.zeros(self.nT)
         R[: self.nm] = to_host(y)
         R += C @ phi
+        if self.external_rhs is not None:
+            R -= self.external_rhs
         R[self.te_aux] = self.te_residual(phi)
         return R, sides, C, w
 
@@ -199,6 +202,7 @@ class LSNewtonSystem:
         su = torch.where(pm, up_s["s_u"], lo_s["s_u"]) * plain
         su = su * self.is_plain_dev[up.long()].double()
         self.nop.build_term3(grad, up, su, lim, inv_u2)
+        self.t3_inputs = (grad, up, su * lim[up.long()].double(), inv_u2)
         # host correction: D_keep C + exact Kutta + Terms 2/3 (special)
         keep = np.ones(nT)
         keep[self.te_aux] = 0.0
@@ -316,7 +320,8 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
                              u_inf: float = 1.0, gamma_air: float = 1.4,
                              upwind_c: float = 1.5, m_crit: float = 0.95,
                              m_cap: float = 3.0, rho_floor: float = 0.05,
-                             farfield: str = "vortex", vortex_center=(0.25, 0.0),
+                             farfield: str = "vortex", lift_axis: int = 1,
+                             vortex_center=(0.25, 0.0),
                              phi_init: Optional[np.ndarray] = None,
                              gamma_init: float = 0.0, n_seed: int = 40,
                              seed_damping_theta: Optional[float] = 0.2,
@@ -327,20 +332,22 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
                              freeze_max_reverts: int = 3, stall_window: int = 8,
                              linear_rtol: float = 1e-10, gmres_restart: int = 60,
                              gmres_maxiter: int = 50, linear: str = "block",
+                             external_rhs: Optional[np.ndarray] = None,
                              verbose: bool = False) -> Dict[str, object]:
     """Level-set lifting Newton with a Picard seed, safety line search on
     ||R_free||_inf (halving down to lam_min, best-of on failure) and the
     freeze / honesty / refresh logic of the reference driver; the far-field
     vortex takes the current circulation at every step.  linear='block'
     solves each step on the whole extended system (AMG main block + exact
-    aux block), 'schur' on the aux-eliminated main system.  Returns the
+    aux block), 'schur' on the aux-eliminated main system.  external_rhs
+    (n_total): a boundary load (R = E phi - f), e.g. a Neumann outflow.  Returns the
     reference's result keys (phi_ext, phi, gamma, cl_kj, te_jump, converged,
     residual_history, n_newton, n_limited, n_floored, nu_max, mach2_max,
     n_gmres_total, n_gmres_stalled, accept_reason, froze, n_freeze_refresh,
     n_freeze_reverts, residual_unfrozen, n_assignment_stale,
     assignment_cycle, timings)."""
-    if farfield != "vortex":
-        raise NotImplementedError("only the vortex far field is built")
+    if farfield not in ("vortex", "freestream"):
+        raise NotImplementedError(f"farfield={farfield!r}: 'vortex' or 'freestream'")
     t_wall0 = time.perf_counter()
     timings = new_timings()
     t0 = time.perf_counter()
@@ -348,14 +355,15 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
         mvop, mesh, m_inf, alpha_deg=alpha_deg, u_inf=u_inf, gamma_air=gamma_air,
         upwind_c=upwind_c, m_crit=m_crit, damping_theta=seed_damping_theta,
         omega_rho=seed_omega_rho, n_outer_max=n_seed, gamma_init=gamma_init,
-        phi_init=phi_init, vortex_center=vortex_center)
+        phi_init=phi_init, vortex_center=vortex_center, farfield=farfield,
+        lift_axis=lift_axis, external_rhs=external_rhs)
     phi = seed["phi_ext"].copy()
     gamma = float(seed["gamma"])
     timings["seed"] = time.perf_counter() - t0
     beta = float(np.sqrt(1.0 - m_inf ** 2))
     nodes = np.asarray(mesh.nodes, dtype=np.float64)
     sysm = LSNewtonSystem(mvop, m_inf, upwind_c, m_crit, gamma_air, u_inf,
-                          m_cap, rho_floor)
+                          m_cap, rho_floor, external_rhs)
     residual_history = []
     converged = False
     n_gmres_total = n_gmres_stalled = 0
@@ -372,7 +380,8 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
     ff = None
     for it in range(n_newton_max):
         ff, vals = _farfield_main(nodes, mesh.boundary_faces["farfield"], alpha_deg,
-                                  gamma, u_inf, vortex_center, beta)
+                                  gamma, u_inf, vortex_center, beta, farfield,
+                                  lift_axis)
         phi[ff] = vals
         free = np.ones(mvop.n_total, dtype=bool)
         free[ff] = False

@@ -44,7 +44,8 @@ class ExtendedSystem:
 
     def __init__(self, mvop, rho_main_dev, special_rho, phi_ext, dir_dofs,
                  dir_vals, closure="wake_ls", te_kutta="pressure",
-                 amg=None, damping_theta=None, damp_rows=None, x_prev=None):
+                 amg=None, damping_theta=None, damp_rows=None, x_prev=None,
+                 external_rhs=None):
         op = mvop.op
         st = op.store
         self.mvop = mvop
@@ -88,6 +89,8 @@ class ExtendedSystem:
         y[nm:].zero_()
         yh = to_host(y) + C @ xD
         b = -yh
+        if external_rhs is not None:
+            b += external_rhs
         if d_damp is not None:
             # (E + D) x = b + D x_prev; D vanishes on the Dirichlet rows
             b += d_damp * np.asarray(x_prev)
@@ -143,11 +146,30 @@ class ExtendedSystem:
         return z
 
 
-def _farfield_main(nodes, farfield_faces, alpha_deg, gamma, u_inf, center, beta):
+def _freestream(points, alpha_deg, u_inf, lift_axis=1):
+    """U (x cos(alpha) + x_l sin(alpha)), x_l the coordinate along
+    `lift_axis` (1: the reference's x-y convention, 2: x-z)."""
+    if lift_axis == 1:
+        return freestream_phi(points, alpha_deg, u_inf)
+    a = np.deg2rad(alpha_deg)
+    return u_inf * (points[:, 0] * np.cos(a) + points[:, lift_axis] * np.sin(a))
+
+
+def _farfield_main(nodes, farfield_faces, alpha_deg, gamma, u_inf, center, beta,
+                   farfield="vortex", lift_axis=1):
+    """(far-field main DOFs, Dirichlet values): freestream plus, for
+    farfield='vortex', the Prandtl-Glauert 2-D vortex of circulation gamma
+    (x-y plane only)."""
     ff = np.unique(np.asarray(farfield_faces))
     xy = nodes[ff]
-    return ff, freestream_phi(xy, alpha_deg, u_inf) + vortex_phi_2d(
-        xy, gamma, center, beta=beta)
+    vals = _freestream(xy, alpha_deg, u_inf, lift_axis)
+    if farfield == "vortex":
+        if lift_axis != 1:
+            raise ValueError("the vortex far field is the x-y plane's (lift_axis=1)")
+        vals = vals + vortex_phi_2d(xy, gamma, center, beta=beta)
+    elif farfield != "freestream":
+        raise NotImplementedError(f"farfield={farfield!r}: 'vortex' or 'freestream'")
+    return ff, vals
 
 
 def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
@@ -158,7 +180,7 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
                               n_outer_max: int = 80, gamma_init: float = 0.0,
                               gamma_farfield_fixed: Optional[float] = None,
                               te_kutta: str = "pressure",
-                              farfield: str = "vortex",
+                              farfield: str = "vortex", lift_axis: int = 1,
                               upwind_c: float = 0.0, m_crit: float = 0.95,
                               m_cap: float = 3.0, rho_floor: float = 0.05,
                               damping_theta: Optional[float] = None,
@@ -167,8 +189,12 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
                               linear_rtol: float = 1e-10,
                               gmres_restart: int = 60, gmres_maxiter: int = 50,
                               amg_rebuild_every: int = 5,
+                              external_rhs: Optional[np.ndarray] = None,
                               verbose: bool = False) -> Dict[str, object]:
-    """`mesh` needs .nodes and .boundary_faces["farfield"].  upwind_c > 0
+    """`mesh` needs .nodes and .boundary_faces["farfield"]; farfield
+    'freestream' holds the far field at the freestream (no circulation
+    feedback), in the x-`lift_axis` plane; external_rhs (n_total) is a
+    boundary load on the right-hand side (a Neumann outflow).  upwind_c > 0
     (with m_inf > 0) runs the per-side artificial density with side-
     restricted walks, and damping_theta the theta*diag damping on the main
     rows of nu > 0 elements (the reference's supersonic-zone scope).
@@ -176,8 +202,8 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
     te_jump, n_outer, converged, gamma_history, drho_history,
     residual_history, residual_norm, timings, mach2_max, nu_max,
     n_nu_active, n_limited, n_floored, n_gmres_total, n_gmres_stalled."""
-    if farfield != "vortex":
-        raise NotImplementedError("only the vortex far field is built")
+    if farfield not in ("vortex", "freestream"):
+        raise NotImplementedError(f"farfield={farfield!r}: 'vortex' or 'freestream'")
     if not 0.0 <= m_inf < 1.0:
         raise ValueError(f"needs 0 <= M_inf < 1, got {m_inf}")
     if len(mvop.cm.te_nodes) == 0:
@@ -194,7 +220,7 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
     if phi_init is not None:
         phi_ext[:] = phi_init
     else:
-        phi_ext[: mvop.n_main] = freestream_phi(nodes, alpha_deg, u_inf)
+        phi_ext[: mvop.n_main] = _freestream(nodes, alpha_deg, u_inf, lift_axis)
         phi_ext[mvop.cut_aux] = phi_ext[mvop.cut_nodes]
     gamma = float(gamma_init)
     gamma_history, drho_history, residual_history = [gamma], [], []
@@ -223,12 +249,13 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
         with phase(timings, "assembly"):
             g_ff = gamma if gamma_farfield_fixed is None else gamma_farfield_fixed
             ff, vals = _farfield_main(nodes, ffaces, alpha_deg, g_ff, u_inf,
-                                      vortex_center, beta)
+                                      vortex_center, beta, farfield, lift_axis)
             reuse = amg if (amg is not None and outer % amg_rebuild_every) else None
             sysm = ExtendedSystem(mvop, w_dev, special, phi_ext, ff, vals,
                                   "wake_ls", te_kutta, amg=reuse,
                                   damping_theta=damping_theta,
-                                  damp_rows=damp_rows, x_prev=phi_ext)
+                                  damp_rows=damp_rows, x_prev=phi_ext,
+                                  external_rhs=external_rhs)
             amg = sysm.amg
             x_prev = to_dev(np.where(sysm.free, phi_ext, 0.0))
             x_prev[to_dev(ff, torch.int64)] = to_dev(vals)

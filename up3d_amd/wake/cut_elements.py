@@ -11,6 +11,9 @@ Rules (same as the reference):
     (0 <= q <= span_length [+ outboard_fringe]);
   * the below-TE fan (elements with a TE node whose other nodes are all "-")
     is never cut; it references the TE through its aux DOF;
+  * `minus_nodes` (no reference counterpart): nodes forced to the "-"
+    side, for the lower-face copies of a zero-thickness wall that lies on
+    the sheet plane upstream of the TE (a cracked embedded plate);
   * one aux DOF per distinct cut-element node, numbered n_main + k in
     ascending node order; dofs_upper = main at "+" nodes, aux at "-" nodes,
     dofs_lower the complement.
@@ -41,7 +44,8 @@ def node_min_edge_length(nodes, elements) -> np.ndarray:
 class CutElementMap:
     def __init__(self, nodes, elements, levelset, wall_nodes=None,
                  eps_rel: float = 1e-6, te_tol_rel: float = 1e-3,
-                 inboard_clip=None, outboard_fringe: float = 0.0):
+                 inboard_clip=None, outboard_fringe: float = 0.0,
+                 minus_nodes=None):
         nodes = np.asarray(nodes, dtype=np.float64)
         el = np.asarray(elements, dtype=np.int64)
         n = len(nodes)
@@ -67,6 +71,13 @@ class CutElementMap:
         shift = (np.abs(s_raw) < eps) | te
         s = s_raw.copy()
         s[shift] = eps[shift]
+        if minus_nodes is not None:
+            # lower-face copies of a cracked surface lying on the sheet
+            mn = np.asarray(minus_nodes, dtype=np.int64)
+            if np.any(te[mn]):
+                raise ValueError("a minus node is a TE node")
+            s[mn] = -np.maximum(np.abs(s_raw[mn]), eps[mn])
+            shift[mn] = True
         self.s = s
         self.shifted_nodes = np.flatnonzero(shift)
         self.node_side = np.where(s > 0.0, 1, -1).astype(np.int8)

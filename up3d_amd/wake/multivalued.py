@@ -161,8 +161,12 @@ class MultivaluedOperator:
             w[cm.wall_nodes] = True
             on_wall = w[el].sum(axis=1) >= 3
         cvs = []
+        is_te = np.zeros(self.n_main, dtype=bool)
+        is_te[cm.te_nodes] = True
+        near = np.flatnonzero(is_te[el].any(axis=1) & on_wall)
+        el_near = el[near]
         for t in cm.te_nodes:
-            fan = np.flatnonzero((el == t).any(axis=1) & on_wall)
+            fan = near[(el_near == t).any(axis=1)]
             ue, ud, le, ld = [], [], [], []
             for e in fan:
                 k = cut_index[e]
