@@ -236,22 +236,29 @@ inline int rblocks(int64_t n) {
   return (int)b;
 }
 
-// partial[k*nb + b] = sum over block b's rows of V_k . w.  Vectors are
-// taken kMdotChunk at a time so each thread keeps that many independent
-// loads in flight; per chunk: wave shuffles, one LDS exchange, fixed order.
-// cond != NULL makes the call a CGS2 re-orthogonalisation pass that runs
-// only when the first pass cancelled (DGKS: ||w'||^2 = ||w||^2 - ||h||^2 <
-// eta2 ||w||^2 with h = cond[0..nv), ||w||^2 = cond[nv]); otherwise the
-// partials are written as exact zeros.
-constexpr int kMdotChunk = 8;
+// partial[k*nb + b] = sum over block b's row slice of V_k . w.  Block b
+// owns the contiguous rows [b*seg, min(n, (b+1)*seg)); its slice of w is
+// staged once in LDS (when seg <= kMdotLds) and each wave streams one basis
+// vector's slice at a time (64 lanes x 8 B contiguous per load, four
+// independent accumulators per lane), so the only block-wide barrier is the
+// staging one.  Per (k, b) the sum is lane-strided then a fixed shuffle tree:
+// deterministic.  cond != NULL makes the call a CGS2 re-orthogonalisation
+// pass that runs only when the first pass cancelled (DGKS: ||w'||^2 =
+// ||w||^2 - ||h||^2 < eta2 ||w||^2 with h = cond[0..nv), ||w||^2 =
+// cond[nv]); otherwise the partials are written as exact zeros.
+constexpr int kMdotLds = 2048;
+constexpr int64_t kMdotSegMin = 1024;
+template <bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
                                                    const double *__restrict__ V,
                                                    int64_t ld,
                                                    const double *__restrict__ w,
                                                    double *__restrict__ part,
                                                    const double *__restrict__ cond,
-                                                   double eta2) {
-  __shared__ double sh[kMdotChunk][kBlock / 64];
+                                                   double eta2, int64_t seg) {
+  // STAGED reads the slice of w from LDS (a separate instantiation, so the
+  // loads stay in the LDS address space rather than generic)
+  __shared__ double ws[STAGED ? kMdotLds : 1];
   const int nb = gridDim.x;
   if (cond) {
     double h2 = 0.0;
@@ -263,31 +270,45 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
       return;
     }
   }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int k0 = 0; k0 < nv; k0 += kMdotChunk) {
-    double acc[kMdotChunk];
-#pragma unroll
-    for (int c = 0; c < kMdotChunk; ++c) acc[c] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)nb * blockDim.x) {
-      const double wi = w[i];
-#pragma unroll
-      for (int c = 0; c < kMdotChunk; ++c)
-        if (k0 + c < nv) acc[c] += V[(int64_t)(k0 + c) * ld + i] * wi;
-    }
-#pragma unroll
-    for (int c = 0; c < kMdotChunk; ++c) {
-      double v = wsum(acc[c]);
-      if (lane == 0) sh[c][wave] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kMdotChunk && k0 + (int)threadIdx.x < nv) {
-      double t = 0.0;
-      for (int q = 0; q < kBlock / 64; ++q) t += sh[threadIdx.x][q];
-      part[(int64_t)(k0 + threadIdx.x) * nb + blockIdx.x] = t;
-    }
+  const int64_t r0 = (int64_t)blockIdx.x * seg;
+  const int64_t r1 = r0 + seg < n ? r0 + seg : n;
+  const int len = (int)(r1 - r0);
+  if (STAGED) {
+    for (int i = threadIdx.x; i < len; i += blockDim.x) ws[i] = w[r0 + i];
     __syncthreads();
   }
+  const double *__restrict__ wg = w + r0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nwave = blockDim.x >> 6;
+  for (int k = wave; k < nv; k += nwave) {
+    const double *__restrict__ v = V + (int64_t)k * ld + r0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = lane;
+    for (; i + 192 < len; i += 256) {
+      if (STAGED) {
+        a0 += v[i] * ws[i];
+        a1 += v[i + 64] * ws[i + 64];
+        a2 += v[i + 128] * ws[i + 128];
+        a3 += v[i + 192] * ws[i + 192];
+      } else {
+        a0 += v[i] * wg[i];
+        a1 += v[i + 64] * wg[i + 64];
+        a2 += v[i + 128] * wg[i + 128];
+        a3 += v[i + 192] * wg[i + 192];
+      }
+    }
+    for (; i < len; i += 64) a0 += v[i] * (STAGED ? ws[i] : wg[i]);
+    double t = wsum((a0 + a1) + (a2 + a3));
+    if (lane == 0) part[(int64_t)k * nb + blockIdx.x] = t;
+  }
+}
+inline int64_t mdot_seg(int64_t n) {
+  int64_t seg = (n + kReduceBlocks - 1) / kReduceBlocks;
+  return seg < kMdotSegMin ? kMdotSegMin : seg;
+}
+inline int mdot_blocks(int64_t n, int64_t seg) {
+  int64_t b = (n + seg - 1) / seg;
+  return (int)(b < 1 ? 1 : b);
 }
 // one block per k, sums the nb partials in index order (wave-parallel then
 // fixed-order combine)
@@ -801,9 +822,14 @@ int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
   FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && work && out && ld >= n,
                "multi_dot: bad arguments");
   hipStream_t s = FP3D_STREAM(stream);
-  int nb = rblocks(n);
-  hipLaunchKernelGGL(k_mdot_p1, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld, w,
-                     work, (const double *)nullptr, 0.0);
+  int64_t seg = mdot_seg(n);
+  int nb = mdot_blocks(n, seg);
+  if (seg <= kMdotLds)
+    hipLaunchKernelGGL(k_mdot_p1<true>, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld,
+                       w, work, (const double *)nullptr, 0.0, seg);
+  else
+    hipLaunchKernelGGL(k_mdot_p1<false>, dim3(nb), dim3(kBlock), 0, s, n, nv, V,
+                       ld, w, work, (const double *)nullptr, 0.0, seg);
   hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
   return check_launch("multi_dot");
 }
@@ -815,9 +841,14 @@ int fp3d_multi_dot_reorth(int64_t n, int nv, const double *V, int64_t ld,
                "multi_dot_reorth: bad arguments");
   FP3D_REQUIRE(eta2 > 0.0 && eta2 <= 1.0, "multi_dot_reorth: eta2 in (0, 1]");
   hipStream_t s = FP3D_STREAM(stream);
-  int nb = rblocks(n);
-  hipLaunchKernelGGL(k_mdot_p1, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld, w,
-                     work, cond, eta2);
+  int64_t seg = mdot_seg(n);
+  int nb = mdot_blocks(n, seg);
+  if (seg <= kMdotLds)
+    hipLaunchKernelGGL(k_mdot_p1<true>, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld,
+                       w, work, cond, eta2, seg);
+  else
+    hipLaunchKernelGGL(k_mdot_p1<false>, dim3(nb), dim3(kBlock), 0, s, n, nv, V,
+                       ld, w, work, cond, eta2, seg);
   hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
   return check_launch("multi_dot_reorth");
 }
