@@ -326,6 +326,26 @@ int fp3d_multi_dot_reorth(int64_t n, int nv, const double *V, int64_t ld,
                           double *work, double *out, void *stream);
 int fp3d_multi_axpy(int64_t n, int nv, const double *V, int64_t ld,
                     const double *h, double sign, double *w, void *stream);
+/* FP32 Krylov basis (no reference counterpart): the same three operations
+ * with the basis V stored as float32 and every product and sum in float64
+ * (w, h and out stay float64), for GMRES in the FP32-preconditioned solve.
+ * fp3d_scale_to_f32: y = (float)(a x); fp3d_f32_to_f64: y = (double)x. */
+int fp3d_multi_dot_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                        const double *w, double *work, double *out, void *stream);
+int fp3d_multi_dot_reorth_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                               const double *w, const double *cond, double eta2,
+                               double *work, double *out, void *stream);
+int fp3d_multi_axpy_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                         const double *h, double sign, double *w, void *stream);
+int fp3d_scale_to_f32(int64_t n, double a, const double *x, float *y, void *stream);
+int fp3d_f32_to_f64(int64_t n, const float *x, double *y, void *stream);
+/* y = x / nrm[0], the divisor read on the device (no reference
+ * counterpart): GMRES normalises the new Arnoldi vector without a host
+ * round trip, so the next step is queued while the host applies Givens. */
+int fp3d_scale_inv_dev(int64_t n, const double *nrm, const double *x, double *y,
+                       void *stream);
+int fp3d_scale_inv_dev_to_f32(int64_t n, const double *nrm, const double *x,
+                              float *y, void *stream);
 /* out2 = {||x||_2, ||x||_inf} on device */
 int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
                void *stream);

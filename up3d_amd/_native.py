@@ -100,6 +100,13 @@ _SIGS = {
     "fp3d_multi_dot": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _vp, _vp]),
     "fp3d_multi_dot_work_doubles": (_int, [_int]),
     "fp3d_multi_axpy": (_int, [_i64, _int, _vp, _i64, _vp, _dbl, _vp, _vp]),
+    "fp3d_multi_dot_f32v": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "fp3d_multi_dot_reorth_f32v": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _dbl, _vp, _vp, _vp]),
+    "fp3d_multi_axpy_f32v": (_int, [_i64, _int, _vp, _i64, _vp, _dbl, _vp, _vp]),
+    "fp3d_scale_to_f32": (_int, [_i64, _dbl, _vp, _vp, _vp]),
+    "fp3d_f32_to_f64": (_int, [_i64, _vp, _vp, _vp]),
+    "fp3d_scale_inv_dev": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "fp3d_scale_inv_dev_to_f32": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_norms": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_amg_cheb_smooth": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
                                     _vp, _vp, _vp, _vp, _int, _dbl, _dbl, _int,

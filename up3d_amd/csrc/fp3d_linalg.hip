@@ -161,6 +161,22 @@ __global__ void k_to_f32(int64_t n, const double *__restrict__ x,
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (float)x[i];
 }
+__global__ void k_scale_cvt(int64_t n, double a, const double *x, float *y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)(a * x[i]);
+}
+__global__ void k_widen(int64_t n, const float *x, double *y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+// y = x / nrm[0] with the divisor read on the device (so the next Arnoldi
+// step can be queued before the host has seen the norm)
+template <typename TY>
+__global__ void k_scale_inv_dev(int64_t n, const double *__restrict__ nrm,
+                                const double *__restrict__ x, TY *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (TY)((1.0 / nrm[0]) * x[i]);
+}
 __global__ void k_scale(int64_t n, double a, const double *x, double *y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i];
@@ -248,9 +264,9 @@ inline int rblocks(int64_t n) {
 // cond[nv]); otherwise the partials are written as exact zeros.
 constexpr int kMdotLds = 2048;
 constexpr int64_t kMdotSegMin = 1024;
-template <bool STAGED>
+template <bool STAGED, typename TV>
 __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
-                                                   const double *__restrict__ V,
+                                                   const TV *__restrict__ V,
                                                    int64_t ld,
                                                    const double *__restrict__ w,
                                                    double *__restrict__ part,
@@ -281,23 +297,23 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwave = blockDim.x >> 6;
   for (int k = wave; k < nv; k += nwave) {
-    const double *__restrict__ v = V + (int64_t)k * ld + r0;
+    const TV *__restrict__ v = V + (int64_t)k * ld + r0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int i = lane;
     for (; i + 192 < len; i += 256) {
       if (STAGED) {
-        a0 += v[i] * ws[i];
-        a1 += v[i + 64] * ws[i + 64];
-        a2 += v[i + 128] * ws[i + 128];
-        a3 += v[i + 192] * ws[i + 192];
+        a0 += (double)v[i] * ws[i];
+        a1 += (double)v[i + 64] * ws[i + 64];
+        a2 += (double)v[i + 128] * ws[i + 128];
+        a3 += (double)v[i + 192] * ws[i + 192];
       } else {
-        a0 += v[i] * wg[i];
-        a1 += v[i + 64] * wg[i + 64];
-        a2 += v[i + 128] * wg[i + 128];
-        a3 += v[i + 192] * wg[i + 192];
+        a0 += (double)v[i] * wg[i];
+        a1 += (double)v[i + 64] * wg[i + 64];
+        a2 += (double)v[i + 128] * wg[i + 128];
+        a3 += (double)v[i + 192] * wg[i + 192];
       }
     }
-    for (; i < len; i += 64) a0 += v[i] * (STAGED ? ws[i] : wg[i]);
+    for (; i < len; i += 64) a0 += (double)v[i] * (STAGED ? ws[i] : wg[i]);
     double t = wsum((a0 + a1) + (a2 + a3));
     if (lane == 0) part[(int64_t)k * nb + blockIdx.x] = t;
   }
@@ -366,9 +382,10 @@ __global__ __launch_bounds__(kBlock) void k_norms_p2(
   }
 }
 
-// w -= sum_k h[k] V_k   (h in LDS)
+// w -= sum_k h[k] V_k   (h in LDS; V float64 or float32, arithmetic float64)
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
-                                                 const double *__restrict__ V,
+                                                 const TV *__restrict__ V,
                                                  int64_t ld,
                                                  const double *__restrict__ h,
                                                  double *__restrict__ w,
@@ -387,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int k = 0; k < nv; ++k) acc += hs[k] * V[(int64_t)k * ld + i];
+  for (int k = 0; k < nv; ++k) acc += hs[k] * (double)V[(int64_t)k * ld + i];
   w[i] += sign * acc;
 }
 
@@ -620,6 +637,34 @@ using namespace fp3d;
 // ======================================================================
 // C-ABI
 // ======================================================================
+template <typename TV>
+static int multi_dot_any(int64_t n, int nv, const TV *V, int64_t ld, const double *w,
+                         const double *cond, double eta2, double *work, double *out,
+                         void *stream, const char *what) {
+  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && work && out && ld >= n, what);
+  hipStream_t s = FP3D_STREAM(stream);
+  int64_t seg = mdot_seg(n);
+  int nb = mdot_blocks(n, seg);
+  if (seg <= kMdotLds)
+    hipLaunchKernelGGL((k_mdot_p1<true, TV>), dim3(nb), dim3(kBlock), 0, s, n, nv,
+                       V, ld, w, work, cond, eta2, seg);
+  else
+    hipLaunchKernelGGL((k_mdot_p1<false, TV>), dim3(nb), dim3(kBlock), 0, s, n, nv,
+                       V, ld, w, work, cond, eta2, seg);
+  hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
+  return check_launch(what);
+}
+
+template <typename TV>
+static int multi_axpy_any(int64_t n, int nv, const TV *V, int64_t ld, const double *h,
+                          double sign, double *w, void *stream, const char *what) {
+  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && h && w && ld >= n, what);
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL((k_maxpy<TV>), dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, nv, V, ld, h, w, sign);
+  return check_launch(what);
+}
+
 extern "C" {
 
 int fp3d_csr_spmv(int64_t n_rows, int64_t nnz, const int32_t *indptr,
@@ -819,50 +864,79 @@ int fp3d_mask_copy(int64_t n, const uint8_t *mask, const double *x, double *y,
 
 int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
                    const double *w, double *work, double *out, void *stream) {
-  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && work && out && ld >= n,
-               "multi_dot: bad arguments");
-  hipStream_t s = FP3D_STREAM(stream);
-  int64_t seg = mdot_seg(n);
-  int nb = mdot_blocks(n, seg);
-  if (seg <= kMdotLds)
-    hipLaunchKernelGGL(k_mdot_p1<true>, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld,
-                       w, work, (const double *)nullptr, 0.0, seg);
-  else
-    hipLaunchKernelGGL(k_mdot_p1<false>, dim3(nb), dim3(kBlock), 0, s, n, nv, V,
-                       ld, w, work, (const double *)nullptr, 0.0, seg);
-  hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
-  return check_launch("multi_dot");
+  return multi_dot_any(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
+                       "multi_dot: bad arguments");
+}
+
+int fp3d_multi_dot_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                        const double *w, double *work, double *out, void *stream) {
+  return multi_dot_any(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
+                       "multi_dot_f32v: bad arguments");
 }
 
 int fp3d_multi_dot_reorth(int64_t n, int nv, const double *V, int64_t ld,
                           const double *w, const double *cond, double eta2,
                           double *work, double *out, void *stream) {
-  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && cond && work && out && ld >= n,
-               "multi_dot_reorth: bad arguments");
-  FP3D_REQUIRE(eta2 > 0.0 && eta2 <= 1.0, "multi_dot_reorth: eta2 in (0, 1]");
-  hipStream_t s = FP3D_STREAM(stream);
-  int64_t seg = mdot_seg(n);
-  int nb = mdot_blocks(n, seg);
-  if (seg <= kMdotLds)
-    hipLaunchKernelGGL(k_mdot_p1<true>, dim3(nb), dim3(kBlock), 0, s, n, nv, V, ld,
-                       w, work, cond, eta2, seg);
-  else
-    hipLaunchKernelGGL(k_mdot_p1<false>, dim3(nb), dim3(kBlock), 0, s, n, nv, V,
-                       ld, w, work, cond, eta2, seg);
-  hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
-  return check_launch("multi_dot_reorth");
+  FP3D_REQUIRE(cond && eta2 > 0.0 && eta2 <= 1.0,
+               "multi_dot_reorth: cond and eta2 in (0, 1] required");
+  return multi_dot_any(n, nv, V, ld, w, cond, eta2, work, out, stream,
+                       "multi_dot_reorth: bad arguments");
+}
+
+int fp3d_multi_dot_reorth_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                               const double *w, const double *cond, double eta2,
+                               double *work, double *out, void *stream) {
+  FP3D_REQUIRE(cond && eta2 > 0.0 && eta2 <= 1.0,
+               "multi_dot_reorth_f32v: cond and eta2 in (0, 1] required");
+  return multi_dot_any(n, nv, V, ld, w, cond, eta2, work, out, stream,
+                       "multi_dot_reorth_f32v: bad arguments");
 }
 
 int fp3d_multi_dot_work_doubles(int nv) { return nv * kReduceBlocks; }
 
 int fp3d_multi_axpy(int64_t n, int nv, const double *V, int64_t ld,
                     const double *h, double sign, double *w, void *stream) {
-  FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && h && w && ld >= n,
-               "multi_axpy: bad arguments");
+  return multi_axpy_any(n, nv, V, ld, h, sign, w, stream, "multi_axpy: bad arguments");
+}
+
+int fp3d_multi_axpy_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                         const double *h, double sign, double *w, void *stream) {
+  return multi_axpy_any(n, nv, V, ld, h, sign, w, stream,
+                        "multi_axpy_f32v: bad arguments");
+}
+
+int fp3d_scale_to_f32(int64_t n, double a, const double *x, float *y, void *stream) {
+  FP3D_REQUIRE(n == 0 || (x && y), "scale_to_f32: NULL pointer");
   if (n == 0) return FP3D_OK;
-  hipLaunchKernelGGL(k_maxpy, dim3(grid_for(n)), dim3(kBlock), 0,
-                     FP3D_STREAM(stream), n, nv, V, ld, h, w, sign);
-  return check_launch("multi_axpy");
+  hipLaunchKernelGGL(k_scale_cvt, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, a, x, y);
+  return check_launch("scale_to_f32");
+}
+
+int fp3d_scale_inv_dev(int64_t n, const double *nrm, const double *x, double *y,
+                       void *stream) {
+  FP3D_REQUIRE(n == 0 || (nrm && x && y), "scale_inv_dev: NULL pointer");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_scale_inv_dev<double>, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, nrm, x, y);
+  return check_launch("scale_inv_dev");
+}
+
+int fp3d_scale_inv_dev_to_f32(int64_t n, const double *nrm, const double *x,
+                              float *y, void *stream) {
+  FP3D_REQUIRE(n == 0 || (nrm && x && y), "scale_inv_dev_to_f32: NULL pointer");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_scale_inv_dev<float>, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, nrm, x, y);
+  return check_launch("scale_inv_dev_to_f32");
+}
+
+int fp3d_f32_to_f64(int64_t n, const float *x, double *y, void *stream) {
+  FP3D_REQUIRE(n == 0 || (x && y), "f32_to_f64: NULL pointer");
+  if (n == 0) return FP3D_OK;
+  hipLaunchKernelGGL(k_widen, dim3(grid_for(n)), dim3(kBlock), 0,
+                     FP3D_STREAM(stream), n, x, y);
+  return check_launch("f32_to_f64");
 }
 
 int fp3d_norms(int64_t n, const double *x, double *work, double *out2,

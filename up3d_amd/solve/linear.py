@@ -80,9 +80,23 @@ def scale(a: float, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
 def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
               x0: Optional[torch.Tensor] = None, rtol: float = 1e-5,
               atol: float = 0.0, restart: int = 20, maxiter: Optional[int] = None,
-              callback: Optional[Callable] = None):
+              callback: Optional[Callable] = None, basis: str = "f64"):
     """Solve A x = b on the device.  matvec(x, y) / psolve(x, y) write into y.
-    Returns (x, info, n_inner) with SciPy's info convention."""
+    Returns (x, info, n_inner) with SciPy's info convention (left
+    preconditioning, restart / ptol logic of scipy.sparse.linalg.gmres).
+
+    The Arnoldi loop runs one step ahead of the host: each step's
+    orthogonalisation (CGS plus a device-side DGKS-conditional second pass)
+    and its normalisation (the norm read on the device) are queued, the
+    step's coefficients go to pinned host memory asynchronously, and the
+    host applies the Givens rotations of step j while the device already
+    computes step j + 1 (a step queued past convergence is discarded).
+    basis="f32" stores the Krylov basis in float32 (half the
+    orthogonalisation traffic); dot products, the Hessenberg matrix, the
+    Givens rotations and the solution update stay float64."""
+    if basis not in ("f64", "f32"):
+        raise ValueError(f"basis={basis!r}: 'f64' or 'f32'")
+    f32 = basis == "f32"
     n = b.numel()
     if maxiter is None:
         maxiter = 10 * n
@@ -108,12 +122,58 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
     ptol_max_factor = 1.0
     ptol = Mb_nrm2 * min(ptol_max_factor, atol / bnrm2)
     presid = 0.0
-    V = empty((restart + 1, n))
+    if f32:
+        V = torch.empty((restart + 1, n), dtype=torch.float32, device=b.device)
+        v64 = empty(n)
+    else:
+        V = empty((restart + 1, n))
+    w_buf = [empty(n), empty(n)]
+    nout = 2 * restart + 4
+    outs = [empty(nout), empty(nout)]
+    hbuf = [torch.empty(nout, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+    evs = [torch.cuda.Event(), torch.cuda.Event()]
     h = np.zeros((restart, restart + 1))
     givens = np.zeros((restart, 2))
     inner = 0
     rnorm = np.inf
     x_nonzero = x0 is not None and bool(torch.any(x != 0).item())
+
+    def enqueue(col):
+        """Device work of Arnoldi step col: w = M^-1 A v_col orthogonalised
+        against v_0..v_col and normalised into v_{col+1}; its coefficients
+        (out[2:2+k] = h, out[2+k] = ||w||^2 before, out[3+k:3+2k] = second
+        pass, out[0] = ||w|| after) copied to pinned host memory."""
+        k = col + 1
+        o = outs[col % 2]
+        w = w_buf[col % 2]
+        if f32:
+            _native.call("fp3d_f32_to_f64", n, ptr(V[col]), ptr(v64), stream_handle())
+            matvec(v64, tmp)
+        else:
+            matvec(V[col], tmp)
+        prec(tmp, w)
+        sh = stream_handle()
+        if f32:
+            dot_, reo_, axpy_ = ("fp3d_multi_dot_f32v", "fp3d_multi_dot_reorth_f32v",
+                                 "fp3d_multi_axpy_f32v")
+        else:
+            dot_, reo_, axpy_ = "fp3d_multi_dot", "fp3d_multi_dot_reorth", "fp3d_multi_axpy"
+        _native.call(dot_, n, k, ptr(V), V.stride(0), ptr(w), ptr(red.work),
+                     ptr(o[2:]), sh)
+        _native.call("fp3d_multi_dot", n, 1, ptr(w), n, ptr(w), ptr(red.work),
+                     ptr(o[2 + k:]), sh)
+        _native.call(axpy_, n, k, ptr(V), V.stride(0), ptr(o[2:]), -1.0, ptr(w), sh)
+        # second CGS pass only where the first one cancelled (device-side DGKS
+        # test on h and ||w||^2; a skipped pass leaves exact zeros and w as is)
+        _native.call(reo_, n, k, ptr(V), V.stride(0), ptr(w), ptr(o[2:]), 0.5,
+                     ptr(red.work), ptr(o[3 + k:]), sh)
+        _native.call(axpy_, n, k, ptr(V), V.stride(0), ptr(o[3 + k:]), -1.0, ptr(w), sh)
+        _native.call("fp3d_norms", n, ptr(w), ptr(red.work), ptr(o), sh)
+        _native.call("fp3d_scale_inv_dev_to_f32" if f32 else "fp3d_scale_inv_dev", n,
+                     ptr(o), ptr(w), ptr(V[col + 1]), sh)
+        hbuf[col % 2][: 3 + 2 * k].copy_(o[: 3 + 2 * k], non_blocking=True)
+        evs[col % 2].record()
+
     for iteration in range(maxiter):
         if iteration == 0:
             if x_nonzero:
@@ -124,45 +184,33 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
                 r.copy_(b)
             if norm2(r, red) < atol:
                 return x, 0, 0
-        prec(r, V[0])
-        t0 = norm2(V[0], red)
-        scale(1.0 / t0, V[0], V[0])
+        w0 = w_buf[1]
+        prec(r, w0)
+        t0 = norm2(w0, red)
+        if f32:
+            _native.call("fp3d_scale_to_f32", n, 1.0 / t0, ptr(w0), ptr(V[0]),
+                         stream_handle())
+        else:
+            scale(1.0 / t0, w0, V[0])
         S = np.zeros(restart + 1)
         S[0] = t0
         breakdown = False
         col = 0
+        enqueue(0)
         for col in range(restart):
-            matvec(V[col], tmp)
-            w = V[col + 1]
-            prec(tmp, w)
+            if col + 1 < restart:
+                enqueue(col + 1)            # runs while the host does step col
+            evs[col % 2].synchronize()
             k = col + 1
-            # one fused pass gives V_j . w for j < k AND w . w (V[k] is w):
-            # out[2:2+k] = h, out[2+k] = ||w||^2 before; re-orthogonalisation
-            # h2 -> out[3+k:3+2k]; ||w|| after -> out[0]
-            sh = stream_handle()
-            red.dots(V, k + 1, w, 2)
-            _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
-                         ptr(red.out[2:]), -1.0, ptr(w), sh)
-            # second CGS pass only where the first one cancelled (device-side
-            # DGKS test on out[2:2+k] = h and out[2+k] = ||w||^2; skipped
-            # passes leave exact zeros and an untouched w)
-            _native.call("fp3d_multi_dot_reorth", n, k, ptr(V), V.stride(0),
-                         ptr(w), ptr(red.out[2:]), 0.5, ptr(red.work),
-                         ptr(red.out[3 + k:]), sh)
-            _native.call("fp3d_multi_axpy", n, k, ptr(V), V.stride(0),
-                         ptr(red.out[3 + k:]), -1.0, ptr(w), sh)
-            red.norm(w, 0)
-            host = to_host(red.out[: 3 + 2 * k])
+            host = hbuf[col % 2][: 3 + 2 * k].numpy()
             h0 = float(np.sqrt(max(host[2 + k], 0.0)))
-            h1 = host[0]
+            h1 = float(host[0])
             hv = host[2:2 + k] + host[3 + k:3 + 2 * k]
             h[col, :k] = hv
             h[col, col + 1] = h1
             if h1 <= eps * h0:
                 h[col, col + 1] = 0.0
                 breakdown = True
-            else:
-                scale(1.0 / h1, w, w)
             for j in range(col):
                 c, s_ = givens[j, 0], givens[j, 1]
                 n0, n1 = h[col, j], h[col, j + 1]
@@ -189,8 +237,8 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
         if y[0] != 0.0:
             y[0] /= h[0, 0]
         yd = to_dev(y, torch.float64)
-        _native.call("fp3d_multi_axpy", n, col + 1, ptr(V), V.stride(0), ptr(yd),
-                     1.0, ptr(x), stream_handle())
+        _native.call("fp3d_multi_axpy_f32v" if f32 else "fp3d_multi_axpy", n, col + 1,
+                     ptr(V), V.stride(0), ptr(yd), 1.0, ptr(x), stream_handle())
         matvec(x, tmp)
         r.copy_(b)
         axpby(-1.0, tmp, 1.0, r)

@@ -19,8 +19,9 @@ Per Newton step (all device-resident):
   line search: safety-only backtracking on ||R||^2, best-of on failure.
 
 precision="fp64" is the all-FP64 solve; precision="fp32" keeps the
-residual, norms, Arnoldi/Givens and the coarsest AMG solve in float64 and
-moves the GMRES Jacobian values and every AMG matrix to float32; the outer
+residual, norms, the Arnoldi dot products / Hessenberg / Givens arithmetic
+and the coarsest AMG solve in float64, and moves the GMRES Jacobian values,
+every AMG matrix and the storage of the Krylov basis to float32; the outer
 Newton iteration on the float64 residual acts as iterative refinement.
 """
 
@@ -505,7 +506,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
         t0 = time.perf_counter()
         scale(-1.0, cur.R, rhs)
         x, info, n_it = gmres_dev(ws.jac.matvec, rhs, M_pre.apply, rtol=eta,
-                                  restart=gmres_restart, maxiter=gmres_maxiter)
+                                  restart=gmres_restart, maxiter=gmres_maxiter,
+                                  basis="f32" if ws.precision == "fp32" else "f64")
         dphi.copy_(x)
         sync()
         timings["linsolve"] += time.perf_counter() - t0
