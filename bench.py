@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the Newton solves (FP64 and FP32-precond)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="solve with the fixed defaults instead of the autotuned parameters")
     ap.add_argument("--no-plate", action="store_true",
                     help="skip the embedded-plate lifting case (128x96x96 box)")
     ap.add_argument("--cpu-passes", type=int, default=2)
@@ -218,7 +220,7 @@ def main():
 
     solve = None
     if not args.no_solve:
-        solve = run_solves(mesh, args.config)
+        solve = run_solves(mesh, args.config, tune=not args.no_tune)
 
     plate = None
     if not args.no_plate:
@@ -298,7 +300,7 @@ def cpu_baseline(st, mesh, passes):
             "s_per_pass": round(dt, 3)}
 
 
-def run_solves(mesh, config):
+def run_solves(mesh, config, tune=True):
     """Newton solve of the M_inf = 0.84 channel to ||R_free||_inf < 1e-10 in
     both precisions.  For the 192x64x64 workload the solve is grid-sequenced
     48x16x16 -> 96x32x32 -> 192x64x64 (jittered, RCM): Mach ramp at
@@ -307,7 +309,10 @@ def run_solves(mesh, config):
     interpolated solution.  Seconds run from the start of the coarsest solve
     to the fine-mesh acceptance and include every AMG set-up and transfer;
     the first (cold) call also includes the per-mesh host set-up
-    (colouring, CSR, geometry, upload), later (warm) calls reuse it."""
+    (colouring, CSR, geometry, upload) and the autotuning of the finest mesh
+    (element block size, GMRES restart, AMG degree and coarsest size, timed
+    on its interpolated initial state; `tuning` in each entry), later
+    (warm) calls reuse both (the tuning from its cache)."""
     import torch
 
     from up3d_amd.mesh.channel import boundary_nodes, bump_channel, config_mesh
@@ -339,7 +344,9 @@ def run_solves(mesh, config):
             upwind_c_post=(2.0, 1.5), post_stages_on="coarser",
             newton_kw={"precision": prec, "gmres_restart": 200, "gmres_maxiter": 5,
                        "n_newton_max": 30, "amg_rebuild_every": 0,
-                       "precond_matrix": "jacobian_spd"})
+                       "precond_matrix": "jacobian_spd"},
+            tune=tune, tune_kw={"restarts": (100, 200), "degrees": (1, 2),
+                                "max_coarse": (10, 1000)})
         torch.cuda.synchronize()
         return r, time.perf_counter() - t0
 
@@ -356,7 +363,8 @@ def run_solves(mesh, config):
                              "levels": [(round(a, 4), c, n, float("%.3e" % rr), ok, it)
                                         for a, c, n, rr, ok, it in mr["levels"]]}
                             for mr in r["mesh_results"]],
-                 "timings_total": {k: round(v, 2) for k, v in r["timings_total"].items()}}
+                 "timings_total": {k: round(v, 2) for k, v in r["timings_total"].items()},
+                 "tuning": r.get("tuning")}
         if kind == "warm":
             wf = wall_force_coefficients(mesh.nodes, mesh.elements, floor, r["phi"],
                                          m_inf=M_INF)

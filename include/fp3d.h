@@ -443,6 +443,12 @@ int fp3d_host_locate_points(int64_t n_tets, const int32_t *elements,
                             int64_t n_pts, const double *pts,
                             int32_t *out_elem, double *out_bary);
 
+/* Threads per block of the element-wise kernels (state, walk, upwind,
+ * sensitivity and element-term sweeps): 64, 128, 192 or 256, process-wide
+ * (no reference counterpart; set by up3d_amd.tune). */
+int fp3d_set_element_block(int threads);
+int fp3d_get_element_block(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -909,11 +909,17 @@ int fp3d_stream_sync(void *stream) {
 
 int fp3d_reduce_work_doubles(void) { return 3 * kReduceBlocks; }
 
+// Threads per block of the element-wise kernels (every one of them indexes
+// with blockDim.x and uses no LDS): 64, 128, 192 or 256, process-wide, set
+// by the autotuner through fp3d_set_element_block.
+static int g_elem_block = kBlock;
+
 #define ELEMWISE(kernel, n, ...)                                       \
   do {                                                                 \
     FP3D_REQUIRE(n >= 0, #kernel ": negative size");                   \
     if (n == 0) return FP3D_OK;                                        \
-    hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kBlock), 0,     \
+    hipLaunchKernelGGL(kernel, dim3(grid_for(n, g_elem_block)),        \
+                       dim3(g_elem_block), 0,                          \
                        FP3D_STREAM(stream), n, __VA_ARGS__);           \
     return check_launch(#kernel);                                      \
   } while (0)
@@ -1222,8 +1228,8 @@ int fp3d_element_residual_terms(int64_t n_tets, const double *B,
   FP3D_REQUIRE(B && V && rho_tilde && grad && terms,
                "element_residual_terms: NULL pointer");
   FP3D_REQUIRE(((uintptr_t)terms & 31) == 0, "terms must be 32-byte aligned");
-  hipLaunchKernelGGL(k_elem_residual_terms, dim3(grid_for(n_tets)),
-                     dim3(kBlock), 0, FP3D_STREAM(stream), n_tets, B, V,
+  hipLaunchKernelGGL(k_elem_residual_terms, dim3(grid_for(n_tets, g_elem_block)),
+                     dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets, B, V,
                      rho_tilde, grad, terms);
   return check_launch("element_residual_terms");
 }
@@ -1255,7 +1261,8 @@ int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
   FP3D_REQUIRE(((uintptr_t)K & 31) == 0, "K must be 32-byte aligned");
   FP3D_REQUIRE(clamp == 0.0 || (clamp > 0.0 && clamp < 1.0 && s_e),
                "element_matrix_terms: clamp must be 0 or in (0, 1) with s_e");
-  hipLaunchKernelGGL(k_elem_matrix_terms, dim3(grid_for(n_tets)), dim3(kBlock),
+  hipLaunchKernelGGL(k_elem_matrix_terms, dim3(grid_for(n_tets, g_elem_block)),
+                     dim3(g_elem_block),
                      0, FP3D_STREAM(stream), n_tets, B, V, rho_tilde,
                      s_e ? grad : nullptr, s_e, lim, inv_u2, clamp, K);
   return check_launch("element_matrix_terms");
@@ -1299,5 +1306,14 @@ int fp3d_fill(int64_t n, double value, double *x, void *stream) {
   FP3D_REQUIRE(n == 0 || x, "fill: NULL");
   ELEMWISE(k_fill, n, value, x);
 }
+
+int fp3d_set_element_block(int threads) {
+  FP3D_REQUIRE(threads == 64 || threads == 128 || threads == 192 || threads == 256,
+               "set_element_block: threads must be 64, 128, 192 or 256");
+  g_elem_block = threads;
+  return FP3D_OK;
+}
+
+int fp3d_get_element_block(void) { return g_elem_block; }
 
 }  // extern "C"

@@ -408,15 +408,20 @@ __global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
   w[i] += sign * acc;
 }
 
+// x = M b, M dense row-major: one wave per row, lanes stride the row
+// (coalesced), fixed shuffle-tree sum
 __global__ __launch_bounds__(kBlock) void k_dense_mv(int64_t n,
                                                     const double *__restrict__ M,
                                                     const double *__restrict__ b,
                                                     double *__restrict__ x) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= n) return;
+  const double *__restrict__ row = M + i * n;
   double s = 0.0;
-  for (int64_t j = 0; j < n; ++j) s += M[i * n + j] * b[j];
-  x[i] = s;
+  for (int64_t j = lane; j < n; j += 64) s += row[j] * b[j];
+  s = wsum(s);
+  if (lane == 0) x[i] = s;
 }
 
 // Chebyshev first step from x = 0: x = 0; r = dinv b; d = r/theta
@@ -1019,8 +1024,9 @@ int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
                       void *stream) {
   FP3D_REQUIRE(n > 0 && M && b && x, "dense_matvec: bad arguments");
-  hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(n)), dim3(kBlock), 0,
-                     FP3D_STREAM(stream), n, M, b, x);
+  const int64_t rows_per_block = kBlock / 64;
+  hipLaunchKernelGGL(k_dense_mv, dim3((n + rows_per_block - 1) / rows_per_block),
+                     dim3(kBlock), 0, FP3D_STREAM(stream), n, M, b, x);
   return check_launch("dense_matvec");
 }
 

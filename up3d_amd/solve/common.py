@@ -58,6 +58,7 @@ def masked(free_mask, x, out):
     return out
 
 
-def amg_from_device(store, data, precision="f64") -> AMGHierarchy:
-    """SA hierarchy (host set-up) of a masked device CSR matrix."""
-    return AMGHierarchy(csr_to_host(store, data), precision=precision, seed=0)
+def amg_from_device(store, data, precision="f64", **amg_kw) -> AMGHierarchy:
+    """SA hierarchy (host set-up) of a masked device CSR matrix; amg_kw go to
+    AMGHierarchy (degree, max_coarse, ...)."""
+    return AMGHierarchy(csr_to_host(store, data), precision=precision, seed=0, **amg_kw)

@@ -220,7 +220,8 @@ class NewtonWorkspace:
         self._diag_shift(self.A_pic, ptc_scale)
         mask_dirichlet(self.st, self.fm, self.A_pic)
         self.precond = amg_from_device(
-            self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64")
+            self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64",
+            **getattr(self, "amg_kw", {}))
         return self.precond
 
     def freeze(self, s: _State, upwind_c, m_crit, rho_floor):
@@ -260,7 +261,8 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                  external_rhs: Optional[np.ndarray] = None,
                  precision: str = "fp64", precond_matrix: str = "picard",
                  precond_clamp: float = 0.9, ptc_dt0: Optional[float] = None,
-                 ptc_dt_max: float = 1e6, verbose: bool = False
+                 ptc_dt_max: float = 1e6, amg_degree: Optional[int] = None,
+                 amg_max_coarse: Optional[int] = None, verbose: bool = False
                  ) -> Dict[str, object]:
     """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
     n_st = 0; precond is always the device AMG).  Converged when
@@ -280,7 +282,12 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     (J + M_lumped / dt) dphi = -R with the switched-evolution-relaxation
     rule dt = ptc_dt0 * ||R_0||_2 / ||R||_2 (R_0 the level's first
     residual); once dt reaches ptc_dt_max the steps are plain Newton.  The
-    converged state is the same discrete solution (R = 0)."""
+    converged state is the same discrete solution (R = 0).
+
+    amg_degree / amg_max_coarse (None: the hierarchy's defaults, Chebyshev
+    degree 2 and a coarsest level of at most 10 rows) set the AMG smoother
+    degree and the size at which the hierarchy stops and solves densely --
+    the knobs `up3d_amd.tune` searches."""
     if m_inf > 0.0 and upwind_c <= 0.0:
         raise ValueError("the Newton driver runs with the walk upwind machinery "
                          "active for m_inf > 0; pass upwind_c > 0")
@@ -291,6 +298,14 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     elif ws.precision != precision:
         raise ValueError(f"workspace precision {ws.precision!r} != {precision!r}")
     ws.set_mach(m_inf, m_cap)
+    amg_kw = dict(getattr(ws, "amg_kw", {}))
+    if amg_degree is not None:
+        amg_kw["degree"] = int(amg_degree)
+    if amg_max_coarse is not None:
+        amg_kw["max_coarse"] = int(amg_max_coarse)
+    if amg_kw != getattr(ws, "amg_kw", {}):
+        ws.amg_kw = amg_kw
+        ws.precond = None             # a kept hierarchy was built with other options
     t_wall0 = time.perf_counter()
     timings = new_timings()
     step_records = []

@@ -34,6 +34,7 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
                            m_crit: float = 0.95,
                            post_stages_on: str = "finest",
                            newton_kw: Optional[dict] = None,
+                           tune: bool = False, tune_kw: Optional[dict] = None,
                            verbose: bool = False) -> Dict[str, object]:
     """meshes: coarse-to-fine list of dicts with keys nodes, elements,
     dirichlet_nodes and either dirichlet_values or a callable
@@ -46,7 +47,13 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
     only; "coarser" runs them on every mesh after the first and lets the
     finest mesh start directly at the last (production) upwind_c from the
     interpolated production-C solution, so the expensive mesh runs one
-    Newton level instead of len(upwind_c_post) + 1."""
+    Newton level instead of len(upwind_c_post) + 1.
+
+    tune=True runs `up3d_amd.tune.tune_newton_workspace` on the finest mesh
+    at its interpolated initial state (element block size, GMRES restart,
+    AMG degree and coarsest size; cached per mesh shape and device) and
+    solves the finest mesh with the result; `tuning` in the returned dict
+    holds the choice, whether it came from the cache and its seconds."""
     if post_stages_on not in ("finest", "coarser"):
         raise ValueError(f"unknown post_stages_on {post_stages_on!r}")
     if len(meshes) < 1:
@@ -86,6 +93,28 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
                 c_level = upwind_c_post[-1]
             else:
                 post = list(upwind_c_post)
+        tuning = None
+        if tune and finest:
+            from ..tune import tune_newton_workspace, tuned_newton_kwargs
+            from .newton import NewtonWorkspace
+
+            t0 = time.perf_counter()
+            ws_t = NewtonWorkspace(nodes, elements, dn, dv,
+                                   precision=kw.get("precision", "fp64"))
+            tuning = tune_newton_workspace(
+                ws_t, phi0, m_inf, upwind_c=c_level, m_crit=m_crit,
+                precond_matrix=kw.get("precond_matrix", "picard"),
+                **(tune_kw or {}))
+            del ws_t
+            lvl_kw.update(tuned_newton_kwargs(tuning))
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+            tuning = {"element_block": tuning["element_block"]["best"],
+                      "solver": tuning["solver"]["best"],
+                      "cached": bool(tuning.get("cached")),
+                      "seconds": time.perf_counter() - t0}
+            if verbose:
+                print(f"[sequence] tuning {tuning}", flush=True)
         t0 = time.perf_counter()
         result = solve_newton_transonic(
             nodes, elements, dn, dv, m_inf,
@@ -114,6 +143,7 @@ def solve_newton_sequenced(meshes: Sequence[dict], m_inf: float,
         phi_prev = result["phi"]
         prev = (nodes, elements)
     out = dict(result)
+    out["tuning"] = tuning
     out["mesh_results"] = mesh_results
     out["timings_total"] = sum_timings(timing_parts)
     out["wall_s"] = time.perf_counter() - t_all
