@@ -84,6 +84,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="solve with the fixed defaults instead of the autotuned parameters")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also run the batched sweep: 64 conditions on 4 channel meshes "
+                         "(about 3 minutes)")
     ap.add_argument("--no-plate", action="store_true",
                     help="skip the embedded-plate lifting case (128x96x96 box)")
     ap.add_argument("--cpu-passes", type=int, default=2)
@@ -230,6 +233,11 @@ def main():
             plate = {"error": f"{type(exc).__name__}: {exc}"}
         log(f"[bench] plate {json.dumps(plate)}")
 
+    sweep = None
+    if args.sweep:
+        sweep = run_sweep()
+        log(f"[bench] sweep {json.dumps({k: v for k, v in sweep.items() if k != 'instances'})}")
+
     out = {
         "metric": "Newton solve s + assembly Mtet/s: FP32-precond vs FP64 GPU vs CPU, M∞=0.84",
         "value": round(value, 2), "unit": "Mtet/s", "n_gpus": 1,
@@ -244,6 +252,8 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "phases": phase_rep,
         "setup_s": round(setup_s, 2), "solve": solve, "plate": plate,
     }
+    if sweep is not None:
+        out["sweep"] = sweep
     print(json.dumps(out), flush=True)
 
 
@@ -378,6 +388,42 @@ def run_solves(mesh, config, tune=True):
     res["fp32_vs_fp64_cf_x_abs"] = abs(res["fp32"]["wall_cf"][0] - res["fp64"]["wall_cf"][0])
     res["fp64_over_fp32_seconds"] = round(res["fp64"]["seconds"] / res["fp32"]["seconds"], 3)
     return res
+
+
+def run_sweep():
+    """Batched sweep: 16 seeded (M_inf in [0.60, 0.86], alpha in [0, 3] deg)
+    conditions on each of 4 jittered RCM bump channels (144,000 / 589,824 /
+    1,152,000 / 2,387,718 tets), FP32-preconditioned Newton, every
+    instance to ||R_free||_inf < 1e-10 with 0 limited / floored."""
+    import torch
+
+    from up3d_amd.solve.sweep import solve_sweep, sweep_conditions, sweep_meshes
+
+    t0 = time.perf_counter()
+    meshes = sweep_meshes()
+    mesh_s = time.perf_counter() - t0
+    conds = sweep_conditions(len(meshes), 16)
+    r = solve_sweep(meshes, conds, log=log)
+    torch.cuda.synchronize()
+    inst = r["instances"]
+    tet_steps = sum(i["n_tets"] * i["n_newton"] for i in inst)
+    return {
+        "workload": "64 conditions: M_inf U[0.60,0.86], alpha U[0,3] deg, 16 on each of "
+                    "4 jittered bump channels (144,000 / 589,824 / 1,152,000 / 2,387,718 tets)",
+        "mesh_build_s": round(mesh_s, 2), "wall_s": round(r["wall_s"], 2),
+        "n_conditions": len(conds), "n_converged": r["n_converged"],
+        "conditions_per_s": round(len(conds) / r["wall_s"], 4),
+        "newton_steps": int(sum(i["n_newton"] for i in inst)),
+        "gmres_iterations": int(sum(i["n_lin_iters"] for i in inst)),
+        "mtet_newton_steps_per_s": round(tet_steps / r["wall_s"] / 1e6, 2),
+        "routes": {k: sum(1 for i in inst if i["route"] == k) for k in ("warm", "cold")},
+        "per_mesh": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in pm.items()}
+                     for pm in r["per_mesh"]],
+        "instances": [{"mesh": i["mesh"], "m_inf": round(i["m_inf"], 4),
+                       "alpha_deg": round(i["alpha_deg"], 4), "converged": i["converged"],
+                       "n_newton": i["n_newton"], "seconds": round(i["seconds"], 2),
+                       "cf": [round(c, 8) for c in i["cf"][:2]]} for i in inst],
+    }
 
 
 def run_plate(dims=(128, 96, 96), m_inf=0.78, alpha_deg=2.0):
