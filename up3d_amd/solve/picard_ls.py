@@ -349,7 +349,7 @@ def solve_multivalued_transonic(mvop, mesh, m_target: float,
                                 farfield_lag: str = "live",
                                 **kwargs) -> Dict[str, object]:
     """Mach continuation of the level-set Picard (pyfp3d/solve/picard_ls.py:
-    919-1044): each level is a warm-started solve_multivalued_lifting from
+    895-1044): each level is a warm-started solve_multivalued_lifting from
     the previous level's full extended state and circulation;
     farfield_lag="level" freezes the far-field vortex at the previous
     level's circulation.  Returns the final level's dict plus `levels` and
