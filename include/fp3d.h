@@ -227,8 +227,9 @@ int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
                               const double *s_e, const uint8_t *lim,
                               double inv_u2, double clamp, double *K,
                               void *stream);
-/* rel8[16e + 4a + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]]
- * (uint8: every row is at most 256 long), 4-byte aligned. */
+/* rel8[4q + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]] where
+ * inc_ent[q] = 4e + a (incidence order; uint8: every row is at most 256
+ * long), 4-byte aligned. */
 int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                           const int32_t *inc_ptr, const int32_t *inc_ent,
                           const uint8_t *rel8, const double *K, double *data,

@@ -797,9 +797,10 @@ __global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
     o[i] = make_double4(k[4 * i], k[4 * i + 1], k[4 * i + 2], k[4 * i + 3]);
 }
 
-// rel8[16e + 4a + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]]: the
-// slot's offset inside its row (rows are at most 256 long), read as one
-// 32-bit word per incidence entry -- a quarter of the elem_to_csr bytes.
+// rel8[4q + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]] for the
+// incidence entry q = inc_ent index of (e, a): the slot's offset inside its
+// row (rows are at most 256 long), one 32-bit word per incidence, stored in
+// incidence order so each row reads its words contiguously.
 __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
     int64_t nn, int64_t nb_pad, int max_seg, const int32_t *__restrict__ indptr,
     const int32_t *__restrict__ iptr, const int32_t *__restrict__ ient,
@@ -826,7 +827,7 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
       for (int k = 0; k < U; ++k) {
         if (q + k < q1) {
           const int64_t ent = ient[q + k];   // 4 e + a: row a of element e
-          of[k] = rel8[ent];
+          of[k] = rel8[q + k];
           kv[k] = *reinterpret_cast<const double4 *>(K + 4 * ent);
         }
       }

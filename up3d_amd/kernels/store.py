@@ -16,7 +16,9 @@ Device layout (element e, node v):
   B        f64   (n_tets, 12)    shape gradients, row-major (i, d)
   V        f64   (n_tets,)
   e2c      int32 (n_tets, 16)    CSR slot of (elements[e,i], elements[e,j])
-  rel8     uint8 (n_tets, 16)    that slot's offset inside row elements[e,i]
+  rel8     uint8 (4 n_tets, 4)   rel8[q, j]: offset inside row v of the slot
+                                 of (v, elements[e, j]) for incidence
+                                 q = (v, 4 e + i) of inc_ent
   fn       int32 (n_tets, 4)     face neighbours, -1 on the boundary
   cen      f64   (n_tets, 3)     centroids
   dcen     f64   (n_tets, 12)    centroid(fn[e,f]) - centroid(e), per face
@@ -143,8 +145,10 @@ class MeshStore:
         rel = e2c.reshape(self.n_tets, 4, 4) - indptr[el][:, :, None]
         if self.n_tets and (rel.min() < 0 or rel.max() > 255):
             raise ValueError("a CSR row is longer than 256 entries")
-        self.rel8 = to_dev(np.ascontiguousarray(rel.astype(np.uint8)).reshape(self.n_tets, 16),
-                           torch.uint8)
+        # stored in node-incidence order (entry q holds the four offsets of
+        # incidence inc_ent[q]), so a row's gather reads them contiguously
+        rel_inc = np.ascontiguousarray(rel.astype(np.uint8)).reshape(-1, 4)[order]
+        self.rel8 = to_dev(np.ascontiguousarray(rel_inc).reshape(-1), torch.uint8)
         self.fn = to_dev(fn.astype(np.int32), torch.int32)
         self.cen = to_dev(cen, torch.float64)
         # per-face centroid displacement (0 on the boundary)
