@@ -458,6 +458,11 @@ def run_plate(dims=(128, 96, 96), m_inf=0.78, alpha_deg=2.0):
     adjoint_s = call_s - sens.get("seconds_tangent", 0.0)
     step_s = (newton_s - seed_s) / n_steps
     y, g = case.gamma_distribution(r["phi_ext"])
+    from up3d_amd.post.section_cut import section_cp_curve
+
+    sec = section_cp_curve(mesh, case.mvop.main_potential(r["phi_ext"]), eta=0.5,
+                           b_semi=case.b, m_inf=m_inf, upper_hint=(0.0, 0.0, 1.0),
+                           span_axis=1, side_by="normal")
     return {
         "workload": f"embedded swept plate, {dims[0]}x{dims[1]}x{dims[2]} hexes -> "
                     f"{len(mesh.elements):,} tets, M_inf {m_inf}, alpha {alpha_deg} deg",
@@ -475,6 +480,10 @@ def run_plate(dims=(128, 96, 96), m_inf=0.78, alpha_deg=2.0):
         "dcl_dalpha_tangent": sens.get("dcl_dalpha_tangent"),
         "adjoint_s": round(adjoint_s, 3), "adjoint_gmres": sens["n_gmres_adjoint"],
         "adjoint_over_newton_step": round(adjoint_s / step_s, 3),
+        "section_eta_0.5": {"n_upper": int(len(sec["cp_upper"])),
+                            "cp_upper_min": round(float(sec["cp_upper"].min()), 5),
+                            "cp_lower_max": round(float(sec["cp_lower"].max()), 5),
+                            "chord": round(sec["chord"], 5)},
     }
 
 
