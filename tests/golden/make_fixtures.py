@@ -32,7 +32,9 @@ def main():
         np.savez_compressed(HERE / f"naca0012_{lv}_bc.npz", nodes=m.nodes,
                             elements=m.elements,
                             wall=m.boundary_faces["wall"],
-                            farfield=m.boundary_faces["farfield"])
+                            farfield=m.boundary_faces["farfield"],
+                            wake=m.boundary_faces["wake"],
+                            symmetry=m.boundary_faces["symmetry"])
     g73 = {}
     for ln in (REF / "cases/demo/p7_diff_flux/results/g73_constructed.csv").read_text().splitlines()[1:]:
         k, v = ln.split(",")
