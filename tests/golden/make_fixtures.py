@@ -69,6 +69,13 @@ def main():
         # C = 1.5, vortex far field, converged circulation (the reference's
         # measured table: M0.60 upwind on/off, then the M0.65-0.75 coarse and
         # M0.70 medium LS column)
+        # conforming explicit-circulation wake, coupled (phi, Gamma) Newton,
+        # alpha 1.25 deg, C = 1.5, vortex far field (the reference's
+        # same-mesh "conf Newton" column)
+        "conforming_newton_naca_a125": {"source": "docs/design_track_b.md:873-879",
+                                        "values": {"coarse": {"0.65": 0.1038, "0.70": 0.1151,
+                                                              "0.75": 0.1377},
+                                                   "medium": {"0.70": 0.1190}}},
         "ls_gamma_naca_a125": {"source": "docs/design_track_b.md:799,873-878",
                                "values": {"coarse": {"0.60": 0.0968, "0.65": 0.1043,
                                                      "0.70": 0.1158, "0.75": 0.1391},
