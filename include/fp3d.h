@@ -206,7 +206,11 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
  *
  * Two stages: the element pass writes terms[4e + a] = rho_tilde V
  * (grad . B_a), or the 16 matrix values K[16e + 4a + b]; the node pass sums
- * them per node / per CSR slot.  Matrix terms: s_e == NULL gives the Picard
+ * them per node / per CSR slot.  With inc_pos (int32[4 n_tets], the inverse
+ * of inc_ent, 16-byte aligned) the element pass writes in incidence order
+ * instead: terms[inc_pos[4e + a]], matrix row a at K[4 inc_pos[4e + a]];
+ * the node pass then takes inc_ent == NULL and reads terms / K in order
+ * (same sums, same order: bitwise equal results).  Matrix terms: s_e == NULL gives the Picard
  * matrix A = sum rho_tilde V B_i.B_j (grad unused); else Terms 1+2 with s_e
  * masked by lim (lim may be NULL); clamp in (0, 1) gives the SPD
  * preconditioner variant (as fp3d_assemble_jacobian_spd_colored), 0 the
@@ -217,19 +221,19 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
 int fp3d_gather_rows_per_block(void);
 int fp3d_element_residual_terms(int64_t n_tets, const double *B,
                                 const double *V, const double *rho_tilde,
-                                const double *grad, double *terms,
-                                void *stream);
+                                const double *grad, const int32_t *inc_pos,
+                                double *terms, void *stream);
 int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
                            const int32_t *inc_ent, const double *terms,
                            double *R, void *stream);
 int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
                               const double *rho_tilde, const double *grad,
                               const double *s_e, const uint8_t *lim,
-                              double inv_u2, double clamp, double *K,
-                              void *stream);
+                              double inv_u2, double clamp,
+                              const int32_t *inc_pos, double *K, void *stream);
 /* rel8[4q + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]] where
  * inc_ent[q] = 4e + a (incidence order; uint8: every row is at most 256
- * long), 4-byte aligned. */
+ * long), 4-byte aligned.  inc_ent == NULL: K holds row q at K[4q]. */
 int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                           const int32_t *inc_ptr, const int32_t *inc_ent,
                           const uint8_t *rel8, const double *K, double *data,

@@ -171,6 +171,27 @@ def test_gather_assembly_bitwise(mesh, ora):
     Js = to_host(op.jacobian_spd_dev(rtd, gd, -(sed.abs() * 40.0), limd, 1.0, 0.9, out))
     assert np.array_equal(Js, oo.jacobian_spd_terms12(g, rt, -np.abs(se) * 40.0, 1.0,
                                                       0.9, lim))
+    # element-order terms (inc_pos NULL) + indexed gathers (inc_ent): the
+    # same sums as the incidence-ordered path the operator runs
+    from up3d_amd import _native
+    from up3d_amd.device import ptr, stream_handle
+
+    st = op.store
+    t = torch.empty(4 * n, dtype=torch.float64, device=rtd.device)
+    R2 = torch.empty(mesh.n_nodes, dtype=torch.float64, device=rtd.device)
+    _native.call("fp3d_element_residual_terms", n, ptr(st.B), ptr(st.V), ptr(rtd),
+                 ptr(gd), None, ptr(t), stream_handle())
+    _native.call("fp3d_gather_node_terms", mesh.n_nodes, ptr(st.inc_ptr),
+                 ptr(st.inc_ent), ptr(t), ptr(R2), stream_handle())
+    assert np.array_equal(to_host(R2), R)
+    K = torch.empty(16 * n, dtype=torch.float64, device=rtd.device)
+    out.fill_(np.nan)
+    _native.call("fp3d_element_matrix_terms", n, ptr(st.B), ptr(st.V), ptr(rtd), ptr(gd),
+                 ptr(sed), ptr(limd), 1.0, 0.0, None, ptr(K), stream_handle())
+    _native.call("fp3d_gather_csr_terms", mesh.n_nodes, st.gather_max_seg, ptr(st.indptr),
+                 ptr(st.inc_ptr), ptr(st.inc_ent), ptr(st.rel8), ptr(K), ptr(out),
+                 stream_handle())
+    assert np.array_equal(to_host(out), J)
     # serial order vs the reference's colour order: same sums to 1e-13
     Jc = ora["opc"].jacobian_terms12(g, rt, se, 1.0)
     assert np.abs(to_host(op.jacobian_dev(rtd, gd, sed, None, 1.0, out)) - Jc).max() \

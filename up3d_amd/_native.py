@@ -85,10 +85,10 @@ _SIGS = {
     "fp3d_csr_mask_dirichlet": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_to_f32": (_int, [_i64, _vp, _vp, _vp]),
     "fp3d_gather_rows_per_block": (_int, []),
-    "fp3d_element_residual_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_element_residual_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_gather_node_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_element_matrix_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl,
-                                         _vp, _vp]),
+                                         _vp, _vp, _vp]),
     "fp3d_gather_csr_terms": (_int, [_i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_multi_dot_reorth": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _dbl, _vp, _vp, _vp]),
     "fp3d_lumped_volume": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),

@@ -714,7 +714,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nb_pad) {
 __global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
     int64_t n, const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ rt, const double *__restrict__ grad,
-    double *__restrict__ c) {
+    const int32_t *__restrict__ pos, double *__restrict__ c) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double w = rt[e] * V[e];
@@ -725,7 +725,17 @@ __global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
   o.y = w * (gx * b[3] + gy * b[4] + gz * b[5]);
   o.z = w * (gx * b[6] + gy * b[7] + gz * b[8]);
   o.w = w * (gx * b[9] + gy * b[10] + gz * b[11]);
-  *reinterpret_cast<double4 *>(c + 4 * e) = o;
+  if (pos) {
+    // incidence order: term a of e goes to the slot of (e, a) in its node's
+    // incidence list, so the node pass reads its terms contiguously
+    const int4 p = *reinterpret_cast<const int4 *>(pos + 4 * e);
+    c[p.x] = o.x;
+    c[p.y] = o.y;
+    c[p.z] = o.z;
+    c[p.w] = o.w;
+  } else {
+    *reinterpret_cast<double4 *>(c + 4 * e) = o;
+  }
 }
 
 // The loads of kUnroll consecutive entries are issued together (they are
@@ -745,7 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_terms(
     double t[kUnroll];
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k)
-      t[k] = (q + k < q1) ? c[ient[q + k]] : 0.0;
+      t[k] = (q + k < q1) ? c[ient ? ient[q + k] : q + k] : 0.0;
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k)
       if (q + k < q1) r += t[k];
@@ -757,7 +767,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
     int64_t n, const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ rt, const double *__restrict__ grad,
     const double *__restrict__ s_e, const uint8_t *__restrict__ lim,
-    double inv_u2, double clamp, double *__restrict__ K) {
+    double inv_u2, double clamp, const int32_t *__restrict__ pos,
+    double *__restrict__ K) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double w1 = rt[e] * V[e];
@@ -791,10 +802,20 @@ __global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
       k[4 * i + j] = val;
     }
   }
-  double4 *o = reinterpret_cast<double4 *>(K + 16 * e);
+  if (pos) {
+    // incidence order: row a of e goes to the slot of (e, a)
+    const int4 p = *reinterpret_cast<const int4 *>(pos + 4 * e);
+    const int32_t pa[4] = {p.x, p.y, p.z, p.w};
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-    o[i] = make_double4(k[4 * i], k[4 * i + 1], k[4 * i + 2], k[4 * i + 3]);
+    for (int i = 0; i < 4; ++i)
+      *reinterpret_cast<double4 *>(K + 4 * (int64_t)pa[i]) =
+          make_double4(k[4 * i], k[4 * i + 1], k[4 * i + 2], k[4 * i + 3]);
+  } else {
+    double4 *o = reinterpret_cast<double4 *>(K + 16 * e);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      o[i] = make_double4(k[4 * i], k[4 * i + 1], k[4 * i + 2], k[4 * i + 3]);
+  }
 }
 
 // rel8[4q + b] = elem_to_csr[e][a][b] - indptr[elements[e][a]] for the
@@ -826,7 +847,9 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         if (q + k < q1) {
-          const int64_t ent = ient[q + k];   // 4 e + a: row a of element e
+          // row a of element e, ient[q] = 4 e + a (or the row's own slot q
+          // when the element pass wrote in incidence order)
+          const int64_t ent = ient ? (int64_t)ient[q + k] : (int64_t)(q + k);
           of[k] = rel8[q + k];
           kv[k] = *reinterpret_cast<const double4 *>(K + 4 * ent);
         }
@@ -1222,16 +1245,18 @@ int fp3d_gather_rows_per_block(void) { return kGatherRows; }
 
 int fp3d_element_residual_terms(int64_t n_tets, const double *B,
                                 const double *V, const double *rho_tilde,
-                                const double *grad, double *terms,
-                                void *stream) {
+                                const double *grad, const int32_t *inc_pos,
+                                double *terms, void *stream) {
   FP3D_REQUIRE(n_tets >= 0, "element_residual_terms: n_tets < 0");
   if (n_tets == 0) return FP3D_OK;
   FP3D_REQUIRE(B && V && rho_tilde && grad && terms,
                "element_residual_terms: NULL pointer");
   FP3D_REQUIRE(((uintptr_t)terms & 31) == 0, "terms must be 32-byte aligned");
+  FP3D_REQUIRE(!inc_pos || ((uintptr_t)inc_pos & 15) == 0,
+               "inc_pos must be 16-byte aligned");
   hipLaunchKernelGGL(k_elem_residual_terms, dim3(grid_for(n_tets, g_elem_block)),
                      dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets, B, V,
-                     rho_tilde, grad, terms);
+                     rho_tilde, grad, inc_pos, terms);
   return check_launch("element_residual_terms");
 }
 
@@ -1240,8 +1265,7 @@ int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
                            double *R, void *stream) {
   FP3D_REQUIRE(n_nodes >= 0, "gather_node_terms: n_nodes < 0");
   if (n_nodes == 0) return FP3D_OK;
-  FP3D_REQUIRE(inc_ptr && inc_ent && terms && R,
-               "gather_node_terms: NULL pointer");
+  FP3D_REQUIRE(inc_ptr && terms && R, "gather_node_terms: NULL pointer");
   int64_t nb = (n_nodes + kBlock - 1) / kBlock;
   unsigned g = xcd_grid(nb);
   hipLaunchKernelGGL(k_gather_terms, dim3(g), dim3(kBlock), 0,
@@ -1253,8 +1277,8 @@ int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
 int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
                               const double *rho_tilde, const double *grad,
                               const double *s_e, const uint8_t *lim,
-                              double inv_u2, double clamp, double *K,
-                              void *stream) {
+                              double inv_u2, double clamp,
+                              const int32_t *inc_pos, double *K, void *stream) {
   FP3D_REQUIRE(n_tets >= 0, "element_matrix_terms: n_tets < 0");
   if (n_tets == 0) return FP3D_OK;
   FP3D_REQUIRE(B && V && rho_tilde && K, "element_matrix_terms: NULL pointer");
@@ -1265,7 +1289,7 @@ int fp3d_element_matrix_terms(int64_t n_tets, const double *B, const double *V,
   hipLaunchKernelGGL(k_elem_matrix_terms, dim3(grid_for(n_tets, g_elem_block)),
                      dim3(g_elem_block),
                      0, FP3D_STREAM(stream), n_tets, B, V, rho_tilde,
-                     s_e ? grad : nullptr, s_e, lim, inv_u2, clamp, K);
+                     s_e ? grad : nullptr, s_e, lim, inv_u2, clamp, inc_pos, K);
   return check_launch("element_matrix_terms");
 }
 
@@ -1275,7 +1299,7 @@ int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                           void *stream) {
   FP3D_REQUIRE(n_nodes >= 0, "gather_csr_terms: n_nodes < 0");
   if (n_nodes == 0) return FP3D_OK;
-  FP3D_REQUIRE(indptr && inc_ptr && inc_ent && rel8 && K && data,
+  FP3D_REQUIRE(indptr && inc_ptr && rel8 && K && data,
                "gather_csr_terms: NULL pointer");
   FP3D_REQUIRE(((uintptr_t)rel8 & 3) == 0, "rel8 must be 4-byte aligned");
   FP3D_REQUIRE(((uintptr_t)K & 31) == 0, "K must be 32-byte aligned");

@@ -168,11 +168,13 @@ class PicardOperator:
         element terms, then the node gather in ascending element order."""
         st = self.store
         R = self.d_R if out is None else out
+        # terms written in incidence order (slot inc_pos[4e + a]), so the
+        # node pass streams them
         _native.call("fp3d_element_residual_terms", self.n_tets, ptr(st.B),
-                     ptr(st.V), ptr(rho_tilde), ptr(grad), ptr(self.d_terms),
-                     stream_handle())
+                     ptr(st.V), ptr(rho_tilde), ptr(grad), ptr(st.inc_pos),
+                     ptr(self.d_terms), stream_handle())
         _native.call("fp3d_gather_node_terms", self.n_nodes, ptr(st.inc_ptr),
-                     ptr(st.inc_ent), ptr(self.d_terms), ptr(R), stream_handle())
+                     None, ptr(self.d_terms), ptr(R), stream_handle())
         return R
 
     def _gather_matrix(self, rho_tilde, grad, s_e, lim, inv_u2, clamp, out):
@@ -184,9 +186,9 @@ class PicardOperator:
                      ptr(grad) if grad is not None else None,
                      ptr(s_e) if s_e is not None else None,
                      ptr(lim) if lim is not None else None, float(inv_u2),
-                     float(clamp), ptr(self.d_K), stream_handle())
+                     float(clamp), ptr(st.inc_pos), ptr(self.d_K), stream_handle())
         _native.call("fp3d_gather_csr_terms", self.n_nodes, st.gather_max_seg,
-                     ptr(st.indptr), ptr(st.inc_ptr), ptr(st.inc_ent),
+                     ptr(st.indptr), ptr(st.inc_ptr), None,
                      ptr(st.rel8), ptr(self.d_K), ptr(out), stream_handle())
         return out
 

@@ -26,6 +26,7 @@ Device layout (element e, node v):
   nodes    f64   (n_nodes, 3)
   indptr   int32 (n_nodes+1,), indices int32 (nnz,)
   inc_ptr  int32 (n_nodes+1,), inc_ent int32 (4 n_tets,)  node -> 4e+i
+  inc_pos  int32 (4 n_tets,)   inverse of inc_ent: 4e+i -> incidence slot
 """
 
 from __future__ import annotations
@@ -162,6 +163,11 @@ class MeshStore:
         self.indices = to_dev(indices, torch.int32)
         self.inc_ptr = to_dev(inc_ptr.astype(np.int32), torch.int32)
         self.inc_ent = to_dev(order.astype(np.int32), torch.int32)
+        # inverse of inc_ent: slot of (e, a) in the incidence list, where the
+        # element passes write their terms so the node passes read in order
+        inc_pos = np.empty(len(order), dtype=np.int32)
+        inc_pos[order] = np.arange(len(order), dtype=np.int32)
+        self.inc_pos = to_dev(inc_pos, torch.int32)
         self.offsets_ct = self.color_offsets.ctypes.data_as(
             C.POINTER(C.c_int64))
 
