@@ -763,6 +763,50 @@ __global__ __launch_bounds__(kBlock) void k_gather_terms(
   R[v] = r;
 }
 
+// Incidence-ordered terms: the terms of a block's kBlock nodes are one
+// contiguous range [iptr[v0], iptr[v1]).  The block streams it through LDS
+// in coalesced chunks (the next chunk's loads are issued before the current
+// one is summed), and each lane adds its own node's terms in entry order:
+// the same sums, in the same order, as k_gather_terms.
+constexpr int kTermChunk = 2048;               // doubles per chunk (16 KiB)
+__global__ __launch_bounds__(kBlock) void k_gather_terms_staged(
+    int64_t nn, int64_t nb_pad, const int32_t *__restrict__ iptr,
+    const double *__restrict__ c, double *__restrict__ R) {
+  __shared__ double buf[kTermChunk];
+  constexpr int P = kTermChunk / kBlock;
+  const int64_t v0 = xcd_block(blockIdx.x, nb_pad) * kBlock;
+  if (v0 >= nn) return;                        // uniform over the block
+  const int64_t v1 = v0 + kBlock < nn ? v0 + kBlock : nn;
+  const int64_t v = v0 + threadIdx.x;
+  const int64_t qb = iptr[v0], qe = iptr[v1];
+  int64_t q = 0, q1 = 0;
+  if (v < v1) {
+    q = iptr[v];
+    q1 = iptr[v + 1];
+  }
+  double nxt[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t i = qb + threadIdx.x + k * kBlock;
+    nxt[k] = i < qe ? c[i] : 0.0;
+  }
+  double r = 0.0;
+  for (int64_t c0 = qb; c0 < qe; c0 += kTermChunk) {
+    __syncthreads();                           // previous chunk consumed
+#pragma unroll
+    for (int k = 0; k < P; ++k) buf[threadIdx.x + k * kBlock] = nxt[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int64_t i = c0 + kTermChunk + threadIdx.x + k * kBlock;
+      nxt[k] = i < qe ? c[i] : 0.0;
+    }
+    const int64_t hi = q1 < c0 + kTermChunk ? q1 : c0 + kTermChunk;
+    for (; q < hi; ++q) r += buf[q - c0];
+  }
+  if (v < v1) R[v] = r;
+}
+
 __global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
     int64_t n, const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ rt, const double *__restrict__ grad,
@@ -863,6 +907,79 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
           seg[rb + (int)(of[k] >> 24)] += kv[k].w;
         }
       }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < len; t += kGatherRows) data[base + t] = seg[t];
+}
+
+// Incidence-ordered matrix rows: the rows and slot offsets of a block's
+// kGatherRows CSR rows are one contiguous range [iptr[r0], iptr[r1]),
+// streamed through LDS in coalesced chunks of kRowChunk (next chunk in
+// flight while the current one is added); each lane adds its own row's
+// entries in entry order into the LDS-staged CSR block, as k_gather_csr.
+constexpr int kRowChunk = 512;                 // matrix rows per chunk (16 KiB)
+__global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
+    int64_t nn, int64_t nb_pad, int max_seg, const int32_t *__restrict__ indptr,
+    const int32_t *__restrict__ iptr, const uint32_t *__restrict__ rel8,
+    const double4 *__restrict__ K4, double *__restrict__ data) {
+  extern __shared__ double seg[];
+  __shared__ double4 kb[kRowChunk];
+  __shared__ uint32_t ob[kRowChunk];
+  constexpr int P = kRowChunk / kGatherRows;
+  const int64_t r0 = xcd_block(blockIdx.x, nb_pad) * kGatherRows;
+  if (r0 >= nn) return;                        // uniform over the block
+  const int64_t r1 = r0 + kGatherRows < nn ? r0 + kGatherRows : nn;
+  const int32_t base = indptr[r0];
+  const int32_t len = indptr[r1] - base;
+  if (len > max_seg) return;
+  for (int t = threadIdx.x; t < len; t += kGatherRows) seg[t] = 0.0;
+  const int64_t v = r0 + threadIdx.x;
+  const int64_t qb = iptr[r0], qe = iptr[r1];
+  int64_t q = 0, q1 = 0;
+  int32_t rb = 0;
+  if (v < r1) {
+    rb = indptr[v] - base;
+    q = iptr[v];
+    q1 = iptr[v + 1];
+  }
+  double4 nk[P];
+  uint32_t no[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t i = qb + threadIdx.x + k * kGatherRows;
+    if (i < qe) {
+      nk[k] = K4[i];
+      no[k] = rel8[i];
+    } else {
+      nk[k] = make_double4(0.0, 0.0, 0.0, 0.0);
+      no[k] = 0u;
+    }
+  }
+  for (int64_t c0 = qb; c0 < qe; c0 += kRowChunk) {
+    __syncthreads();                 // previous chunk consumed, seg zeroed
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      kb[threadIdx.x + k * kGatherRows] = nk[k];
+      ob[threadIdx.x + k * kGatherRows] = no[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int64_t i = c0 + kRowChunk + threadIdx.x + k * kGatherRows;
+      if (i < qe) {
+        nk[k] = K4[i];
+        no[k] = rel8[i];
+      }
+    }
+    const int64_t hi = q1 < c0 + kRowChunk ? q1 : c0 + kRowChunk;
+    for (; q < hi; ++q) {
+      const double4 kv = kb[q - c0];
+      const uint32_t of = ob[q - c0];
+      seg[rb + (int)(of & 0xffu)] += kv.x;
+      seg[rb + (int)((of >> 8) & 0xffu)] += kv.y;
+      seg[rb + (int)((of >> 16) & 0xffu)] += kv.z;
+      seg[rb + (int)(of >> 24)] += kv.w;
     }
   }
   __syncthreads();
@@ -1268,9 +1385,14 @@ int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
   FP3D_REQUIRE(inc_ptr && terms && R, "gather_node_terms: NULL pointer");
   int64_t nb = (n_nodes + kBlock - 1) / kBlock;
   unsigned g = xcd_grid(nb);
-  hipLaunchKernelGGL(k_gather_terms, dim3(g), dim3(kBlock), 0,
-                     FP3D_STREAM(stream), n_nodes, (int64_t)g, inc_ptr,
-                     inc_ent, terms, R);
+  if (inc_ent)
+    hipLaunchKernelGGL(k_gather_terms, dim3(g), dim3(kBlock), 0,
+                       FP3D_STREAM(stream), n_nodes, (int64_t)g, inc_ptr,
+                       inc_ent, terms, R);
+  else
+    hipLaunchKernelGGL(k_gather_terms_staged, dim3(g), dim3(kBlock), 0,
+                       FP3D_STREAM(stream), n_nodes, (int64_t)g, inc_ptr,
+                       terms, R);
   return check_launch("gather_node_terms");
 }
 
@@ -1307,10 +1429,16 @@ int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                "gather_csr_terms: max_seg must lie in (0, 8192]");
   int64_t nb = (n_nodes + kGatherRows - 1) / kGatherRows;
   unsigned g = xcd_grid(nb);
-  hipLaunchKernelGGL(k_gather_csr, dim3(g), dim3(kGatherRows),
-                     (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
-                     n_nodes, (int64_t)g, max_seg, indptr, inc_ptr, inc_ent,
-                     (const uint32_t *)rel8, K, data);
+  if (inc_ent)
+    hipLaunchKernelGGL(k_gather_csr, dim3(g), dim3(kGatherRows),
+                       (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
+                       n_nodes, (int64_t)g, max_seg, indptr, inc_ptr, inc_ent,
+                       (const uint32_t *)rel8, K, data);
+  else
+    hipLaunchKernelGGL(k_gather_csr_staged, dim3(g), dim3(kGatherRows),
+                       (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
+                       n_nodes, (int64_t)g, max_seg, indptr, inc_ptr,
+                       (const uint32_t *)rel8, (const double4 *)K, data);
   return check_launch("gather_csr_terms");
 }
 
