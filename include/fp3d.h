@@ -253,20 +253,25 @@ int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
  * the active elements (s_u != 0 after the limiter mask lim[upstream[e]]),
  * stores their coefficients and a row-gather table built from the static
  * node -> (4*element + local) incidence CSR (inc_ptr int32[n_nodes+1],
- * inc_ent int32[4*n_tets], entries of a node in ascending element order),
+ * inc_ent int32[4*n_tets], entries of a node in ascending element order;
+ * inc_pos int32[4*n_tets] its inverse, 16-byte aligned),
  * so fp3d_csr_spmv_term3 adds Term 3 row by row in a fixed order: no
  * atomics, bitwise repeatable.  Rows with free_mask[row] == 0 (nullable)
  * get no Term-3 entries.  `t3` is caller-owned device memory of
- * fp3d_term3_workspace_bytes bytes; *n_active_out (host) receives the
- * active count (the call synchronises the stream once to read it). */
+ * fp3d_term3_workspace_bytes bytes.  With n_active_out != NULL the call
+ * synchronises the stream once and stores the active count there; with
+ * NULL it returns without synchronising, and fp3d_term3_active_count
+ * reads the count of the last build later (one synchronisation). */
 int64_t fp3d_term3_workspace_bytes(int64_t n_tets, int64_t n_nodes);
 int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
                      const int32_t *inc_ptr, const int32_t *inc_ent,
-                     const double *B, const double *V, const double *grad,
+                     const int32_t *inc_pos, const double *B, const double *V, const double *grad,
                      const int32_t *upstream, const double *s_u,
                      const uint8_t *lim, const uint8_t *free_mask,
                      double inv_u2, void *t3, int64_t *n_active_out,
                      void *stream);
+int fp3d_term3_active_count(const void *t3, int64_t n_tets, int64_t n_nodes,
+                            int64_t *n_active_out, void *stream);
 
 /* ------------------------------------------------------ sparse / BLAS-1
  * CSR SpMV, G lanes of a wave64 per row (G chosen from nnz / n_rows). */

@@ -77,7 +77,8 @@ _SIGS = {
                                    _dbl, _vp, _vp, _vp, _vp]),
     "fp3d_term3_workspace_bytes": (_i64, [_i64, _i64]),
     "fp3d_term3_build": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                _vp, _vp, _vp, _dbl, _vp, _pi64, _vp]),
+                                _vp, _vp, _vp, _vp, _dbl, _vp, _pi64, _vp]),
+    "fp3d_term3_active_count": (_int, [_vp, _i64, _i64, _pi64, _vp]),
     "fp3d_csr_spmv": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_spmv_f32": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_spmv_term3": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,

@@ -23,6 +23,18 @@ inline unsigned grid_for(int64_t n, int per_block = kBlock) {
 
 int check_launch(const char *what);
 
+// Workgroups are dealt to the 8 XCDs round-robin; xcd_block() maps them so
+// that each XCD walks one contiguous range of row blocks, keeping the
+// elements shared by neighbouring rows in that XCD's L2.  The grid is padded
+// to a multiple of 8 (xcd_grid) so the mapping is a bijection.
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nb_pad) {
+  int64_t per = nb_pad >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+inline unsigned xcd_grid(int64_t n_blocks) {
+  return (unsigned)(((n_blocks + 7) / 8) * 8);
+}
+
 // Blocks used by the deterministic two-pass reductions; the partial-sum
 // layout depends only on n, never on timing, so results are bitwise
 // repeatable.

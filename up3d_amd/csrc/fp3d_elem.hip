@@ -696,15 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
 // serial assembly (pyfp3d/kernels/residual.py:144-189,
 // pyfp3d/kernels/residual.py:106-129 + COO->CSR) -- with no colour passes,
 // no atomics, and each output written exactly once.
-//
-// Workgroups are dealt to the 8 XCDs round-robin; xcd_block() maps them so
-// that each XCD walks one contiguous range of row blocks, keeping the
-// elements shared by neighbouring rows in that XCD's L2.  The grid is padded
-// to a multiple of 8 so the mapping is a bijection.
-__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nb_pad) {
-  int64_t per = nb_pad >> 3;
-  return (bid & 7) * per + (bid >> 3);
-}
+// Blocks are mapped to XCDs by xcd_block().
 
 // Two stages: an element pass writes each element's contributions once,
 // coalesced (4 residual terms, or the 16 matrix values row-major [4a + b]);
@@ -1352,10 +1344,6 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
                        B, V, rho_tilde, grad, s_e, lim, inv_u2, clamp, data);
   }
   return check_launch("assemble_jacobian_spd_colored");
-}
-
-static inline unsigned xcd_grid(int64_t n_blocks) {
-  return (unsigned)(((n_blocks + 7) / 8) * 8);
 }
 
 int fp3d_gather_rows_per_block(void) { return kGatherRows; }

@@ -81,7 +81,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t3(
     int64_t n_rows, const int32_t *__restrict__ ptr,
     const int32_t *__restrict__ idx, const T *__restrict__ val,
     const double *__restrict__ x, const int32_t *__restrict__ t3ptr,
-    const int32_t *__restrict__ t3ent, const double *__restrict__ W,
+    const int32_t *__restrict__ t3ent,
+    const double *__restrict__ W,  // weight of entry q, beside t3ent[q]
     const double *__restrict__ tval, const uint8_t *__restrict__ fmask,
     double *__restrict__ y) {
   constexpr int RPB = kBlock / G;
@@ -95,10 +96,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t3(
   // assignment and reduction tree: deterministic)
   if (t3ptr) {
     const int32_t q1 = t3ptr[row + 1];
-    for (int32_t q = t3ptr[row] + lane; q < q1; q += G) {
-      int32_t ent = t3ent[q];
-      s += W[ent] * tval[ent >> 2];
-    }
+    for (int32_t q = t3ptr[row] + lane; q < q1; q += G)
+      s += W[q] * tval[t3ent[q] >> 2];
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
@@ -445,7 +444,8 @@ __global__ void k_add(int64_t n, const double *__restrict__ d,
 // ----------------------------------------------------------- Term 3 (device)
 // Workspace layout (byte offsets computed by t3_layout).
 struct T3Layout {
-  size_t flag, flag8, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, total;
+  size_t flag, flag8, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, fq, Wq,
+      total;
 };
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 inline T3Layout t3_layout(int64_t nt, int64_t nn) {
@@ -469,18 +469,22 @@ inline T3Layout t3_layout(int64_t nt, int64_t nn) {
   L.cnt = take(sizeof(int32_t) * (nn + 1));
   L.rowptr = take(sizeof(int32_t) * (nn + 2));
   L.ent = take(sizeof(int32_t) * 4 * nt);
+  L.fq = take(4 * nt + 1);
+  L.Wq = take(sizeof(double) * 4 * nt);
   L.total = o + 256;  // header: n_active (int64) lives in the last 256 B
   return L;
 }
 
-// per element: masked s_u, activity flag and the two coefficient sets
+// per element: masked s_u, activity flag and the two coefficient sets (the
+// weights W of the element's four entries in incidence order)
 __global__ __launch_bounds__(kBlock) void k_t3_coef(
     int64_t nt, const int32_t *__restrict__ elements,
     const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ grad, const int32_t *__restrict__ up,
     const double *__restrict__ s_u, const uint8_t *__restrict__ lim,
-    double inv_u2, int32_t *__restrict__ flag, uint8_t *__restrict__ flag8,
-    double *__restrict__ W, double *__restrict__ Cc) {
+    double inv_u2, const int32_t *__restrict__ pos,
+    int32_t *__restrict__ flag, uint8_t *__restrict__ flag8,
+    double *__restrict__ Wi, double *__restrict__ Cc) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
   int64_t u = up[e];
@@ -493,130 +497,224 @@ __global__ __launch_bounds__(kBlock) void k_t3_coef(
   double ge0 = grad[3 * e], ge1 = grad[3 * e + 1], ge2 = grad[3 * e + 2];
   double gu0 = grad[3 * u], gu1 = grad[3 * u + 1], gu2 = grad[3 * u + 2];
   double w = 2.0 * inv_u2 * s * V[e];
+  const int4 p = *reinterpret_cast<const int4 *>(pos + 4 * e);
+  const int32_t pa[4] = {p.x, p.y, p.z, p.w};
   for (int i = 0; i < 4; ++i) {
     double gi = ge0 * B[12 * e + 3 * i] + ge1 * B[12 * e + 3 * i + 1] +
                 ge2 * B[12 * e + 3 * i + 2];
-    W[4 * e + i] = w * gi;
+    // weight of entry (e, i), at its slot in node elements[e][i]'s
+    // incidence list
+    Wi[pa[i]] = w * gi;
     Cc[4 * e + i] = gu0 * B[12 * u + 3 * i] + gu1 * B[12 * u + 3 * i + 1] +
                     gu2 * B[12 * u + 3 * i + 2];
   }
 }
 
-// exclusive scan, pass 1: per-block local scan + block totals
+// Block-wide exclusive scan of one int per lane: prefix inside each 64-lane
+// wave by shuffles, then the wave totals through LDS (sh holds kBlock / 64
+// ints; the caller syncs before sh is reused).  Integer sums: exact.
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *sh,
+                                                   int32_t *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  int32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    const int32_t t = sh[k];
+    pre += k < w ? t : 0;
+    tot += t;
+  }
+  *total = tot;
+  return pre + x - v;
+}
+
+// exclusive scan, pass 1: kScanPer consecutive entries per lane, block
+// totals out
+constexpr int kScanPer = 4;
+constexpr int kScanTile = kBlock * kScanPer;
 __global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n,
                                                    const int32_t *__restrict__ in,
                                                    int32_t *__restrict__ out,
                                                    int32_t *__restrict__ bsum) {
-  __shared__ int32_t sh[kBlock];
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int32_t v = i < n ? in[i] : 0;
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    int32_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
+  __shared__ int32_t sh[kBlock / 64];
+  const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  int32_t v[kScanPer], t = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = i0 + k < n ? in[i0 + k] : 0;
+    t += v[k];
   }
-  if (i < n) out[i] = sh[threadIdx.x] - v;
-  if (threadIdx.x == kBlock - 1) bsum[blockIdx.x] = sh[kBlock - 1];
+  int32_t tot;
+  int32_t run = block_excl_scan(t, sh, &tot);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (i0 + k < n) out[i0 + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
-// pass 2: exclusive scan of the block totals in one workgroup: each lane
-// owns a contiguous segment, the 256 segment totals are scanned in LDS,
-// then every lane rewrites its segment (integer sums: order-independent)
+// pass 2: exclusive scan of the block totals in one workgroup, in tiles of
+// kBlock with a running carry (coalesced)
 __global__ __launch_bounds__(kBlock) void k_scan_p2(int nb,
                                                    int32_t *__restrict__ bsum,
                                                    int32_t *__restrict__ total) {
-  __shared__ int32_t sh[kBlock];
-  int seg = (nb + kBlock - 1) / kBlock;
-  int lo = threadIdx.x * seg;
-  int hi = lo + seg < nb ? lo + seg : nb;
-  int32_t t = 0;
-  for (int i = lo; i < hi; ++i) t += bsum[i];
-  sh[threadIdx.x] = t;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    int32_t v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += v;
+  __shared__ int32_t sh[kBlock / 64];
+  int32_t carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+    const int i = b0 + threadIdx.x;
+    const int32_t v = i < nb ? bsum[i] : 0;
+    int32_t tot;
+    const int32_t ex = block_excl_scan(v, sh, &tot);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += tot;
     __syncthreads();
   }
-  int32_t run = sh[threadIdx.x] - t;
-  for (int i = lo; i < hi; ++i) {
-    int32_t v = bsum[i];
-    bsum[i] = run;
-    run += v;
-  }
-  if (threadIdx.x == kBlock - 1) *total = sh[kBlock - 1];
+  if (threadIdx.x == 0) *total = carry;
 }
-__global__ void k_scan_p3(int64_t n, int32_t *__restrict__ out,
-                          const int32_t *__restrict__ bsum) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += bsum[blockIdx.x];
+// pass 3: add the block offsets; with act != NULL also compact: the index
+// of every nonzero entry goes to act[out[i]]
+__global__ __launch_bounds__(kBlock) void k_scan_p3(
+    int64_t n, const int32_t *__restrict__ in, int32_t *__restrict__ out,
+    const int32_t *__restrict__ bsum, int32_t *__restrict__ act) {
+  const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  const int32_t b = bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t i = i0 + k;
+    if (i < n) {
+      const int32_t o = out[i] + b;
+      out[i] = o;
+      if (act && in[i]) act[o] = (int32_t)i;
+    }
+  }
 }
 
 int exclusive_scan(int64_t n, const int32_t *in, int32_t *out, int32_t *bsum,
-                   int32_t *total, hipStream_t s) {
-  int nb = (int)grid_for(n);
+                   int32_t *total, int32_t *act, hipStream_t s) {
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
   hipLaunchKernelGGL(k_scan_p1, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum);
   hipLaunchKernelGGL(k_scan_p2, dim3(1), dim3(kBlock), 0, s, nb, bsum, total);
-  hipLaunchKernelGGL(k_scan_p3, dim3(nb), dim3(kBlock), 0, s, n, out, bsum);
+  hipLaunchKernelGGL(k_scan_p3, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum,
+                     act);
   return check_launch("exclusive_scan");
 }
 
-__global__ void k_t3_compact(int64_t nt, const int32_t *__restrict__ flag,
-                             const int32_t *__restrict__ pos,
-                             int32_t *__restrict__ act) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < nt && flag[e]) act[pos[e]] = (int32_t)e;
-}
-// per node: number of active incident (element, local) entries.  The
-// byte flags of kT3Unroll consecutive entries are fetched together (the
-// node loops are bound by these independent gathers, not by arithmetic).
-constexpr int kT3Unroll = 8;
-__global__ void k_t3_count(int64_t nn, const int32_t *__restrict__ iptr,
-                           const int32_t *__restrict__ ient,
-                           const uint8_t *__restrict__ flag8,
-                           const uint8_t *__restrict__ fmask,
-                           int32_t *__restrict__ cnt) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
+// Per block of kBlock nodes, whose incidence entries are the contiguous
+// range [iptr[v0], iptr[v1]): the active flags of the entries (the
+// element's flag, 0 on rows that fmask marks Dirichlet) are gathered in
+// coalesced chunks through LDS, written in incidence order (fq), and
+// counted per node (cnt).
+constexpr int kT3Chunk = 2048;
+__global__ __launch_bounds__(kBlock) void k_t3_count(
+    int64_t nn, int64_t nb_pad, const int32_t *__restrict__ iptr,
+    const int32_t *__restrict__ ient, const uint8_t *__restrict__ flag8,
+    const uint8_t *__restrict__ fmask, uint8_t *__restrict__ fq,
+    int32_t *__restrict__ cnt) {
+  __shared__ uint8_t fl[kT3Chunk];
+  constexpr int P = kT3Chunk / kBlock;
+  const int64_t v0 = xcd_block(blockIdx.x, nb_pad) * kBlock;
+  if (v0 >= nn) return;                        // uniform over the block
+  const int64_t v1 = v0 + kBlock < nn ? v0 + kBlock : nn;
+  const int64_t v = v0 + threadIdx.x;
+  const int64_t qb = iptr[v0], qe = iptr[v1];
+  int64_t q = 0, q1 = 0;
+  bool on = false;
+  if (v < v1) {
+    q = iptr[v];
+    q1 = iptr[v + 1];
+    on = !fmask || fmask[v];
+  }
+  uint8_t nf[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t i = qb + threadIdx.x + k * kBlock;
+    nf[k] = i < qe ? flag8[ient[i] >> 2] : (uint8_t)0;
+  }
   int32_t c = 0;
-  if (!fmask || fmask[v]) {
-    const int32_t q1 = iptr[v + 1];
-    for (int32_t q = iptr[v]; q < q1; q += kT3Unroll) {
-      uint8_t f[kT3Unroll];
+  for (int64_t c0 = qb; c0 < qe; c0 += kT3Chunk) {
+    __syncthreads();                           // previous chunk written out
 #pragma unroll
-      for (int k = 0; k < kT3Unroll; ++k)
-        f[k] = (q + k < q1) ? flag8[ient[q + k] >> 2] : (uint8_t)0;
+    for (int k = 0; k < P; ++k) fl[threadIdx.x + k * kBlock] = nf[k];
+    __syncthreads();
 #pragma unroll
-      for (int k = 0; k < kT3Unroll; ++k) c += f[k];
+    for (int k = 0; k < P; ++k) {
+      const int64_t i = c0 + kT3Chunk + threadIdx.x + k * kBlock;
+      nf[k] = i < qe ? flag8[ient[i] >> 2] : (uint8_t)0;
+    }
+    const int64_t hi = q1 < c0 + kT3Chunk ? q1 : c0 + kT3Chunk;
+    for (; q < hi; ++q) {
+      if (on)
+        c += fl[q - c0];
+      else
+        fl[q - c0] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int64_t i = c0 + threadIdx.x + k * kBlock;
+      if (i < qe) fq[i] = fl[threadIdx.x + k * kBlock];
     }
   }
-  cnt[v] = c;
+  if (v < v1) cnt[v] = c;
 }
-__global__ void k_t3_fill(int64_t nn, const int32_t *__restrict__ iptr,
-                          const int32_t *__restrict__ ient,
-                          const uint8_t *__restrict__ flag8,
-                          const uint8_t *__restrict__ fmask,
-                          const int32_t *__restrict__ rowptr,
-                          int32_t *__restrict__ ent) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nn) return;
-  if (fmask && !fmask[v]) return;
-  int32_t o = rowptr[v];
-  const int32_t q1 = iptr[v + 1];
-  for (int32_t q = iptr[v]; q < q1; q += kT3Unroll) {
-    int32_t en[kT3Unroll];
-    uint8_t f[kT3Unroll];
+// Per block of kBlock nodes: the active entries of the block's incidence
+// range, kept in incidence order (= by node, then ascending element), go
+// to ent[rowptr[v0] + running count] with their weight beside them (Wq), so
+// the matvec reads both contiguously.  Lane t of a chunk holds entries
+// c0 + 256 k + t: each wave compacts its 64 entries by ballot, the 32
+// (k, wave) counts are prefix-summed from LDS, and loads and stores stay
+// coalesced.
+constexpr int kT3FillP = 8;
+constexpr int kT3FillChunk = kBlock * kT3FillP;
+__global__ __launch_bounds__(kBlock) void k_t3_fill(
+    int64_t nn, int64_t nb_pad, const int32_t *__restrict__ iptr,
+    const int32_t *__restrict__ ient, const uint8_t *__restrict__ fq,
+    const double *__restrict__ Wi, const int32_t *__restrict__ rowptr,
+    int32_t *__restrict__ ent, double *__restrict__ Wq) {
+  constexpr int NW = kBlock / 64;
+  __shared__ int32_t wc[kT3FillP * NW];
+  const int64_t v0 = xcd_block(blockIdx.x, nb_pad) * kBlock;
+  if (v0 >= nn) return;                        // uniform over the block
+  const int64_t v1 = v0 + kBlock < nn ? v0 + kBlock : nn;
+  const int64_t qb = iptr[v0], qe = iptr[v1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int64_t run = rowptr[v0];
+  for (int64_t c0 = qb; c0 < qe; c0 += kT3FillChunk) {
+    uint64_t m[kT3FillP];
 #pragma unroll
-    for (int k = 0; k < kT3Unroll; ++k) {
-      en[k] = (q + k < q1) ? ient[q + k] : 0;
-      f[k] = (q + k < q1) ? flag8[en[k] >> 2] : (uint8_t)0;
+    for (int k = 0; k < kT3FillP; ++k) {
+      const int64_t q = c0 + k * kBlock + threadIdx.x;
+      const bool f = q < qe && fq[q] != 0;
+      m[k] = __ballot(f);
+      if (lane == 0) wc[k * NW + w] = __popcll(m[k]);
     }
+    __syncthreads();
+    // tot = active entries of the chunk before slot (k, j), slots taken in
+    // q order (k-major, then wave)
+    int32_t tot = 0;
 #pragma unroll
-    for (int k = 0; k < kT3Unroll; ++k)
-      if (f[k]) ent[o++] = en[k];
+    for (int k = 0; k < kT3FillP; ++k) {
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        if (j == w && ((m[k] >> lane) & 1ull)) {
+          const int64_t q = c0 + k * kBlock + threadIdx.x;
+          const int64_t o = run + tot + __popcll(m[k] & below);
+          ent[o] = ient[q];
+          Wq[o] = Wi[q];
+        }
+        tot += wc[k * NW + j];
+      }
+    }
+    run += tot;
+    __syncthreads();                           // wc reused next chunk
   }
 }
 // t_e = sum_k (grad_u . B_u,k) x[elements[u,k]] for the active elements
@@ -699,15 +797,16 @@ int64_t fp3d_term3_workspace_bytes(int64_t n_tets, int64_t n_nodes) {
 
 int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
                      const int32_t *inc_ptr, const int32_t *inc_ent,
-                     const double *B, const double *V, const double *grad,
+                     const int32_t *inc_pos, const double *B, const double *V, const double *grad,
                      const int32_t *upstream, const double *s_u,
                      const uint8_t *lim, const uint8_t *free_mask,
                      double inv_u2, void *t3, int64_t *n_active_out,
                      void *stream) {
   FP3D_REQUIRE(n_tets > 0 && n_nodes > 0, "term3_build: empty mesh");
-  FP3D_REQUIRE(elements && inc_ptr && inc_ent && B && V && grad && upstream &&
-                   s_u && t3 && n_active_out,
+  FP3D_REQUIRE(elements && inc_ptr && inc_ent && inc_pos && B && V && grad &&
+                   upstream && s_u && t3,
                "term3_build: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(inc_pos), "inc_pos must be 16-byte aligned");
   hipStream_t s = FP3D_STREAM(stream);
   T3Layout L = t3_layout(n_tets, n_nodes);
   char *base = (char *)t3;
@@ -721,22 +820,34 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
   int32_t *cnt = (int32_t *)(base + L.cnt);
   int32_t *rowptr = (int32_t *)(base + L.rowptr);
   int32_t *ent = (int32_t *)(base + L.ent);
+  uint8_t *fq = (uint8_t *)(base + L.fq);
+  double *Wq = (double *)(base + L.Wq);
   int32_t *hdr = (int32_t *)(base + L.total - 256);
   hipLaunchKernelGGL(k_t3_coef, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
                      n_tets, elements, B, V, grad, upstream, s_u, lim, inv_u2,
-                     flag, flag8, W, Cc);
-  int rc = exclusive_scan(n_tets, flag, scan, bsum, hdr, s);
+                     inc_pos, flag, flag8, W, Cc);
+  int rc = exclusive_scan(n_tets, flag, scan, bsum, hdr, act, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_t3_compact, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
-                     n_tets, flag, scan, act);
-  hipLaunchKernelGGL(k_t3_count, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s,
-                     n_nodes, inc_ptr, inc_ent, flag8, free_mask, cnt);
-  rc = exclusive_scan(n_nodes, cnt, rowptr, bsum, rowptr + n_nodes, s);
+  const unsigned gn = xcd_grid((n_nodes + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_t3_count, dim3(gn), dim3(kBlock), 0, s, n_nodes,
+                     (int64_t)gn, inc_ptr, inc_ent, flag8, free_mask, fq, cnt);
+  rc = exclusive_scan(n_nodes, cnt, rowptr, bsum, rowptr + n_nodes, nullptr, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_t3_fill, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s,
-                     n_nodes, inc_ptr, inc_ent, flag8, free_mask, rowptr, ent);
+  hipLaunchKernelGGL(k_t3_fill, dim3(gn), dim3(kBlock), 0, s, n_nodes,
+                     (int64_t)gn, inc_ptr, inc_ent, fq, W, rowptr, ent, Wq);
   rc = check_launch("term3_build");
   if (rc) return rc;
+  if (!n_active_out) return FP3D_OK;
+  return fp3d_term3_active_count(t3, n_tets, n_nodes, n_active_out, stream);
+}
+
+int fp3d_term3_active_count(const void *t3, int64_t n_tets, int64_t n_nodes,
+                            int64_t *n_active_out, void *stream) {
+  FP3D_REQUIRE(t3 && n_active_out && n_tets > 0 && n_nodes > 0,
+               "term3_active_count: bad argument");
+  hipStream_t s = FP3D_STREAM(stream);
+  const T3Layout L = t3_layout(n_tets, n_nodes);
+  const int32_t *hdr = (const int32_t *)((const char *)t3 + L.total - 256);
   int32_t na = 0;
   FP3D_HIP(hipMemcpyAsync(&na, hdr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   FP3D_HIP(hipStreamSynchronize(s));
@@ -768,7 +879,7 @@ int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
                        n_active, act, elements, upstream, Cc, x, tv);
     t3ptr = (const int32_t *)(base + L.rowptr);
     t3ent = (const int32_t *)(base + L.ent);
-    W = (const double *)(base + L.W);
+    W = (const double *)(base + L.Wq);
     tval = tv;
   }
   int G = lanes_for(n_rows, nnz);

@@ -314,7 +314,7 @@ class NewtonOperator:
                                                           store.n_nodes)
         self.t3 = torch.empty(int(nbytes), dtype=torch.uint8,
                               device=store.device)
-        self.n_active = 0
+        self._n_active: Optional[int] = 0
         self.upstream = None
 
     def finalize(self, f32: bool = False) -> None:
@@ -329,19 +329,36 @@ class NewtonOperator:
                      ptr(self.data32) if f32 else None, stream_handle())
         self.is_f32 = 1 if f32 else 0
 
-    def build_term3(self, grad, upstream, s_u, lim, inv_u2) -> int:
-        import ctypes as C
+    @property
+    def n_active(self) -> int:
+        """Active Term-3 elements of the last build (read from the device on
+        first use after a build: one stream synchronisation)."""
+        if self._n_active is None:
+            import ctypes as C
 
+            st = self.store
+            na = C.c_int64(0)
+            _native.check(_native.lib().fp3d_term3_active_count(
+                ptr(self.t3), st.n_tets, st.n_nodes, C.byref(na), stream_handle()),
+                "term3_active_count")
+            self._n_active = int(na.value)
+        return self._n_active
+
+    @n_active.setter
+    def n_active(self, value: int) -> None:
+        self._n_active = int(value)
+
+    def build_term3(self, grad, upstream, s_u, lim, inv_u2) -> None:
+        """Select the active elements and build the Term-3 row table on the
+        stream (no synchronisation; n_active is read when first needed)."""
         st = self.store
-        na = C.c_int64(0)
         _native.check(_native.lib().fp3d_term3_build(
             st.n_tets, st.n_nodes, ptr(st.el), ptr(st.inc_ptr), ptr(st.inc_ent),
-            ptr(st.B), ptr(st.V), ptr(grad), ptr(upstream), ptr(s_u), ptr(lim),
-            ptr(self.free_mask), float(inv_u2), ptr(self.t3), C.byref(na),
+            ptr(st.inc_pos), ptr(st.B), ptr(st.V), ptr(grad), ptr(upstream), ptr(s_u),
+            ptr(lim), ptr(self.free_mask), float(inv_u2), ptr(self.t3), None,
             stream_handle()), "term3_build")
-        self.n_active = int(na.value)
+        self._n_active = None
         self.upstream = upstream
-        return self.n_active
 
     def matvec(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
         st = self.store
