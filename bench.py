@@ -52,15 +52,15 @@ def algorithmic_bytes(n_tets, n_nodes, nnz):
     """Compulsory HBM bytes of each phase (each input read once, each
     output written once; float64 state, int32 connectivity)."""
     return {
-        # elements 16, B 96, write grad 24 + q2n 8 + q2l 8 + rho 8 + lim 1;
-        # phi 8/node
-        "state": 161 * n_tets + 8 * n_nodes,
+        # elements 16, B 96, write grad 24 + q2n 8 + q2l 8 + rho 8 + lim 1
+        # + grad.B_a 32; phi 8/node
+        "state": 193 * n_tets + 8 * n_nodes,
         # fn 16, own centroid 24, elements 16, grad 24, q2 8, rho 8; write
         # upstream 4, nu 8, rho_tilde 8; nodes 24/node
         "upwind": 116 * n_tets + 24 * n_nodes,
-        # node gather: incidence 16, B 96, V 8, rho_tilde 8, grad 24;
+        # node gather: incidence 16, grad.B_a 32, V 8, rho_tilde 8;
         # inc_ptr 4 + R 8 per node
-        "residual": 152 * n_tets + 12 * n_nodes,
+        "residual": 64 * n_tets + 12 * n_nodes,
         # upstream 4, q2 8, rho 8, write s_e 8, s_u 8 (+ q2/rho of u(e),
         # non-local, counted once above)
         "sensitivities": 36 * n_tets,
@@ -129,20 +129,22 @@ def main():
     q2l = torch.empty_like(q2n)
     rho = torch.empty_like(q2n)
     rt = torch.empty_like(q2n)
+    gb = torch.empty(4 * nt, dtype=torch.float64, device=dev)
     lim = torch.empty(nt, dtype=torch.uint8, device=dev)
     fm = torch.ones(nn, dtype=torch.uint8, device=dev)
     jac = NewtonOperator(st, fm)
     q2cap = q2_at_mach(3.0, M_INF, GAMMA)
 
     def ph_state():
-        op.state_dev(phi, 1.0, M_INF, GAMMA, q2cap, grad, q2n, q2l, rho, lim)
+        op.state_dev(phi, 1.0, M_INF, GAMMA, q2cap, grad, q2n, q2l, rho, lim,
+                     gdotb=gb)
 
     def ph_upwind():
         upw.rho_tilde_dev(grad, q2l, rho, M_INF, UPWIND_C, M_CRIT, GAMMA,
                           RHO_FLOOR, monitors=False)
 
     def ph_residual():
-        op.residual_dev(upw.d_rt, grad)
+        op.residual_dev(upw.d_rt, grad, gdotb=gb)
 
     def ph_sens():
         # the Newton driver reuses the residual's walk (same grad -> same u)
@@ -153,7 +155,7 @@ def main():
         op.jacobian_dev(upw.d_rt, grad, upw.d_se, lim, 1.0, jac.data)
 
     def ph_t3():
-        jac.build_term3(grad, upw.d_up, upw.d_su, lim, 1.0)
+        jac.build_term3(grad, upw.d_up, upw.d_su, lim, 1.0, gdotb=gb)
 
     phases = [("state", ph_state), ("upwind", ph_upwind),
               ("residual", ph_residual), ("sensitivities", ph_sens),

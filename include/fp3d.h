@@ -68,12 +68,16 @@ int fp3d_element_velocity_q2(int64_t n_tets, const int32_t *elements,
 
 /* Fused state sweep of NewtonWorkspace.eval_residual
  * (pyfp3d/solve/newton.py:345-351): grad, q2n = q2/u2, q2l = min(q2n, q2cap),
- * rho = density(q2l), lim = (q2l == q2n).  lim_out may be NULL. */
+ * rho = density(q2l), lim = (q2l == q2n).  lim_out may be NULL.
+ * gdotb_out (nullable, float64[4 n_tets], 32-byte aligned) receives
+ * grad_e . B[e,a,:] for a = 0..3, which fp3d_element_residual_terms and
+ * fp3d_term3_build take in place of grad and B (bitwise the same values
+ * they would compute). */
 int fp3d_element_state(int64_t n_tets, const int32_t *elements, const double *B,
                        const double *phi, double u2, double m_inf, double gamma,
                        double q2cap, double *grad_out, double *q2n_out,
                        double *q2l_out, double *rho_out, uint8_t *lim_out,
-                       void *stream);
+                       double *gdotb_out, void *stream);
 
 /* ------------------------------------------------------------ upwinding */
 /* Multi-hop upstream walk u(e); pyfp3d/kernels/upwind.py:68-125
@@ -205,7 +209,8 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
  * classes (pyfp3d/kernels/jacobian.py:231-375 PicardOperator).
  *
  * Two stages: the element pass writes terms[4e + a] = rho_tilde V
- * (grad . B_a), or the 16 matrix values K[16e + 4a + b]; the node pass sums
+ * (grad . B_a) (grad . B_a read from gdotb when that is given, B and grad
+ * may then be NULL), or the 16 matrix values K[16e + 4a + b]; the node pass sums
  * them per node / per CSR slot.  With inc_pos (int32[4 n_tets], the inverse
  * of inc_ent, 16-byte aligned) the element pass writes in incidence order
  * instead: terms[inc_pos[4e + a]], matrix row a at K[4 inc_pos[4e + a]];
@@ -221,8 +226,9 @@ int fp3d_assemble_jacobian_spd_colored(int n_colors,
 int fp3d_gather_rows_per_block(void);
 int fp3d_element_residual_terms(int64_t n_tets, const double *B,
                                 const double *V, const double *rho_tilde,
-                                const double *grad, const int32_t *inc_pos,
-                                double *terms, void *stream);
+                                const double *grad, const double *gdotb,
+                                const int32_t *inc_pos, double *terms,
+                                void *stream);
 int fp3d_gather_node_terms(int64_t n_nodes, const int32_t *inc_ptr,
                            const int32_t *inc_ent, const double *terms,
                            double *R, void *stream);
@@ -254,7 +260,8 @@ int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
  * stores their coefficients and a row-gather table built from the static
  * node -> (4*element + local) incidence CSR (inc_ptr int32[n_nodes+1],
  * inc_ent int32[4*n_tets], entries of a node in ascending element order;
- * inc_pos int32[4*n_tets] its inverse, 16-byte aligned),
+ * inc_pos int32[4*n_tets] its inverse, 16-byte aligned; gdotb nullable:
+ * the state sweep's grad . B_i per element, used in place of grad and B),
  * so fp3d_csr_spmv_term3 adds Term 3 row by row in a fixed order: no
  * atomics, bitwise repeatable.  Rows with free_mask[row] == 0 (nullable)
  * get no Term-3 entries.  `t3` is caller-owned device memory of
@@ -265,7 +272,8 @@ int fp3d_fill_term3_coo(int64_t n_active, const int32_t *active,
 int64_t fp3d_term3_workspace_bytes(int64_t n_tets, int64_t n_nodes);
 int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
                      const int32_t *inc_ptr, const int32_t *inc_ent,
-                     const int32_t *inc_pos, const double *B, const double *V, const double *grad,
+                     const int32_t *inc_pos, const double *B,
+                     const double *V, const double *grad, const double *gdotb,
                      const int32_t *upstream, const double *s_u,
                      const uint8_t *lim, const uint8_t *free_mask,
                      double inv_u2, void *t3, int64_t *n_active_out,

@@ -161,6 +161,18 @@ def test_gather_assembly_bitwise(mesh, ora):
     R = to_host(op.residual_dev(rtd, gd, torch.empty(mesh.n_nodes, dtype=torch.float64,
                                                      device=rtd.device)))
     assert np.array_equal(R, oo.assemble_residual(ora["phi"], rt, g))
+    # residual terms from the state sweep's grad . B (gdotb): the same bits
+    ph = to_dev(ora["phi"])
+    gst = torch.empty((n, 3), dtype=torch.float64, device=rtd.device)
+    q2n, q2l, rh = (torch.empty(n, dtype=torch.float64, device=rtd.device)
+                    for _ in range(3))
+    gb = torch.empty(4 * n, dtype=torch.float64, device=rtd.device)
+    op.state_dev(ph, 1.0, M, 1.4, 1e30, gst, q2n, q2l, rh, None, gdotb=gb)
+    Ra = to_host(op.residual_dev(rtd, gst, torch.empty(mesh.n_nodes, dtype=torch.float64,
+                                                       device=rtd.device)))
+    Rb = to_host(op.residual_dev(rtd, gst, torch.empty(mesh.n_nodes, dtype=torch.float64,
+                                                       device=rtd.device), gdotb=gb))
+    assert np.array_equal(Ra, Rb) and np.abs(Ra).max() > 0.0
     out = torch.full((op.nnz,), np.nan, dtype=torch.float64, device=rtd.device)
     assert np.array_equal(to_host(op.matrix_dev(rtd, out)), oo.assemble_matrix(rt).data)
     out.fill_(np.nan)
@@ -180,7 +192,7 @@ def test_gather_assembly_bitwise(mesh, ora):
     t = torch.empty(4 * n, dtype=torch.float64, device=rtd.device)
     R2 = torch.empty(mesh.n_nodes, dtype=torch.float64, device=rtd.device)
     _native.call("fp3d_element_residual_terms", n, ptr(st.B), ptr(st.V), ptr(rtd),
-                 ptr(gd), None, ptr(t), stream_handle())
+                 ptr(gd), None, None, ptr(t), stream_handle())
     _native.call("fp3d_gather_node_terms", mesh.n_nodes, ptr(st.inc_ptr),
                  ptr(st.inc_ent), ptr(t), ptr(R2), stream_handle())
     assert np.array_equal(to_host(R2), R)
@@ -295,6 +307,20 @@ def test_term3_device_operator(mesh, ora):
     # repeatability
     y1 = to_host(jop.matvec(xd, yd)).copy()
     assert np.array_equal(y1, to_host(jop.matvec(xd, yd)))
+    # the build fed the state sweep's grad . B (gdotb) instead of grad and B:
+    # the same table, so the same bits
+    ph = to_dev(ora["phi"])
+    gst = torch.empty((mesh.n_tets, 3), dtype=torch.float64, device=st.device)
+    q2n, q2l, rh = (torch.empty(mesh.n_tets, dtype=torch.float64, device=st.device)
+                    for _ in range(3))
+    gb = torch.empty(4 * mesh.n_tets, dtype=torch.float64, device=st.device)
+    op.state_dev(ph, 1.0, M, 1.4, 1e30, gst, q2n, q2l, rh, None, gdotb=gb)
+    ys = []
+    for g_arg in (None, gb):
+        jop.build_term3(gst, up, su, lim, 1.0, gdotb=g_arg)
+        ys.append(to_host(jop.matvec(xd, yd)).copy())
+    assert jop.n_active == na
+    assert np.array_equal(ys[0], ys[1])
 
 
 def test_gmres_matches_scipy(mesh, ora, gpu):

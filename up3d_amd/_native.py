@@ -38,7 +38,7 @@ _SIGS = {
     "fp3d_density_derivative_field": (_int, [_i64, _vp, _dbl, _dbl, _vp, _vp]),
     "fp3d_element_velocity_q2": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_element_state": (_int, [_i64, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
-                                  _vp, _vp, _vp, _vp, _vp, _vp]),
+                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_upstream_elements": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp,
                                       _vp]),
     "fp3d_upstream_elements_d": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _int,
@@ -77,7 +77,7 @@ _SIGS = {
                                    _dbl, _vp, _vp, _vp, _vp]),
     "fp3d_term3_workspace_bytes": (_i64, [_i64, _i64]),
     "fp3d_term3_build": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                _vp, _vp, _vp, _vp, _dbl, _vp, _pi64, _vp]),
+                                _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _pi64, _vp]),
     "fp3d_term3_active_count": (_int, [_vp, _i64, _i64, _pi64, _vp]),
     "fp3d_csr_spmv": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_spmv_f32": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -86,7 +86,8 @@ _SIGS = {
     "fp3d_csr_mask_dirichlet": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_csr_to_f32": (_int, [_i64, _vp, _vp, _vp]),
     "fp3d_gather_rows_per_block": (_int, []),
-    "fp3d_element_residual_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_element_residual_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _vp]),
     "fp3d_gather_node_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_element_matrix_terms": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl,
                                          _vp, _vp, _vp]),

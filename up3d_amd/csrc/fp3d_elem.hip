@@ -115,7 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_element_state(
     const double *__restrict__ B, const double *__restrict__ phi, double u2,
     double m, double g, double cap, double *__restrict__ grad,
     double *__restrict__ q2n_out, double *__restrict__ q2l_out,
-    double *__restrict__ rho_out, uint8_t *__restrict__ lim_out) {
+    double *__restrict__ rho_out, uint8_t *__restrict__ lim_out,
+    double *__restrict__ gdotb) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double gx, gy, gz;
@@ -123,6 +124,18 @@ __global__ __launch_bounds__(kBlock) void k_element_state(
   grad[3 * e] = gx;
   grad[3 * e + 1] = gy;
   grad[3 * e + 2] = gz;
+  if (gdotb) {
+    // grad . B_a, the factor every later pass applies to node a: the same
+    // expression (and so the same bits) the residual and Term-3 passes
+    // evaluate when they are not handed it
+    const double *b = B + 12 * e;
+    double4 o;
+    o.x = gx * b[0] + gy * b[1] + gz * b[2];
+    o.y = gx * b[3] + gy * b[4] + gz * b[5];
+    o.z = gx * b[6] + gy * b[7] + gz * b[8];
+    o.w = gx * b[9] + gy * b[10] + gz * b[11];
+    *reinterpret_cast<double4 *>(gdotb + 4 * e) = o;
+  }
   double q2 = gx * gx + gy * gy + gz * gz;
   double q2n = q2 / u2;
   double q2l = q2n < cap ? q2n : cap;
@@ -706,17 +719,26 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
 __global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
     int64_t n, const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ rt, const double *__restrict__ grad,
-    const int32_t *__restrict__ pos, double *__restrict__ c) {
+    const double *__restrict__ gdotb, const int32_t *__restrict__ pos,
+    double *__restrict__ c) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double w = rt[e] * V[e];
-  double gx = grad[3 * e], gy = grad[3 * e + 1], gz = grad[3 * e + 2];
-  const double *b = B + 12 * e;
   double4 o;
-  o.x = w * (gx * b[0] + gy * b[1] + gz * b[2]);
-  o.y = w * (gx * b[3] + gy * b[4] + gz * b[5]);
-  o.z = w * (gx * b[6] + gy * b[7] + gz * b[8]);
-  o.w = w * (gx * b[9] + gy * b[10] + gz * b[11]);
+  if (gdotb) {
+    const double4 gb = *reinterpret_cast<const double4 *>(gdotb + 4 * e);
+    o.x = w * gb.x;
+    o.y = w * gb.y;
+    o.z = w * gb.z;
+    o.w = w * gb.w;
+  } else {
+    double gx = grad[3 * e], gy = grad[3 * e + 1], gz = grad[3 * e + 2];
+    const double *b = B + 12 * e;
+    o.x = w * (gx * b[0] + gy * b[1] + gz * b[2]);
+    o.y = w * (gx * b[3] + gy * b[4] + gz * b[5]);
+    o.z = w * (gx * b[6] + gy * b[7] + gz * b[8]);
+    o.w = w * (gx * b[9] + gy * b[10] + gz * b[11]);
+  }
   if (pos) {
     // incidence order: term a of e goes to the slot of (e, a) in its node's
     // incidence list, so the node pass reads its terms contiguously
@@ -1091,14 +1113,16 @@ int fp3d_element_state(int64_t n_tets, const int32_t *elements, const double *B,
                        const double *phi, double u2, double m_inf, double gamma,
                        double q2cap, double *grad_out, double *q2n_out,
                        double *q2l_out, double *rho_out, uint8_t *lim_out,
-                       void *stream) {
+                       double *gdotb_out, void *stream) {
   FP3D_REQUIRE(n_tets == 0 || (elements && B && phi && grad_out && q2n_out &&
                                q2l_out && rho_out),
                "element_state: NULL pointer");
   FP3D_REQUIRE(FP3D_ALIGNED16(elements), "elements must be 16-byte aligned");
+  FP3D_REQUIRE(((uintptr_t)gdotb_out & 31) == 0,
+               "gdotb must be 32-byte aligned");
   FP3D_REQUIRE(u2 > 0.0, "element_state: u_inf^2 must be > 0");
   ELEMWISE(k_element_state, n_tets, elements, B, phi, u2, m_inf, gamma, q2cap,
-           grad_out, q2n_out, q2l_out, rho_out, lim_out);
+           grad_out, q2n_out, q2l_out, rho_out, lim_out, gdotb_out);
 }
 
 int fp3d_upstream_elements(int64_t n_tets, const int32_t *face_neighbors,
@@ -1350,18 +1374,20 @@ int fp3d_gather_rows_per_block(void) { return kGatherRows; }
 
 int fp3d_element_residual_terms(int64_t n_tets, const double *B,
                                 const double *V, const double *rho_tilde,
-                                const double *grad, const int32_t *inc_pos,
-                                double *terms, void *stream) {
+                                const double *grad, const double *gdotb,
+                                const int32_t *inc_pos, double *terms,
+                                void *stream) {
   FP3D_REQUIRE(n_tets >= 0, "element_residual_terms: n_tets < 0");
   if (n_tets == 0) return FP3D_OK;
-  FP3D_REQUIRE(B && V && rho_tilde && grad && terms,
+  FP3D_REQUIRE(V && rho_tilde && terms && (gdotb || (B && grad)),
                "element_residual_terms: NULL pointer");
+  FP3D_REQUIRE(((uintptr_t)gdotb & 31) == 0, "gdotb must be 32-byte aligned");
   FP3D_REQUIRE(((uintptr_t)terms & 31) == 0, "terms must be 32-byte aligned");
   FP3D_REQUIRE(!inc_pos || ((uintptr_t)inc_pos & 15) == 0,
                "inc_pos must be 16-byte aligned");
   hipLaunchKernelGGL(k_elem_residual_terms, dim3(grid_for(n_tets, g_elem_block)),
                      dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets, B, V,
-                     rho_tilde, grad, inc_pos, terms);
+                     rho_tilde, grad, gdotb, inc_pos, terms);
   return check_launch("element_residual_terms");
 }
 

@@ -477,10 +477,12 @@ inline T3Layout t3_layout(int64_t nt, int64_t nn) {
 
 // per element: masked s_u, activity flag and the two coefficient sets (the
 // weights W of the element's four entries in incidence order)
+template <bool HAS_GB>
 __global__ __launch_bounds__(kBlock) void k_t3_coef(
     int64_t nt, const int32_t *__restrict__ elements,
     const double *__restrict__ B, const double *__restrict__ V,
-    const double *__restrict__ grad, const int32_t *__restrict__ up,
+    const double *__restrict__ grad, const double *__restrict__ gdotb,
+    const int32_t *__restrict__ up,
     const double *__restrict__ s_u, const uint8_t *__restrict__ lim,
     double inv_u2, const int32_t *__restrict__ pos,
     int32_t *__restrict__ flag, uint8_t *__restrict__ flag8,
@@ -494,11 +496,22 @@ __global__ __launch_bounds__(kBlock) void k_t3_coef(
   flag[e] = act ? 1 : 0;
   flag8[e] = act ? 1 : 0;
   if (!act) return;
-  double ge0 = grad[3 * e], ge1 = grad[3 * e + 1], ge2 = grad[3 * e + 2];
-  double gu0 = grad[3 * u], gu1 = grad[3 * u + 1], gu2 = grad[3 * u + 2];
   double w = 2.0 * inv_u2 * s * V[e];
   const int4 p = *reinterpret_cast<const int4 *>(pos + 4 * e);
   const int32_t pa[4] = {p.x, p.y, p.z, p.w};
+  if (HAS_GB) {
+    // grad . B_i of e and of u, from the state sweep (same bits)
+    const double4 ge = *reinterpret_cast<const double4 *>(gdotb + 4 * e);
+    const double4 gu = *reinterpret_cast<const double4 *>(gdotb + 4 * u);
+    Wi[pa[0]] = w * ge.x;
+    Wi[pa[1]] = w * ge.y;
+    Wi[pa[2]] = w * ge.z;
+    Wi[pa[3]] = w * ge.w;
+    *reinterpret_cast<double4 *>(Cc + 4 * e) = gu;
+    return;
+  }
+  double ge0 = grad[3 * e], ge1 = grad[3 * e + 1], ge2 = grad[3 * e + 2];
+  double gu0 = grad[3 * u], gu1 = grad[3 * u + 1], gu2 = grad[3 * u + 2];
   for (int i = 0; i < 4; ++i) {
     double gi = ge0 * B[12 * e + 3 * i] + ge1 * B[12 * e + 3 * i + 1] +
                 ge2 * B[12 * e + 3 * i + 2];
@@ -797,7 +810,8 @@ int64_t fp3d_term3_workspace_bytes(int64_t n_tets, int64_t n_nodes) {
 
 int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
                      const int32_t *inc_ptr, const int32_t *inc_ent,
-                     const int32_t *inc_pos, const double *B, const double *V, const double *grad,
+                     const int32_t *inc_pos, const double *B,
+                     const double *V, const double *grad, const double *gdotb,
                      const int32_t *upstream, const double *s_u,
                      const uint8_t *lim, const uint8_t *free_mask,
                      double inv_u2, void *t3, int64_t *n_active_out,
@@ -807,6 +821,8 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
                    upstream && s_u && t3,
                "term3_build: NULL pointer");
   FP3D_REQUIRE(FP3D_ALIGNED16(inc_pos), "inc_pos must be 16-byte aligned");
+  FP3D_REQUIRE(((uintptr_t)gdotb & 31) == 0 && ((uintptr_t)t3 & 255) == 0,
+               "term3_build: gdotb 32-byte / t3 256-byte alignment");
   hipStream_t s = FP3D_STREAM(stream);
   T3Layout L = t3_layout(n_tets, n_nodes);
   char *base = (char *)t3;
@@ -823,9 +839,14 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
   uint8_t *fq = (uint8_t *)(base + L.fq);
   double *Wq = (double *)(base + L.Wq);
   int32_t *hdr = (int32_t *)(base + L.total - 256);
-  hipLaunchKernelGGL(k_t3_coef, dim3(grid_for(n_tets)), dim3(kBlock), 0, s,
-                     n_tets, elements, B, V, grad, upstream, s_u, lim, inv_u2,
-                     inc_pos, flag, flag8, W, Cc);
+  if (gdotb)
+    hipLaunchKernelGGL(k_t3_coef<true>, dim3(grid_for(n_tets)), dim3(kBlock), 0,
+                       s, n_tets, elements, B, V, grad, gdotb, upstream, s_u,
+                       lim, inv_u2, inc_pos, flag, flag8, W, Cc);
+  else
+    hipLaunchKernelGGL(k_t3_coef<false>, dim3(grid_for(n_tets)), dim3(kBlock),
+                       0, s, n_tets, elements, B, V, grad, gdotb, upstream,
+                       s_u, lim, inv_u2, inc_pos, flag, flag8, W, Cc);
   int rc = exclusive_scan(n_tets, flag, scan, bsum, hdr, act, s);
   if (rc) return rc;
   const unsigned gn = xcd_grid((n_nodes + kBlock - 1) / kBlock);

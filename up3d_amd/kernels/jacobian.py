@@ -154,24 +154,32 @@ class PicardOperator:
                      ptr(st.B), ptr(phi), ptr(grad), ptr(q2), stream_handle())
         return grad, q2
 
-    def state_dev(self, phi, u2, m_inf, gamma, q2cap, grad, q2n, q2l, rho, lim):
-        """Fused grad / q2 / limiter / density sweep."""
+    def state_dev(self, phi, u2, m_inf, gamma, q2cap, grad, q2n, q2l, rho, lim,
+                  gdotb=None):
+        """Fused grad / q2 / limiter / density sweep; gdotb (n_tets*4,
+        optional) receives grad_e . B_e,a for residual_dev / Term 3 of the
+        same state."""
         st = self.store
         _native.call("fp3d_element_state", self.n_tets, ptr(st.el), ptr(st.B),
                      ptr(phi), float(u2), float(m_inf), float(gamma),
                      float(q2cap), ptr(grad), ptr(q2n), ptr(q2l), ptr(rho),
-                     ptr(lim), stream_handle())
+                     ptr(lim), ptr(gdotb) if gdotb is not None else None,
+                     stream_handle())
 
     def residual_dev(self, rho_tilde: torch.Tensor, grad: torch.Tensor,
-                     out: Optional[torch.Tensor] = None) -> torch.Tensor:
+                     out: Optional[torch.Tensor] = None,
+                     gdotb: Optional[torch.Tensor] = None) -> torch.Tensor:
         """R_v = sum over incident (e, i) of rho_tilde_e V_e (grad_e . B_e,i):
-        element terms, then the node gather in ascending element order."""
+        element terms, then the node gather in ascending element order.
+        gdotb: grad . B of the same state from state_dev (then read instead
+        of recomputed; the same bits)."""
         st = self.store
         R = self.d_R if out is None else out
         # terms written in incidence order (slot inc_pos[4e + a]), so the
         # node pass streams them
         _native.call("fp3d_element_residual_terms", self.n_tets, ptr(st.B),
-                     ptr(st.V), ptr(rho_tilde), ptr(grad), ptr(st.inc_pos),
+                     ptr(st.V), ptr(rho_tilde), ptr(grad),
+                     ptr(gdotb) if gdotb is not None else None, ptr(st.inc_pos),
                      ptr(self.d_terms), stream_handle())
         _native.call("fp3d_gather_node_terms", self.n_nodes, ptr(st.inc_ptr),
                      None, ptr(self.d_terms), ptr(R), stream_handle())

@@ -348,13 +348,15 @@ class NewtonOperator:
     def n_active(self, value: int) -> None:
         self._n_active = int(value)
 
-    def build_term3(self, grad, upstream, s_u, lim, inv_u2) -> None:
+    def build_term3(self, grad, upstream, s_u, lim, inv_u2, gdotb=None) -> None:
         """Select the active elements and build the Term-3 row table on the
-        stream (no synchronisation; n_active is read when first needed)."""
+        stream (no synchronisation; n_active is read when first needed).
+        gdotb: grad . B of the same state (state sweep output), optional."""
         st = self.store
         _native.check(_native.lib().fp3d_term3_build(
             st.n_tets, st.n_nodes, ptr(st.el), ptr(st.inc_ptr), ptr(st.inc_ent),
-            ptr(st.inc_pos), ptr(st.B), ptr(st.V), ptr(grad), ptr(upstream), ptr(s_u),
+            ptr(st.inc_pos), ptr(st.B), ptr(st.V), ptr(grad),
+            ptr(gdotb) if gdotb is not None else None, ptr(upstream), ptr(s_u),
             ptr(lim), ptr(self.free_mask), float(inv_u2), ptr(self.t3), None,
             stream_handle()), "term3_build")
         self._n_active = None

@@ -51,6 +51,7 @@ class _State:
     def __init__(self, n, nt):
         self.phi = empty(n)
         self.grad = empty((nt, 3))
+        self.gb = empty(4 * nt)    # grad . B_a per element (state sweep)
         self.q2n = empty(nt)
         self.q2l = empty(nt)
         self.rho = empty(nt)
@@ -145,7 +146,7 @@ class NewtonWorkspace:
         the full potential (Dirichlet entries set)."""
         op, upw = self.op, self.upw
         op.state_dev(s.phi, self.u_inf ** 2, self.m_inf, self.gamma_air,
-                     self.q2cap, s.grad, s.q2n, s.q2l, s.rho, s.lim)
+                     self.q2cap, s.grad, s.q2n, s.q2l, s.rho, s.lim, gdotb=s.gb)
         _native.call("fp3d_count_limited", self.nt, ptr(s.lim), ptr(self.work),
                      ptr(self.cnt), stream_handle())
         s.n_limited = int(self.cnt.item())
@@ -163,7 +164,7 @@ class NewtonWorkspace:
         s.n_floored = upw.n_floored
         s.nu_max = upw.nu_max
         s.n_nu_active = upw.n_supersonic
-        R = op.residual_dev(s.rt, s.grad)
+        R = op.residual_dev(s.rt, s.grad, gdotb=s.gb)
         if self.b_ext is not None:
             axpby(-1.0, self.b_ext, 1.0, R)
         masked(self.fm, R, s.R)
@@ -200,7 +201,7 @@ class NewtonWorkspace:
         self._diag_shift(self.jac.data, ptc_scale)
         self.jac.finalize(f32=self.precision == "fp32")
         if self.m_inf > 0.0:
-            self.jac.build_term3(s.grad, upstream, su, s.lim, inv_u2)
+            self.jac.build_term3(s.grad, upstream, su, s.lim, inv_u2, gdotb=s.gb)
         else:
             self.jac.n_active = 0
         return self.jac
