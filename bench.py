@@ -4,10 +4,11 @@ on the M_inf = 0.84 bump channel, 192x64x64 hexes -> 4,718,592 tets,
 interior nodes jittered +-15 %, RCM-renumbered.
 
 A "step" is one Newton linearisation pass over the whole mesh with inputs
-resident in HBM: fused state sweep (grad, q2, limiter, density), upstream
-walk + nu switch + rho_tilde, coloured residual scatter, walk + exact
-sensitivities, coloured Newton Jacobian (Terms 1+2) into the fixed CSR and
-the Term-3 gather table.  `value` = n_tets / step time (Mtet/s).
+resident in HBM: fused state sweep (grad, q2, limiter, density, grad.B),
+upstream walk + nu switch + rho_tilde + the exact sensitivities of that
+selection, the residual (element terms + node gather), the Newton Jacobian
+Terms 1+2 (element matrices + CSR row gather) and the Term-3 gather table.
+`value` = n_tets / step time (Mtet/s).
 
 The `solve` object holds the Newton solve seconds to ||R_free||_inf < 1e-10
 at M_inf = 0.84 (grid-sequenced 48x16x16 -> 96x32x32 -> 192x64x64): FP64
@@ -56,14 +57,12 @@ def algorithmic_bytes(n_tets, n_nodes, nnz):
         # + grad.B_a 32; phi 8/node
         "state": 193 * n_tets + 8 * n_nodes,
         # fn 16, own centroid 24, elements 16, grad 24, q2 8, rho 8; write
-        # upstream 4, nu 8, rho_tilde 8; nodes 24/node
-        "upwind": 116 * n_tets + 24 * n_nodes,
+        # upstream 4, nu 8, rho_tilde 8, s_e 8, s_u 8 (+ q2/rho of u(e),
+        # non-local); nodes 24/node
+        "upwind": 132 * n_tets + 24 * n_nodes,
         # node gather: incidence 16, grad.B_a 32, V 8, rho_tilde 8;
         # inc_ptr 4 + R 8 per node
         "residual": 64 * n_tets + 12 * n_nodes,
-        # upstream 4, q2 8, rho 8, write s_e 8, s_u 8 (+ q2/rho of u(e),
-        # non-local, counted once above)
-        "sensitivities": 36 * n_tets,
         # node gather: incidence 16, elem_to_csr 64, B 96, V 8, rho_tilde 8,
         # grad 24, s_e 8, lim 1; indptr + inc_ptr 8 per node; CSR values
         # written once 8/nnz
@@ -139,27 +138,26 @@ def main():
         op.state_dev(phi, 1.0, M_INF, GAMMA, q2cap, grad, q2n, q2l, rho, lim,
                      gdotb=gb)
 
+    se = torch.empty_like(q2n)
+    su = torch.empty_like(q2n)
+
     def ph_upwind():
+        # the walk's sweep also writes the exact s_e, s_u of its selection,
+        # as the Newton residual evaluation does
         upw.rho_tilde_dev(grad, q2l, rho, M_INF, UPWIND_C, M_CRIT, GAMMA,
-                          RHO_FLOOR, monitors=False)
+                          RHO_FLOOR, monitors=False, se_out=se, su_out=su)
 
     def ph_residual():
         op.residual_dev(upw.d_rt, grad, gdotb=gb)
 
-    def ph_sens():
-        # the Newton driver reuses the residual's walk (same grad -> same u)
-        upw.sensitivities_at_dev(q2l, rho, upw.d_up, M_INF, UPWIND_C, M_CRIT,
-                                 GAMMA, RHO_FLOOR)
-
     def ph_jac():
-        op.jacobian_dev(upw.d_rt, grad, upw.d_se, lim, 1.0, jac.data)
+        op.jacobian_dev(upw.d_rt, grad, se, lim, 1.0, jac.data)
 
     def ph_t3():
-        jac.build_term3(grad, upw.d_up, upw.d_su, lim, 1.0, gdotb=gb)
+        jac.build_term3(grad, upw.d_up, su, lim, 1.0, gdotb=gb)
 
     phases = [("state", ph_state), ("upwind", ph_upwind),
-              ("residual", ph_residual), ("sensitivities", ph_sens),
-              ("jacobian", ph_jac), ("term3", ph_t3)]
+              ("residual", ph_residual), ("jacobian", ph_jac), ("term3", ph_t3)]
 
     def step():
         for _, f in phases:

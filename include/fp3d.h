@@ -103,6 +103,17 @@ int fp3d_upwind_rho_tilde_d(int64_t n_tets, const int32_t *face_neighbors,
                             double rho_floor, int32_t *upstream_out,
                             double *nu_out, double *rho_tilde_out,
                             void *stream);
+/* The same sweep plus the exact sensitivities of the selection it makes
+ * (s_e, s_u: pyfp3d/kernels/upwind.py:733-920, as
+ * fp3d_rho_tilde_sensitivities on upstream_out), from one read of q2 / rho
+ * of e and of u. */
+int fp3d_upwind_rho_tilde_sens_d(
+    int64_t n_tets, const int32_t *face_neighbors, const double *face_disp,
+    const double *nodes, const int32_t *elements, const double *grad,
+    const double *q2, const double *rho, int max_hops, double m_inf,
+    double m_crit, double C, double gamma, double rho_floor,
+    int32_t *upstream_out, double *nu_out, double *rho_tilde_out,
+    double *s_e_out, double *s_u_out, void *stream);
 /* pyfp3d/kernels/upwind.py:456-500 rho_tilde_sweep. */
 int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,
                          const int32_t *upstream, double m_inf, double m_crit,

@@ -323,6 +323,39 @@ def test_term3_device_operator(mesh, ora):
     assert np.array_equal(ys[0], ys[1])
 
 
+def test_fused_walk_sensitivities_bitwise(mesh, ora):
+    """Walk + rho_tilde + (s_e, s_u) in one sweep == the walk sweep followed
+    by the sensitivities at its upstream (same bits), and both match the
+    oracle's upstream / rho_tilde / sensitivities."""
+    import torch
+
+    from up3d_amd.device import to_host
+    from up3d_amd.kernels.upwind import UpwindOperator
+
+    upw = UpwindOperator(mesh.nodes, mesh.elements)
+    st = upw.store
+    q2, rho, g = ora["q2"], ora["rho"], ora["g"]
+    gs, q2s, rhos = st.to_storage(g.reshape(-1, 3)), st.to_storage(q2), st.to_storage(rho)
+    n = mesh.n_tets
+    for floor in (0.05, 0.9):
+        up_a = torch.empty(n, dtype=torch.int32, device=st.device)
+        rt_a = upw.rho_tilde_dev(gs, q2s, rhos, M, C, MC, rho_floor=floor,
+                                 up_out=up_a, rt_out=torch.empty_like(q2s)).clone()
+        se_a, su_a = (t.clone() for t in upw.sensitivities_at_dev(q2s, rhos, up_a, M, C, MC,
+                                                                   rho_floor=floor))
+        up_b = torch.empty_like(up_a)
+        se_b, su_b = torch.empty_like(q2s), torch.empty_like(q2s)
+        rt_b = upw.rho_tilde_dev(gs, q2s, rhos, M, C, MC, rho_floor=floor, up_out=up_b,
+                                 rt_out=torch.empty_like(q2s), se_out=se_b, su_out=su_b)
+        assert torch.equal(up_a, up_b) and torch.equal(rt_a, rt_b)
+        assert torch.equal(se_a, se_b) and torch.equal(su_a, su_b)
+        assert np.array_equal(to_host(up_b).astype(np.int64), ora["up"])
+        se_o, su_o = O.rho_tilde_sensitivities(q2, rho, ora["up"], M, MC, C, G, floor)
+        assert _ulp_close(to_host(se_b), se_o) and _ulp_close(to_host(su_b), su_o)
+        if floor > 0.5:
+            assert np.count_nonzero(to_host(rt_b) == floor) > 0
+
+
 def test_gmres_matches_scipy(mesh, ora, gpu):
     import scipy.sparse.linalg as spla
 

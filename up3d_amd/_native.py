@@ -46,6 +46,9 @@ _SIGS = {
     "fp3d_upwind_rho_tilde_d": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                        _int, _dbl, _dbl, _dbl, _dbl, _dbl, _vp,
                                        _vp, _vp, _vp]),
+    "fp3d_upwind_rho_tilde_sens_d": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _int, _dbl, _dbl, _dbl, _dbl, _dbl, _vp,
+                                            _vp, _vp, _vp, _vp, _vp]),
     "fp3d_rho_tilde_sweep": (_int, [_i64, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
                                     _dbl, _vp, _vp, _vp]),
     "fp3d_upwind_rho_tilde": (_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -380,6 +380,63 @@ __device__ __forceinline__ double d_dnu(double q2, double m2, double m,
   return C * (mc2 / (m2 * m2)) * d_dmach2(q2, m, g);
 }
 
+// Walk + switch + rho_tilde + the exact sensitivities (s_e, s_u) of the
+// same selection in one sweep: the Newton residual evaluation keeps them for
+// the Jacobian of an accepted state, so the linearisation skips its own
+// pass over q2 / rho of e and of u.  The same expressions as sweep_one and
+// k_sens, so the same bits.
+__global__ __launch_bounds__(kBlock) void k_upwind_sens_d(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, const double *__restrict__ q2,
+    const double *__restrict__ rho, int max_hops, double m, double mc2,
+    double C, double g, double floor_, int32_t *__restrict__ up_out,
+    double *__restrict__ nu_out, double *__restrict__ rt_out,
+    double *__restrict__ se, double *__restrict__ su) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int32_t u = walk_upstream_d(e, fn, dcen, nodes, elements, grad[3 * e],
+                                    grad[3 * e + 1], grad[3 * e + 2], max_hops);
+  if (up_out) up_out[e] = u;
+  const double q2e = q2[e], q2u = q2[u];
+  const double rho_e = rho[e], rho_u = rho[u];
+  const double m2 = d_mach2(q2e, m, g);
+  const double m2u = d_mach2(q2u, m, g);
+  const double nu_e = d_nu(m2, mc2, C);
+  const double nu_u = d_nu(m2u, mc2, C);
+  const double nu = nu_e >= nu_u ? nu_e : nu_u;
+  const double rt = rho_e - nu * (rho_e - rho_u);
+  nu_out[e] = nu;
+  rt_out[e] = rt > floor_ ? rt : floor_;
+  const double drho_e = d_drho(q2e, m, g);
+  if (u == e) {
+    se[e] = rho_e <= floor_ ? 0.0 : drho_e;
+    su[e] = 0.0;
+    return;
+  }
+  if (rt <= floor_) {
+    se[e] = 0.0;
+    su[e] = 0.0;
+    return;
+  }
+  if (nu == 0.0) {
+    se[e] = drho_e;
+    su[e] = 0.0;
+    return;
+  }
+  const double drho_u = d_drho(q2u, m, g);
+  const double djump = rho_e - rho_u;
+  if (nu_e >= nu_u) {
+    const double dnu = m2 > mc2 ? d_dnu(q2e, m2, m, mc2, C, g) : 0.0;
+    se[e] = drho_e * (1.0 - nu) - djump * dnu;
+    su[e] = nu * drho_u;
+  } else {
+    const double dnu = d_dnu(q2u, m2u, m, mc2, C, g);
+    se[e] = drho_e * (1.0 - nu);
+    su[e] = nu * drho_u - djump * dnu;
+  }
+}
+
 // pyfp3d/kernels/upwind.py:596-643
 __global__ __launch_bounds__(kBlock) void k_frozen_sens(
     int64_t n, const double *__restrict__ q2, const double *__restrict__ rho,
@@ -1169,6 +1226,25 @@ int fp3d_upwind_rho_tilde_d(int64_t n_tets, const int32_t *face_neighbors,
   ELEMWISE(k_upwind_fused_d, n_tets, face_neighbors, face_disp, nodes, elements,
            grad, q2, rho, max_hops, m_inf, m_crit * m_crit, C, gamma,
            rho_floor, upstream_out, nu_out, rho_tilde_out);
+}
+
+int fp3d_upwind_rho_tilde_sens_d(
+    int64_t n_tets, const int32_t *face_neighbors, const double *face_disp,
+    const double *nodes, const int32_t *elements, const double *grad,
+    const double *q2, const double *rho, int max_hops, double m_inf,
+    double m_crit, double C, double gamma, double rho_floor,
+    int32_t *upstream_out, double *nu_out, double *rho_tilde_out,
+    double *s_e_out, double *s_u_out, void *stream) {
+  FP3D_REQUIRE(n_tets == 0 || (face_neighbors && face_disp && nodes &&
+                               elements && grad && q2 && rho && nu_out &&
+                               rho_tilde_out && s_e_out && s_u_out),
+               "upwind_rho_tilde_sens_d: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements) && FP3D_ALIGNED16(face_neighbors) &&
+                   FP3D_ALIGNED16(face_disp),
+               "upwind_rho_tilde_sens_d: arrays must be 16-byte aligned");
+  ELEMWISE(k_upwind_sens_d, n_tets, face_neighbors, face_disp, nodes, elements,
+           grad, q2, rho, max_hops, m_inf, m_crit * m_crit, C, gamma,
+           rho_floor, upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out);
 }
 
 int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,

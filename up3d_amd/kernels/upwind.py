@@ -152,17 +152,29 @@ class UpwindOperator:
         return out
 
     def rho_tilde_dev(self, grad, q2, rho, m_inf, C, m_crit, gamma=GAMMA,
-                      rho_floor=0.05, monitors=True, up_out=None, rt_out=None):
+                      rho_floor=0.05, monitors=True, up_out=None, rt_out=None,
+                      se_out=None, su_out=None):
         """Fused walk + switch + rho_tilde; monitors (nu_max, n_supersonic,
-        n_floored) refreshed by a deterministic device reduction."""
+        n_floored) refreshed by a deterministic device reduction.  With
+        se_out / su_out the same sweep also writes the exact sensitivities
+        of its selection (as sensitivities_at_dev on the walk's upstream)."""
         st = self.store
         up = self.d_up if up_out is None else up_out
         rt = self.d_rt if rt_out is None else rt_out
-        _native.call("fp3d_upwind_rho_tilde_d", st.n_tets, ptr(st.fn),
-                     ptr(st.dcen), ptr(st.nodes), ptr(st.el), ptr(grad), ptr(q2),
-                     ptr(rho), 4, float(m_inf), float(m_crit), float(C),
-                     float(gamma), float(rho_floor), ptr(up), ptr(self.d_nu),
-                     ptr(rt), stream_handle())
+        if (se_out is None) != (su_out is None):
+            raise ValueError("rho_tilde_dev: give both se_out and su_out or neither")
+        if se_out is not None:
+            _native.call("fp3d_upwind_rho_tilde_sens_d", st.n_tets, ptr(st.fn),
+                         ptr(st.dcen), ptr(st.nodes), ptr(st.el), ptr(grad), ptr(q2),
+                         ptr(rho), 4, float(m_inf), float(m_crit), float(C),
+                         float(gamma), float(rho_floor), ptr(up), ptr(self.d_nu),
+                         ptr(rt), ptr(se_out), ptr(su_out), stream_handle())
+        else:
+            _native.call("fp3d_upwind_rho_tilde_d", st.n_tets, ptr(st.fn),
+                         ptr(st.dcen), ptr(st.nodes), ptr(st.el), ptr(grad), ptr(q2),
+                         ptr(rho), 4, float(m_inf), float(m_crit), float(C),
+                         float(gamma), float(rho_floor), ptr(up), ptr(self.d_nu),
+                         ptr(rt), stream_handle())
         if monitors:
             self._monitors(self.d_nu, rt, rho_floor)
         return rt

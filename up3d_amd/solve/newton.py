@@ -58,6 +58,9 @@ class _State:
         self.rt = empty(nt)
         self.lim = empty(nt, torch.uint8)
         self.up = empty(nt, torch.int32)   # upstream of the live walk
+        self.se = empty(nt)        # exact s_e, s_u of that walk's selection
+        self.su = empty(nt)
+        self.sens_key = None       # (M, C, M_crit, gamma, floor) they are for
         self.R = empty(n)          # free-masked residual
         self.n_limited = 0
         self.n_floored = 0
@@ -150,14 +153,18 @@ class NewtonWorkspace:
         _native.call("fp3d_count_limited", self.nt, ptr(s.lim), ptr(self.work),
                      ptr(self.cnt), stream_handle())
         s.n_limited = int(self.cnt.item())
+        s.sens_key = None
         if self.m_inf > 0.0 and frozen is not None:
             upw.frozen_dev(s.q2l, s.rho, frozen[0], frozen[1], self.m_inf,
                            upwind_c, m_crit, self.gamma_air, rho_floor)
             s.rt.copy_(upw.d_rt)
         elif self.m_inf > 0.0:
+            # the walk's sweep also keeps the exact sensitivities of its
+            # selection, which assemble_coupled reuses for this state
             upw.rho_tilde_dev(s.grad, s.q2l, s.rho, self.m_inf, upwind_c, m_crit,
                               self.gamma_air, rho_floor, up_out=s.up,
-                              rt_out=s.rt)
+                              rt_out=s.rt, se_out=s.se, su_out=s.su)
+            s.sens_key = (self.m_inf, upwind_c, m_crit, self.gamma_air, rho_floor)
         else:
             s.rt.copy_(s.rho)
             upw.nu_max, upw.n_supersonic, upw.n_floored = 0.0, 0, 0
@@ -190,13 +197,17 @@ class NewtonWorkspace:
             # the live walk on this state's grad already ran in eval_residual
             # (pyfp3d re-walks the same grad, which yields the same u(e))
             upstream = s.up
-            se, su = upw.sensitivities_at_dev(
-                s.q2l, s.rho, upstream, self.m_inf, upwind_c, m_crit,
-                self.gamma_air, rho_floor)
+            if s.sens_key == (self.m_inf, upwind_c, m_crit, self.gamma_air, rho_floor):
+                se, su = s.se, s.su
+            else:
+                se, su = upw.sensitivities_at_dev(
+                    s.q2l, s.rho, upstream, self.m_inf, upwind_c, m_crit,
+                    self.gamma_air, rho_floor)
         else:
             se = upw.d_se.zero_()
             su = upw.d_su.zero_()
             upstream = upw.walk_dev(s.grad)
+        self.se_used = se          # the preconditioner's SPD matrix reuses it
         self.op.jacobian_dev(s.rt, s.grad, se, s.lim, inv_u2, self.jac.data)
         self._diag_shift(self.jac.data, ptc_scale)
         self.jac.finalize(f32=self.precision == "fp32")
@@ -508,7 +519,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                            and it - last_lin[3] >= 3))
         if rebuild:
             M_pre = ws.build_preconditioner(cur, precond_matrix, precond_clamp,
-                                            se=ws.upw.d_se, ptc_scale=ptc_scale)
+                                            se=ws.se_used, ptc_scale=ptc_scale)
             last_lin[2] = -1
             last_lin[3] = it
         sync()
