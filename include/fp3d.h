@@ -446,6 +446,26 @@ int fp3d_host_csr_pattern(int64_t n_tets, const int32_t *elements,
 int fp3d_host_elem_to_csr(int64_t n_tets, const int32_t *elements,
                           const int32_t *indptr, const int32_t *indices,
                           int32_t *out);
+/* Row-parallel host products of the AMG set-up (pyfp3d/solve/linear.py:
+ * 61-81 builds the hierarchy with pyamg on scipy.sparse).  C = A B (CSR,
+ * int32 indices) in two passes: fp3d_host_spgemm writes C's row pointer
+ * (n_rows + 1, caller array) and *nnz_out, fp3d_host_spgemm_fill the
+ * caller's indices / data.  Each row is accumulated exactly as
+ * scipy.sparse's csr_matmat (A entries in stored order, then B entries,
+ * sums from 0; exact zeros dropped), so C carries scipy's values; columns
+ * ascend.  fp3d_host_csr_matvec: y = A x, each row summed from 0 in stored
+ * order (scipy's csr_matvec).  Threads: at most 16, or FP3D_HOST_THREADS;
+ * results do not depend on the count. */
+int fp3d_host_spgemm(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
+                     const int32_t *Aj, const double *Ax, const int32_t *Bp,
+                     const int32_t *Bj, const double *Bx, int32_t *indptr,
+                     int64_t *nnz_out);
+int fp3d_host_spgemm_fill(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
+                          const int32_t *Aj, const double *Ax, const int32_t *Bp,
+                          const int32_t *Bj, const double *Bx,
+                          const int32_t *indptr, int32_t *indices, double *data);
+int fp3d_host_csr_matvec(int64_t n_rows, const int32_t *Ap, const int32_t *Aj,
+                         const double *Ax, const double *x, double *y);
 /* pyfp3d/mesh/metrics.py:207-278 build_face_adjacency (int32 (n,4)). */
 int fp3d_host_face_adjacency(int64_t n_tets, const int32_t *elements,
                              int32_t *face_neighbors);

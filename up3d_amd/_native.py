@@ -133,6 +133,11 @@ _SIGS = {
     "fp3d_host_greedy_coloring": (_int, [_i64, _vp, _i64, _vp, _pi32]),
     "fp3d_host_csr_pattern": (_int, [_i64, _vp, _i64, _vp, _vp, _pi64]),
     "fp3d_host_elem_to_csr": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "fp3d_host_spgemm": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _pi64]),
+    "fp3d_host_spgemm_fill": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp, _vp]),
+    "fp3d_host_csr_matvec": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_host_face_adjacency": (_int, [_i64, _vp, _vp]),
     "fp3d_host_element_geometry": (_int, [_i64, _vp, _vp, _vp, _vp, _pi64]),
     "fp3d_host_locate_points": (_int, [_i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),

@@ -48,14 +48,15 @@ def build(verbose: bool = False) -> Path:
         cmd = [hipcc, *HIP_FLAGS, "-c", str(HERE / src), "-o", str(obj)]
         if src.endswith(".cpp"):
             cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off",
-                   "-c", str(HERE / src), "-o", str(obj)]
+                   "-pthread", "-c", str(HERE / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(str(obj))
     out = LIB_DIR / LIB_NAME
     tmp = LIB_DIR / (LIB_NAME + ".tmp")
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-pthread",
+           "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

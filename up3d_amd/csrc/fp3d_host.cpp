@@ -10,6 +10,9 @@
 #include <numeric>
 #include <string>
 #include <vector>
+#include <climits>
+#include <cstdlib>
+#include <thread>
 
 #include "../../include/fp3d.h"
 
@@ -48,9 +51,145 @@ int check_elements(int64_t n_tets, const int32_t *el, int64_t n_nodes) {
   return FP3D_OK;
 }
 
+// Host worker threads for the row-parallel set-up products: at most 16, or
+// FP3D_HOST_THREADS.  Each output row is produced by one thread in a fixed
+// order, so results do not depend on the thread count.
+unsigned host_threads() {
+  unsigned t = std::thread::hardware_concurrency();
+  if (t == 0) t = 1;
+  if (t > 16) t = 16;
+  if (const char *e = std::getenv("FP3D_HOST_THREADS")) {
+    int v = std::atoi(e);
+    if (v >= 1) t = (unsigned)v;
+  }
+  return t;
+}
+
+// Rows split over threads in proportion to the work (stored entries of
+// the rows): a thread costs tens of microseconds to start, so one thread
+// per 256K entries at most.
+template <class F>
+void parallel_rows(int64_t n, int64_t work, F f) {
+  int64_t want = work / (1 << 18);
+  unsigned T = host_threads();
+  if (want < (int64_t)T) T = want < 1 ? 1u : (unsigned)want;
+  if (T <= 1 || n < 2 * (int64_t)T) {
+    f(0, n, 0u);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (unsigned t = 0; t < T; ++t)
+    th.emplace_back(f, n * (int64_t)t / T, n * (int64_t)(t + 1) / T, t);
+  for (auto &x : th) x.join();
+}
+
+// One row of C = A B accumulated as scipy.sparse's csr_matmat does (A
+// entries in stored order, then the B entries of that row, sums from 0);
+// emit(k, value) is called for the nonzero sums in ascending column k.
+struct RowProduct {
+  std::vector<double> sums;
+  std::vector<uint8_t> seen;
+  std::vector<int32_t> touched;
+  explicit RowProduct(int64_t n_cols) : sums(n_cols, 0.0), seen(n_cols, 0) {}
+  template <class Emit>
+  int32_t run(int64_t i, const int32_t *Ap, const int32_t *Aj, const double *Ax,
+              const int32_t *Bp, const int32_t *Bj, const double *Bx, Emit emit) {
+    touched.clear();
+    for (int32_t jj = Ap[i]; jj < Ap[i + 1]; ++jj) {
+      const int32_t j = Aj[jj];
+      const double v = Ax[jj];
+      for (int32_t kk = Bp[j]; kk < Bp[j + 1]; ++kk) {
+        const int32_t k = Bj[kk];
+        sums[k] += v * Bx[kk];
+        if (!seen[k]) {
+          seen[k] = 1;
+          touched.push_back(k);
+        }
+      }
+    }
+    std::sort(touched.begin(), touched.end());
+    int32_t cnt = 0;
+    for (const int32_t k : touched) {
+      if (sums[k] != 0.0) emit(cnt++, k, sums[k]);
+      sums[k] = 0.0;
+      seen[k] = 0;
+    }
+    return cnt;
+  }
+};
+
 }  // namespace
 
 extern "C" {
+
+/* Row-parallel sparse product C = A B on the host, in two passes: the
+ * first writes the row pointer of C (caller array, n_rows + 1) and its
+ * entry count, the second fills the caller's indices and data.  Row i is
+ * accumulated exactly as scipy.sparse's csr_matmat does (A entries in
+ * stored order, then B entries, sums from 0) and exact zeros are dropped,
+ * so C carries scipy's values; columns ascend. */
+int fp3d_host_spgemm(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
+                     const int32_t *Aj, const double *Ax, const int32_t *Bp,
+                     const int32_t *Bj, const double *Bx, int32_t *indptr,
+                     int64_t *nnz_out) {
+  HREQ(n_rows >= 0 && n_cols >= 0 && indptr && nnz_out, "spgemm: bad arguments");
+  HREQ(n_rows == 0 || (Ap && Bp), "spgemm: NULL pointer");
+  indptr[0] = 0;
+  parallel_rows(n_rows, n_rows ? (int64_t)Ap[n_rows] : 0,
+                [&](int64_t lo, int64_t hi, unsigned) {
+    RowProduct rp(n_cols);
+    for (int64_t i = lo; i < hi; ++i)
+      indptr[i + 1] = rp.run(i, Ap, Aj, Ax, Bp, Bj, Bx,
+                             [](int32_t, int32_t, double) {});
+  });
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < n_rows; ++i) {
+    nnz += indptr[i + 1];
+    HREQ(nnz <= INT32_MAX, "spgemm: product has more than 2^31 entries");
+    indptr[i + 1] = (int32_t)nnz;
+  }
+  *nnz_out = nnz;
+  return FP3D_OK;
+}
+
+int fp3d_host_spgemm_fill(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
+                          const int32_t *Aj, const double *Ax, const int32_t *Bp,
+                          const int32_t *Bj, const double *Bx,
+                          const int32_t *indptr, int32_t *indices, double *data) {
+  HREQ(n_rows >= 0 && n_cols >= 0 && indptr, "spgemm_fill: bad arguments");
+  HREQ(n_rows == 0 || indptr[n_rows] == 0 || (indices && data),
+       "spgemm_fill: NULL output");
+  parallel_rows(n_rows, n_rows ? (int64_t)Ap[n_rows] : 0,
+                [&](int64_t lo, int64_t hi, unsigned) {
+    RowProduct rp(n_cols);
+    for (int64_t i = lo; i < hi; ++i) {
+      int32_t *ci = indices + indptr[i];
+      double *cx = data + indptr[i];
+      rp.run(i, Ap, Aj, Ax, Bp, Bj, Bx, [ci, cx](int32_t c, int32_t k, double v) {
+        ci[c] = k;
+        cx[c] = v;
+      });
+    }
+  });
+  return FP3D_OK;
+}
+
+/* y = A x on the host, row-parallel; row i sums from 0 in stored order, as
+ * scipy.sparse's csr_matvec. */
+int fp3d_host_csr_matvec(int64_t n_rows, const int32_t *Ap, const int32_t *Aj,
+                         const double *Ax, const double *x, double *y) {
+  HREQ(n_rows >= 0 && (n_rows == 0 || (Ap && x && y)), "csr_matvec: bad arguments");
+  parallel_rows(n_rows, n_rows ? (int64_t)Ap[n_rows] : 0,
+                [&](int64_t lo, int64_t hi, unsigned) {
+    for (int64_t i = lo; i < hi; ++i) {
+      double s = 0.0;
+      for (int32_t jj = Ap[i]; jj < Ap[i + 1]; ++jj) s += Ax[jj] * x[Aj[jj]];
+      y[i] = s;
+    }
+  });
+  return FP3D_OK;
+}
 
 int fp3d_host_greedy_coloring(int64_t n_tets, const int32_t *elements,
                               int64_t n_nodes, int32_t *colors,

@@ -7,8 +7,9 @@ strength (theta = 0), standard aggregation, constant near-nullspace,
 tentative prolongator smoothed by damped Jacobi with
 omega = (4/3) / rho(D^-1 A), Galerkin coarse operators R A P with
 R = P^T, at most 10 levels, coarsest size <= 10 solved by a pseudo-inverse.
-This module follows that recipe for the set-up (aggregation in the
-library's host code, products with scipy.sparse) and replaces pyamg's
+This module follows that recipe for the set-up (aggregation and the
+sparse products in the library's row-parallel host code, which reproduces
+scipy.sparse's values) and replaces pyamg's
 sequential Gauss-Seidel smoother by Chebyshev polynomial smoothing on
 D^-1 A, which is fully parallel on the GPU and keeps the V-cycle symmetric
 (same polynomial before and after the coarse correction), so it remains a
@@ -46,6 +47,62 @@ def standard_aggregation(A: sp.csr_matrix):
     return agg, int(na.value)
 
 
+def _csr32(M: sp.csr_matrix):
+    return (np.ascontiguousarray(M.indptr, dtype=np.int32),
+            np.ascontiguousarray(M.indices, dtype=np.int32),
+            np.ascontiguousarray(M.data, dtype=np.float64))
+
+
+# below this many stored entries of A the single-pass scipy product is the
+# faster of the two (the values are the same)
+_HOST_PARALLEL_NNZ = 1 << 21
+
+
+def spgemm(A: sp.csr_matrix, B: sp.csr_matrix, force_native: bool = False
+           ) -> sp.csr_matrix:
+    """A @ B with the library's row-parallel host product: the values of
+    scipy's product (same per-row accumulation order, exact zeros dropped),
+    columns sorted.  Small products go to scipy itself."""
+    A, B = A.tocsr(), B.tocsr()
+    if A.shape[1] != B.shape[0]:
+        raise ValueError(f"spgemm: shapes {A.shape} and {B.shape} do not align")
+    if A.nnz < _HOST_PARALLEL_NNZ and not force_native:
+        out = (A @ B).tocsr()
+        out.sort_indices()
+        return out
+    ap, aj, ax = _csr32(A)
+    bp, bj, bx = _csr32(B)
+    args = (A.shape[0], B.shape[1], ap.ctypes.data, aj.ctypes.data, ax.ctypes.data,
+            bp.ctypes.data, bj.ctypes.data, bx.ctypes.data)
+    L = _native.lib()
+    ip = np.empty(A.shape[0] + 1, dtype=np.int32)
+    nnz = C.c_int64(0)
+    _native.check(L.fp3d_host_spgemm(*args, ip.ctypes.data, C.byref(nnz)),
+                  "host_spgemm")
+    ix = np.empty(int(nnz.value), dtype=np.int32)
+    dv = np.empty(int(nnz.value), dtype=np.float64)
+    _native.check(L.fp3d_host_spgemm_fill(*args, ip.ctypes.data, ix.ctypes.data,
+                                          dv.ctypes.data), "host_spgemm_fill")
+    out = sp.csr_matrix((dv, ix, ip), shape=(A.shape[0], B.shape[1]))
+    out.has_sorted_indices = True
+    return out
+
+
+def csr_matvec(A: sp.csr_matrix, x: np.ndarray, force_native: bool = False
+               ) -> np.ndarray:
+    """A @ x with the library's row-parallel host matvec (scipy's values);
+    small matrices go to scipy itself."""
+    if A.nnz < _HOST_PARALLEL_NNZ and not force_native:
+        return A @ x
+    ap, aj, ax = _csr32(A)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(A.shape[0], dtype=np.float64)
+    _native.check(_native.lib().fp3d_host_csr_matvec(
+        A.shape[0], ap.ctypes.data, aj.ctypes.data, ax.ctypes.data, x.ctypes.data,
+        y.ctypes.data), "host_csr_matvec")
+    return y
+
+
 def spectral_radius_dinv_a(A: sp.csr_matrix, dinv: np.ndarray, iters: int = 12,
                            seed: int = 0) -> float:
     """Power-iteration estimate of rho(D^-1 A) from a seeded start vector."""
@@ -53,7 +110,7 @@ def spectral_radius_dinv_a(A: sp.csr_matrix, dinv: np.ndarray, iters: int = 12,
     x = rng.random(A.shape[0]) + 0.5
     lam = 1.0
     for _ in range(iters):
-        y = dinv * (A @ x)
+        y = dinv * csr_matvec(A, x)
         ny = np.linalg.norm(y)
         if ny == 0.0:
             return 1.0
@@ -92,13 +149,12 @@ def sa_setup(A: sp.csr_matrix, max_levels: int = 10, max_coarse: int = 10,
         cnt = np.bincount(agg[rows], minlength=n_agg).astype(np.float64)
         T = sp.csr_matrix((1.0 / np.sqrt(cnt[agg[rows]]), (rows, agg[rows])),
                           shape=(n, n_agg))
-        AT = A @ T
+        AT = spgemm(A, T)
         P = (T - sp.diags((omega / rho) * dinv) @ AT).tocsr()
         P.sort_indices()
         R = P.T.tocsr()
         R.sort_indices()
-        Ac = (R @ (A @ P)).tocsr()
-        Ac.sort_indices()
+        Ac = spgemm(R, spgemm(A, P))
         levels.append(HostLevel(A, P, R, dinv, rho))
         A = Ac
     coarse = np.linalg.pinv(levels[-1].A.toarray())
