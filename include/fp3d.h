@@ -381,7 +381,9 @@ int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
 
 /* ----------------------------------------------------- AMG V-cycle kernels
  * Chebyshev smoothing (degree >= 1) of A x = b on D^-1 A with eigenvalue
- * interval [lmin, lmax]; zero_init != 0 starts from x = 0 (x is written).
+ * interval [lmin, lmax]; zero_init = 1 starts from x = 0 (x is written),
+ * 0 from the given x, 2 continues after a first step from x = 0 that the
+ * caller already took (x, r, d0 as fp3d_amg_restrict_start leaves them).
  * r, d0, d1 are n-vectors of scratch. */
 int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
                          const int32_t *indices, const void *data, int is_f32,
@@ -389,6 +391,16 @@ int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
                          double *r, double *d0, double *d1, int degree,
                          double lmin, double lmax, int zero_init,
                          void *stream);
+/* bc = R r, and the coarse level's first Chebyshev step from x_c = 0 on
+ * bc (dinv_c, degree_c, [lmin_c, lmax_c]): r_c, d_c and x_c as
+ * fp3d_amg_cheb_smooth(zero_init = 1) leaves them before its second step.
+ * Then fp3d_amg_cheb_smooth(zero_init = 2) on the coarse level. */
+int fp3d_amg_restrict_start(int64_t nc, int64_t nnz, const int32_t *indptr,
+                            const int32_t *indices, const void *data,
+                            int is_f32, const double *r, double *bc,
+                            const double *dinv_c, int degree_c, double lmin_c,
+                            double lmax_c, double *x_c, double *r_c,
+                            double *d_c, void *stream);
 /* r = b - A x */
 int fp3d_amg_residual(int64_t n, int64_t nnz, const int32_t *indptr,
                       const int32_t *indices, const void *data, int is_f32,

@@ -399,8 +399,10 @@ def test_cg_amg_laplace(gpu):
 
 
 def test_amg_graph_replay_matches_eager(gpu):
-    """The V-cycle replayed from its HIP graph equals the eager V-cycle on
-    every replay, with eager device work in between."""
+    """The V-cycle replayed from its HIP graphs equals the eager V-cycle on
+    every replay, with eager device work in between: graphs captured on the
+    caller's own buffers (up to four pairs) and the copy-in/out graph used
+    beyond them, for Chebyshev degrees 1, 2 and 3 and both precisions."""
     import torch
 
     from up3d_amd.device import to_dev, to_host
@@ -412,18 +414,22 @@ def test_amg_graph_replay_matches_eager(gpu):
     op = O.OraclePicardOperator(nodes, el)
     free = np.nonzero(~cube_boundary_mask(nodes))[0]
     A = op.assemble_matrix(np.linspace(0.8, 1.2, len(el)))[free][:, free].tocsr()
-    for prec in ("f64", "f32"):
-        ml = AMGHierarchy(A, precision=prec)
-        assert len(ml.levels) >= 2
+    n = A.shape[0]
+    for prec, degree in (("f64", 2), ("f32", 2), ("f64", 1), ("f64", 3)):
+        ml = AMGHierarchy(A, precision=prec, degree=degree)
+        assert len(ml.levels) >= 3          # restriction into a smoothed level
         rng = np.random.default_rng(1)
-        xe = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
-        xg = torch.empty_like(xe)
-        for rep in range(4):
-            b = to_dev(rng.standard_normal(A.shape[0]))
+        xe = torch.empty(n, dtype=torch.float64, device="cuda")
+        srcs = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+        dsts = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+        for rep in range(12):
+            b, xg = srcs[rep % 3], dsts[(rep // 3) % 3]
+            b.copy_(to_dev(rng.standard_normal(n)))
             ml.apply_eager(b, xe)
             ref = to_host(xe).copy()
-            ml.apply(b, xg)                    # first call eager, then graph
+            ml.apply(b, xg)                    # first call eager, then graphs
             torch.cuda.synchronize()
-            assert np.array_equal(to_host(xg), ref), (prec, rep)
+            assert np.array_equal(to_host(xg), ref), (prec, degree, rep)
             _ = (b * 2.0).sum().item()         # eager work between replays
-        assert ml._graph is not None
+        # nine buffer pairs were used: four direct graphs, then the copy graph
+        assert len(ml._graphs) == ml._MAX_DIRECT_GRAPHS and ml._graph is not None

@@ -122,6 +122,8 @@ _SIGS = {
                                  _vp]),
     "fp3d_amg_restrict": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
                                  _vp]),
+    "fp3d_amg_restrict_start": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
+                                       _vp, _int, _dbl, _dbl, _vp, _vp, _vp, _vp]),
     "fp3d_amg_prolong_add": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
                                     _vp]),
     "fp3d_dense_matvec": (_int, [_i64, _vp, _vp, _vp, _vp]),

@@ -56,6 +56,42 @@ struct EpChebIter {
   }
 };
 
+// Last recurrence step: as EpChebIter, then the new direction is added to
+// x as well -- the same two additions, in the same order, that the step
+// and a separate x += d_last pass make.
+struct EpChebLast {
+  const double *dinv, *d_old;
+  double *x, *r, *d_new;
+  double c1, c2;
+  __device__ void operator()(int64_t i, double s) const {
+    double dold = d_old[i];
+    double ri = r[i] - dinv[i] * s;
+    r[i] = ri;
+    double dn = c1 * dold + c2 * ri;
+    d_new[i] = dn;
+    x[i] = (x[i] + dold) + dn;
+  }
+};
+// Restriction b_c = R r that also takes the coarse level's first smoothing
+// step from x_c = 0 (as k_cheb_zero on b_c): r_c = dinv b_c,
+// d_c = r_c / theta, and x_c = 0, or x_c = 0 + d_c when that step is the
+// only one.
+struct EpRestrictStart {
+  double *bc;
+  const double *dinv;
+  double *x, *r, *d;
+  double inv_theta;
+  int x_is_d;
+  __device__ void operator()(int64_t i, double s) const {
+    bc[i] = s;
+    double ri = dinv[i] * s;
+    r[i] = ri;
+    double di = ri * inv_theta;
+    d[i] = di;
+    x[i] = x_is_d ? 0.0 + di : 0.0;
+  }
+};
+
 template <int G, typename T, typename Ep>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
                                                  const int32_t *__restrict__ ptr,
@@ -423,17 +459,19 @@ __global__ __launch_bounds__(kBlock) void k_dense_mv(int64_t n,
   if (lane == 0) x[i] = s;
 }
 
-// Chebyshev first step from x = 0: x = 0; r = dinv b; d = r/theta
+// Chebyshev first step from x = 0: x = 0; r = dinv b; d = r/theta (x = 0 + d
+// when this step is the only one)
 __global__ void k_cheb_zero(int64_t n, const double *__restrict__ b,
                             const double *__restrict__ dinv, double inv_theta,
                             double *__restrict__ x, double *__restrict__ r,
-                            double *__restrict__ d) {
+                            double *__restrict__ d, int x_is_d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double ri = dinv[i] * b[i];
   r[i] = ri;
-  d[i] = ri * inv_theta;
-  x[i] = 0.0;
+  double di = ri * inv_theta;
+  d[i] = di;
+  x[i] = x_is_d ? 0.0 + di : 0.0;
 }
 __global__ void k_add(int64_t n, const double *__restrict__ d,
                       double *__restrict__ x) {
@@ -1095,34 +1133,65 @@ int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
   FP3D_REQUIRE(n > 0 && indptr && indices && data && dinv && b && x && r &&
                    d0 && d1 && degree >= 1 && lmax > lmin && lmin > 0.0,
                "amg_cheb_smooth: bad arguments");
+  FP3D_REQUIRE(zero_init >= 0 && zero_init <= 2,
+               "amg_cheb_smooth: zero_init must be 0, 1 or 2");
   hipStream_t s = FP3D_STREAM(stream);
   double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   double sigma = theta / delta;
   double rho0 = 1.0 / sigma;
-  int rc;
-  if (zero_init) {
+  int rc = FP3D_OK;
+  // the last direction goes into x inside the last step (EpChebLast), or in
+  // the first step itself from x = 0 when degree == 1; only the first step
+  // from a nonzero x (which reads x in its SpMV) needs a separate x += d
+  if (zero_init == 1) {
     hipLaunchKernelGGL(k_cheb_zero, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
-                       b, dinv, 1.0 / theta, x, r, d0);
+                       b, dinv, 1.0 / theta, x, r, d0, degree == 1 ? 1 : 0);
     rc = check_launch("cheb_zero");
-  } else {
+  } else if (zero_init == 0) {
     rc = spmv_any(n, nnz, indptr, indices, data, is_f32, x,
                   EpChebFirst{b, dinv, r, d0, 1.0 / theta}, s);
   }
+  // zero_init == 2: the caller took the first step from x = 0 (x, r, d0 set,
+  // as fp3d_amg_restrict_start does)
   if (rc) return rc;
   double *dold = d0, *dnew = d1;
   for (int k = 1; k < degree; ++k) {
     double rho1 = 1.0 / (2.0 * sigma - rho0);
     double c1 = rho1 * rho0, c2 = 2.0 * rho1 / delta;
-    rc = spmv_any(n, nnz, indptr, indices, data, is_f32, dold,
-                  EpChebIter{dinv, dold, x, r, dnew, c1, c2}, s);
+    if (k == degree - 1)
+      rc = spmv_any(n, nnz, indptr, indices, data, is_f32, dold,
+                    EpChebLast{dinv, dold, x, r, dnew, c1, c2}, s);
+    else
+      rc = spmv_any(n, nnz, indptr, indices, data, is_f32, dold,
+                    EpChebIter{dinv, dold, x, r, dnew, c1, c2}, s);
     if (rc) return rc;
     rho0 = rho1;
     double *t = dold;
     dold = dnew;
     dnew = t;
   }
-  hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dold, x);
-  return check_launch("cheb_smooth");
+  if (degree == 1 && zero_init == 0) {
+    hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dold, x);
+    return check_launch("cheb_smooth");
+  }
+  return FP3D_OK;
+}
+
+int fp3d_amg_restrict_start(int64_t nc, int64_t nnz, const int32_t *indptr,
+                            const int32_t *indices, const void *data,
+                            int is_f32, const double *r, double *bc,
+                            const double *dinv_c, int degree_c, double lmin_c,
+                            double lmax_c, double *x_c, double *r_c,
+                            double *d_c, void *stream) {
+  FP3D_REQUIRE(nc > 0 && indptr && indices && data && r && bc && dinv_c &&
+                   x_c && r_c && d_c && degree_c >= 1 && lmax_c > lmin_c &&
+                   lmin_c > 0.0,
+               "amg_restrict_start: bad arguments");
+  const double theta = 0.5 * (lmax_c + lmin_c);
+  return spmv_any(nc, nnz, indptr, indices, data, is_f32, r,
+                  EpRestrictStart{bc, dinv_c, x_c, r_c, d_c, 1.0 / theta,
+                                  degree_c == 1 ? 1 : 0},
+                  FP3D_STREAM(stream));
 }
 
 int fp3d_amg_residual(int64_t n, int64_t nnz, const int32_t *indptr,

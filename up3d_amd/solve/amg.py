@@ -234,6 +234,7 @@ class AMGHierarchy:
         # the first apply, after one eager V-cycle has loaded the kernels)
         self.use_graph = use_graph
         self._graph = None
+        self._graphs = {}
         self._gb = empty(self.levels[0].n)
         self._gx = empty(self.levels[0].n)
         self._eager_calls = 0
@@ -242,7 +243,10 @@ class AMGHierarchy:
     def operator_complexity(self) -> float:
         return float(sum(self.nnzs)) / float(self.nnzs[0])
 
-    def _vcycle(self, l: int, b: torch.Tensor, x: torch.Tensor) -> None:
+    def _vcycle(self, l: int, b: torch.Tensor, x: torch.Tensor,
+                started: bool = False) -> None:
+        """V-cycle from level l on b into x (x = 0 initially); started: the
+        restriction into this level already took the smoother's first step."""
         L = self.levels[l]
         s = stream_handle()
         if l == len(self.levels) - 1:
@@ -253,16 +257,25 @@ class AMGHierarchy:
         _native.call("fp3d_amg_cheb_smooth", L.n, A.nnz, ptr(A.indptr),
                      ptr(A.indices), ptr(A.values), A.is_f32, ptr(L.dinv),
                      ptr(b), ptr(x), ptr(L.r), ptr(L.d0), ptr(L.d1), L.degree,
-                     L.lmin, L.lmax, 1, s)
+                     L.lmin, L.lmax, 2 if started else 1, s)
         _native.call("fp3d_amg_residual", L.n, A.nnz, ptr(A.indptr),
                      ptr(A.indices), ptr(A.values), A.is_f32, ptr(b), ptr(x),
                      ptr(L.r), s)
         Lc = self.levels[l + 1]
         R = L.R
-        _native.call("fp3d_amg_restrict", R.n, R.nnz, ptr(R.indptr),
-                     ptr(R.indices), ptr(R.values), R.is_f32, ptr(L.r),
-                     ptr(Lc.b), s)
-        self._vcycle(l + 1, Lc.b, Lc.x)
+        coarse_next = l + 1 == len(self.levels) - 1
+        if coarse_next:
+            _native.call("fp3d_amg_restrict", R.n, R.nnz, ptr(R.indptr),
+                         ptr(R.indices), ptr(R.values), R.is_f32, ptr(L.r),
+                         ptr(Lc.b), s)
+        else:
+            # the restriction's epilogue also takes the coarse smoother's
+            # first step from x = 0 (one launch fewer per level)
+            _native.call("fp3d_amg_restrict_start", R.n, R.nnz, ptr(R.indptr),
+                         ptr(R.indices), ptr(R.values), R.is_f32, ptr(L.r),
+                         ptr(Lc.b), ptr(Lc.dinv), Lc.degree, Lc.lmin, Lc.lmax,
+                         ptr(Lc.x), ptr(Lc.r), ptr(Lc.d0), s)
+        self._vcycle(l + 1, Lc.b, Lc.x, started=not coarse_next)
         P = L.P
         _native.call("fp3d_amg_prolong_add", P.n, P.nnz, ptr(P.indptr),
                      ptr(P.indices), ptr(P.values), P.is_f32, ptr(Lc.x), ptr(x),
@@ -277,22 +290,40 @@ class AMGHierarchy:
         self._vcycle(0, b, x)
         return x
 
-    def _capture(self):
+    def _capture(self, b: torch.Tensor, x: torch.Tensor):
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self._vcycle(0, self._gb, self._gx)
-        self._graph = g
+            self._vcycle(0, b, x)
+        return g
+
+    # graphs captured directly on callers' (b, x) buffers; a Krylov solve
+    # passes one source and two alternating destinations, and the caching
+    # allocator hands the same buffers to the next solve of the same size
+    _MAX_DIRECT_GRAPHS = 4
 
     def apply(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
         """x = M b (one V-cycle from a zero initial guess).  After the first
-        call the V-cycle is replayed as one HIP graph on fixed buffers."""
+        call the V-cycle is replayed as a HIP graph: one captured on the
+        caller's own b and x for up to four buffer pairs, else one on fixed
+        buffers with copies in and out."""
         if not self.use_graph:
             return self.apply_eager(b, x)
+        if self._eager_calls == 0:
+            self._eager_calls += 1
+            return self.apply_eager(b, x)
+        direct = (b.is_contiguous() and x.is_contiguous() and b.dtype == torch.float64
+                  and x.dtype == torch.float64 and b.numel() == x.numel() == self._gb.numel()
+                  and b.data_ptr() != x.data_ptr())
+        if direct:
+            key = (b.data_ptr(), x.data_ptr())
+            g = self._graphs.get(key)
+            if g is None and len(self._graphs) < self._MAX_DIRECT_GRAPHS:
+                g = self._graphs[key] = self._capture(b, x)
+            if g is not None:
+                g.replay()
+                return x
         if self._graph is None:
-            if self._eager_calls == 0:
-                self._eager_calls += 1
-                return self.apply_eager(b, x)
-            self._capture()
+            self._graph = self._capture(self._gb, self._gx)
         self._gb.copy_(b)
         self._graph.replay()
         x.copy_(self._gx)
