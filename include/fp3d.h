@@ -343,6 +343,16 @@ int fp3d_mask_copy(int64_t n, const uint8_t *mask, const double *x, double *y,
 int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
                    const double *w, double *work, double *out, void *stream);
 int fp3d_multi_dot_work_doubles(int nv);
+/* As fp3d_multi_dot, and out[nv] = w . w in the same pass (the value a
+ * one-row call on w gives; no reference counterpart: the Arnoldi step's
+ * ||w||^2 for the DGKS test).  work = fp3d_multi_dot_work_doubles(nv + 1);
+ * _f32v reads float32 basis vectors (products and sums in float64). */
+int fp3d_multi_dot_norm(int64_t n, int nv, const double *V, int64_t ld,
+                        const double *w, double *work, double *out,
+                        void *stream);
+int fp3d_multi_dot_norm_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                             const double *w, double *work, double *out,
+                             void *stream);
 /* w += sign * sum_k h[k] v_k  (h on device, nv <= 256) */
 /* Re-orthogonalisation pass of CGS2 (no reference counterpart; the
  * reference's GMRES is scipy's): out[k] = V_k . w only when the first pass

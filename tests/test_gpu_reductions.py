@@ -33,6 +33,22 @@ def test_multi_dot_matches_exact_sums(gpu, n, nv):
     again = to_host(red.out[2:3 + nv]).copy()
     red.dots(V, nv + 1, V[nv], 2)
     assert np.array_equal(to_host(red.out[2:3 + nv]), again)     # bitwise repeatable
+    # the fused self row (fp3d_multi_dot_norm): the same V . w and, in
+    # out[nv], the bits of a one-row call on w
+    import torch
+
+    fused = torch.empty(nv + 1, dtype=torch.float64, device=V.device)
+    _native.call("fp3d_multi_dot_norm", n, nv, ptr(V), V.stride(0), ptr(V[nv]),
+                 ptr(red.work), ptr(fused), stream_handle())
+    single = torch.empty(1, dtype=torch.float64, device=V.device)
+    _native.call("fp3d_multi_dot", n, 1, ptr(V[nv]), n, ptr(V[nv]), ptr(red.work),
+                 ptr(single), stream_handle())
+    fh = to_host(fused)
+    assert np.array_equal(fh[:nv], again[:nv]) and fh[nv] == to_host(single)[0]
+    V32 = V[:nv].to(torch.float32).contiguous()
+    _native.call("fp3d_multi_dot_norm_f32v", n, nv, ptr(V32), V32.stride(0), ptr(V[nv]),
+                 ptr(red.work), ptr(fused), stream_handle())
+    assert to_host(fused)[nv] == fh[nv]
     # DGKS-conditional pass: no cancellation -> exact zeros
     cond = to_dev(np.concatenate([np.zeros(nv), [1.0]]))
     _native.call("fp3d_multi_dot_reorth", n, nv, ptr(V), V.stride(0), ptr(V[nv]),

@@ -306,9 +306,11 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
                                                    const double *__restrict__ w,
                                                    double *__restrict__ part,
                                                    const double *__restrict__ cond,
-                                                   double eta2, int64_t seg) {
+                                                   double eta2, int64_t seg,
+                                                   int with_self) {
   // STAGED reads the slice of w from LDS (a separate instantiation, so the
-  // loads stay in the LDS address space rather than generic)
+  // loads stay in the LDS address space rather than generic).  with_self:
+  // row nv is w itself (w . w, summed exactly as a one-row call on w)
   __shared__ double ws[STAGED ? kMdotLds : 1];
   const int nb = gridDim.x;
   if (cond) {
@@ -351,6 +353,28 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
     for (; i < len; i += 64) a0 += (double)v[i] * (STAGED ? ws[i] : wg[i]);
     double t = wsum((a0 + a1) + (a2 + a3));
     if (lane == 0) part[(int64_t)k * nb + blockIdx.x] = t;
+  }
+  if (with_self && wave == nv % nwave) {
+    // the row after the last basis vector: w itself, read from global as
+    // the V operand of a one-row call would be
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = lane;
+    for (; i + 192 < len; i += 256) {
+      if (STAGED) {
+        a0 += wg[i] * ws[i];
+        a1 += wg[i + 64] * ws[i + 64];
+        a2 += wg[i + 128] * ws[i + 128];
+        a3 += wg[i + 192] * ws[i + 192];
+      } else {
+        a0 += wg[i] * wg[i];
+        a1 += wg[i + 64] * wg[i + 64];
+        a2 += wg[i + 128] * wg[i + 128];
+        a3 += wg[i + 192] * wg[i + 192];
+      }
+    }
+    for (; i < len; i += 64) a0 += wg[i] * (STAGED ? ws[i] : wg[i]);
+    double t = wsum((a0 + a1) + (a2 + a3));
+    if (lane == 0) part[(int64_t)nv * nb + blockIdx.x] = t;
   }
 }
 inline int64_t mdot_seg(int64_t n) {
@@ -794,18 +818,19 @@ using namespace fp3d;
 template <typename TV>
 static int multi_dot_any(int64_t n, int nv, const TV *V, int64_t ld, const double *w,
                          const double *cond, double eta2, double *work, double *out,
-                         void *stream, const char *what) {
+                         void *stream, const char *what, int with_self = 0) {
   FP3D_REQUIRE(nv >= 1 && nv <= 256 && V && w && work && out && ld >= n, what);
   hipStream_t s = FP3D_STREAM(stream);
   int64_t seg = mdot_seg(n);
   int nb = mdot_blocks(n, seg);
   if (seg <= kMdotLds)
     hipLaunchKernelGGL((k_mdot_p1<true, TV>), dim3(nb), dim3(kBlock), 0, s, n, nv,
-                       V, ld, w, work, cond, eta2, seg);
+                       V, ld, w, work, cond, eta2, seg, with_self);
   else
     hipLaunchKernelGGL((k_mdot_p1<false, TV>), dim3(nb), dim3(kBlock), 0, s, n, nv,
-                       V, ld, w, work, cond, eta2, seg);
-  hipLaunchKernelGGL(k_mdot_p2, dim3(nv), dim3(kBlock), 0, s, nb, work, out);
+                       V, ld, w, work, cond, eta2, seg, with_self);
+  hipLaunchKernelGGL(k_mdot_p2, dim3(nv + with_self), dim3(kBlock), 0, s, nb, work,
+                     out);
   return check_launch(what);
 }
 
@@ -1041,6 +1066,20 @@ int fp3d_multi_dot(int64_t n, int nv, const double *V, int64_t ld,
                    const double *w, double *work, double *out, void *stream) {
   return multi_dot_any(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
                        "multi_dot: bad arguments");
+}
+
+int fp3d_multi_dot_norm(int64_t n, int nv, const double *V, int64_t ld,
+                        const double *w, double *work, double *out,
+                        void *stream) {
+  return multi_dot_any<double>(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
+                               "multi_dot_norm: bad arguments", 1);
+}
+
+int fp3d_multi_dot_norm_f32v(int64_t n, int nv, const float *V, int64_t ld,
+                             const double *w, double *work, double *out,
+                             void *stream) {
+  return multi_dot_any<float>(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
+                              "multi_dot_norm_f32v: bad arguments", 1);
 }
 
 int fp3d_multi_dot_f32v(int64_t n, int nv, const float *V, int64_t ld,

@@ -154,14 +154,14 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
         prec(tmp, w)
         sh = stream_handle()
         if f32:
-            dot_, reo_, axpy_ = ("fp3d_multi_dot_f32v", "fp3d_multi_dot_reorth_f32v",
+            dot_, reo_, axpy_ = ("fp3d_multi_dot_norm_f32v", "fp3d_multi_dot_reorth_f32v",
                                  "fp3d_multi_axpy_f32v")
         else:
-            dot_, reo_, axpy_ = "fp3d_multi_dot", "fp3d_multi_dot_reorth", "fp3d_multi_axpy"
+            dot_, reo_, axpy_ = ("fp3d_multi_dot_norm", "fp3d_multi_dot_reorth",
+                                 "fp3d_multi_axpy")
+        # h = V^T w into out[2:2+k] and ||w||^2 into out[2+k], one pass over w
         _native.call(dot_, n, k, ptr(V), V.stride(0), ptr(w), ptr(red.work),
                      ptr(o[2:]), sh)
-        _native.call("fp3d_multi_dot", n, 1, ptr(w), n, ptr(w), ptr(red.work),
-                     ptr(o[2 + k:]), sh)
         _native.call(axpy_, n, k, ptr(V), V.stride(0), ptr(o[2:]), -1.0, ptr(w), sh)
         # second CGS pass only where the first one cancelled (device-side DGKS
         # test on h and ||w||^2; a skipped pass leaves exact zeros and w as is)
