@@ -203,6 +203,27 @@ def main():
                      "GBs": round(ab[k] / (per[k] * 1e-3) / 1e9, 1)}
                  for k in per}
     log("[bench] phases " + json.dumps(phase_rep))
+    # the two other rates the reference quotes: a residual evaluation (state
+    # sweep, walk + rho_tilde + sensitivities, residual) and the Picard
+    # matrix alone (A = sum rho_tilde V B_i.B_j into the fixed CSR; the
+    # reference's anchor is its coloured Picard assembly)
+    res_ms = per["state"] + per["upwind"] + per["residual"]
+    pic_out = torch.empty(st.nnz, dtype=torch.float64, device=dev)
+    op.matrix_dev(upw.d_rt, pic_out)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(max(3, args.steps)):
+        op.matrix_dev(upw.d_rt, pic_out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    pic_ms = e0.elapsed_time(e1) / max(3, args.steps)
+    rates = {"residual_evaluation_mtet_s": round(nt / (res_ms * 1e-3) / 1e6, 1),
+             "linearisation_mtet_s": round(value, 1),
+             "picard_matrix_mtet_s": round(nt / (pic_ms * 1e-3) / 1e6, 1),
+             "picard_matrix_ms": round(pic_ms, 4)}
+    log("[bench] rates " + json.dumps(rates))
     dom = max((k for k in per if k != "term3"), key=lambda k: per[k])
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
@@ -250,6 +271,7 @@ def main():
                    "n_tets": nt, "n_nodes": nn, "nnz": st.nnz,
                    "n_colors": st.n_colors, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "phases": phase_rep,
+        "rates": rates,
         "setup_s": round(setup_s, 2), "solve": solve, "plate": plate,
     }
     if sweep is not None:
@@ -303,11 +325,30 @@ def cpu_baseline(st, mesh, passes):
     for _ in range(passes):
         one()
     dt = (time.perf_counter() - t0) / passes
+    # the Picard matrix alone (zero fill + coloured assembly into the fixed
+    # CSR values), the rate the reference's anchor quotes
+    rt1 = np.linspace(0.8, 1.2, st.n_tets)
+    data = np.zeros(len(st.indices_host))
+    import ctypes
+
+    def picard():
+        data.fill(0.0)
+        O._lib().or_assemble_matrix_colored(
+            ctypes.c_int(len(off) - 1), O._p(off, O._i64), O._p(ce, O._i64),
+            O._p(e2c, O._i32), O._p(B, O._d), O._p(V, O._d), O._p(rt1, O._d),
+            O._p(data, O._d))
+
+    picard()
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        picard()
+    dtp = (time.perf_counter() - t0) / passes
     return {"value": round(st.n_tets / dt / 1e6, 3), "unit": "Mtet/s",
             "cores": threads, "kind": "port",
             "sample": f"{passes} full linearisation passes over the same "
                       f"{st.n_tets}-tet mesh (oracle C, OpenMP {threads} threads)",
-            "s_per_pass": round(dt, 3)}
+            "s_per_pass": round(dt, 3),
+            "picard_matrix_mtet_s": round(st.n_tets / dtp / 1e6, 3)}
 
 
 def run_solves(mesh, config, tune=True):
