@@ -99,3 +99,22 @@ def test_sa_setup_matches_scipy_recipe():
         Ar = Ac
     assert np.array_equal(levels[-1].A.data, Ar.data)
     assert len(levels) >= 2
+
+
+@pytest.mark.parametrize("shape_a,shape_b", [((0, 5), (5, 3)), ((4, 5), (5, 0)),
+                                             ((3, 0), (0, 2))])
+def test_spgemm_empty_shapes(shape_a, shape_b):
+    A = sp.csr_matrix(shape_a)
+    B = sp.csr_matrix(shape_b)
+    C = amg.spgemm(A, B, force_native=True)
+    assert C.shape == (shape_a[0], shape_b[1]) and C.nnz == 0
+    assert np.array_equal(amg.csr_matvec(A, np.ones(shape_a[1]), force_native=True),
+                          np.zeros(shape_a[0]))
+
+
+def test_spgemm_all_cancelling_rows():
+    # every product entry cancels exactly: no entries, like scipy
+    A = sp.csr_matrix(np.array([[1.0, -1.0], [2.0, -2.0]]))
+    B = sp.csr_matrix(np.array([[3.0, 1.0], [3.0, 1.0]]))
+    C = amg.spgemm(A, B, force_native=True)
+    assert C.nnz == (A @ B).nnz == 0
