@@ -519,6 +519,12 @@ int fp3d_host_locate_points(int64_t n_tets, const int32_t *elements,
  * (no reference counterpart; set by up3d_amd.tune). */
 int fp3d_set_element_block(int threads);
 int fp3d_get_element_block(void);
+/* Rows per staged chunk of the incidence-ordered CSR gather
+ * (fp3d_gather_csr_terms with inc_ent == NULL): 256, 512, 1024 (default)
+ * or 2048.  Changes only the LDS staging; the sums and their order are the
+ * same.  No reference counterpart (autotuner). */
+int fp3d_set_gather_chunk(int rows);
+int fp3d_get_gather_chunk(void);
 
 #ifdef __cplusplus
 }

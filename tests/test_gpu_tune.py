@@ -46,6 +46,36 @@ def test_element_block_is_bitwise_neutral(gpu):
         apply_element_block(100)
 
 
+def test_gather_chunk_is_bitwise_neutral(gpu):
+    from up3d_amd import _native
+    from up3d_amd.device import to_dev, to_host
+    from up3d_amd.solve.newton import NewtonWorkspace
+    from up3d_amd.tune import apply_gather_chunk
+
+    m, dn, dv = _channel()
+    ws = NewtonWorkspace(m.nodes, m.elements, dn, dv)
+    ws.set_mach(0.8)
+    s = ws.states[0]
+    phi = m.nodes[:, 0] + 0.05 * np.sin(3 * m.nodes[:, 0]) * m.nodes[:, 1]
+    s.phi.copy_(to_dev(phi))
+    s.phi.index_copy_(0, ws.dn, ws.dv)
+    ws.eval_residual(s, 1.5, 0.95, 0.05)
+    out = {}
+    prev = int(_native.lib().fp3d_get_gather_chunk())
+    try:
+        for c in (256, 512, 1024, 2048):
+            apply_gather_chunk(c)
+            ws.jac.data.fill_(float("nan"))
+            ws.assemble_coupled(s, 1.5, 0.95, 0.05)
+            out[c] = to_host(ws.jac.data).copy()
+    finally:
+        apply_gather_chunk(prev)
+    for c in (256, 512, 2048):
+        assert np.array_equal(out[c], out[1024])
+    with pytest.raises(ValueError):
+        apply_gather_chunk(300)
+
+
 def test_tune_newton_workspace_and_cache(gpu, tmp_path):
     from up3d_amd import _native
     from up3d_amd.solve.newton import NewtonWorkspace, solve_newton

@@ -115,6 +115,8 @@ _SIGS = {
     "fp3d_scale_inv_dev": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_set_element_block": (_int, [_int]),
     "fp3d_get_element_block": (_int, []),
+    "fp3d_set_gather_chunk": (_int, [_int]),
+    "fp3d_get_gather_chunk": (_int, []),
     "fp3d_scale_inv_dev_to_f32": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_norms": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_amg_cheb_smooth": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,

@@ -986,10 +986,12 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
 
 // Incidence-ordered matrix rows: the rows and slot offsets of a block's
 // kGatherRows CSR rows are one contiguous range [iptr[r0], iptr[r1]),
-// streamed through LDS in coalesced chunks of kRowChunk (next chunk in
-// flight while the current one is added); each lane adds its own row's
+// streamed through LDS in coalesced chunks of kRowChunk rows (next chunk
+// in flight while the current one is added; 1024 rows = 32 KiB by
+// default, 256, 512 or 2048 by fp3d_set_gather_chunk); each lane adds its own row's
 // entries in entry order into the LDS-staged CSR block, as k_gather_csr.
-constexpr int kRowChunk = 512;                 // matrix rows per chunk (16 KiB)
+// The chunk size changes only the staging, never the sums or their order.
+template <int kRowChunk>
 __global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
     int64_t nn, int64_t nb_pad, int max_seg, const int32_t *__restrict__ indptr,
     const int32_t *__restrict__ iptr, const uint32_t *__restrict__ rel8,
@@ -1125,6 +1127,8 @@ int fp3d_reduce_work_doubles(void) { return 3 * kReduceBlocks; }
 // with blockDim.x and uses no LDS): 64, 128, 192 or 256, process-wide, set
 // by the autotuner through fp3d_set_element_block.
 static int g_elem_block = kBlock;
+// rows per staged chunk of the incidence-ordered CSR gather (autotuned)
+static int g_gather_chunk = 1024;
 
 #define ELEMWISE(kernel, n, ...)                                       \
   do {                                                                 \
@@ -1519,16 +1523,27 @@ int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                "gather_csr_terms: max_seg must lie in (0, 8192]");
   int64_t nb = (n_nodes + kGatherRows - 1) / kGatherRows;
   unsigned g = xcd_grid(nb);
-  if (inc_ent)
+  if (inc_ent) {
     hipLaunchKernelGGL(k_gather_csr, dim3(g), dim3(kGatherRows),
                        (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
                        n_nodes, (int64_t)g, max_seg, indptr, inc_ptr, inc_ent,
                        (const uint32_t *)rel8, K, data);
-  else
-    hipLaunchKernelGGL(k_gather_csr_staged, dim3(g), dim3(kGatherRows),
-                       (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),
-                       n_nodes, (int64_t)g, max_seg, indptr, inc_ptr,
-                       (const uint32_t *)rel8, (const double4 *)K, data);
+  } else {
+#define FP3D_STAGED(CH)                                                      \
+  hipLaunchKernelGGL(k_gather_csr_staged<CH>, dim3(g), dim3(kGatherRows),    \
+                     (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),  \
+                     n_nodes, (int64_t)g, max_seg, indptr, inc_ptr,          \
+                     (const uint32_t *)rel8, (const double4 *)K, data)
+    if (g_gather_chunk == 256)
+      FP3D_STAGED(256);
+    else if (g_gather_chunk == 2048)
+      FP3D_STAGED(2048);
+    else if (g_gather_chunk == 512)
+      FP3D_STAGED(512);
+    else
+      FP3D_STAGED(1024);
+#undef FP3D_STAGED
+  }
   return check_launch("gather_csr_terms");
 }
 
@@ -1558,5 +1573,14 @@ int fp3d_set_element_block(int threads) {
 }
 
 int fp3d_get_element_block(void) { return g_elem_block; }
+
+int fp3d_set_gather_chunk(int rows) {
+  FP3D_REQUIRE(rows == 256 || rows == 512 || rows == 1024 || rows == 2048,
+               "set_gather_chunk: rows must be 256, 512, 1024 or 2048");
+  g_gather_chunk = rows;
+  return FP3D_OK;
+}
+
+int fp3d_get_gather_chunk(void) { return g_gather_chunk; }
 
 }  // extern "C"

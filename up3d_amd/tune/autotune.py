@@ -8,6 +8,8 @@ stream, on the caller's own mesh and state:
     Jacobian element terms) over {64, 128, 192, 256}; the node gathers keep
     their fixed 128-row LDS blocks.  The winner is applied process-wide
     (fp3d_set_element_block).
+  * tune_gather_chunk: rows per LDS-staged chunk of the CSR matrix gather
+    over {512, 1024} (fp3d_set_gather_chunk; the sums are unchanged).
   * tune_linear_solver: GMRES restart x AMG Chebyshev degree x AMG coarsest
     size on one Newton linear system J d = -R, by wall time to the asked
     relative tolerance (AMG set-up included: it is paid once per rebuild).
@@ -29,6 +31,7 @@ import torch
 from .. import _native
 
 ELEMENT_BLOCKS = (64, 128, 192, 256)
+GATHER_CHUNKS = (512, 1024)
 
 
 def _event_time(fn: Callable[[], None], reps: int) -> float:
@@ -67,6 +70,29 @@ def tune_element_block(pipeline: Callable[[], None],
         raise
     best = min(times, key=times.get)
     apply_element_block(best)
+    return {"best": best, "ms": times}
+
+
+def apply_gather_chunk(rows: int) -> None:
+    _native.call("fp3d_set_gather_chunk", int(rows))
+
+
+def tune_gather_chunk(pipeline: Callable[[], None],
+                      candidates: Sequence[int] = GATHER_CHUNKS,
+                      reps: int = 5) -> Dict[str, object]:
+    """Time `pipeline` (a pass through the matrix gather) at each staged
+    chunk size, apply the fastest and return {"best": rows, "ms": {...}}."""
+    prev = int(_native.lib().fp3d_get_gather_chunk())
+    times = {}
+    try:
+        for c in candidates:
+            apply_gather_chunk(c)
+            times[int(c)] = _event_time(pipeline, reps)
+    except Exception:
+        apply_gather_chunk(prev)
+        raise
+    best = min(times, key=times.get)
+    apply_gather_chunk(best)
     return {"best": best, "ms": times}
 
 
@@ -172,6 +198,8 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
     hit = cache.get(key)
     if hit is not None and "solver" in hit and "element_block" in hit:
         apply_element_block(hit["element_block"]["best"])
+        if "gather_chunk" in hit:
+            apply_gather_chunk(hit["gather_chunk"]["best"])
         return dict(hit, cached=True)
     ws.set_mach(m_inf)
     s = ws.states[2]
@@ -183,6 +211,7 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
         ws.assemble_coupled(s, upwind_c, m_crit, rho_floor)
 
     eb = tune_element_block(linearise, element_blocks)
+    gc = tune_gather_chunk(linearise)
     linearise()
     upw = ws.upw
     se, _ = upw.sensitivities_at_dev(s.q2l, s.rho, s.up, ws.m_inf, upwind_c, m_crit,
@@ -199,7 +228,7 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
     prec = "f32" if ws.precision == "fp32" else "f64"
     sol = tune_linear_solver(ws.jac.matvec, rhs, A_pre, precision=prec, rtol=rtol,
                              restarts=restarts, degrees=degrees, max_coarse=max_coarse)
-    entry = {"element_block": eb, "solver": sol, "m_inf": float(m_inf),
+    entry = {"element_block": eb, "gather_chunk": gc, "solver": sol, "m_inf": float(m_inf),
              "precision": ws.precision}
     cache.put(key, entry)
     return dict(entry, cached=False)
