@@ -88,6 +88,9 @@ def main():
                          "(about 3 minutes)")
     ap.add_argument("--no-plate", action="store_true",
                     help="skip the embedded-plate lifting case (128x96x96 box)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also run the adjoint case: 96x32x32 channel (589,824 tets) at "
+                         "M 0.80, gradients checked by central differences (about 2 minutes)")
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scale)")
@@ -254,6 +257,11 @@ def main():
             plate = {"error": f"{type(exc).__name__}: {exc}"}
         log(f"[bench] plate {json.dumps(plate)}")
 
+    adjoint = None
+    if args.adjoint:
+        adjoint = run_adjoint()
+        log(f"[bench] adjoint {json.dumps(adjoint)}")
+
     sweep = None
     if args.sweep:
         sweep = run_sweep()
@@ -271,7 +279,7 @@ def main():
                    "n_tets": nt, "n_nodes": nn, "nnz": st.nnz,
                    "n_colors": st.n_colors, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "phases": phase_rep,
-        "rates": rates,
+        "rates": rates, "adjoint": adjoint,
         "setup_s": round(setup_s, 2), "solve": solve, "plate": plate,
     }
     if sweep is not None:
@@ -429,6 +437,88 @@ def run_solves(mesh, config, tune=True):
     res["fp32_vs_fp64_cf_x_abs"] = abs(res["fp32"]["wall_cf"][0] - res["fp64"]["wall_cf"][0])
     res["fp64_over_fp32_seconds"] = round(res["fp64"]["seconds"] / res["fp32"]["seconds"], 3)
     return res
+
+
+def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
+    """The adjoint case on the 96x32x32 jittered channel (589,824 tets) at
+    M_inf 0.80, alpha 0: the converged Newton state (upward Mach ramp with
+    step halving), then d(cf_x)/d(M_inf, alpha, y of the 97x33 floor nodes)
+    from one adjoint solve, checked by central differences (step h) along
+    n_pert seeded unit-norm floor perturbations: of the frozen-selection
+    solve (the smooth piece the adjoint differentiates) and, for n_free of
+    them, of free re-solves (which may switch upstream selections)."""
+    import torch
+
+    from up3d_amd.constraints.dirichlet import freestream_phi
+    from up3d_amd.mesh.channel import boundary_nodes, config_mesh
+    from up3d_amd.solve.adjoint import ChannelForceProblem
+    from up3d_amd.solve.newton import solve_newton_transonic
+
+    m = config_mesh("c3")
+    dn = boundary_nodes(m, ["inflow", "outflow"])
+    floor = np.unique(m.boundary_faces["floor"])
+    prob = ChannelForceProblem(m.nodes, m.elements, dn, m.boundary_faces["floor"], floor)
+    m_inf, alpha = 0.80, 0.0
+    t0 = time.perf_counter()
+    sol = None
+    for mm in (0.5, 0.6, 0.65, 0.7):
+        sol = prob.solve(None, mm, alpha, phi_init=None if sol is None else sol["phi"])
+    rt = solve_newton_transonic(
+        m.nodes, m.elements, dn, freestream_phi(m.nodes[dn], alpha), m_inf,
+        m_start=0.7, dm=0.05, upwind_c=1.5,
+        newton_kw={"n_newton_max": 40, "tol_residual": 1e-13, "phi_init": sol["phi"],
+                   "n_picard_seed": 0})
+    sol = prob.solve(None, m_inf, alpha, phi_init=rt["phi"])
+    solve_s = time.perf_counter() - t0
+    if not sol["converged"]:
+        return {"error": "no converged state at M 0.80"}
+    t0 = time.perf_counter()
+    g = prob.gradients(sol, weights=(1.0, 0.0))
+    torch.cuda.synchronize()
+    adjoint_s = time.perf_counter() - t0
+    # the forward tangent of the same linearisation: exact derivatives that
+    # the adjoint must reproduce to round-off
+    tan_m = prob.jvp_params(sol, 1.0, 0.0, weights=(1.0, 0.0))
+    tan_a = prob.jvp_params(sol, 0.0, 1.0, weights=(1.0, 0.0))
+    frozen = prob.frozen_selection(sol)
+
+    def f_frozen(y=None, mm=m_inf, a=alpha):
+        return prob.functional(prob.solve_frozen(y, mm, a, frozen, sol["phi"]))[0]
+
+    def f_free(y=None):
+        return prob.functional(prob.solve(y, m_inf, alpha, phi_init=sol["phi"]))[0]
+
+    t0 = time.perf_counter()
+    fd_m = (f_frozen(mm=m_inf + h) - f_frozen(mm=m_inf - h)) / (2 * h)
+    fd_a = (f_frozen(a=alpha + h) - f_frozen(a=alpha - h)) / (2 * h)
+    y0 = prob.base_nodes[prob.floor_nodes, 1]
+    rng = np.random.default_rng(16)
+    rel_frozen, rel_free = [], []
+    for k in range(n_pert):
+        d = rng.standard_normal(len(y0))
+        d /= np.linalg.norm(d)
+        ad = float(g["d_y_floor"] @ d)
+        fd = (f_frozen(y0 + h * d) - f_frozen(y0 - h * d)) / (2 * h)
+        rel_frozen.append(abs(ad - fd) / max(abs(fd), 1e-30))
+        if k < n_free:
+            fr = (f_free(y0 + h * d) - f_free(y0 - h * d)) / (2 * h)
+            rel_free.append(abs(ad - fr) / max(abs(fr), 1e-30))
+    fd_s = time.perf_counter() - t0
+    step_s = solve_s / max(1, sum(n for _, n, _ in rt["level_history"]))
+    return {"workload": "bump channel 96x32x32 hexes -> 589,824 tets, jitter 15%, RCM, "
+                        "M_inf 0.80, alpha 0, output wall x-force",
+            "n_tets": int(m.n_tets), "n_floor_nodes": int(len(floor)),
+            "solve_s": round(solve_s, 2), "adjoint_s": round(adjoint_s, 3),
+            "adjoint_over_newton_step": round(adjoint_s / step_s, 2),
+            "d_m_inf": g["d_m_inf"], "tangent_m_inf": tan_m, "fd_m_inf_frozen": fd_m,
+            "d_alpha_deg": g["d_alpha_deg"], "tangent_alpha_deg": tan_a,
+            "fd_alpha_frozen": fd_a,
+            "rel_adjoint_vs_tangent": max(abs(g["d_m_inf"] - tan_m) / abs(tan_m),
+                                          abs(g["d_alpha_deg"] - tan_a) / abs(tan_a)),
+            "fd_step": h, "n_perturbations": n_pert,
+            "rel_err_frozen_max": max(rel_frozen),
+            "rel_err_frozen_median": float(np.median(rel_frozen)),
+            "rel_err_free": rel_free, "fd_seconds": round(fd_s, 1)}
 
 
 def run_sweep():

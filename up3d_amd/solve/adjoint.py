@@ -135,6 +135,52 @@ class ChannelForceProblem:
         r["alpha_deg"] = float(alpha_deg)
         return r
 
+    def frozen_selection(self, sol, upwind_c: float = 1.5, m_crit: float = 0.95,
+                         rho_floor: float = 0.05):
+        """(upstream, branch) of the converged state `sol`: the selection the
+        gradients are taken at (device int32 / int8 tensors)."""
+        L = self._linearisation(sol, upwind_c, m_crit, rho_floor)
+        return L["up"].clone(), L["br"].clone()
+
+    def solve_frozen(self, y_floor, m_inf: float, alpha_deg: float, frozen,
+                     phi_init, tol_residual: float = 1e-13, n_newton_max: int = 20,
+                     upwind_c: float = 1.5, m_crit: float = 0.95,
+                     rho_floor: float = 0.05):
+        """Newton on the system with the upstream selection and branch held at
+        `frozen` (frozen_selection of a converged state) -- the smooth piece of
+        the solution map whose derivative gradients() returns, so central
+        differences of it check the adjoint without the selection switches a
+        free re-solve may make.  Full Newton steps from phi_init (a nearby
+        converged state), GMRES to 1e-12 with the AMG of the clamped Jacobian."""
+        from .linear import scale
+        from .newton import NewtonWorkspace
+
+        nodes = self.nodes_for(y_floor)
+        dv = freestream_phi(nodes[self.dn], alpha_deg, self.u_inf)
+        ws = NewtonWorkspace(nodes, self.elements, self.dn, dv, u_inf=self.u_inf)
+        ws.set_mach(m_inf)
+        s = ws.states[0]
+        s.phi.copy_(to_dev(np.asarray(phi_init, dtype=np.float64)))
+        s.phi.index_copy_(0, ws.dn, ws.dv)
+        rhs = empty(ws.n)
+        hist = []
+        converged = False
+        for _ in range(n_newton_max):
+            ws.eval_residual(s, upwind_c, m_crit, rho_floor, frozen=frozen)
+            hist.append(s.r_norm)
+            if s.r_norm < tol_residual:
+                converged = True
+                break
+            ws.assemble_coupled(s, upwind_c, m_crit, rho_floor, frozen)
+            M = ws.build_preconditioner(s, "jacobian_spd", 0.9, se=ws.se_used)
+            scale(-1.0, s.R, rhs)
+            x, info, _ = gmres_dev(ws.jac.matvec, rhs, M.apply, rtol=1e-12,
+                                   restart=200, maxiter=20)
+            axpby(1.0, x, 1.0, s.phi)
+        return {"phi": to_host(s.phi).copy(), "nodes": nodes, "m_inf": float(m_inf),
+                "alpha_deg": float(alpha_deg), "converged": converged,
+                "residual_history": hist, "n_limited": int(s.n_limited)}
+
     def functional(self, sol, weights=(1.0, 0.0)):
         t = lambda a: torch.tensor(a, dtype=torch.float64)  # noqa: E731
         val, cf, cl = _wall_functional(t(sol["nodes"]), t(sol["phi"]),
