@@ -257,6 +257,9 @@ def main():
             plate = {"error": f"{type(exc).__name__}: {exc}"}
         log(f"[bench] plate {json.dumps(plate)}")
 
+    config1 = run_config1(with_cpu=not args.no_cpu)
+    log(f"[bench] config1 {json.dumps(config1)}")
+
     adjoint = None
     if args.adjoint:
         adjoint = run_adjoint()
@@ -279,7 +282,7 @@ def main():
                    "n_tets": nt, "n_nodes": nn, "nnz": st.nnz,
                    "n_colors": st.n_colors, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "phases": phase_rep,
-        "rates": rates, "adjoint": adjoint,
+        "rates": rates, "config1": config1, "adjoint": adjoint,
         "setup_s": round(setup_s, 2), "solve": solve, "plate": plate,
     }
     if sweep is not None:
@@ -437,6 +440,45 @@ def run_solves(mesh, config, tune=True):
     res["fp32_vs_fp64_cf_x_abs"] = abs(res["fp32"]["wall_cf"][0] - res["fp64"]["wall_cf"][0])
     res["fp64_over_fp32_seconds"] = round(res["fp64"]["seconds"] / res["fp32"]["seconds"], 3)
     return res
+
+
+def run_config1(with_cpu: bool = True):
+    """The plain subsonic case: 48x16x16 channel (73,728 tets), M_inf 0.5,
+    freestream Neumann flux in and out, one pinned node; Picard to
+    ||drho|| < 1e-10 on the device in both AMG precisions, and the oracle's
+    CPU Picard (direct sparse solves) on the same problem beside it."""
+    import torch
+
+    from up3d_amd.mesh.channel import config_mesh, neumann_flux_rhs
+    from up3d_amd.solve.picard import solve_subsonic
+
+    m = config_mesh("c1")
+    b = neumann_flux_rhs(m)
+    pin = np.array([int(np.argmin(np.linalg.norm(m.nodes - [0, 0.5, 0.5], axis=1)))])
+    vals = m.nodes[pin, 0]
+    out = {"workload": "bump channel 48x16x16 hexes -> 73,728 tets, M_inf 0.5, "
+                       "Neumann flux in/out, Picard", "n_tets": int(m.n_tets)}
+    phis = {}
+    for prec in ("f64", "f32", "f64"):            # the first f64 run warms up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = solve_subsonic(m.nodes, m.elements, pin, vals, 0.5, body_source_rhs=b,
+                           precision=prec)
+        torch.cuda.synchronize()
+        out[f"{prec}_s"] = round(time.perf_counter() - t0, 3)
+        out[f"{prec}_picard_its"] = int(r["n_picard"])
+        out[f"{prec}_converged"] = bool(r["converged"])
+        phis[prec] = r["phi"]
+    out["f32_vs_f64_phi_maxabs"] = float(np.abs(phis["f32"] - phis["f64"]).max())
+    if with_cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        ref = O.solve_subsonic(m.nodes, m.elements, pin, vals, 0.5, body_source_rhs=b)
+        out["cpu_oracle_s"] = round(time.perf_counter() - t0, 3)
+        out["gpu_vs_cpu_phi_rel"] = float(np.abs(phis["f64"] - ref["phi"]).max()
+                                          / np.abs(ref["phi"]).max())
+    return out
 
 
 def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
