@@ -1049,10 +1049,16 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
     for (; q < hi; ++q) {
       const double4 kv = kb[q - c0];
       const uint32_t of = ob[q - c0];
-      seg[rb + (int)(of & 0xffu)] += kv.x;
-      seg[rb + (int)((of >> 8) & 0xffu)] += kv.y;
-      seg[rb + (int)((of >> 16) & 0xffu)] += kv.z;
-      seg[rb + (int)(of >> 24)] += kv.w;
+      // the four slots are those of one element's four distinct nodes in
+      // row v, so they never alias: read all four, then write the sums (the
+      // same additions, without four chained read-modify-writes)
+      const int o0 = rb + (int)(of & 0xffu), o1 = rb + (int)((of >> 8) & 0xffu);
+      const int o2 = rb + (int)((of >> 16) & 0xffu), o3 = rb + (int)(of >> 24);
+      const double s0 = seg[o0], s1 = seg[o1], s2 = seg[o2], s3 = seg[o3];
+      seg[o0] = s0 + kv.x;
+      seg[o1] = s1 + kv.y;
+      seg[o2] = s2 + kv.z;
+      seg[o3] = s3 + kv.w;
     }
   }
   __syncthreads();
