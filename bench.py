@@ -79,7 +79,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-solve", action="store_true",
-                    help="skip the Newton solves (FP64 and FP32-precond)")
+                    help="skip the solves (config-2 Newton in FP64 and FP32-precond, "
+                         "config-1 Picard)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="solve with the fixed defaults instead of the autotuned parameters")
@@ -257,8 +258,10 @@ def main():
             plate = {"error": f"{type(exc).__name__}: {exc}"}
         log(f"[bench] plate {json.dumps(plate)}")
 
-    config1 = run_config1(with_cpu=not args.no_cpu)
-    log(f"[bench] config1 {json.dumps(config1)}")
+    config1 = None
+    if not args.no_solve:
+        config1 = run_config1(with_cpu=not args.no_cpu)
+        log(f"[bench] config1 {json.dumps(config1)}")
 
     adjoint = None
     if args.adjoint:

@@ -972,10 +972,16 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         if (q + k < q1) {
-          seg[rb + (int)(of[k] & 0xffu)] += kv[k].x;
-          seg[rb + (int)((of[k] >> 8) & 0xffu)] += kv[k].y;
-          seg[rb + (int)((of[k] >> 16) & 0xffu)] += kv[k].z;
-          seg[rb + (int)(of[k] >> 24)] += kv[k].w;
+          // four distinct slots (one element's four nodes): read, then write
+          const int o0 = rb + (int)(of[k] & 0xffu);
+          const int o1 = rb + (int)((of[k] >> 8) & 0xffu);
+          const int o2 = rb + (int)((of[k] >> 16) & 0xffu);
+          const int o3 = rb + (int)(of[k] >> 24);
+          const double s0 = seg[o0], s1 = seg[o1], s2 = seg[o2], s3 = seg[o3];
+          seg[o0] = s0 + kv[k].x;
+          seg[o1] = s1 + kv[k].y;
+          seg[o2] = s2 + kv[k].z;
+          seg[o3] = s3 + kv[k].w;
         }
       }
     }
