@@ -525,6 +525,12 @@ int fp3d_get_element_block(void);
  * same.  No reference counterpart (autotuner). */
 int fp3d_set_gather_chunk(int rows);
 int fp3d_get_gather_chunk(void);
+/* Upstream walks per lane in fp3d_upwind_rho_tilde_sens_d: 1, 2 (default),
+ * 3 or 4.  Each lane advances that many elements' walks together so their
+ * random reads overlap; the selections and values are the same for every
+ * width.  No reference counterpart (autotuner). */
+int fp3d_set_walk_width(int walks);
+int fp3d_get_walk_width(void);
 
 #ifdef __cplusplus
 }

@@ -90,3 +90,13 @@ def test_check_maps_einval_to_value_error():
     with pytest.raises(ValueError, match="max_seg"):
         _native.check(L.fp3d_gather_csr_terms(10, 0, P, P, None, P, P, P, None),
                       "gather_csr_terms")
+
+
+def test_tuning_setters_validate():
+    L = _lib()
+    prev = int(L.fp3d_get_walk_width())
+    for bad in (0, 5):
+        assert L.fp3d_set_walk_width(bad) == EINVAL and "walk_width" in _err()
+    for w in (1, 2, 3, 4):
+        assert L.fp3d_set_walk_width(w) == OK and L.fp3d_get_walk_width() == w
+    assert L.fp3d_set_walk_width(prev) == OK

@@ -356,6 +356,42 @@ def test_fused_walk_sensitivities_bitwise(mesh, ora):
             assert np.count_nonzero(to_host(rt_b) == floor) > 0
 
 
+def test_walk_width_is_bitwise_neutral(mesh, ora):
+    """1 to 4 upstream walks per lane give the same selection, rho_tilde and
+    sensitivities, bit for bit, over full and partial blocks (the mesh size
+    is not a multiple of W x block), and match the oracle's upstream."""
+    import torch
+
+    from up3d_amd import _native
+    from up3d_amd.device import to_host
+    from up3d_amd.kernels.upwind import UpwindOperator
+
+    upw = UpwindOperator(mesh.nodes, mesh.elements)
+    st = upw.store
+    q2, rho, g = ora["q2"], ora["rho"], ora["g"]
+    gs, q2s, rhos = st.to_storage(g.reshape(-1, 3)), st.to_storage(q2), st.to_storage(rho)
+    L = _native.lib()
+    prev = int(L.fp3d_get_walk_width())
+    out = {}
+    try:
+        for w in (1, 2, 3, 4):
+            _native.check(L.fp3d_set_walk_width(w), "set_walk_width")
+            up = torch.full((mesh.n_tets,), -7, dtype=torch.int32, device=st.device)
+            se, su = torch.full_like(q2s, float("nan")), torch.full_like(q2s, float("nan"))
+            rt = upw.rho_tilde_dev(gs, q2s, rhos, M, C, MC, rho_floor=0.05, up_out=up,
+                                   rt_out=torch.full_like(q2s, float("nan")), se_out=se,
+                                   su_out=su)
+            out[w] = [to_host(t).copy() for t in (up, rt, se, su)]
+    finally:
+        _native.check(L.fp3d_set_walk_width(prev), "set_walk_width")
+    assert mesh.n_tets % (2 * int(L.fp3d_get_element_block())) != 0
+    for w in (1, 3, 4):
+        for a, b in zip(out[w], out[2]):
+            assert np.array_equal(a, b)
+    assert np.array_equal(out[2][0].astype(np.int64), ora["up"])
+    assert L.fp3d_set_walk_width(5) != 0 and "walk_width" in L.fp3d_last_error().decode()
+
+
 def test_gmres_matches_scipy(mesh, ora, gpu):
     import scipy.sparse.linalg as spla
 

@@ -117,6 +117,8 @@ _SIGS = {
     "fp3d_get_element_block": (_int, []),
     "fp3d_set_gather_chunk": (_int, [_int]),
     "fp3d_get_gather_chunk": (_int, []),
+    "fp3d_set_walk_width": (_int, [_int]),
+    "fp3d_get_walk_width": (_int, []),
     "fp3d_scale_inv_dev_to_f32": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_norms": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_amg_cheb_smooth": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
@@ -166,6 +168,12 @@ def lib():
             raise NativeMissing(
                 f"{LIB_PATH} is missing: build it with "
                 "`python -m up3d_amd.csrc.build` (hipcc, gfx950)")
+        # torch ships its own libamdhip64.so.7 (same soname as the system
+        # one the library links): load torch first so the process holds one
+        # HIP runtime, the one torch's streams and allocations belong to.
+        # Loaded the other way round, torch binds to the system runtime and
+        # the first launch fails with "no ROCm-capable device".
+        import torch  # noqa: F401
         h = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(h, name)

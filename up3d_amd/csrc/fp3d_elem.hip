@@ -385,19 +385,12 @@ __device__ __forceinline__ double d_dnu(double q2, double m2, double m,
 // the Jacobian of an accepted state, so the linearisation skips its own
 // pass over q2 / rho of e and of u.  The same expressions as sweep_one and
 // k_sens, so the same bits.
-__global__ __launch_bounds__(kBlock) void k_upwind_sens_d(
-    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
-    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
-    const double *__restrict__ grad, const double *__restrict__ q2,
-    const double *__restrict__ rho, int max_hops, double m, double mc2,
-    double C, double g, double floor_, int32_t *__restrict__ up_out,
-    double *__restrict__ nu_out, double *__restrict__ rt_out,
+// nu, rho_tilde and the exact (s_e, s_u) of element e with upstream u
+__device__ __forceinline__ void sens_of_selection(
+    int64_t e, int32_t u, const double *__restrict__ q2,
+    const double *__restrict__ rho, double m, double mc2, double C, double g,
+    double floor_, double *__restrict__ nu_out, double *__restrict__ rt_out,
     double *__restrict__ se, double *__restrict__ su) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int32_t u = walk_upstream_d(e, fn, dcen, nodes, elements, grad[3 * e],
-                                    grad[3 * e + 1], grad[3 * e + 2], max_hops);
-  if (up_out) up_out[e] = u;
   const double q2e = q2[e], q2u = q2[u];
   const double rho_e = rho[e], rho_u = rho[u];
   const double m2 = d_mach2(q2e, m, g);
@@ -434,6 +427,140 @@ __global__ __launch_bounds__(kBlock) void k_upwind_sens_d(
     const double dnu = d_dnu(q2u, m2u, m, mc2, C, g);
     se[e] = drho_e * (1.0 - nu);
     su[e] = nu * drho_u - djump * dnu;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_upwind_sens_d(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, const double *__restrict__ q2,
+    const double *__restrict__ rho, int max_hops, double m, double mc2,
+    double C, double g, double floor_, int32_t *__restrict__ up_out,
+    double *__restrict__ nu_out, double *__restrict__ rt_out,
+    double *__restrict__ se, double *__restrict__ su) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int32_t u = walk_upstream_d(e, fn, dcen, nodes, elements, grad[3 * e],
+                                    grad[3 * e + 1], grad[3 * e + 2], max_hops);
+  if (up_out) up_out[e] = u;
+  sens_of_selection(e, u, q2, rho, m, mc2, C, g, floor_, nu_out, rt_out, se, su);
+}
+
+// One walk's state, for the two-walk lanes below.
+struct WalkState {
+  int32_t cur;
+  int hops;
+  double reach, target, gx, gy, gz;
+  bool active;
+};
+
+__device__ __forceinline__ void walk_begin(WalkState &w, int64_t e, bool valid,
+                                           const double *__restrict__ nodes,
+                                           const int32_t *__restrict__ elements,
+                                           const double *__restrict__ grad,
+                                           int max_hops) {
+  w.cur = (int32_t)e;
+  w.hops = 0;
+  w.reach = 0.0;
+  w.active = false;
+  if (!valid) return;
+  w.gx = grad[3 * e];
+  w.gy = grad[3 * e + 1];
+  w.gz = grad[3 * e + 2];
+  double pmin = 1e300, pmax = -1e300;
+  const int4 nd4 = *reinterpret_cast<const int4 *>(elements + 4 * e);
+  const int32_t nds[4] = {nd4.x, nd4.y, nd4.z, nd4.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int64_t nd = nds[k];
+    double p = nodes[3 * nd] * w.gx + nodes[3 * nd + 1] * w.gy + nodes[3 * nd + 2] * w.gz;
+    if (p < pmin) pmin = p;
+    if (p > pmax) pmax = p;
+  }
+  w.target = 0.8 * (pmax - pmin);
+  w.active = w.hops < max_hops && w.reach < w.target;
+}
+
+// One hop from rows already loaded: the same comparisons and updates as
+// walk_upstream_d.
+__device__ __forceinline__ void walk_hop(WalkState &w, const int4 nb4,
+                                         const double2 *dp, int max_hops) {
+  const int32_t nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+  const double dx[4] = {dp[0].x, dp[1].y, dp[3].x, dp[4].y};
+  const double dy[4] = {dp[0].y, dp[2].x, dp[3].y, dp[5].x};
+  const double dz[4] = {dp[1].x, dp[2].y, dp[4].x, dp[5].y};
+  int32_t best = -1;
+  double best_disp = 0.0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (nbs[f] >= 0) {
+      double d = dx[f] * w.gx + dy[f] * w.gy + dz[f] * w.gz;
+      if (d < best_disp) {
+        best_disp = d;
+        best = nbs[f];
+      }
+    }
+  }
+  if (best < 0) {
+    w.active = false;
+    return;
+  }
+  w.cur = best;
+  w.reach += -best_disp;
+  w.hops += 1;
+  w.active = w.hops < max_hops && w.reach < w.target;
+}
+
+// W elements per lane (e + k blockDim.x, k < W, of the block's W blockDim.x
+// range, each coalesced): their walks advance together, every active walk's
+// rows loaded before any is used, so W dependent chains of random reads are
+// in flight per lane.  Per element the same u(e), nu, rho_tilde and
+// (s_e, s_u) as k_upwind_sens_d; W trades chains per lane against
+// registers (occupancy), chosen by fp3d_set_walk_width.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, const double *__restrict__ q2,
+    const double *__restrict__ rho, int max_hops, double m, double mc2,
+    double C, double g, double floor_, int32_t *__restrict__ up_out,
+    double *__restrict__ nu_out, double *__restrict__ rt_out,
+    double *__restrict__ se, double *__restrict__ su) {
+  const int64_t e0 = (int64_t)blockIdx.x * W * blockDim.x + threadIdx.x;
+  if (e0 >= n) return;
+  WalkState w[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    walk_begin(w[k], e0 + (int64_t)k * blockDim.x, e0 + (int64_t)k * blockDim.x < n,
+               nodes, elements, grad, max_hops);
+  bool any = true;
+  while (any) {
+    int4 nb[W];
+    double2 d[W][6];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (w[k].active) {
+        nb[k] = *reinterpret_cast<const int4 *>(fn + 4 * (int64_t)w[k].cur);
+        const double2 *p = reinterpret_cast<const double2 *>(dcen + 12 * (int64_t)w[k].cur);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) d[k][j] = p[j];
+      }
+    }
+    any = false;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      if (w[k].active) walk_hop(w[k], nb[k], d[k], max_hops);
+      any = any || w[k].active;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int64_t e = e0 + (int64_t)k * blockDim.x;
+    if (e < n) {
+      if (up_out) up_out[e] = w[k].cur;
+      sens_of_selection(e, w[k].cur, q2, rho, m, mc2, C, g, floor_, nu_out, rt_out,
+                        se, su);
+    }
   }
 }
 
@@ -1141,6 +1268,7 @@ int fp3d_reduce_work_doubles(void) { return 3 * kReduceBlocks; }
 static int g_elem_block = kBlock;
 // rows per staged chunk of the incidence-ordered CSR gather (autotuned)
 static int g_gather_chunk = 1024;
+static int g_walk_width = 2;
 
 #define ELEMWISE(kernel, n, ...)                                       \
   do {                                                                 \
@@ -1258,9 +1386,28 @@ int fp3d_upwind_rho_tilde_sens_d(
   FP3D_REQUIRE(FP3D_ALIGNED16(elements) && FP3D_ALIGNED16(face_neighbors) &&
                    FP3D_ALIGNED16(face_disp),
                "upwind_rho_tilde_sens_d: arrays must be 16-byte aligned");
-  ELEMWISE(k_upwind_sens_d, n_tets, face_neighbors, face_disp, nodes, elements,
-           grad, q2, rho, max_hops, m_inf, m_crit * m_crit, C, gamma,
-           rho_floor, upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out);
+  FP3D_REQUIRE(n_tets >= 0, "upwind_rho_tilde_sens_d: n_tets < 0");
+  if (n_tets == 0) return FP3D_OK;
+#define FP3D_WALKW(W)                                                           \
+  hipLaunchKernelGGL(k_upwind_sens_dw<W>, dim3(grid_for(n_tets, W * g_elem_block)), \
+                     dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets,          \
+                     face_neighbors, face_disp, nodes, elements, grad, q2, rho,   \
+                     max_hops, m_inf, m_crit * m_crit, C, gamma, rho_floor,       \
+                     upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out)
+  if (g_walk_width == 1)
+    hipLaunchKernelGGL(k_upwind_sens_d, dim3(grid_for(n_tets, g_elem_block)),
+                       dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets,
+                       face_neighbors, face_disp, nodes, elements, grad, q2, rho,
+                       max_hops, m_inf, m_crit * m_crit, C, gamma, rho_floor,
+                       upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out);
+  else if (g_walk_width == 3)
+    FP3D_WALKW(3);
+  else if (g_walk_width == 4)
+    FP3D_WALKW(4);
+  else
+    FP3D_WALKW(2);
+#undef FP3D_WALKW
+  return check_launch("upwind_rho_tilde_sens_d");
 }
 
 int fp3d_rho_tilde_sweep(int64_t n, const double *q2, const double *rho,
@@ -1594,5 +1741,13 @@ int fp3d_set_gather_chunk(int rows) {
 }
 
 int fp3d_get_gather_chunk(void) { return g_gather_chunk; }
+
+int fp3d_set_walk_width(int walks) {
+  FP3D_REQUIRE(walks >= 1 && walks <= 4, "set_walk_width: walks must be 1, 2, 3 or 4");
+  g_walk_width = walks;
+  return FP3D_OK;
+}
+
+int fp3d_get_walk_width(void) { return g_walk_width; }
 
 }  // extern "C"
