@@ -382,6 +382,11 @@ def test_walk_width_is_bitwise_neutral(mesh, ora):
                                    rt_out=torch.full_like(q2s, float("nan")), se_out=se,
                                    su_out=su)
             out[w] = [to_host(t).copy() for t in (up, rt, se, su)]
+            # the walk without sensitivities, same width
+            up2 = torch.full_like(up, -7)
+            rt2 = upw.rho_tilde_dev(gs, q2s, rhos, M, C, MC, rho_floor=0.05, up_out=up2,
+                                    rt_out=torch.full_like(q2s, float("nan")))
+            assert torch.equal(up2, up) and torch.equal(rt2, rt)
     finally:
         _native.check(L.fp3d_set_walk_width(prev), "set_walk_width")
     assert mesh.n_tets % (2 * int(L.fp3d_get_element_block())) != 0

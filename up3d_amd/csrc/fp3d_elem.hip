@@ -385,7 +385,8 @@ __device__ __forceinline__ double d_dnu(double q2, double m2, double m,
 // the Jacobian of an accepted state, so the linearisation skips its own
 // pass over q2 / rho of e and of u.  The same expressions as sweep_one and
 // k_sens, so the same bits.
-// nu, rho_tilde and the exact (s_e, s_u) of element e with upstream u
+// nu, rho_tilde and (SENS) the exact (s_e, s_u) of element e with upstream u
+template <bool SENS>
 __device__ __forceinline__ void sens_of_selection(
     int64_t e, int32_t u, const double *__restrict__ q2,
     const double *__restrict__ rho, double m, double mc2, double C, double g,
@@ -401,6 +402,7 @@ __device__ __forceinline__ void sens_of_selection(
   const double rt = rho_e - nu * (rho_e - rho_u);
   nu_out[e] = nu;
   rt_out[e] = rt > floor_ ? rt : floor_;
+  if (!SENS) return;
   const double drho_e = d_drho(q2e, m, g);
   if (u == e) {
     se[e] = rho_e <= floor_ ? 0.0 : drho_e;
@@ -443,7 +445,8 @@ __global__ __launch_bounds__(kBlock) void k_upwind_sens_d(
   const int32_t u = walk_upstream_d(e, fn, dcen, nodes, elements, grad[3 * e],
                                     grad[3 * e + 1], grad[3 * e + 2], max_hops);
   if (up_out) up_out[e] = u;
-  sens_of_selection(e, u, q2, rho, m, mc2, C, g, floor_, nu_out, rt_out, se, su);
+  sens_of_selection<true>(e, u, q2, rho, m, mc2, C, g, floor_, nu_out, rt_out, se,
+                          su);
 }
 
 // One walk's state, for the two-walk lanes below.
@@ -515,9 +518,10 @@ __device__ __forceinline__ void walk_hop(WalkState &w, const int4 nb4,
 // range, each coalesced): their walks advance together, every active walk's
 // rows loaded before any is used, so W dependent chains of random reads are
 // in flight per lane.  Per element the same u(e), nu, rho_tilde and
-// (s_e, s_u) as k_upwind_sens_d; W trades chains per lane against
+// (s_e, s_u) as k_upwind_sens_d (SENS; without, the same u, nu and
+// rho_tilde as k_upwind_fused_d); W trades chains per lane against
 // registers (occupancy), chosen by fp3d_set_walk_width.
-template <int W>
+template <int W, bool SENS>
 __global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
     int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
     const double *__restrict__ nodes, const int32_t *__restrict__ elements,
@@ -558,8 +562,8 @@ __global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
     const int64_t e = e0 + (int64_t)k * blockDim.x;
     if (e < n) {
       if (up_out) up_out[e] = w[k].cur;
-      sens_of_selection(e, w[k].cur, q2, rho, m, mc2, C, g, floor_, nu_out, rt_out,
-                        se, su);
+      sens_of_selection<SENS>(e, w[k].cur, q2, rho, m, mc2, C, g, floor_, nu_out,
+                              rt_out, se, su);
     }
   }
 }
@@ -1270,6 +1274,40 @@ static int g_elem_block = kBlock;
 static int g_gather_chunk = 1024;
 static int g_walk_width = 2;
 
+// The walk sweep at the process-wide width (1: one walk per lane, the
+// single-walk kernels, which need the fewest registers).
+extern "C++" template <bool SENS>
+void launch_walk_width(int64_t n, const int32_t *fn, const double *dcen,
+                       const double *nodes, const int32_t *elements,
+                       const double *grad, const double *q2, const double *rho,
+                       int max_hops, double m, double mc2, double C, double g,
+                       double floor_, int32_t *up, double *nu, double *rt,
+                       double *se, double *su, hipStream_t st) {
+#define FP3D_WALKW(W)                                                          \
+  hipLaunchKernelGGL((k_upwind_sens_dw<W, SENS>), dim3(grid_for(n, W * g_elem_block)), \
+                     dim3(g_elem_block), 0, st, n, fn, dcen, nodes, elements,    \
+                     grad, q2, rho, max_hops, m, mc2, C, g, floor_, up, nu, rt,  \
+                     se, su)
+  if (g_walk_width == 1) {
+    if (SENS)
+      hipLaunchKernelGGL(k_upwind_sens_d, dim3(grid_for(n, g_elem_block)),
+                         dim3(g_elem_block), 0, st, n, fn, dcen, nodes, elements,
+                         grad, q2, rho, max_hops, m, mc2, C, g, floor_, up, nu, rt,
+                         se, su);
+    else
+      hipLaunchKernelGGL(k_upwind_fused_d, dim3(grid_for(n, g_elem_block)),
+                         dim3(g_elem_block), 0, st, n, fn, dcen, nodes, elements,
+                         grad, q2, rho, max_hops, m, mc2, C, g, floor_, up, nu, rt);
+  } else if (g_walk_width == 3) {
+    FP3D_WALKW(3);
+  } else if (g_walk_width == 4) {
+    FP3D_WALKW(4);
+  } else {
+    FP3D_WALKW(2);
+  }
+#undef FP3D_WALKW
+}
+
 #define ELEMWISE(kernel, n, ...)                                       \
   do {                                                                 \
     FP3D_REQUIRE(n >= 0, #kernel ": negative size");                   \
@@ -1367,9 +1405,13 @@ int fp3d_upwind_rho_tilde_d(int64_t n_tets, const int32_t *face_neighbors,
   FP3D_REQUIRE(FP3D_ALIGNED16(elements) && FP3D_ALIGNED16(face_neighbors) &&
                    FP3D_ALIGNED16(face_disp),
                "upwind_rho_tilde_d: arrays must be 16-byte aligned");
-  ELEMWISE(k_upwind_fused_d, n_tets, face_neighbors, face_disp, nodes, elements,
-           grad, q2, rho, max_hops, m_inf, m_crit * m_crit, C, gamma,
-           rho_floor, upstream_out, nu_out, rho_tilde_out);
+  FP3D_REQUIRE(n_tets >= 0, "upwind_rho_tilde_d: n_tets < 0");
+  if (n_tets == 0) return FP3D_OK;
+  launch_walk_width<false>(n_tets, face_neighbors, face_disp, nodes, elements, grad, q2,
+                           rho, max_hops, m_inf, m_crit * m_crit, C, gamma, rho_floor,
+                           upstream_out, nu_out, rho_tilde_out, nullptr, nullptr,
+                           FP3D_STREAM(stream));
+  return check_launch("upwind_rho_tilde_d");
 }
 
 int fp3d_upwind_rho_tilde_sens_d(
@@ -1388,25 +1430,10 @@ int fp3d_upwind_rho_tilde_sens_d(
                "upwind_rho_tilde_sens_d: arrays must be 16-byte aligned");
   FP3D_REQUIRE(n_tets >= 0, "upwind_rho_tilde_sens_d: n_tets < 0");
   if (n_tets == 0) return FP3D_OK;
-#define FP3D_WALKW(W)                                                           \
-  hipLaunchKernelGGL(k_upwind_sens_dw<W>, dim3(grid_for(n_tets, W * g_elem_block)), \
-                     dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets,          \
-                     face_neighbors, face_disp, nodes, elements, grad, q2, rho,   \
-                     max_hops, m_inf, m_crit * m_crit, C, gamma, rho_floor,       \
-                     upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out)
-  if (g_walk_width == 1)
-    hipLaunchKernelGGL(k_upwind_sens_d, dim3(grid_for(n_tets, g_elem_block)),
-                       dim3(g_elem_block), 0, FP3D_STREAM(stream), n_tets,
-                       face_neighbors, face_disp, nodes, elements, grad, q2, rho,
-                       max_hops, m_inf, m_crit * m_crit, C, gamma, rho_floor,
-                       upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out);
-  else if (g_walk_width == 3)
-    FP3D_WALKW(3);
-  else if (g_walk_width == 4)
-    FP3D_WALKW(4);
-  else
-    FP3D_WALKW(2);
-#undef FP3D_WALKW
+  launch_walk_width<true>(n_tets, face_neighbors, face_disp, nodes, elements, grad, q2,
+                          rho, max_hops, m_inf, m_crit * m_crit, C, gamma, rho_floor,
+                          upstream_out, nu_out, rho_tilde_out, s_e_out, s_u_out,
+                          FP3D_STREAM(stream));
   return check_launch("upwind_rho_tilde_sens_d");
 }
 
