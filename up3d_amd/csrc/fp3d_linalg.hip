@@ -610,14 +610,69 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *sh,
   return pre + x - v;
 }
 
-// exclusive scan, pass 1: kScanPer consecutive entries per lane, block
-// totals out
+// exclusive scan (reduce, scan the block totals, rescan), pass 1:
+// kScanPer consecutive entries per lane, block totals out; the entries are
+// read again in pass 3 instead of writing and re-reading a partial scan
 constexpr int kScanPer = 4;
 constexpr int kScanTile = kBlock * kScanPer;
 __global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n,
                                                    const int32_t *__restrict__ in,
-                                                   int32_t *__restrict__ out,
                                                    int32_t *__restrict__ bsum) {
+  __shared__ int32_t sh[kBlock / 64];
+  const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  int32_t t = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) t += i0 + k < n ? in[i0 + k] : 0;
+  int32_t tot;
+  (void)block_excl_scan(t, sh, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+// pass 2: exclusive scan of the block totals in one workgroup: tiles of
+// kScanP2Per * kBlock totals staged through LDS (coalesced), kScanP2Per
+// consecutive totals per lane, one block scan per tile, running carry
+constexpr int kScanP2Per = 16;
+__global__ __launch_bounds__(kBlock) void k_scan_p2(int nb,
+                                                   int32_t *__restrict__ bsum,
+                                                   int32_t *__restrict__ total) {
+  constexpr int kTile = kScanP2Per * kBlock;
+  __shared__ int32_t tile[kTile];
+  __shared__ int32_t sh[kBlock / 64];
+  int32_t carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kTile) {
+#pragma unroll
+    for (int k = 0; k < kScanP2Per; ++k) {
+      const int i = b0 + k * kBlock + threadIdx.x;
+      tile[k * kBlock + threadIdx.x] = i < nb ? bsum[i] : 0;
+    }
+    __syncthreads();
+    int32_t *mine = tile + threadIdx.x * kScanP2Per;
+    int32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < kScanP2Per; ++k) t += mine[k];
+    int32_t tot;
+    int32_t run = carry + block_excl_scan(t, sh, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanP2Per; ++k) {
+      const int32_t v = mine[k];
+      mine[k] = run;
+      run += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanP2Per; ++k) {
+      const int i = b0 + k * kBlock + threadIdx.x;
+      if (i < nb) bsum[i] = tile[k * kBlock + threadIdx.x];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+// pass 3: scan the block's entries again from its offset; with act != NULL
+// also compact: the index of every nonzero entry goes to act[out[i]]
+__global__ __launch_bounds__(kBlock) void k_scan_p3(
+    int64_t n, const int32_t *__restrict__ in, int32_t *__restrict__ out,
+    const int32_t *__restrict__ bsum, int32_t *__restrict__ act) {
   __shared__ int32_t sh[kBlock / 64];
   const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
   int32_t v[kScanPer], t = 0;
@@ -627,54 +682,22 @@ __global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n,
     t += v[k];
   }
   int32_t tot;
-  int32_t run = block_excl_scan(t, sh, &tot);
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    if (i0 + k < n) out[i0 + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
-// pass 2: exclusive scan of the block totals in one workgroup, in tiles of
-// kBlock with a running carry (coalesced)
-__global__ __launch_bounds__(kBlock) void k_scan_p2(int nb,
-                                                   int32_t *__restrict__ bsum,
-                                                   int32_t *__restrict__ total) {
-  __shared__ int32_t sh[kBlock / 64];
-  int32_t carry = 0;
-  for (int b0 = 0; b0 < nb; b0 += kBlock) {
-    const int i = b0 + threadIdx.x;
-    const int32_t v = i < nb ? bsum[i] : 0;
-    int32_t tot;
-    const int32_t ex = block_excl_scan(v, sh, &tot);
-    if (i < nb) bsum[i] = carry + ex;
-    carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
-// pass 3: add the block offsets; with act != NULL also compact: the index
-// of every nonzero entry goes to act[out[i]]
-__global__ __launch_bounds__(kBlock) void k_scan_p3(
-    int64_t n, const int32_t *__restrict__ in, int32_t *__restrict__ out,
-    const int32_t *__restrict__ bsum, int32_t *__restrict__ act) {
-  const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-  const int32_t b = bsum[blockIdx.x];
+  int32_t run = bsum[blockIdx.x] + block_excl_scan(t, sh, &tot);
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
     const int64_t i = i0 + k;
     if (i < n) {
-      const int32_t o = out[i] + b;
-      out[i] = o;
-      if (act && in[i]) act[o] = (int32_t)i;
+      out[i] = run;
+      if (act && v[k]) act[run] = (int32_t)i;
     }
+    run += v[k];
   }
 }
 
 int exclusive_scan(int64_t n, const int32_t *in, int32_t *out, int32_t *bsum,
                    int32_t *total, int32_t *act, hipStream_t s) {
   const int nb = (int)((n + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_p1, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum);
+  hipLaunchKernelGGL(k_scan_p1, dim3(nb), dim3(kBlock), 0, s, n, in, bsum);
   hipLaunchKernelGGL(k_scan_p2, dim3(1), dim3(kBlock), 0, s, nb, bsum, total);
   hipLaunchKernelGGL(k_scan_p3, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum,
                      act);
