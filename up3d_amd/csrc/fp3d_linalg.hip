@@ -506,7 +506,7 @@ __global__ void k_add(int64_t n, const double *__restrict__ d,
 // ----------------------------------------------------------- Term 3 (device)
 // Workspace layout (byte offsets computed by t3_layout).
 struct T3Layout {
-  size_t flag, flag8, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, fq, Wq,
+  size_t flag8, scan, bsum, act, W, Cc, tval, cnt, rowptr, ent, fq, Wq,
       total;
 };
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -520,7 +520,6 @@ inline T3Layout t3_layout(int64_t nt, int64_t nn) {
   };
   int64_t nmax = nt > nn ? nt : nn;
   int64_t nbs = (nmax + 1 + kBlock - 1) / kBlock + 1;
-  L.flag = take(sizeof(int32_t) * (nt + 1));
   L.flag8 = take(nt + 1);
   L.scan = take(sizeof(int32_t) * (nmax + 2));
   L.bsum = take(sizeof(int32_t) * (nbs + 1));
@@ -547,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_t3_coef(
     const int32_t *__restrict__ up,
     const double *__restrict__ s_u, const uint8_t *__restrict__ lim,
     double inv_u2, const int32_t *__restrict__ pos,
-    int32_t *__restrict__ flag, uint8_t *__restrict__ flag8,
+    uint8_t *__restrict__ flag8,
     double *__restrict__ Wi, double *__restrict__ Cc) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
@@ -555,7 +554,6 @@ __global__ __launch_bounds__(kBlock) void k_t3_coef(
   double s = s_u[e];
   if (lim) s = s * (lim[u] ? 1.0 : 0.0);
   bool act = s != 0.0;
-  flag[e] = act ? 1 : 0;
   flag8[e] = act ? 1 : 0;
   if (!act) return;
   double w = 2.0 * inv_u2 * s * V[e];
@@ -615,14 +613,14 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *sh,
 // read again in pass 3 instead of writing and re-reading a partial scan
 constexpr int kScanPer = 4;
 constexpr int kScanTile = kBlock * kScanPer;
-__global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n,
-                                                   const int32_t *__restrict__ in,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_p1(int64_t n, const T *__restrict__ in,
                                                    int32_t *__restrict__ bsum) {
   __shared__ int32_t sh[kBlock / 64];
   const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
   int32_t t = 0;
 #pragma unroll
-  for (int k = 0; k < kScanPer; ++k) t += i0 + k < n ? in[i0 + k] : 0;
+  for (int k = 0; k < kScanPer; ++k) t += i0 + k < n ? (int32_t)in[i0 + k] : 0;
   int32_t tot;
   (void)block_excl_scan(t, sh, &tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -670,15 +668,16 @@ __global__ __launch_bounds__(kBlock) void k_scan_p2(int nb,
 }
 // pass 3: scan the block's entries again from its offset; with act != NULL
 // also compact: the index of every nonzero entry goes to act[out[i]]
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scan_p3(
-    int64_t n, const int32_t *__restrict__ in, int32_t *__restrict__ out,
+    int64_t n, const T *__restrict__ in, int32_t *__restrict__ out,
     const int32_t *__restrict__ bsum, int32_t *__restrict__ act) {
   __shared__ int32_t sh[kBlock / 64];
   const int64_t i0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
   int32_t v[kScanPer], t = 0;
 #pragma unroll
   for (int k = 0; k < kScanPer; ++k) {
-    v[k] = i0 + k < n ? in[i0 + k] : 0;
+    v[k] = i0 + k < n ? (int32_t)in[i0 + k] : 0;
     t += v[k];
   }
   int32_t tot;
@@ -694,12 +693,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_p3(
   }
 }
 
-int exclusive_scan(int64_t n, const int32_t *in, int32_t *out, int32_t *bsum,
+// in: int32 counts or uint8 flags (the element selection's byte flags)
+template <typename T>
+int exclusive_scan(int64_t n, const T *in, int32_t *out, int32_t *bsum,
                    int32_t *total, int32_t *act, hipStream_t s) {
   const int nb = (int)((n + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_p1, dim3(nb), dim3(kBlock), 0, s, n, in, bsum);
+  hipLaunchKernelGGL(k_scan_p1<T>, dim3(nb), dim3(kBlock), 0, s, n, in, bsum);
   hipLaunchKernelGGL(k_scan_p2, dim3(1), dim3(kBlock), 0, s, nb, bsum, total);
-  hipLaunchKernelGGL(k_scan_p3, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum,
+  hipLaunchKernelGGL(k_scan_p3<T>, dim3(nb), dim3(kBlock), 0, s, n, in, out, bsum,
                      act);
   return check_launch("exclusive_scan");
 }
@@ -912,7 +913,6 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
   hipStream_t s = FP3D_STREAM(stream);
   T3Layout L = t3_layout(n_tets, n_nodes);
   char *base = (char *)t3;
-  int32_t *flag = (int32_t *)(base + L.flag);
   uint8_t *flag8 = (uint8_t *)(base + L.flag8);
   int32_t *scan = (int32_t *)(base + L.scan);
   int32_t *bsum = (int32_t *)(base + L.bsum);
@@ -928,12 +928,12 @@ int fp3d_term3_build(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
   if (gdotb)
     hipLaunchKernelGGL(k_t3_coef<true>, dim3(grid_for(n_tets)), dim3(kBlock), 0,
                        s, n_tets, elements, B, V, grad, gdotb, upstream, s_u,
-                       lim, inv_u2, inc_pos, flag, flag8, W, Cc);
+                       lim, inv_u2, inc_pos, flag8, W, Cc);
   else
     hipLaunchKernelGGL(k_t3_coef<false>, dim3(grid_for(n_tets)), dim3(kBlock),
                        0, s, n_tets, elements, B, V, grad, gdotb, upstream,
-                       s_u, lim, inv_u2, inc_pos, flag, flag8, W, Cc);
-  int rc = exclusive_scan(n_tets, flag, scan, bsum, hdr, act, s);
+                       s_u, lim, inv_u2, inc_pos, flag8, W, Cc);
+  int rc = exclusive_scan(n_tets, flag8, scan, bsum, hdr, act, s);
   if (rc) return rc;
   const unsigned gn = xcd_grid((n_nodes + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_t3_count, dim3(gn), dim3(kBlock), 0, s, n_nodes,
