@@ -163,9 +163,21 @@ def main():
     phases = [("state", ph_state), ("upwind", ph_upwind),
               ("residual", ph_residual), ("jacobian", ph_jac), ("term3", ph_t3)]
 
+    side = torch.cuda.Stream()
+
     def step():
-        for _, f in phases:
-            f()
+        # as NewtonWorkspace.assemble_coupled: the Term-3 build runs on a
+        # second stream beside Terms 1+2 (both read only the state and walk
+        # outputs); the main stream waits for it
+        ph_state()
+        ph_upwind()
+        ph_residual()
+        main = torch.cuda.current_stream()
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            ph_t3()
+        ph_jac()
+        main.wait_stream(side)
         upw._monitors(upw.d_nu, upw.d_rt, RHO_FLOOR)
 
     if args.calib:

@@ -208,14 +208,29 @@ class NewtonWorkspace:
             su = upw.d_su.zero_()
             upstream = upw.walk_dev(s.grad)
         self.se_used = se          # the preconditioner's SPD matrix reuses it
+        main = side = None
+        if self.m_inf > 0.0:
+            # Term 3 (its own table, read-only inputs) on a second stream,
+            # overlapping Terms 1+2; the main stream waits for it below
+            main = torch.cuda.current_stream()
+            side = self._side_stream()
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                self.jac.build_term3(s.grad, upstream, su, s.lim, inv_u2, gdotb=s.gb)
+        else:
+            self.jac.n_active = 0
         self.op.jacobian_dev(s.rt, s.grad, se, s.lim, inv_u2, self.jac.data)
         self._diag_shift(self.jac.data, ptc_scale)
         self.jac.finalize(f32=self.precision == "fp32")
-        if self.m_inf > 0.0:
-            self.jac.build_term3(s.grad, upstream, su, s.lim, inv_u2, gdotb=s.gb)
-        else:
-            self.jac.n_active = 0
+        if side is not None:
+            main.wait_stream(side)
         return self.jac
+
+    def _side_stream(self):
+        """Second stream of this workspace's device (default priority)."""
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream(device=self.st.device)
+        return self._side
 
     def build_preconditioner(self, s: _State, precond_matrix: str = "picard",
                              clamp: float = 0.9, se=None, ptc_scale: float = 0.0):
