@@ -85,6 +85,7 @@ def test_tune_newton_workspace_and_cache(gpu, tmp_path):
     ws = NewtonWorkspace(m.nodes, m.elements, dn, dv)
     cache = TuningCache(str(tmp_path / "tune.json"))
     prev = int(_native.lib().fp3d_get_element_block())
+    prev_w = int(_native.lib().fp3d_get_walk_width())
     try:
         r = tune_newton_workspace(ws, m.nodes[:, 0].copy(), 0.7, precond_matrix="picard",
                                   restarts=(30, 60), degrees=(1, 2), max_coarse=(10, 500),
@@ -93,6 +94,8 @@ def test_tune_newton_workspace_and_cache(gpu, tmp_path):
         assert r["cached"] is False
         assert r["element_block"]["best"] in (64, 128, 192, 256)
         assert int(_native.lib().fp3d_get_element_block()) == r["element_block"]["best"]
+        assert r["walk_width"]["best"] in (1, 2, 3) and len(r["walk_width"]["ms"]) == 3
+        assert int(_native.lib().fp3d_get_walk_width()) == r["walk_width"]["best"]
         assert len(r["solver"]["trials"]) == 8 and any(t["converged"] for t in r["solver"]["trials"])
         again = tune_newton_workspace(ws, m.nodes[:, 0].copy(), 0.7, cache=cache)
         assert again["cached"] is True and again["solver"]["best"] == r["solver"]["best"]
@@ -104,6 +107,7 @@ def test_tune_newton_workspace_and_cache(gpu, tmp_path):
         assert res["converged"] and base["converged"]
         assert np.abs(res["phi"] - base["phi"]).max() < 1e-8
     finally:
-        from up3d_amd.tune import apply_element_block
+        from up3d_amd.tune import apply_element_block, apply_walk_width
 
         apply_element_block(prev)
+        apply_walk_width(prev_w)

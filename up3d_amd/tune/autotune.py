@@ -10,6 +10,8 @@ stream, on the caller's own mesh and state:
     (fp3d_set_element_block).
   * tune_gather_chunk: rows per LDS-staged chunk of the CSR matrix gather
     over {512, 1024} (fp3d_set_gather_chunk; the sums are unchanged).
+  * tune_walk_width: upstream walks per lane over {1, 2, 3}
+    (fp3d_set_walk_width; the selections are unchanged).
   * tune_linear_solver: GMRES restart x AMG Chebyshev degree x AMG coarsest
     size on one Newton linear system J d = -R, by wall time to the asked
     relative tolerance (AMG set-up included: it is paid once per rebuild).
@@ -32,6 +34,7 @@ from .. import _native
 
 ELEMENT_BLOCKS = (64, 128, 192, 256)
 GATHER_CHUNKS = (512, 1024)
+WALK_WIDTHS = (1, 2, 3)
 
 
 def _event_time(fn: Callable[[], None], reps: int) -> float:
@@ -93,6 +96,29 @@ def tune_gather_chunk(pipeline: Callable[[], None],
         raise
     best = min(times, key=times.get)
     apply_gather_chunk(best)
+    return {"best": best, "ms": times}
+
+
+def apply_walk_width(walks: int) -> None:
+    _native.call("fp3d_set_walk_width", int(walks))
+
+
+def tune_walk_width(pipeline: Callable[[], None],
+                    candidates: Sequence[int] = WALK_WIDTHS,
+                    reps: int = 5) -> Dict[str, object]:
+    """Time `pipeline` (a pass through the upstream walk) at each number of
+    walks per lane, apply the fastest and return {"best": w, "ms": {...}}."""
+    prev = int(_native.lib().fp3d_get_walk_width())
+    times = {}
+    try:
+        for w in candidates:
+            apply_walk_width(w)
+            times[int(w)] = _event_time(pipeline, reps)
+    except Exception:
+        apply_walk_width(prev)
+        raise
+    best = min(times, key=times.get)
+    apply_walk_width(best)
     return {"best": best, "ms": times}
 
 
@@ -200,6 +226,8 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
         apply_element_block(hit["element_block"]["best"])
         if "gather_chunk" in hit:
             apply_gather_chunk(hit["gather_chunk"]["best"])
+        if "walk_width" in hit:
+            apply_walk_width(hit["walk_width"]["best"])
         return dict(hit, cached=True)
     ws.set_mach(m_inf)
     s = ws.states[2]
@@ -212,6 +240,7 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
 
     eb = tune_element_block(linearise, element_blocks)
     gc = tune_gather_chunk(linearise)
+    ww = tune_walk_width(linearise)
     linearise()
     upw = ws.upw
     se, _ = upw.sensitivities_at_dev(s.q2l, s.rho, s.up, ws.m_inf, upwind_c, m_crit,
@@ -228,7 +257,8 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
     prec = "f32" if ws.precision == "fp32" else "f64"
     sol = tune_linear_solver(ws.jac.matvec, rhs, A_pre, precision=prec, rtol=rtol,
                              restarts=restarts, degrees=degrees, max_coarse=max_coarse)
-    entry = {"element_block": eb, "gather_chunk": gc, "solver": sol, "m_inf": float(m_inf),
+    entry = {"element_block": eb, "gather_chunk": gc, "walk_width": ww, "solver": sol,
+             "m_inf": float(m_inf),
              "precision": ws.precision}
     cache.put(key, entry)
     return dict(entry, cached=False)
