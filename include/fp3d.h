@@ -353,6 +353,19 @@ int fp3d_multi_dot_norm(int64_t n, int nv, const double *V, int64_t ld,
 int fp3d_multi_dot_norm_f32v(int64_t n, int nv, const float *V, int64_t ld,
                              const double *w, double *work, double *out,
                              void *stream);
+/* Allocation-free fp3d_multi_dot_norm (no reference counterpart): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments without
+ * `work` and the same bits; the partial sums live in a library-owned device
+ * array of 257 x 1024 doubles (2,105,344 bytes per GPU, held until the
+ * process ends), so nv > 256 is rejected (FP3D_EINVAL).  The array's
+ * address is looked up on the first call on each device (make that call
+ * outside graph capture).  Nothing needs clearing between calls, but all
+ * calls on one GPU share the array: calls that overlap in time on one GPU
+ * (two streams, two host threads) race and must be serialised by the
+ * caller. */
+int fp3d_multi_dot_norm_noalloc(int64_t n, int nv, const double *V, int64_t ld,
+                                const double *w, double *out, void *stream);
 /* w += sign * sum_k h[k] v_k  (h on device, nv <= 256) */
 /* Re-orthogonalisation pass of CGS2 (no reference counterpart; the
  * reference's GMRES is scipy's): out[k] = V_k . w only when the first pass

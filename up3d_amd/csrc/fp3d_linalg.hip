@@ -9,6 +9,7 @@
 //     upstream coupling is applied inside the SpMV without atomics and
 //     without rebuilding the CSR pattern.
 #include <cmath>
+#include <mutex>
 
 #include "fp3d_common.h"
 
@@ -858,6 +859,48 @@ static int multi_dot_any(int64_t n, int nv, const TV *V, int64_t ld, const doubl
   return check_launch(what);
 }
 
+// ---------------------------------------------- allocation-free multi-dot
+// Partial sums of fp3d_multi_dot_norm_noalloc live in this file-scope
+// device array instead of a caller workspace: (kNoallocMaxNv + 1) rows of
+// kReduceBlocks doubles = 2,105,344 bytes, held by every GPU the library
+// runs on until the process ends.  Host code never passes `scratch` itself
+// (in host code the name is a stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, once per device, and is cached in
+// g_scratch_dev[device].
+constexpr int kNoallocMaxNv = 256;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[(kNoallocMaxNv + 1) * kReduceBlocks];
+static double *g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch` (looked up on the first call on
+// that device, with that device current), or nullptr with the error set.
+static double *scratch_on_current_device() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error(std::string("multi_dot_norm_noalloc: hipGetDevice: ") +
+              hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("multi_dot_norm_noalloc: device ordinal beyond 63");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    double *p = nullptr;
+    e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("multi_dot_norm_noalloc: hipGetSymbolAddress: ") +
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  return g_scratch_dev[dev];
+}
+
 template <typename TV>
 static int multi_axpy_any(int64_t n, int nv, const TV *V, int64_t ld, const double *h,
                           double sign, double *w, void *stream, const char *what) {
@@ -1103,6 +1146,20 @@ int fp3d_multi_dot_norm_f32v(int64_t n, int nv, const float *V, int64_t ld,
                              void *stream) {
   return multi_dot_any<float>(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
                               "multi_dot_norm_f32v: bad arguments", 1);
+}
+
+int fp3d_multi_dot_norm_noalloc(int64_t n, int nv, const double *V, int64_t ld,
+                                const double *w, double *out, void *stream) {
+  // every partial the second pass reads, (nv + 1) x nb with nb <=
+  // kReduceBlocks, is written by the first pass of the same call, so no
+  // clearing is needed; nv is checked here so the rows always fit
+  FP3D_REQUIRE(nv >= 1 && nv <= kNoallocMaxNv,
+               "multi_dot_norm_noalloc: nv must be in [1, 256] (scratch holds "
+               "257 x 1024 partial sums)");
+  double *work = scratch_on_current_device();
+  if (!work) return FP3D_EHIP;
+  return multi_dot_any<double>(n, nv, V, ld, w, nullptr, 0.0, work, out, stream,
+                               "multi_dot_norm_noalloc: bad arguments", 1);
 }
 
 int fp3d_multi_dot_f32v(int64_t n, int nv, const float *V, int64_t ld,
