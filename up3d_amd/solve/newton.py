@@ -476,11 +476,14 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                 accept_reason = accept
                 break
             # honesty check: the live residual at the frozen solution
+            t0 = time.perf_counter()
             live = ws.free_state(cur)
             live.phi.copy_(cur.phi)
             ws.eval_residual(live, upwind_c, m_crit, rho_floor)
             residual_unfrozen = live.r_norm
             up_new, br_new = ws.freeze(live, upwind_c, m_crit, rho_floor)
+            sync()
+            timings["residual"] += time.perf_counter() - t0
             n_assignment_stale = int(
                 (up_new != frozen[0]).sum().item()
                 + ((br_new != frozen[1]) & (up_new == frozen[0])).sum().item())

@@ -362,8 +362,9 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
     timings["seed"] = time.perf_counter() - t0
     beta = float(np.sqrt(1.0 - m_inf ** 2))
     nodes = np.asarray(mesh.nodes, dtype=np.float64)
-    sysm = LSNewtonSystem(mvop, m_inf, upwind_c, m_crit, gamma_air, u_inf,
-                          m_cap, rho_floor, external_rhs)
+    with phase(timings, "assembly"):
+        sysm = LSNewtonSystem(mvop, m_inf, upwind_c, m_crit, gamma_air, u_inf,
+                              m_cap, rho_floor, external_rhs)
     residual_history = []
     converged = False
     n_gmres_total = n_gmres_stalled = 0
@@ -379,12 +380,13 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
     armed = bool(freeze) and freeze_tol is not None
     ff = None
     for it in range(n_newton_max):
-        ff, vals = _farfield_main(nodes, mesh.boundary_faces["farfield"], alpha_deg,
-                                  gamma, u_inf, vortex_center, beta, farfield,
-                                  lift_axis)
-        phi[ff] = vals
-        free = np.ones(mvop.n_total, dtype=bool)
-        free[ff] = False
+        with phase(timings, "assembly", sync=False):
+            ff, vals = _farfield_main(nodes, mesh.boundary_faces["farfield"], alpha_deg,
+                                      gamma, u_inf, vortex_center, beta, farfield,
+                                      lift_axis)
+            phi[ff] = vals
+            free = np.ones(mvop.n_total, dtype=bool)
+            free[ff] = False
         with phase(timings, "residual"):
             R, sides, C, w = sysm.residual(phi, frozen, sens=True)
         res = float(np.abs(R[free]).max())
@@ -418,9 +420,10 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
             if frozen is None:
                 converged, accept_reason = True, "tol"
                 break
-            R_l, _, _, _ = sysm.residual(phi, None)
-            residual_unfrozen = float(np.abs(R_l[free]).max())
-            fz_new = sysm.freeze(phi)
+            with phase(timings, "residual"):
+                R_l, _, _, _ = sysm.residual(phi, None)
+                residual_unfrozen = float(np.abs(R_l[free]).max())
+                fz_new = sysm.freeze(phi)
             n_assignment_stale = int(sum(
                 int((ua != ub).sum().item()) + int(((ba != bb) & (ua == ub)).sum().item())
                 for (ua, ba), (ub, bb) in zip(frozen, fz_new)))
@@ -442,10 +445,11 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
             continue
         if (armed and frozen is None and freeze_cooldown == 0 and m_inf > 0.0
                 and n_lim + n_flr == 0 and (res < freeze_tol or live_stalled)):
-            frozen = sysm.freeze(phi)
-            freeze_point = (phi.copy(), res)
-            r_best = np.inf
-            R, sides, C, w = sysm.residual(phi, frozen, sens=True)
+            with phase(timings, "residual"):
+                frozen = sysm.freeze(phi)
+                freeze_point = (phi.copy(), res)
+                r_best = np.inf
+                R, sides, C, w = sysm.residual(phi, frozen, sens=True)
         with phase(timings, "assembly"):
             sysm.linearize(phi, sides, C, w, ff)
         with phase(timings, "linsolve"):
@@ -473,8 +477,9 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
         phi = best[0]
         with phase(timings, "kutta"):
             gamma = float(np.mean(mvop.te_jump(phi)))
-    sysm.residual(phi, None)
-    mach2_max = mvop.own_side_mach2_max(phi, m_inf, gamma_air, u_inf)
+    with phase(timings, "residual"):
+        sysm.residual(phi, None)
+        mach2_max = mvop.own_side_mach2_max(phi, m_inf, gamma_air, u_inf)
     finalize(timings, time.perf_counter() - t_wall0)
     return {
         "phi_ext": phi, "phi": mvop.main_potential(phi), "gamma": gamma,

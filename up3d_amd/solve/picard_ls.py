@@ -319,7 +319,8 @@ def solve_multivalued_lifting(mvop, mesh, m_inf: float, alpha_deg: float = 2.0,
         if outer > 0 and ((dgamma < tol_gamma and drho < tol_rho) or res_ok):
             converged = True
             break
-    mach2_max = mvop.own_side_mach2_max(phi_ext, m_inf, gamma_air, u_inf)
+    with phase(timings, "residual"):
+        mach2_max = mvop.own_side_mach2_max(phi_ext, m_inf, gamma_air, u_inf)
     finalize(timings, time.perf_counter() - t_wall0)
     return {
         "phi_ext": phi_ext, "phi": mvop.main_potential(phi_ext),
