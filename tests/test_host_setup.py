@@ -122,6 +122,34 @@ def test_channel_sizes_closed_form(shape):
     assert abs(b.sum()) < 1e-14 and b.min() < 0 < b.max()
 
 
+@pytest.mark.parametrize("alpha", [0.0, 25.0])
+def test_neumann_flux_rhs_is_the_boundary_term_of_a_linear_field(alpha):
+    """For phi = u . x (rho = 1, M 0) the Galerkin residual is
+    R_i = int grad phi . grad N_i = oint N_i (u . n) dS (divergence theorem),
+    the consistent P1 load of the flux through every boundary face: with the
+    loads of all six faces of a flat channel it equals R to rounding, at
+    every node (interior nodes jittered).  Per face the load sums to
+    rho u . n times the area: -cos(alpha) at the inflow, +cos(alpha) at the
+    outflow (area 1), -3 sin(alpha) at the floor (area 3)."""
+    m = channel.bump_channel(6, 4, 3, bump_height=0.0, jitter=0.15, seed=9)
+    a = np.deg2rad(alpha)
+    phi = m.nodes[:, 0] * np.cos(a) + m.nodes[:, 1] * np.sin(a)
+    tags = ("inflow", "outflow", "floor", "ceiling", "side_lo", "side_hi")
+    b = channel.neumann_flux_rhs(m, tags=tags, alpha_deg=alpha)
+    R = O.OraclePicardOperator(m.nodes, m.elements).assemble_residual(
+        phi, np.ones(m.n_tets))
+    assert np.abs(R - b).max() < 1e-13 * np.abs(b).max()      # rounding only
+    for tag, total in (("inflow", -np.cos(a)), ("outflow", np.cos(a)),
+                       ("floor", -3.0 * np.sin(a)), ("ceiling", 3.0 * np.sin(a)),
+                       ("side_lo", 0.0)):
+        bt = channel.neumann_flux_rhs(m, tags=(tag,), alpha_deg=alpha, rho_inf=1.0)
+        assert bt.sum() == pytest.approx(total, abs=1e-14)
+    # the default (inflow + outflow) is what the subsonic channel carries;
+    # rho_inf scales it
+    b2 = channel.neumann_flux_rhs(m, rho_inf=2.0)
+    assert np.allclose(b2, 2.0 * channel.neumann_flux_rhs(m), rtol=0, atol=1e-16)
+
+
 def test_bump_profile():
     x = np.array([0.5, 1.0, 1.5, 2.0, 2.5])
     y = channel.bump_profile(x, 0.1)

@@ -370,3 +370,112 @@ def test_wall_force_uniform_flow_zero():
                                     mesh.boundary_faces["floor"], phi, m_inf=0.5)
     assert np.abs(out["cp_tri"]).max() < 1e-12
     assert np.abs(out["cf"]).max() < 1e-12
+
+
+# ------------------------------------------- pressure coefficient, wall force
+def _cp_local_mach(q2, m, g=1.4):
+    """Isentropic Cp written through the local Mach number (the textbook
+    route, independent of the density form the code uses):
+    M_l^2 = q2 M^2 / (1 + (g-1)/2 M^2 (1 - q2)) from the energy equation,
+    p_l / p_inf = ((1 + (g-1)/2 M^2) / (1 + (g-1)/2 M_l^2))^(g/(g-1))."""
+    k = 0.5 * (g - 1.0)
+    ml2 = q2 * m * m / (1.0 + k * m * m * (1.0 - q2))
+    return 2.0 / (g * m * m) * (((1.0 + k * m * m) / (1.0 + k * ml2)) ** (g / (g - 1.0))
+                                - 1.0)
+
+
+def _cp_impls():
+    from up3d_amd.post.surface import cp_from_q2
+
+    return {"oracle": O.pressure_coefficient,
+            "product": lambda q2, m: float(cp_from_q2(np.asarray(q2), m))}
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_pressure_coefficient_closed_forms(impl):
+    cp = _cp_impls()[impl]
+    # stagnation point at M 0.5: isentropic p0/p = 1.05^3.5 = 1.186212..., so
+    # Cp = (p0/p - 1) / (0.7 * 0.25) = 1.0640722 (tables: p/p0 = 0.8430)
+    assert abs(cp(0.0, 0.5) - 1.0640722) < 1e-7
+    # small-M series of the stagnation value: 1 + M^2/4 + M^4/40 + ...
+    m = 0.05
+    assert abs(cp(0.0, m) - (1.0 + m * m / 4.0 + m ** 4 / 40.0)) < 1e-10
+    # incompressible limit 1 - q2 and freestream Cp = 0
+    assert abs(cp(0.64, 1e-4) - 0.36) < 1e-7
+    assert abs(cp(1.0, 0.8)) < 1e-15
+    # the local-Mach route at q2 != 1, sub- and supersonic
+    for q2, mm in [(1.21, 0.7), (0.5, 0.84), (1.6, 0.8), (0.2, 0.3)]:
+        assert abs(cp(q2, mm) - _cp_local_mach(q2, mm)) < 1e-12, (q2, mm)
+    # sonic pressure coefficient Cp* = 2/(g M^2) [((2 + (g-1) M^2)/(g+1))^(g/(g-1)) - 1]
+    # at the sonic speed q*^2 (M 0.8: -0.4346 in the tables)
+    for mm in (0.6, 0.8):
+        cps = 2.0 / (1.4 * mm * mm) * (((2.0 + 0.4 * mm * mm) / 2.4) ** 3.5 - 1.0)
+        assert abs(cp(O.q2_at_mach(1.0, mm), mm) - cps) < 1e-12
+    assert abs(cp(O.q2_at_mach(1.0, 0.8), 0.8) + 0.4346) < 1e-4
+
+
+def _wall_force_impls():
+    from up3d_amd.post.surface import wall_force_coefficients
+
+    return {"oracle": O.wall_force_coefficients, "product": wall_force_coefficients}
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_wall_force_closed_forms(impl):
+    from up3d_amd.mesh.channel import bump_channel
+
+    wfc = _wall_force_impls()[impl]
+    mesh = bump_channel(12, 4, 3)                  # 10 % arc bump on the floor
+    floor = mesh.boundary_faces["floor"]
+    nodes, el = mesh.nodes, mesh.elements
+    n = len(nodes)
+    # phi = 0 at M 0: Cp = 1 on every triangle and cf = -(vector area of the
+    # floor, normals into the fluid).  A surface's vector area depends only
+    # on its boundary loop, here the 3 x 1 rectangle in y = 0, so the bumped
+    # floor gives exactly (0, -3, 0): pins the normal's side and the 1/2
+    out = wfc(nodes, el, floor, np.zeros(n), m_inf=0.0)
+    assert np.allclose(out["cp_tri"], 1.0, atol=1e-15)
+    assert np.allclose(out["cf"], [0.0, -3.0, 0.0], atol=1e-12)
+    # phi = c z: its gradient is tangent to the spanwise-uniform bump, so
+    # q2 = c^2 on every triangle and cf = (0, -3 Cp(c^2), 0)
+    c, m = 1.1, 0.5
+    out = wfc(nodes, el, floor, c * nodes[:, 2], m_inf=m)
+    cp = _cp_local_mach(c * c, m)
+    assert np.allclose(out["cp_tri"], cp, rtol=0, atol=1e-12)
+    assert np.allclose(out["cf"], [0.0, -3.0 * cp, 0.0], atol=1e-11)
+    # the same field at twice the reference speed with phi scaled alike, and S_ref
+    out2 = wfc(nodes, el, floor, 2.0 * c * nodes[:, 2], m_inf=m, u_inf=2.0, s_ref=1.5)
+    assert np.allclose(out2["cf"], out["cf"] / 1.5, atol=1e-11)
+    # lift / pressure-drag axes rotate with alpha
+    a = 3.0
+    ar = np.deg2rad(a)
+    outa = wfc(nodes, el, floor, c * nodes[:, 2], m_inf=m, alpha_deg=a)
+    assert abs(outa["cl"] - (-np.sin(ar) * outa["cf"][0] + np.cos(ar) * outa["cf"][1])) < 1e-14
+    assert abs(outa["cd_pressure"] - (np.cos(ar) * outa["cf"][0]
+                                      + np.sin(ar) * outa["cf"][1])) < 1e-14
+    # a nonuniform field on a rotated copy of the mesh: Cp is invariant and
+    # the force rotates with the mesh
+    phi = nodes[:, 0] + 0.3 * np.sin(2.0 * nodes[:, 0]) + 0.2 * nodes[:, 2]
+    base = wfc(nodes, el, floor, phi, m_inf=0.6)
+    th = np.deg2rad(37.0)
+    Rz = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0],
+                   [0.0, 0.0, 1.0]])
+    rot = wfc(nodes @ Rz.T, el, floor, phi, m_inf=0.6)
+    assert np.allclose(rot["cp_tri"], base["cp_tri"], rtol=0, atol=1e-12)
+    assert np.allclose(rot["cf"], Rz @ base["cf"], rtol=0, atol=1e-12)
+    assert abs(base["cf"][0]) > 1e-3               # a real x-force, not zero
+
+
+def test_wall_force_oracle_equals_product_on_a_transonic_state():
+    from up3d_amd.mesh.channel import bump_channel
+    from up3d_amd.post.surface import wall_force_coefficients
+
+    mesh = bump_channel(12, 4, 3, jitter=0.15, seed=4)
+    x = mesh.nodes[:, 0]
+    phi = 1.25 * x + 0.35 * np.sin(1.7 * x) + 0.05 * mesh.nodes[:, 1] ** 2
+    a = O.wall_force_coefficients(mesh.nodes, mesh.elements, mesh.boundary_faces["floor"],
+                                  phi, m_inf=0.8, alpha_deg=1.0)
+    b = wall_force_coefficients(mesh.nodes, mesh.elements, mesh.boundary_faces["floor"],
+                                phi, m_inf=0.8, alpha_deg=1.0)
+    assert np.allclose(a["cp_tri"], b["cp_tri"], rtol=1e-13, atol=1e-14)
+    assert np.allclose(a["cf"], b["cf"], rtol=1e-12, atol=1e-14)
