@@ -13,10 +13,13 @@ reference's NACA0012 2.5D meshes:
     873-878; 4 significant digits), the same state from a Picard seed at the
     target Mach and from the Mach ramp (the discrete solution is isolated),
     0 limited / floored, shock pocket M_max in (1, 1.3);
-  * medium M0.70: converged, 0 limited / floored; the reference's medium
-    figure (0.1193) was taken with a lagged far-field vortex plus a polish
-    pass, a different fixed point from the live vortex solved here, so it
-    is held to 1 % only."""
+  * medium M0.70: the reference's figure there (0.1193,
+    docs/design_track_b.md:879,882) is a level-set PICARD iterate stopped at
+    ||R||inf ~ 1e-7; the same procedure here gives it to 0.1 %, and Newton
+    started from that very Picard state converges (||R|| < 1e-10) to the
+    same root as the Newton Mach ramp, 0.118312: the Picard's circulation
+    mode converges slowly, and continued past that stop the Picard iterate
+    walks onto the Newton root (0.118307 at ||R|| 2e-9 after 3000 outers)."""
 
 import json
 from pathlib import Path
@@ -115,17 +118,47 @@ def test_ls_newton_transonic_matches_reference_table(gpu):
 
 
 def test_ls_newton_medium_m070(gpu):
-    from up3d_amd.solve.newton_ls import solve_multivalued_newton_transonic
+    from up3d_amd.solve.newton_ls import (LSNewtonSystem, solve_multivalued_newton,
+                                          solve_multivalued_newton_transonic)
+    from up3d_amd.solve.picard_ls import _farfield_main, solve_multivalued_transonic
 
     mesh = _mesh("medium")
     print(f"[wake-newton] medium M0.70 n_tets={len(mesh.elements)}", flush=True)
     _, mv = _setup(mesh)
-    r = solve_multivalued_newton_transonic(mv, mesh, 0.70)
-    print(f"[wake-newton] medium M0.70: gamma={r['gamma']:.6f} "
-          f"levels={[(lv['m_inf'], lv['n_newton']) for lv in r['levels']]}", flush=True)
-    assert r["target_reached"] and r["n_limited"] == 0 and r["n_floored"] == 0
+    # the reference's procedure: level-set Picard Mach ramp, stopped at
+    # ||R||inf < 1e-7 (tol_residual); it reproduces the quoted 0.1193 to 0.2 %
+    # (the stop falls at 0.1192-0.1194 depending on the far-field variant)
+    pic = solve_multivalued_transonic(mv, mesh, 0.70, n_outer_level=600)
     ref = _gold()["medium"]["0.70"]
-    assert abs(r["gamma"] - ref) <= 0.01 * ref
+    print(f"[wake-newton] medium M0.70 Picard (stop at 1e-7): gamma={pic['gamma']:.6f}",
+          flush=True)
+    assert pic["converged"] and pic["n_limited"] == 0 and pic["n_floored"] == 0
+    assert abs(pic["gamma"] - ref) <= 0.002 * ref
+    # ...but that state is not a discrete solution at the 1e-10 bar
+    phi = pic["phi_ext"].copy()
+    ff, vals = _farfield_main(mesh.nodes, mesh.boundary_faces["farfield"], 1.25,
+                              float(pic["gamma"]), 1.0, (0.25, 0.0), np.sqrt(1 - 0.49))
+    phi[ff] = vals
+    free = np.ones(mv.n_total, dtype=bool)
+    free[ff] = False
+    R, _, _, _ = LSNewtonSystem(mv, 0.70).residual(phi, None)
+    r_pic = float(np.abs(R[free]).max())
+    assert r_pic > 1e-9, r_pic
+    # Newton from the Picard state and the Newton Mach ramp reach one root
+    from_pic = solve_multivalued_newton(mv, mesh, 0.70, phi_init=pic["phi_ext"],
+                                        gamma_init=float(pic["gamma"]), n_seed=0,
+                                        freeze_tol=1e-6)
+    r = solve_multivalued_newton_transonic(mv, mesh, 0.70)
+    print(f"[wake-newton] medium M0.70: Newton from Picard gamma={from_pic['gamma']:.8f} "
+          f"(|R| {from_pic['residual_history'][-1]:.1e}), ramp gamma={r['gamma']:.8f}, "
+          f"Picard state |R|={r_pic:.1e}", flush=True)
+    for x in (from_pic, r):
+        assert x["converged"] and x["residual_history"][-1] < 1e-10
+        assert x["n_limited"] == 0 and x["n_floored"] == 0
+    assert r["target_reached"]
+    assert abs(from_pic["gamma"] - r["gamma"]) < 1e-8
+    # the root sits below the Picard stop, as the continued Picard shows
+    assert r["gamma"] < pic["gamma"]
 
 
 def test_schur_reduced_system_solves_fused_matrix(gpu):

@@ -348,12 +348,17 @@ def solve_multivalued_transonic(mvop, mesh, m_target: float,
                                 tol_gamma: float = 1e-6,
                                 tol_residual: Optional[float] = 1e-7,
                                 farfield_lag: str = "live",
+                                n_ff_polish: int = 0, n_outer_polish: int = 200,
                                 **kwargs) -> Dict[str, object]:
     """Mach continuation of the level-set Picard (pyfp3d/solve/picard_ls.py:
     895-1044): each level is a warm-started solve_multivalued_lifting from
     the previous level's full extended state and circulation;
     farfield_lag="level" freezes the far-field vortex at the previous
-    level's circulation.  Returns the final level's dict plus `levels` and
+    level's circulation (level 0 keeps the live vortex), and n_ff_polish
+    rounds at the target Mach then re-converge with the vortex set to the
+    newest circulation, n_outer_polish outers each (one step of the
+    far-field outer map per round).  Returns the final level's dict plus
+    `levels` (polish rounds appended at the target Mach) and
     `mach_schedule`."""
     from .continuation import mach_schedule
     from .timing import sum_timings
@@ -363,15 +368,20 @@ def solve_multivalued_transonic(mvop, mesh, m_target: float,
     schedule = mach_schedule(m_target, m_start=m_start, dm=dm)
     phi_ext, gamma, levels, res = None, 0.0, [], None
     t0 = time.perf_counter()
-    for k, m in enumerate(schedule):
+    lagged = farfield_lag == "level" and kwargs.get("farfield", "vortex") == "vortex"
+    runs = [(m, n_outer_seed if k == 0 else n_outer_level, lagged and k > 0)
+            for k, m in enumerate(schedule)]
+    if lagged:
+        runs += [(schedule[-1], n_outer_polish, True)] * int(n_ff_polish)
+    for m, budget, fixed in runs:
         tl = time.perf_counter()
         res = solve_multivalued_lifting(
             mvop, mesh, m, alpha_deg=alpha_deg, u_inf=u_inf, gamma_air=gamma_air,
             upwind_c=upwind_c, m_crit=m_crit, damping_theta=damping_theta,
-            omega_rho=omega_rho, n_outer_max=n_outer_seed if k == 0 else n_outer_level,
+            omega_rho=omega_rho, n_outer_max=budget,
             tol_rho=tol_rho, tol_gamma=tol_gamma, tol_residual=tol_residual,
             gamma_init=gamma, phi_init=phi_ext,
-            gamma_farfield_fixed=gamma if (farfield_lag == "level" and k > 0) else None,
+            gamma_farfield_fixed=gamma if fixed else None,
             **kwargs)
         phi_ext, gamma = res["phi_ext"], res["gamma"]
         levels.append({"m_inf": float(m), "gamma": float(gamma),
