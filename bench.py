@@ -49,26 +49,37 @@ def constructed_phi(nodes):
     return 1.25 * x + 0.35 * np.sin(1.7 * x)
 
 
-def algorithmic_bytes(n_tets, n_nodes, nnz):
-    """Compulsory HBM bytes of each phase (each input read once, each
-    output written once; float64 state, int32 connectivity)."""
+def algorithmic_bytes(n_tets, n_nodes, nnz, n_active=0):
+    """Compulsory HBM bytes per pass (each input read once, each output
+    written once; float64 state, int32 connectivity, the precomputed B_e
+    layout).  The Jacobian, the residual evaluation and Term 3 use the
+    survey's per-unit model of the reference layout:
+      residual_evaluation  B_R = 180 N_t + 40 N_n (state sweep, walk,
+                           rho_tilde, residual scatter, end to end)
+      jacobian             B_J = 204 N_t + 8 N_n + 8 nnz (Terms 1+2 into the
+                           fixed CSR: elements 16, B 96, V 8, rho_tilde 8,
+                           s_e 8, elem_to_csr 64, color_elems 4 per tet;
+                           phi 8 per node; values 8 per CSR entry)
+      term3                448 per active element (16 entries x (8 value +
+                           4 row + 4 col) + 2 x (grad 24 + s_u 8 +
+                           elem_to_csr 64)) plus the flag scan (s_u 8,
+                           upstream 4, lim 1, flags 4 + 16 incidence per
+                           tet, 12 per node).
+    The three pieces of the residual evaluation are counted per kernel
+    (inputs and outputs of each sweep) for the per-phase rates."""
     return {
         # elements 16, B 96, write grad 24 + q2n 8 + q2l 8 + rho 8 + lim 1
         # + grad.B_a 32; phi 8/node
         "state": 193 * n_tets + 8 * n_nodes,
         # fn 16, own centroid 24, elements 16, grad 24, q2 8, rho 8; write
-        # upstream 4, nu 8, rho_tilde 8, s_e 8, s_u 8 (+ q2/rho of u(e),
-        # non-local); nodes 24/node
+        # upstream 4, nu 8, rho_tilde 8, s_e 8, s_u 8; nodes 24/node
         "upwind": 132 * n_tets + 24 * n_nodes,
         # node gather: incidence 16, grad.B_a 32, V 8, rho_tilde 8;
         # inc_ptr 4 + R 8 per node
         "residual": 64 * n_tets + 12 * n_nodes,
-        # node gather: incidence 16, elem_to_csr 64, B 96, V 8, rho_tilde 8,
-        # grad 24, s_e 8, lim 1; indptr + inc_ptr 8 per node; CSR values
-        # written once 8/nnz
-        "jacobian": 225 * n_tets + 8 * n_nodes + 8 * nnz,
-        # s_u 8, upstream 4, lim 1, flags 4 + incidence scan 16/tet + rows
-        "term3": 33 * n_tets + 12 * n_nodes,
+        "residual_evaluation": 180 * n_tets + 40 * n_nodes,
+        "jacobian": 204 * n_tets + 8 * n_nodes + 8 * nnz,
+        "term3": 33 * n_tets + 12 * n_nodes + 448 * n_active,
     }
 
 
@@ -89,9 +100,12 @@ def main():
                          "(about 3 minutes)")
     ap.add_argument("--no-plate", action="store_true",
                     help="skip the embedded-plate lifting case (128x96x96 box)")
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also run the adjoint case: 96x32x32 channel (589,824 tets) at "
-                         "M 0.80, gradients checked by central differences (about 2 minutes)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint case: 96x32x32 channel (589,824 tets) at M 0.80, "
+                         "gradients checked by central differences (about 40 s)")
+    ap.add_argument("--adjoint-perturbations", type=int, default=4,
+                    help="floor perturbations the adjoint is checked along (16 in the "
+                         "full study)")
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scale)")
@@ -214,16 +228,22 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         per[name] = e0.elapsed_time(e1) / reps       # ms
-    ab = algorithmic_bytes(nt, nn, st.nnz)
+    n_active = int(jac.n_active)
+    ab = algorithmic_bytes(nt, nn, st.nnz, n_active)
     phase_rep = {k: {"ms": round(per[k], 4),
                      "GBs": round(ab[k] / (per[k] * 1e-3) / 1e9, 1)}
                  for k in per}
+    phase_rep["term3"]["n_active"] = n_active
     log("[bench] phases " + json.dumps(phase_rep))
     # the two other rates the reference quotes: a residual evaluation (state
     # sweep, walk + rho_tilde + sensitivities, residual) and the Picard
     # matrix alone (A = sum rho_tilde V B_i.B_j into the fixed CSR; the
     # reference's anchor is its coloured Picard assembly)
     res_ms = per["state"] + per["upwind"] + per["residual"]
+    phase_rep["residual_evaluation"] = {
+        "ms": round(res_ms, 4),
+        "GBs": round(ab["residual_evaluation"] / (res_ms * 1e-3) / 1e9, 1),
+        "frac": round(ab["residual_evaluation"] / (res_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     pic_out = torch.empty(st.nnz, dtype=torch.float64, device=dev)
     op.matrix_dev(upw.d_rt, pic_out)
     e0 = torch.cuda.Event(enable_timing=True)
@@ -242,7 +262,11 @@ def main():
     log("[bench] rates " + json.dumps(rates))
     dom = max((k for k in per if k != "term3"), key=lambda k: per[k])
     achieved = ab[dom] / (per[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+    kernels = {"jacobian": "k_elem_matrix_terms + k_gather_csr_staged",
+               "upwind": "k_upwind_sens_dw", "state": "k_element_state",
+               "residual": "k_elem_residual_terms + k_gather_terms_staged"}
+    roofline = {"bound": "hbm", "kernel": f"{dom} ({kernels[dom]})",
+                "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": int(ab[dom])}
@@ -276,8 +300,8 @@ def main():
         log(f"[bench] config1 {json.dumps(config1)}")
 
     adjoint = None
-    if args.adjoint:
-        adjoint = run_adjoint()
+    if not args.no_adjoint:
+        adjoint = run_adjoint(n_pert=args.adjoint_perturbations, n_free=1)
         log(f"[bench] adjoint {json.dumps(adjoint)}")
 
     sweep = None
@@ -305,12 +329,30 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def cpu_threads_plan():
+    """Thread counts of the CPU baseline: one thread, all the cores this
+    process may use (its affinity set, capped by OMP_NUM_THREADS where that
+    is set: the share of the host the GPU job is given), and 16 threads (the
+    reference's anchors were taken at <= 16 threads), without repeats."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    share = min(affinity, int(env)) if env and env.isdigit() and int(env) > 0 else affinity
+    plan = sorted({1, share, min(16, affinity)})
+    return plan, share, affinity
+
+
 def cpu_baseline(st, mesh, passes):
     """The oracle's C restatement (OpenMP, colour-serial like numba prange)
-    timed on the same linearisation pass over the same mesh."""
+    timed on the same linearisation pass over the same mesh, at one thread,
+    at the cores available to this process and at 16 threads."""
+    import ctypes
+
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    plan, share, affinity = cpu_threads_plan()
     el = st.elements_host
     B, V = st.B_host, st.V_host
     off, ce = st.color_offsets, st.color_elems
@@ -346,16 +388,10 @@ def cpu_baseline(st, mesh, passes):
         op.jacobian_terms12(g, rt, se, 1.0)
         op.term3_coo(g, up2, su, 1.0)
 
-    one()                                   # first touch / warm caches
-    t0 = time.perf_counter()
-    for _ in range(passes):
-        one()
-    dt = (time.perf_counter() - t0) / passes
     # the Picard matrix alone (zero fill + coloured assembly into the fixed
     # CSR values), the rate the reference's anchor quotes
     rt1 = np.linspace(0.8, 1.2, st.n_tets)
     data = np.zeros(len(st.indices_host))
-    import ctypes
 
     def picard():
         data.fill(0.0)
@@ -364,17 +400,36 @@ def cpu_baseline(st, mesh, passes):
             O._p(e2c, O._i32), O._p(B, O._d), O._p(V, O._d), O._p(rt1, O._d),
             O._p(data, O._d))
 
+    prev = O.set_threads(share)
+    one()                                   # first touch / warm caches
     picard()
-    t0 = time.perf_counter()
-    for _ in range(passes):
-        picard()
-    dtp = (time.perf_counter() - t0) / passes
-    return {"value": round(st.n_tets / dt / 1e6, 3), "unit": "Mtet/s",
-            "cores": threads, "kind": "port",
-            "sample": f"{passes} full linearisation passes over the same "
-                      f"{st.n_tets}-tet mesh (oracle C, OpenMP {threads} threads)",
-            "s_per_pass": round(dt, 3),
-            "picard_matrix_mtet_s": round(st.n_tets / dtp / 1e6, 3)}
+    runs = []
+    for t in plan:
+        O.set_threads(t)
+        n_pass = 1 if t == 1 else passes
+        t0 = time.perf_counter()
+        for _ in range(n_pass):
+            one()
+        dt = (time.perf_counter() - t0) / n_pass
+        t0 = time.perf_counter()
+        for _ in range(n_pass):
+            picard()
+        dtp = (time.perf_counter() - t0) / n_pass
+        runs.append({"threads": t, "mtet_s": round(st.n_tets / dt / 1e6, 3),
+                     "s_per_pass": round(dt, 3),
+                     "picard_matrix_mtet_s": round(st.n_tets / dtp / 1e6, 3)})
+        log(f"[bench] cpu baseline {runs[-1]}")
+    O.set_threads(prev)
+    main = next(r for r in runs if r["threads"] == share)
+    return {"value": main["mtet_s"], "unit": "Mtet/s", "cores": share, "kind": "port",
+            "sample": f"full linearisation passes ({passes} per thread count, 1 at one "
+                      f"thread) over the same {st.n_tets}-tet mesh (oracle C, OpenMP); "
+                      f"value at the {share} cores available to this process",
+            "s_per_pass": main["s_per_pass"],
+            "picard_matrix_mtet_s": main["picard_matrix_mtet_s"],
+            "os_cpu_count": os.cpu_count(), "affinity_cores": affinity,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "runs": runs}
 
 
 def run_solves(mesh, config, tune=True):
@@ -529,6 +584,10 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
     solve_s = time.perf_counter() - t0
     if not sol["converged"]:
         return {"error": "no converged state at M 0.80"}
+    # the accepted state settled on the root of the selection the gradients
+    # are taken at (an accepted assignment cycle leaves it off that root)
+    settle = {"frozen_residual_at_accept": sol["frozen_residual_at_accept"],
+              "settling_newton_steps": sol["frozen_polish_steps"]}
     t0 = time.perf_counter()
     g = prob.gradients(sol, weights=(1.0, 0.0))
     torch.cuda.synchronize()
@@ -572,7 +631,7 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
             "fd_alpha_frozen": fd_a,
             "rel_adjoint_vs_tangent": max(abs(g["d_m_inf"] - tan_m) / abs(tan_m),
                                           abs(g["d_alpha_deg"] - tan_a) / abs(tan_a)),
-            "fd_step": h, "n_perturbations": n_pert,
+            "fd_step": h, "n_perturbations": n_pert, "settle": settle,
             "rel_err_frozen_max": max(rel_frozen),
             "rel_err_frozen_median": float(np.median(rel_frozen)),
             "rel_err_free": rel_free, "fd_seconds": round(fd_s, 1)}

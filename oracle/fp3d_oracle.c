@@ -14,9 +14,15 @@
  * int32; elem_to_csr int32 (n,4,4); face_neighbors int32 (n,4).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* OpenMP team size of the parallel loops below (the timed CPU baseline runs
+ * at several thread counts in one process). */
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
 
 /* ---- pyfp3d/physics/isentropic.py ------------------------------------- */
 /* :29-58 density_isentropic */

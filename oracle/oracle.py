@@ -65,6 +65,14 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+def set_threads(n: int) -> int:
+    """OpenMP threads of the C restatement's parallel loops; returns the
+    previous count."""
+    prev = int(_lib().or_get_threads())
+    _lib().or_set_threads(C.c_int(int(n)))
+    return prev
+
+
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 

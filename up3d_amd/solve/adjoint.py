@@ -133,12 +133,39 @@ class ChannelForceProblem:
         r["nodes"] = nodes
         r["m_inf"] = float(m_inf)
         r["alpha_deg"] = float(alpha_deg)
+        if r["converged"]:
+            self._settle(r, tol_residual)
         return r
+
+    def _settle(self, r, tol_residual: float) -> None:
+        """Put the accepted state on the root of the selection it is
+        differentiated at.  An accepted state solves the system of the
+        selection the Newton froze; where the live selection at that state
+        differs (an accepted assignment cycle), the state is not a root of
+        the system frozen at its own selection, and derivatives of that
+        system taken at it are off by the residual's image.  So the
+        selection is frozen at the accepted state and the state re-converged
+        on it (full Newton steps, usually none or one); `frozen` keeps that
+        selection for the gradients."""
+        frozen = self.frozen_selection(r)
+        pol = self.solve_frozen(r["nodes"][self.floor_nodes, 1], r["m_inf"],
+                                r["alpha_deg"], frozen, r["phi"],
+                                tol_residual=tol_residual)
+        r["frozen_residual_at_accept"] = float(pol["residual_history"][0])
+        r["frozen_polish_steps"] = len(pol["residual_history"]) - 1
+        if not pol["converged"]:
+            r["converged"] = False
+            return
+        r["phi"] = pol["phi"]
+        r["frozen"] = frozen
 
     def frozen_selection(self, sol, upwind_c: float = 1.5, m_crit: float = 0.95,
                          rho_floor: float = 0.05):
-        """(upstream, branch) of the converged state `sol`: the selection the
-        gradients are taken at (device int32 / int8 tensors)."""
+        """(upstream, branch) the gradients of `sol` are taken at (device
+        int32 / int8 tensors): the selection `sol` was settled on, else the
+        one frozen at its state."""
+        if sol.get("frozen") is not None:
+            return sol["frozen"][0].clone(), sol["frozen"][1].clone()
         L = self._linearisation(sol, upwind_c, m_crit, rho_floor)
         return L["up"].clone(), L["br"].clone()
 
@@ -209,7 +236,10 @@ class ChannelForceProblem:
 
         op.state_dev(phi, self.u_inf ** 2, m, 1.4, q2_at_mach(3.0, m), grad, q2n,
                      q2l, rho, lim)
-        up, br = upw.freeze_dev(grad, q2l, rho, m, upwind_c, m_crit, 1.4, rho_floor)
+        if sol.get("frozen") is not None:
+            up, br = sol["frozen"]
+        else:
+            up, br = upw.freeze_dev(grad, q2l, rho, m, upwind_c, m_crit, 1.4, rho_floor)
         rt = upw.frozen_dev(q2l, rho, up, br, m, upwind_c, m_crit, 1.4, rho_floor,
                             monitors=False).clone()
         se, su = upw.frozen_sensitivities_dev(q2l, rho, up, br, m, upwind_c,
