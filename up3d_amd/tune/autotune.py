@@ -162,6 +162,12 @@ def tune_linear_solver(matvec: Callable, rhs: torch.Tensor, A_pre,
             "trials": trials}
 
 
+# Version of the cache entries' layout; entries of another version, or
+# missing a tuned dimension, are misses and are tuned again.
+CACHE_VERSION = 2
+TUNED_KEYS = ("element_block", "gather_chunk", "walk_width", "solver")
+
+
 class TuningCache:
     """JSON cache of tuning results keyed by mesh shape and device."""
 
@@ -181,7 +187,13 @@ class TuningCache:
             return {}
 
     def get(self, key: str) -> Optional[dict]:
-        return self.load().get(key)
+        """The entry for `key` if it is complete and of this layout version."""
+        e = self.load().get(key)
+        if not isinstance(e, dict) or e.get("version") != CACHE_VERSION:
+            return None
+        if any(k not in e for k in TUNED_KEYS):
+            return None
+        return e
 
     def put(self, key: str, value: dict) -> None:
         data = self.load()
@@ -222,12 +234,10 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
     key = TuningCache.key(st.n_tets, st.n_nodes, st.nnz)
     cache = cache or TuningCache()
     hit = cache.get(key)
-    if hit is not None and "solver" in hit and "element_block" in hit:
+    if hit is not None:
         apply_element_block(hit["element_block"]["best"])
-        if "gather_chunk" in hit:
-            apply_gather_chunk(hit["gather_chunk"]["best"])
-        if "walk_width" in hit:
-            apply_walk_width(hit["walk_width"]["best"])
+        apply_gather_chunk(hit["gather_chunk"]["best"])
+        apply_walk_width(hit["walk_width"]["best"])
         return dict(hit, cached=True)
     ws.set_mach(m_inf)
     s = ws.states[2]
@@ -257,8 +267,8 @@ def tune_newton_workspace(ws, phi, m_inf: float, upwind_c: float = 1.5,
     prec = "f32" if ws.precision == "fp32" else "f64"
     sol = tune_linear_solver(ws.jac.matvec, rhs, A_pre, precision=prec, rtol=rtol,
                              restarts=restarts, degrees=degrees, max_coarse=max_coarse)
-    entry = {"element_block": eb, "gather_chunk": gc, "walk_width": ww, "solver": sol,
-             "m_inf": float(m_inf),
+    entry = {"version": CACHE_VERSION, "element_block": eb, "gather_chunk": gc,
+             "walk_width": ww, "solver": sol, "m_inf": float(m_inf),
              "precision": ws.precision}
     cache.put(key, entry)
     return dict(entry, cached=False)
