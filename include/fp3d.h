@@ -463,6 +463,54 @@ int fp3d_adjoint_dx(int64_t n_tets, int64_t n_nodes, const int32_t *elements,
                     double *out, void *stream);
 int64_t fp3d_adjoint_dx_work_doubles(int64_t n_tets);
 
+/* ------------------------------------------------ level-set wake sheet
+ * Replaces the per-assembly host COO -> CSR of the sheet terms: the
+ * special-copy mass conservation (pyfp3d/kernels/cut_assembly.py:119-227
+ * mass_conservation_coo, the two copies of each cut element and the fan's
+ * lower copy, pyfp3d/wake/multivalued.py:276-379), the wake least-squares
+ * rows (cut_assembly.py:230-318 wake_ls_coo), the TE pressure Kutta rows
+ * and their Jacobian (cut_assembly.py:321-398, 478-515) and the Newton
+ * Term 2 of the special copies (cut_assembly.py:401-475).  Their extended
+ * CSR pattern is fixed by the cut census; slot s sums its contributions
+ * c in [seg[s], seg[s+1]) in that order, contribution c being
+ *   kind 0: coef*wa[e]   1: coef*wb[e]   2: coef   3: coef*kv[e]
+ *   4: coef*sa[e]*(ga[e].B_a)*(ga[e].B_b)   5: the same with sb, gb
+ * with e = src[c], a = ab[c] >> 2, b = ab[c] & 3, B (n_tets,4,3), ga / gb
+ * (n_tets,3); the slot value is 0 where fmask (nullable) is 0 at its row
+ * srow[s] or column scol[s].  Pointers a kind does not use may be NULL. */
+int fp3d_sheet_fill(int64_t n_slots, const int32_t *seg, const uint8_t *kind,
+                    const int32_t *src, const uint8_t *ab, const double *coef,
+                    const double *wa, const double *wb, const double *kv,
+                    const double *sa, const double *sb, const double *ga,
+                    const double *gb, const double *B, const int32_t *srow,
+                    const int32_t *scol, const uint8_t *fmask, double *out,
+                    void *stream);
+/* Term 3 of the special copies and of plain elements whose upstream element
+ * is special (cut_assembly.py:401-475, the upstream part): per candidate
+ * element cand[c] on side cside[c] (0 upper, 1 lower; per-side grad g,
+ * upstream up, s_u su, limiter lim, DOF table dof and read table rd, all
+ * (n_tets[,4|3])), rows[4c+a] = row_map[dof[e][a]] (-1 when s_u lim[u] = 0,
+ * the row is dropped or fmask (nullable) is 0 there), W[4c+a] =
+ * 2 inv_u2 s_u lim[u] V_e (g_e.B_a), gu[4c+b] = g_u.B_u,b (0 where fmask is 0
+ * at the column), cols[4c+b] = rd[u][b]. */
+int fp3d_sheet_t3_prepare(int64_t n_cand, const int32_t *cand, const uint8_t *cside,
+                          const double *g0, const double *g1, const int32_t *up0,
+                          const int32_t *up1, const double *su0, const double *su1,
+                          const uint8_t *lim0, const uint8_t *lim1,
+                          const int32_t *dof0, const int32_t *dof1,
+                          const int32_t *rd0, const int32_t *rd1,
+                          const int32_t *row_map, const uint8_t *fmask,
+                          const double *B, const double *V, double inv_u2,
+                          int32_t *rows, double *W, double *gu, int32_t *cols,
+                          void *stream);
+/* y += T x for that Term 3: t[c] = sum_b gu[4c+b] x[cols[4c+b]], then per
+ * row r the entries rent[rptr[r]..rptr[r+1]) (entry q = 4c+a, in a fixed
+ * order) add W[q] t[q >> 2].  t is n_cand doubles of scratch; x != y. */
+int fp3d_sheet_t3_apply(int64_t n_cand, int64_t n_rows, const double *gu,
+                        const int32_t *cols, const double *W, const int32_t *rptr,
+                        const int32_t *rent, const double *x, double *t, double *y,
+                        void *stream);
+
 /* --------------------------------------------------- host-side set-up
  * (plain host pointers; run once per mesh, as in the reference) */
 /* pyfp3d/mesh/coloring.py:65-95 _greedy_coloring_kernel + :129-145

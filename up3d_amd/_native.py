@@ -139,6 +139,9 @@ _SIGS = {
     "fp3d_adjoint_dx": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_adjoint_dx_work_doubles": (_i64, [_i64]),
+    "fp3d_sheet_fill": (_int, [_i64] + [_vp] * 17 + [_vp]),
+    "fp3d_sheet_t3_prepare": (_int, [_i64] + [_vp] * 18 + [_dbl] + [_vp] * 4 + [_vp]),
+    "fp3d_sheet_t3_apply": (_int, [_i64, _i64] + [_vp] * 8 + [_vp]),
     "fp3d_host_greedy_coloring": (_int, [_i64, _vp, _i64, _vp, _pi32]),
     "fp3d_host_csr_pattern": (_int, [_i64, _vp, _i64, _vp, _vp, _pi64]),
     "fp3d_host_elem_to_csr": (_int, [_i64, _vp, _vp, _vp, _vp]),

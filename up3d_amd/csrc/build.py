@@ -16,7 +16,7 @@ PKG = HERE.parent
 LIB_DIR = PKG / "lib"
 LIB_NAME = "libfp3d_hip.so"
 
-HIP_SOURCES = ["fp3d_elem.hip", "fp3d_linalg.hip", "fp3d_adjoint.hip"]
+HIP_SOURCES = ["fp3d_elem.hip", "fp3d_linalg.hip", "fp3d_adjoint.hip", "fp3d_sheet.hip"]
 HOST_SOURCES = ["fp3d_host.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so every element's arithmetic is the

@@ -8,19 +8,23 @@ Residual at a given per-side upwind selection (live or frozen):
 
 with E the multivalued Picard matrix (mass conservation per side copy,
 wake least-squares rows, frozen-mean Kutta rows) -- its plain-element part
-assembled on the device, the rest the host correction of
-MultivaluedOperator.correction.  Jacobian:
+and its sheet part (MultivaluedOperator.sheet_fill) both assembled on the
+device.  Jacobian:
 
     J = D_keep E  +  exact Kutta Jacobian  +  Terms 2+3 per side
 
 (D_keep drops the TE aux rows; Term-2/3 rows landing on non-TE aux rows are
 dropped -- those rows are exactly linear -- and TE aux rows re-routed to the
-TE main row, as the mass conservation is).  Device part: Terms 1+2 of the
-plain elements (node-gather CSR) and Term 3 of plain elements whose
-upstream element is plain (the Term-3 gather operator); host correction:
-everything that touches a cut or fan element or an aux row.  The linear
-step is device GMRES with the block preconditioner of the Picard path
-(AMG of the single-valued Picard matrix + exact aux-block solve).
+TE main row, as the mass conservation is).  All of it is assembled on the
+device: Terms 1+2 of the plain elements (node-gather CSR) and Term 3 of
+plain elements whose upstream element is plain (the Term-3 gather
+operator); the sheet part (D_keep C, the exact Kutta Jacobian and Term 2
+of the special copies) filled into its fixed pattern; Term 3 of the
+special copies and of plain elements with a special upstream element as
+the SpecialTerm3 operator, rebuilt per linearisation because its columns
+follow the upwind selection.  The linear step is device GMRES with the
+block preconditioner of the Picard path (AMG of the single-valued Picard
+matrix + exact aux-block solve, the aux block's sparse LU on the host).
 """
 
 from __future__ import annotations
@@ -63,6 +67,9 @@ class LSNewtonSystem:
         self.nop = None
         self._amg = None
         self._amg_age = 0
+        self._jsheet = None
+        self._t3s = None
+        self._jc_host = None
 
     # ------------------------------------------------------------ residual
     def _sides(self, phi, frozen, sens):
@@ -81,21 +88,24 @@ class LSNewtonSystem:
         return np.einsum("ij,ij->i", qu, qu) - np.einsum("ij,ij->i", ql, ql)
 
     def residual(self, phi, frozen=None, sens=False):
-        """(R host (n_total,), side data, host Picard correction C)."""
+        """(R host (n_total,), side data, the Picard sheet terms C (device
+        SheetMatrix), plain-part weights w (device))."""
         mv = self.mv
         op, st = mv.op, mv.op.store
+        nm = self.nm
         sides = self._sides(phi, frozen, sens)
         w, special = mv.side_weights(sides[0]["rho_tilde"], sides[1]["rho_tilde"])
         op.matrix_dev(mv.plain_weight_dev(w), self.vals)
-        C = mv.correction(special, "wake_ls", "pressure", phi)
-        y = empty(self.nm)
-        x = to_dev(phi[: self.nm])
-        _native.call("fp3d_csr_spmv", self.nm, op.nnz, ptr(st.indptr),
-                     ptr(st.indices), ptr(self.vals), ptr(x), ptr(y),
+        C = mv.sheet_fill(special, "pressure", phi, user="newton")
+        y = empty(self.nT)
+        x = to_dev(np.asarray(phi, dtype=np.float64))
+        _native.call("fp3d_csr_spmv", nm, op.nnz, ptr(st.indptr),
+                     ptr(st.indices), ptr(self.vals), ptr(x[:nm]), ptr(y[:nm]),
                      stream_handle())
-        R = np.zeros(self.nT)
-        R[: self.nm] = to_host(y)
-        R += C @ phi
+        if self.nT > nm:
+            y[nm:].zero_()
+        C.add_matvec(x, y)
+        R = to_host(y)
         if self.external_rhs is not None:
             R -= self.external_rhs
         R[self.te_aux] = self.te_residual(phi)
@@ -128,7 +138,8 @@ class LSNewtonSystem:
     def _terms23_host(self, sides):
         """Terms 2+3 of the special copies and Term 3 of plain elements whose
         upstream element is special (pyfp3d/kernels/cut_assembly.py:401-475),
-        rows re-mapped."""
+        rows re-mapped, as a host COO: the restatement the device sheet and
+        SpecialTerm3 are checked against (the solver uses those)."""
         mv = self.mv
         cm = mv.cm
         inv_u2 = 1.0 / (self.u_inf ** 2)
@@ -176,14 +187,58 @@ class LSNewtonSystem:
         keep = rr >= 0
         return rr[keep], c[keep], d[keep]
 
+    def _jacobian_sheet(self):
+        """Device pattern of the sheet part of J: D_keep C (the Picard sheet
+        terms without the TE aux rows), the exact Kutta Jacobian on the TE
+        aux rows (values per call) and Term 2 of the special copies (rows
+        re-mapped, dropped rows removed)."""
+        if self._jsheet is None:
+            from ..wake.sheet import HOSTVAL, TERM2_LOWER, TERM2_UPPER, SheetMatrix
+
+            mv = self.mv
+            cm = mv.cm
+            r, c, k, s, ab, coef = mv.sheet_contributions("pressure")
+            is_te_aux = np.zeros(self.nT, dtype=bool)
+            is_te_aux[self.te_aux] = True
+            keep = ~is_te_aux[r]
+            parts = [tuple(a[keep] for a in (r, c, k, s, ab, coef))]
+            kr, kc, _ = self._kutta_jacobian(np.zeros(self.nT))
+            parts.append((kr, kc, np.full(len(kr), HOSTVAL, dtype=np.uint8),
+                          np.arange(len(kr)), np.zeros(len(kr), dtype=np.int64),
+                          np.ones(len(kr))))
+            inv_u2 = 1.0 / (self.u_inf ** 2)
+            for kind, ids, dofs in (
+                    (TERM2_UPPER, cm.cut_elems, cm.dofs_upper),
+                    (TERM2_LOWER, np.concatenate([cm.cut_elems, cm.te_lower_elems]),
+                     np.concatenate([cm.dofs_lower, mv.tel_dof]))):
+                ids = np.asarray(ids, dtype=np.int64)
+                if not len(ids):
+                    continue
+                dofs = np.asarray(dofs, dtype=np.int64).reshape(-1, 4)
+                rows = self.row_map[np.repeat(dofs, 4, axis=1).reshape(-1)]
+                ok = rows >= 0
+                parts.append((rows[ok], np.tile(dofs, (1, 4)).reshape(-1)[ok],
+                              np.full(int(ok.sum()), kind, dtype=np.uint8),
+                              np.repeat(ids, 16)[ok], np.tile(np.arange(16), len(ids))[ok],
+                              np.repeat(2.0 * inv_u2 * mv.V[ids], 16)[ok]))
+            self._jsheet = SheetMatrix(self.nT, *(np.concatenate([p[i] for p in parts])
+                                                  for i in range(6)))
+        return self._jsheet
+
     def linearize(self, phi, sides, C, w, dir_dofs):
-        """Device Jacobian operator at phi (Dirichlet rows as identity)."""
+        """Device Jacobian operator at phi (Dirichlet rows as identity): the
+        plain part (node-gather Terms 1+2, plain Term 3), the sheet part in
+        its fixed pattern and the special Term 3, all filled on the device;
+        the aux block is copied to the host for its sparse LU."""
+        from ..wake.sheet import SpecialTerm3
+
         mv = self.mv
         op, st = mv.op, mv.op.store
         nm, nT = self.nm, self.nT
         fm = np.ones(nT, dtype=np.uint8)
         fm[dir_dofs] = 0
         fm_main = to_dev(fm[:nm], torch.uint8)
+        fm_dev = to_dev(fm, torch.uint8)
         if self.nop is None:
             self.nop = NewtonOperator(st, fm_main)
         else:
@@ -203,23 +258,17 @@ class LSNewtonSystem:
         su = su * self.is_plain_dev[up.long()].double()
         self.nop.build_term3(grad, up, su, lim, inv_u2)
         self.t3_inputs = (grad, up, su * lim[up.long()].double(), inv_u2)
-        # host correction: D_keep C + exact Kutta + Terms 2/3 (special)
-        keep = np.ones(nT)
-        keep[self.te_aux] = 0.0
-        Ck = (sp.diags(keep) @ C).tocoo()
-        kr, kc, kd = self._kutta_jacobian(phi)
-        tr, tc, td = self._terms23_host(sides)
-        Jc = sp.coo_matrix((np.concatenate([Ck.data, kd, td]),
-                            (np.concatenate([Ck.row, kr, tr]),
-                             np.concatenate([Ck.col, kc, tc]))),
-                           shape=(nT, nT)).tocsr()
-        Dm = sp.diags(fm.astype(np.float64))
-        Jc = (Dm @ Jc @ Dm).tocsr()
-        Jc.eliminate_zeros()
-        self.Jc_host = Jc
-        self.Jc = DeviceCSR(Jc)
-        self.Jc_am = DeviceCSR(Jc[nm:, :nm].tocsr()) if nT > nm else None
-        self.aux_lu = spla.splu(Jc[nm:, nm:].tocsc()) if nT > nm else None
+        # sheet part: D_keep C + exact Kutta + Term 2 of the special copies
+        J = self._jacobian_sheet()
+        J.fill(wa=up_s["rho_tilde"], wb=lo_s["rho_tilde"],
+               kv=self._kutta_jacobian(phi)[2],
+               sa=up_s["s_e"] * up_s["lim"].double(), sb=lo_s["s_e"] * lo_s["lim"].double(),
+               ga=up_s["grad"], gb=lo_s["grad"], B=st.B, fmask=fm_dev)
+        if self._t3s is None:
+            self._t3s = SpecialTerm3(mv, self.row_map)
+        self._t3s.build(sides, inv_u2, fmask=fm_dev)
+        self._jc_host = None
+        self.aux_lu = spla.splu(J.block_csc(nm, nm)) if nT > nm else None
         self._t_aux = empty(max(nT - nm, 1))
         self._schur = None
         # AMG of the single-valued Picard matrix at the current densities
@@ -232,15 +281,24 @@ class LSNewtonSystem:
         self._amg_age += 1
         self.free = fm.astype(bool)
 
+    @property
+    def Jc_host(self) -> sp.csr_matrix:
+        """The sheet part of J plus the special Term 3 as one host CSR (for
+        the Schur mode and the transposed operator of the adjoint)."""
+        if self._jc_host is None:
+            tr, tc, tv = self._t3s.to_coo()
+            J = self._jsheet.to_scipy()
+            T = sp.coo_matrix((tv, (tr, tc)), shape=(self.nT, self.nT)).tocsr()
+            self._jc_host = (J + T).tocsr()
+        return self._jc_host
+
     def matvec(self, x, y):
         nm = self.nm
         self.nop.matvec(x[:nm], y[:nm])
         if self.nT > nm:
             y[nm:].zero_()
-        J = self.Jc
-        _native.call("fp3d_amg_prolong_add", J.n, J.nnz, ptr(J.indptr),
-                     ptr(J.indices), ptr(J.data), 0, ptr(x), ptr(y),
-                     stream_handle())
+        self._jsheet.add_matvec(x, y)
+        self._t3s.add_matvec(x, y)
         return y
 
     # ---------------------------------------------------- linear step
@@ -249,20 +307,20 @@ class LSNewtonSystem:
             nm = self.nm
             Jc = self.Jc_host
             self._schur = (DeviceCSR(Jc[:nm, :nm].tocsr()), DeviceCSR(Jc[:nm, nm:].tocsr()),
-                           empty(nm))
+                           empty(nm), DeviceCSR(Jc[nm:, :nm].tocsr()))
         return self._schur
 
     def schur_matvec(self, x, y):
         """K x = J_mm x - J_ma J_aa^{-1} J_am x on main-length vectors
         (pyfp3d/solve/schur_ls.py:95-103)."""
         nm = self.nm
-        cmm, cma, t = self._schur_parts()
+        cmm, cma, t, cam = self._schur_parts()
         self.nop.matvec(x, y)
         _native.call("fp3d_amg_prolong_add", cmm.n, cmm.nnz, ptr(cmm.indptr),
                      ptr(cmm.indices), ptr(cmm.data), 0, ptr(x), ptr(y),
                      stream_handle())
         if self.nT > nm:
-            C = self.Jc_am
+            C = cam
             _native.call("fp3d_csr_spmv", C.n, C.nnz, ptr(C.indptr), ptr(C.indices),
                          ptr(C.data), ptr(x), ptr(self._t_aux), stream_handle())
             v = to_dev(self.aux_lu.solve(to_host(self._t_aux)))
@@ -285,7 +343,7 @@ class LSNewtonSystem:
             raise ValueError(f"linear mode {mode!r} unknown")
         b_a = rhs[nm:]
         r = rhs[:nm].copy()
-        cmm, cma, t = self._schur_parts()
+        cmm, cma, t, _ = self._schur_parts()
         if self.nT > nm:
             v = to_dev(self.aux_lu.solve(b_a))
             _native.call("fp3d_csr_spmv", cma.n, cma.nnz, ptr(cma.indptr),
@@ -307,10 +365,10 @@ class LSNewtonSystem:
         nm = self.nm
         self._amg.apply(r[:nm].contiguous(), z[:nm])
         if self.nT > nm:
-            C = self.Jc_am
-            _native.call("fp3d_csr_spmv", C.n, C.nnz, ptr(C.indptr),
-                         ptr(C.indices), ptr(C.data), ptr(z[:nm]),
-                         ptr(self._t_aux), stream_handle())
+            # the aux rows of J (only the sheet part has aux rows) applied to
+            # [z_main; 0]: the (aux, main) block
+            z[nm:].zero_()
+            self._jsheet.rows_matvec(nm, z, self._t_aux)
             rhs = to_host(r[nm:]) - to_host(self._t_aux)
             z[nm:].copy_(to_dev(self.aux_lu.solve(rhs)))
         return z

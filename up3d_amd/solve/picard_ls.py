@@ -40,7 +40,10 @@ from .timing import finalize, new_timings, phase
 
 class ExtendedSystem:
     """Device operator, right-hand side and preconditioner of one outer
-    iteration of the extended (main + aux) system."""
+    iteration of the extended (main + aux) system.  Closure "wake_ls" fills
+    the sheet terms on the device into their fixed pattern
+    (MultivaluedOperator.sheet_fill); the "continuity" weld closure keeps
+    the host correction."""
 
     def __init__(self, mvop, rho_main_dev, special_rho, phi_ext, dir_dofs,
                  dir_vals, closure="wake_ls", te_kutta="pressure",
@@ -60,49 +63,83 @@ class ExtendedSystem:
         self.fm_main = to_dev(fm[:nm], torch.uint8)
         xD = np.zeros(nT)
         xD[dir_dofs] = dir_vals
-        # plain part (special elements weighted 0) and host correction
+        # plain part (special elements weighted 0) and the sheet terms
         self.vals = empty(op.nnz)
         op.matrix_dev(mvop.plain_weight_dev(rho_main_dev), self.vals)
-        C = mvop.correction(special_rho, closure, te_kutta, phi_ext)
+        device_sheet = closure == "wake_ls"
+        if device_sheet:
+            S = mvop.sheet_fill(special_rho, te_kutta, phi_ext)
+            C = None
+        else:
+            C = mvop.correction(special_rho, closure, te_kutta, phi_ext)
         d_damp = None
         if damping_theta is not None and damp_rows is not None:
             # theta * diag(E) on the damped free main rows (supersonic-zone
-            # damping, pyfp3d/solve/picard_ls.py:626-640): added to C's
-            # diagonal, with theta diag x_prev on the right-hand side
+            # damping, pyfp3d/solve/picard_ls.py:626-640), with
+            # theta diag x_prev on the right-hand side
             ip, ix = st.indptr_host, st.indices_host
             if not hasattr(mvop, "_diag_pos"):
                 rows_ = np.repeat(np.arange(nm), np.diff(ip))
                 mvop._diag_pos = np.flatnonzero(ix == rows_)
-            dA = to_host(self.vals)[mvop._diag_pos] + C.diagonal()[:nm]
+                mvop._d_diag_pos = to_dev(mvop._diag_pos.astype(np.int32), torch.int32)
+            if device_sheet:
+                if not hasattr(S, "_main_diag"):
+                    ds = S.diagonal_slots(nm)
+                    S._main_diag = (to_dev(np.flatnonzero(ds >= 0), torch.int64),
+                                    to_dev(ds[ds >= 0], torch.int64))
+                cd = torch.zeros(nm, dtype=torch.float64, device=dev)
+                cd[S._main_diag[0]] = S.data[S._main_diag[1]]
+                dA = to_host(self.vals[mvop._d_diag_pos.long()] + cd)
+            else:
+                dA = to_host(self.vals)[mvop._diag_pos] + C.diagonal()[:nm]
             d = np.zeros(nT)
             d[:nm] = damping_theta * dA
             d[~damp_rows] = 0.0
             d[dir_dofs] = 0.0
             np.clip(d, 0.0, None, out=d)
-            C = (C + sp.diags(d)).tocsr()
             d_damp = d
-        # b = -E x_D on free rows, x_D on Dirichlet rows
+        # b = -E x_D on free rows, x_D on Dirichlet rows (D x_D = 0: the
+        # damping vanishes on the Dirichlet rows, x_D off them)
         y = empty(nT)
         xDd = to_dev(xD)
         _native.call("fp3d_csr_spmv", nm, op.nnz, ptr(st.indptr), ptr(st.indices),
                      ptr(self.vals), ptr(xDd[:nm]), ptr(y[:nm]), stream_handle())
         y[nm:].zero_()
-        yh = to_host(y) + C @ xD
-        b = -yh
+        if device_sheet:
+            S.add_matvec(xDd, y)
+            b = -to_host(y)
+        else:
+            b = -(to_host(y) + C @ xD)
         if external_rhs is not None:
             b += external_rhs
         if d_damp is not None:
-            # (E + D) x = b + D x_prev; D vanishes on the Dirichlet rows
+            # (E + D) x = b + D x_prev
             b += d_damp * np.asarray(x_prev)
         b[dir_dofs] = dir_vals
         self.b = to_dev(b)
-        # Dirichlet embedding: identity rows, zero columns
+        # Dirichlet embedding: identity rows, zero columns; the damping on
+        # the plain part's diagonal (free main rows only)
         mask_dirichlet(st, self.fm_main, self.vals)
-        Dm = sp.diags(fm.astype(np.float64))
-        Cm = (Dm @ C @ Dm).tocsr()
-        Cm.eliminate_zeros()
-        self.C = DeviceCSR(Cm)
-        self.C_host = Cm
+        if d_damp is not None:
+            _native.call("fp3d_csr_diag_shift", nm, ptr(mvop._d_diag_pos),
+                         ptr(to_dev(d_damp[:nm])), 1.0, ptr(self.fm_main),
+                         ptr(self.vals), stream_handle())
+        if device_sheet:
+            S.fill(*mvop.special_weights_dev(special_rho),
+                   kv=mvop.kutta_values(phi_ext) if te_kutta == "pressure" else None,
+                   fmask=to_dev(fm, torch.uint8))
+            self.S = S
+            self.C = None
+            Caa = S.block_csc(nm, nm) if nT > nm else None
+        else:
+            Dm = sp.diags(fm.astype(np.float64))
+            Cm = (Dm @ C @ Dm).tocsr()
+            Cm.eliminate_zeros()
+            self.S = None
+            self.C = DeviceCSR(Cm)
+            self.C_host = Cm
+            self.C_am = DeviceCSR(Cm[nm:, :nm].tocsr()) if nT > nm else None
+            Caa = Cm[nm:, nm:].tocsc() if nT > nm else None
         # preconditioner: AMG of the single-valued Picard matrix (all
         # elements, main DOFs) + exact aux-block solve
         if amg is None:
@@ -113,9 +150,7 @@ class ExtendedSystem:
             mask_dirichlet(st, self.fm_main, A_sv)
             amg = amg_from_device(st, A_sv, "f64")
         self.amg = amg
-        Caa = Cm[nm:, nm:].tocsc()
-        self.aux_lu = spla.splu(Caa) if self.nT > nm else None
-        self.C_am = DeviceCSR(Cm[nm:, :nm].tocsr()) if self.nT > nm else None
+        self.aux_lu = spla.splu(Caa) if nT > nm else None
         self._t_aux = empty(max(nT - nm, 1))
 
     def matvec(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
@@ -126,6 +161,9 @@ class ExtendedSystem:
                      stream_handle())
         if self.nT > nm:
             y[nm:].zero_()
+        if self.S is not None:
+            self.S.add_matvec(x, y)
+            return y
         C = self.C
         _native.call("fp3d_amg_prolong_add", C.n, C.nnz, ptr(C.indptr),
                      ptr(C.indices), ptr(C.data), 0, ptr(x), ptr(y),
@@ -136,11 +174,16 @@ class ExtendedSystem:
         nm = self.nm
         self.amg.apply(r[:nm].contiguous(), z[:nm])
         if self.nT > nm:
-            C = self.C_am
             t = self._t_aux
-            _native.call("fp3d_csr_spmv", C.n, C.nnz, ptr(C.indptr),
-                         ptr(C.indices), ptr(C.data), ptr(z[:nm]), ptr(t),
-                         stream_handle())
+            if self.S is not None:
+                # aux rows of C applied to [z_main; 0]: the (aux, main) block
+                z[nm:].zero_()
+                self.S.rows_matvec(nm, z, t)
+            else:
+                C = self.C_am
+                _native.call("fp3d_csr_spmv", C.n, C.nnz, ptr(C.indptr),
+                             ptr(C.indices), ptr(C.data), ptr(z[:nm]), ptr(t),
+                             stream_handle())
             rhs = to_host(r[nm:]) - to_host(t)
             z[nm:].copy_(to_dev(self.aux_lu.solve(rhs)))
         return z

@@ -17,9 +17,13 @@ DOF of a cut-element node the other side's.  Its mass-conservation part is
 
 Design: the plain part is the device Picard-matrix assembly
 (PicardOperator.matrix_dev) with the special (cut + fan) elements given a
-zero weight; everything that involves a special element or an aux row is a
-small host COO correction C (O(sheet area) entries).  The Krylov operator
-is y = [A_plain x_main ; 0] + C x on the device.
+zero weight; everything that involves a special element or an aux row is
+the sheet part C (O(sheet area) entries), whose extended CSR pattern the
+cut census fixes: it is built once (`sheet`) and its values are filled on
+the device per assembly (`sheet_fill`, up3d_amd/wake/sheet.py).  The
+Krylov operator is y = [A_plain x_main ; 0] + C x on the device.
+`correction` is the same C assembled on the host as a COO matrix: the
+"continuity" weld closure and small-case checks use it.
 
 Closures (aux rows):
   * "continuity": aux_k - main_j = 0 (the weld; the extended solve then
@@ -409,16 +413,23 @@ class MultivaluedOperator:
         return np.flatnonzero(to_host(self._nu_active))
 
     def side_weights(self, rt_up_dev, rt_lo_dev):
-        """(element weights on the device for the plain part / the AMG,
-        host special-copy weights)."""
+        """(element weights on the device for the plain part / the AMG, the
+        special-copy weights as the device (upper, lower) pair)."""
         pp = getattr(self, "_d_plain_plus", None)
         if pp is None:
             pp = self._d_plain_plus = to_dev(self.plain_plus.astype(np.uint8),
                                              torch.uint8).bool()
         w = torch.where(pp, rt_up_dev, rt_lo_dev)
-        cm = self.cm
-        ru, rl = to_host(rt_up_dev), to_host(rt_lo_dev)
-        return w, (ru[cm.cut_elems], rl[cm.cut_elems], rl[cm.te_lower_elems])
+        return w, (rt_up_dev, rt_lo_dev)
+
+    def special_weights_host(self, special):
+        """The host triple (cut upper, cut lower, fan lower) of a device
+        (upper, lower) density pair (a host triple passes through)."""
+        if special is not None and len(special) == 2 and isinstance(special[0], torch.Tensor):
+            cm = self.cm
+            ru, rl = to_host(special[0]), to_host(special[1])
+            return ru[cm.cut_elems], rl[cm.cut_elems], rl[cm.te_lower_elems]
+        return special
 
     def own_side_mach2_max(self, phi_ext, m_inf, gamma_air=1.4, u_inf=1.0):
         """Largest own-side element M^2 (main field on plain elements, both
@@ -463,7 +474,7 @@ class MultivaluedOperator:
             r_lc = np.ones(len(cm.cut_elems))
             r_lt = np.ones(len(cm.te_lower_elems))
         else:
-            r_uc, r_lc, r_lt = special_rho
+            r_uc, r_lc, r_lt = self.special_weights_host(special_rho)
         parts = []
         if closure == "continuity":
             # single-valued special elements on main DOFs with the upper
@@ -522,6 +533,124 @@ class MultivaluedOperator:
         C = sp.coo_matrix((data, (rows, cols)), shape=(nT, nT)).tocsr()
         C.sum_duplicates()
         return C
+
+    # ------------------------------------------------ device sheet terms
+    def special_copy_contributions(self):
+        """Contributions (rows, cols, kind, src, ab, coef) of the special
+        copies' mass conservation V_e B_a.B_b (times the copy's density):
+        both copies of each cut element, the fan's lower copy; rows and
+        columns are the copy's DOFs, before any row treatment."""
+        from .sheet import LOWER, UPPER
+
+        cm = self.cm
+        parts = []
+        for ids, dofs, kind in ((cm.cut_elems, cm.dofs_upper, UPPER),
+                                (cm.cut_elems, cm.dofs_lower, LOWER),
+                                (cm.te_lower_elems, self.tel_dof, LOWER)):
+            ids = np.asarray(ids, dtype=np.int64)
+            if not len(ids):
+                continue
+            dofs = np.asarray(dofs, dtype=np.int64).reshape(-1, 4)
+            K = _elem_matrices(self.B[ids], self.V[ids], np.ones(len(ids)))
+            parts.append((np.repeat(dofs, 4, axis=1).reshape(-1),
+                          np.tile(dofs, (1, 4)).reshape(-1),
+                          np.full(16 * len(ids), kind, dtype=np.uint8),
+                          np.repeat(ids, 16), np.tile(np.arange(16), len(ids)),
+                          K.reshape(-1)))
+        if not parts:
+            z = np.empty(0, dtype=np.int64)
+            return z, z, z.astype(np.uint8), z, z, np.empty(0)
+        return tuple(np.concatenate([p[i] for p in parts]) for i in range(6))
+
+    def sheet_contributions(self, te_kutta: str = "pressure"):
+        """Contributions of the correction C of closure 'wake_ls' (the
+        special copies with non-TE aux rows dropped and, for the pressure
+        Kutta, TE aux rows re-routed to the TE main row; the wake
+        least-squares rows; the TE Kutta rows, whose values come per call
+        from kutta_values), in the order `correction` builds them."""
+        from .sheet import HOSTVAL, STATIC
+
+        if self._ls is None:
+            raise ValueError("wake_ls closure needs a level set and cut elements")
+        cm = self.cm
+        nT = self.n_total
+        r, c, k, s, ab, coef = self.special_copy_contributions()
+        drop = np.zeros(nT, dtype=bool)
+        drop[self._nonte_rows] = True
+        keep = ~drop[r]
+        r, c, k, s, ab, coef = (a[keep] for a in (r, c, k, s, ab, coef))
+        parts = []
+        if te_kutta == "pressure":
+            reroute = np.full(nT, -1, dtype=np.int64)
+            reroute[cm.ext_dof_of_node[cm.te_nodes]] = cm.te_nodes
+            hit = reroute[r] >= 0
+            r = r.copy()
+            r[hit] = reroute[r[hit]]
+        elif te_kutta != "mass":
+            raise ValueError(f"te_kutta={te_kutta!r} unknown")
+        parts.append((r, c, k, s, ab, coef))
+        lr, lc, ld = self._ls
+        z = np.zeros(len(lr), dtype=np.int64)
+        parts.append((lr, lc, np.full(len(lr), STATIC, dtype=np.uint8), z, z, ld))
+        if te_kutta == "pressure":
+            kr, kc, _ = self._te_kutta_coo(np.zeros(nT))
+            parts.append((kr, kc, np.full(len(kr), HOSTVAL, dtype=np.uint8),
+                          np.arange(len(kr)), np.zeros(len(kr), dtype=np.int64),
+                          np.ones(len(kr))))
+        return tuple(np.concatenate([p[i] for p in parts]) for i in range(6))
+
+    def sheet(self, te_kutta: str = "pressure", user: str = "picard"):
+        """The device pattern of C (closure 'wake_ls'), built once per
+        (te_kutta, user): each user (the Picard operator, the Newton
+        residual) fills its own values."""
+        from .sheet import SheetMatrix
+
+        cache = self.__dict__.setdefault("_sheets", {})
+        key = (te_kutta, user)
+        if key not in cache:
+            cache[key] = SheetMatrix(self.n_total, *self.sheet_contributions(te_kutta))
+        return cache[key]
+
+    def kutta_values(self, phi_ext) -> np.ndarray:
+        """Values of the TE pressure-Kutta rows at phi_ext, in the order of
+        their contributions."""
+        return self._te_kutta_coo(phi_ext)[2]
+
+    def special_weights_dev(self, special):
+        """(wa, wb) device (n_tets,) arrays of the upper / lower copy
+        densities: `special` is a device pair (rho_tilde upper, lower), or
+        the host triple (cut upper, cut lower, fan lower) of special
+        densities, or None (all ones)."""
+        nt = self.op.store.n_tets
+        if special is None:
+            one = torch.ones(nt, dtype=torch.float64, device=self.d_plain_mask.device)
+            return one, one
+        if len(special) == 2 and isinstance(special[0], torch.Tensor):
+            return special[0], special[1]
+        cm = self.cm
+        wa = torch.zeros(nt, dtype=torch.float64, device=self.d_plain_mask.device)
+        wb = torch.zeros_like(wa)
+        if len(cm.cut_elems):
+            ic = to_dev(np.asarray(cm.cut_elems, dtype=np.int64), torch.int64)
+            wa[ic] = to_dev(np.asarray(special[0], dtype=np.float64))
+            wb[ic] = to_dev(np.asarray(special[1], dtype=np.float64))
+        if len(cm.te_lower_elems):
+            it = to_dev(np.asarray(cm.te_lower_elems, dtype=np.int64), torch.int64)
+            wb[it] = to_dev(np.asarray(special[2], dtype=np.float64))
+        return wa, wb
+
+    def sheet_fill(self, special, te_kutta: str = "pressure", phi_ext=None,
+                   fmask=None, user: str = "picard"):
+        """C of closure 'wake_ls' filled on the device (its SheetMatrix)."""
+        S = self.sheet(te_kutta, user)
+        wa, wb = self.special_weights_dev(special)
+        kv = None
+        if te_kutta == "pressure":
+            if phi_ext is None:
+                raise ValueError("te_kutta='pressure' needs phi_ext")
+            kv = self.kutta_values(phi_ext)
+        S.fill(wa=wa, wb=wb, kv=kv, fmask=fmask)
+        return S
 
     def assemble_matrix(self, rho_main=None, special_rho=None,
                         closure: str = "wake_ls", te_kutta: str = "pressure",
