@@ -109,12 +109,26 @@ def main():
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--calib", action="store_true",
                     help="also run a plain 1 GiB device copy (counter scale)")
+    ap.add_argument("--solve-profile", action="store_true",
+                    help="only profile the Krylov solve: one Newton system of the "
+                         "192x64x64 channel, GMRES(200) + AMG for a fixed iteration "
+                         "count in both precisions, with a per-component budget")
+    ap.add_argument("--profile-iters", type=int, default=400)
+    ap.add_argument("--no-components", action="store_true",
+                    help="with --solve-profile: time only the GMRES solves (for a "
+                         "kernel trace whose dispatches are all GMRES iterations)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the "
               "multi-GPU sharding of this path is a design note, not a run.",
               file=sys.stderr)
         sys.exit(2)
+    if args.solve_profile:
+        if args.calib:
+            _calibration_copy()
+        print(json.dumps(run_solve_profile(args.profile_iters,
+                                           components=not args.no_components)), flush=True)
+        return
 
     import torch
 
@@ -195,13 +209,7 @@ def main():
         upw._monitors(upw.d_nu, upw.d_rt, RHO_FLOOR)
 
     if args.calib:
-        # plain copy of known size: 1 GiB read + 1 GiB written, so rocprof's
-        # FETCH_SIZE / WRITE_SIZE for this dispatch give the counter scale
-        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        _calibration_copy()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -329,6 +337,19 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _calibration_copy():
+    """A plain device copy of known size (1 GiB read + 1 GiB written), so
+    rocprof's FETCH_SIZE / WRITE_SIZE of this dispatch give the counters'
+    scale in the same run."""
+    import torch
+
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def cpu_threads_plan():
     """Thread counts of the CPU baseline: one thread, all the cores this
     process may use (its affinity set, capped by OMP_NUM_THREADS where that
@@ -430,6 +451,125 @@ def cpu_baseline(st, mesh, passes):
             "os_cpu_count": os.cpu_count(), "affinity_cores": affinity,
             "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
             "runs": runs}
+
+
+def run_solve_profile(iters: int = 400, restart: int = 200, components: bool = True):
+    """The Krylov solve of one Newton system on the 192x64x64 channel at
+    M_inf 0.84 (the constructed multi-regime state of the linearisation
+    bench, the clamped-Jacobian AMG with the tuned degree 2 and a 1000-row
+    coarsest level): `iters` GMRES(restart) iterations in each precision
+    (the tolerance is set out of reach so the count is fixed), with HIP
+    events around the whole solve and around each component (Jacobian
+    matvec with Term 3, one V-cycle, one Arnoldi orthogonalisation at
+    k = restart / 2 basis vectors).  Bytes are compulsory traffic per call:
+    SpMV 12 nnz + 20 N (float64 values) or 8 nnz + 20 N (float32 values,
+    float64 vectors); the orthogonalisation reads the k basis vectors twice
+    (dot, axpy) and w three times and writes it twice (CGS, the DGKS second
+    pass skipped): (2k + 5) x 8 N, or 8 k N + 40 N with a float32 basis."""
+    import torch
+
+    from up3d_amd.device import ptr, stream_handle, to_dev
+    from up3d_amd.mesh.channel import boundary_nodes, config_mesh
+    from up3d_amd.solve.linear import gmres_dev, scale
+
+    from up3d_amd import _native
+    from up3d_amd.solve.newton import NewtonWorkspace
+
+    mesh = config_mesh("c2")
+    dn = boundary_nodes(mesh, ["inflow", "outflow"])
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {"workload": "one Newton system of the 192x64x64 channel (4,718,592 tets, "
+                       "815,425 rows) at M_inf 0.84, constructed state; GMRES(%d), AMG of "
+                       "the clamped Jacobian (Chebyshev degree 2, coarsest <= 1000 rows)"
+                       % restart}
+    for prec in ("fp64", "fp32"):
+        log(f"[profile] {prec}: set-up")
+        ws = NewtonWorkspace(mesh.nodes, mesh.elements, dn, mesh.nodes[dn, 0].copy(),
+                             precision=prec)
+        ws.amg_kw = {"degree": 2, "max_coarse": 1000}
+        ws.set_mach(M_INF)
+        s = ws.states[0]
+        s.phi.copy_(to_dev(constructed_phi(mesh.nodes)))
+        s.phi.index_copy_(0, ws.dn, ws.dv)
+        ws.eval_residual(s, UPWIND_C, M_CRIT, RHO_FLOOR)
+        ws.assemble_coupled(s, UPWIND_C, M_CRIT, RHO_FLOOR, None)
+        M = ws.build_preconditioner(s, "jacobian_spd", 0.9, se=ws.se_used)
+        n, nnz = ws.n, ws.st.nnz
+        rhs = torch.empty_like(s.R)
+        scale(-1.0, s.R, rhs)
+        basis = "f32" if prec == "fp32" else "f64"
+        maxiter = max(1, iters // restart)
+        gmres_dev(ws.jac.matvec, rhs, M.apply, rtol=1e-30, restart=restart,
+                  maxiter=1, basis=basis)                      # warm-up, graphs
+        log(f"[profile] {prec}: {maxiter * restart} GMRES iterations")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, info, n_it = gmres_dev(ws.jac.matvec, rhs, M.apply, rtol=1e-30,
+                                  restart=restart, maxiter=maxiter, basis=basis)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per_it = wall / max(n_it, 1) * 1e3
+        out[prec] = {"iterations": int(n_it), "seconds": round(wall, 4),
+                     "ms_per_iteration": round(per_it, 4),
+                     "n_term3_active": int(ws.jac.n_active),
+                     "amg_levels": [{"rows": int(a), "nnz": int(b)}
+                                    for a, b in zip(M.shapes, M.nnzs)]}
+        if not components:
+            log(f"[profile] {prec}: {json.dumps(out[prec])}")
+            del ws, M
+            torch.cuda.empty_cache()
+            continue
+        x = torch.randn(n, dtype=torch.float64, device=rhs.device)
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        t_mv = timed(lambda: ws.jac.matvec(x, y), 50)
+        t_vc = timed(lambda: M.apply(x, z), 50)
+        k = restart // 2
+        Vb = torch.randn((k + 1, n), dtype=torch.float64, device=x.device)
+        if basis == "f32":
+            Vb = Vb.to(torch.float32)
+        w = torch.randn(n, dtype=torch.float64, device=x.device)
+        red_work = torch.empty(_native.lib().fp3d_multi_dot_work_doubles(k + 2),
+                               dtype=torch.float64, device=x.device)
+        o = torch.empty(2 * k + 8, dtype=torch.float64, device=x.device)
+        sfx = "_f32v" if basis == "f32" else ""
+
+        def ortho():
+            sh = stream_handle()
+            _native.call("fp3d_multi_dot_norm" + sfx, n, k, ptr(Vb), Vb.stride(0),
+                         ptr(w), ptr(red_work), ptr(o[2:]), sh)
+            _native.call("fp3d_multi_axpy" + sfx, n, k, ptr(Vb), Vb.stride(0), ptr(o[2:]),
+                         0.0, ptr(w), sh)
+            _native.call("fp3d_norms", n, ptr(w), ptr(red_work), ptr(o), sh)
+
+        t_or = timed(ortho, 20)
+        mv_bytes = (12 if prec == "fp64" else 8) * nnz + 20 * n
+        or_bytes = ((2 * k + 5) * 8 * n) if basis == "f64" else (8 * k * n + 40 * n)
+        out[prec].update({
+            "matvec_ms": round(t_mv, 4),
+            "matvec_GBs": round(mv_bytes / (t_mv * 1e-3) / 1e9, 1),
+            "vcycle_ms": round(t_vc, 4),
+            "orthogonalisation_ms_at_k": round(t_or, 4), "k": k,
+            "orthogonalisation_GBs": round(or_bytes / (t_or * 1e-3) / 1e9, 1),
+            "other_ms_per_iteration": round(per_it - t_mv - t_vc - t_or, 4)})
+        log(f"[profile] {prec}: {json.dumps(out[prec])}")
+        del ws, M, Vb
+        torch.cuda.empty_cache()
+    out["fp64_over_fp32_per_iteration"] = round(
+        out["fp64"]["ms_per_iteration"] / out["fp32"]["ms_per_iteration"], 3)
+    return out
 
 
 def run_solves(mesh, config, tune=True):

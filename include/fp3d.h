@@ -302,9 +302,12 @@ int fp3d_csr_spmv_f32(int64_t n_rows, int64_t nnz, const int32_t *indptr,
                       const double *x, double *y, void *stream);
 /* y = A x + Term3 x (t3 from fp3d_term3_build, skipped when t3 == NULL or
  * n_active == 0); rows with free_mask[row] == 0 give y[row] = x[row].
- * data is float64 (is_f32 == 0) or float32 (is_f32 == 1). */
+ * fmt (no reference counterpart; the reference's matvec is scipy's): bit 1
+ * = data float32 (else float64), bit 4 = indices are int16 offsets from
+ * the row (col = row + indices[j], a square matrix whose bandwidth fits),
+ * else int32 columns.  Vectors are float64. */
 int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
-                        const int32_t *indices, const void *data, int is_f32,
+                        const void *indices, const void *data, int fmt,
                         const int32_t *elements, const int32_t *upstream,
                         const void *t3, int64_t n_active, int64_t n_tets,
                         const uint8_t *free_mask, const double *x, double *y,
@@ -403,43 +406,50 @@ int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
                void *stream);
 
 /* ----------------------------------------------------- AMG V-cycle kernels
+ * (the reference applies pyamg's V-cycle, pyfp3d/solve/linear.py:61-81)
+ * fmt: bit 1 = matrix values float32 (else float64); bit 2 = every vector
+ * argument (dinv, b, x, r, d*, xc, bc) float32 (else float64), arithmetic
+ * float64 either way; bit 4 = indices are int16 offsets from the row (a
+ * square matrix whose bandwidth fits in int16), else int32 columns.
+ *
  * Chebyshev smoothing (degree >= 1) of A x = b on D^-1 A with eigenvalue
  * interval [lmin, lmax]; zero_init = 1 starts from x = 0 (x is written),
  * 0 from the given x, 2 continues after a first step from x = 0 that the
  * caller already took (x, r, d0 as fp3d_amg_restrict_start leaves them).
  * r, d0, d1 are n-vectors of scratch. */
 int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
-                         const int32_t *indices, const void *data, int is_f32,
-                         const double *dinv, const double *b, double *x,
-                         double *r, double *d0, double *d1, int degree,
-                         double lmin, double lmax, int zero_init,
-                         void *stream);
+                         const void *indices, const void *data, int fmt,
+                         const void *dinv, const void *b, void *x, void *r,
+                         void *d0, void *d1, int degree, double lmin, double lmax,
+                         int zero_init, void *stream);
 /* bc = R r, and the coarse level's first Chebyshev step from x_c = 0 on
  * bc (dinv_c, degree_c, [lmin_c, lmax_c]): r_c, d_c and x_c as
  * fp3d_amg_cheb_smooth(zero_init = 1) leaves them before its second step.
  * Then fp3d_amg_cheb_smooth(zero_init = 2) on the coarse level. */
 int fp3d_amg_restrict_start(int64_t nc, int64_t nnz, const int32_t *indptr,
-                            const int32_t *indices, const void *data,
-                            int is_f32, const double *r, double *bc,
-                            const double *dinv_c, int degree_c, double lmin_c,
-                            double lmax_c, double *x_c, double *r_c,
-                            double *d_c, void *stream);
+                            const void *indices, const void *data, int fmt,
+                            const void *r, void *bc, const void *dinv_c,
+                            int degree_c, double lmin_c, double lmax_c, void *x_c,
+                            void *r_c, void *d_c, void *stream);
 /* r = b - A x */
 int fp3d_amg_residual(int64_t n, int64_t nnz, const int32_t *indptr,
-                      const int32_t *indices, const void *data, int is_f32,
-                      const double *b, const double *x, double *r,
-                      void *stream);
+                      const void *indices, const void *data, int fmt,
+                      const void *b, const void *x, void *r, void *stream);
 /* bc = R r (R = P^T stored as CSR, nc rows) */
 int fp3d_amg_restrict(int64_t nc, int64_t nnz, const int32_t *indptr,
-                      const int32_t *indices, const void *data, int is_f32,
-                      const double *r, double *bc, void *stream);
+                      const void *indices, const void *data, int fmt,
+                      const void *r, void *bc, void *stream);
 /* x += P xc */
 int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
-                         const int32_t *indices, const void *data, int is_f32,
-                         const double *xc, double *x, void *stream);
+                         const void *indices, const void *data, int fmt,
+                         const void *xc, void *x, void *stream);
 /* x = M b, M dense row-major n x n (the coarsest-level pseudo-inverse) */
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
                       void *stream);
+/* the same with float32 b and x when fmt bit 2 is set (M and the sums stay
+ * float64: the coarsest solve is float64 in both precisions) */
+int fp3d_dense_matvec_v(int fmt, int64_t n, const double *M, const void *b, void *x,
+                        void *stream);
 
 /* ---------------------------------------------------- discrete adjoint
  * (new functionality: the reference has no adjoint.)  At a frozen upstream

@@ -134,6 +134,7 @@ _SIGS = {
     "fp3d_amg_prolong_add": (_int, [_i64, _i64, _vp, _vp, _vp, _int, _vp, _vp,
                                     _vp]),
     "fp3d_dense_matvec": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "fp3d_dense_matvec_v": (_int, [_int, _i64, _vp, _vp, _vp, _vp]),
     "fp3d_drho_tilde_dmach": (_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl,
                                      _dbl, _vp, _vp]),
     "fp3d_adjoint_dx": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

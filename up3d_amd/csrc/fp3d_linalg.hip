@@ -16,89 +16,116 @@
 namespace fp3d {
 
 // --------------------------------------------------------------- epilogues
+// Every epilogue is templated on the vector type TV of the V-cycle (double
+// or float; the arithmetic is double either way, so TV = double is exactly
+// the float64 sequence).
+template <typename TV>
 struct EpStore {  // y = s
-  double *y;
-  __device__ void operator()(int64_t r, double s) const { y[r] = s; }
+  TV *y;
+  __device__ void operator()(int64_t r, double s) const { y[r] = (TV)s; }
 };
+template <typename TV>
 struct EpResidual {  // r = b - s
-  const double *b;
-  double *r;
-  __device__ void operator()(int64_t i, double s) const { r[i] = b[i] - s; }
+  const TV *b;
+  TV *r;
+  __device__ void operator()(int64_t i, double s) const {
+    r[i] = (TV)((double)b[i] - s);
+  }
 };
+template <typename TV>
 struct EpAdd {  // x += s
-  double *x;
-  __device__ void operator()(int64_t i, double s) const { x[i] += s; }
+  TV *x;
+  __device__ void operator()(int64_t i, double s) const {
+    x[i] = (TV)((double)x[i] + s);
+  }
 };
 // Chebyshev smoothing (three-term recurrence on D^-1 A).  x is never read
 // by the SpMV of a kernel that writes it: each step adds the PREVIOUS
 // direction to x, and the last direction is added by k_add.
 // First step from a nonzero x: r = dinv (b - Ax); d = r/theta.
+template <typename TV>
 struct EpChebFirst {
-  const double *b, *dinv;
-  double *r, *d;
+  const TV *b, *dinv;
+  TV *r, *d;
   double inv_theta;
   __device__ void operator()(int64_t i, double s) const {
-    double ri = dinv[i] * (b[i] - s);
-    r[i] = ri;
-    d[i] = ri * inv_theta;
+    double ri = (double)dinv[i] * ((double)b[i] - s);
+    r[i] = (TV)ri;
+    d[i] = (TV)(ri * inv_theta);
   }
 };
 // Recurrence: x += d_old; r -= dinv A d_old; d_new = c1 d_old + c2 r
+template <typename TV>
 struct EpChebIter {
-  const double *dinv, *d_old;
-  double *x, *r, *d_new;
+  const TV *dinv, *d_old;
+  TV *x, *r, *d_new;
   double c1, c2;
   __device__ void operator()(int64_t i, double s) const {
     double dold = d_old[i];
-    double ri = r[i] - dinv[i] * s;
-    r[i] = ri;
-    d_new[i] = c1 * dold + c2 * ri;
-    x[i] += dold;
+    double ri = (double)r[i] - (double)dinv[i] * s;
+    r[i] = (TV)ri;
+    d_new[i] = (TV)(c1 * dold + c2 * ri);
+    x[i] = (TV)((double)x[i] + dold);
   }
 };
 
 // Last recurrence step: as EpChebIter, then the new direction is added to
 // x as well -- the same two additions, in the same order, that the step
 // and a separate x += d_last pass make.
+template <typename TV>
 struct EpChebLast {
-  const double *dinv, *d_old;
-  double *x, *r, *d_new;
+  const TV *dinv, *d_old;
+  TV *x, *r, *d_new;
   double c1, c2;
   __device__ void operator()(int64_t i, double s) const {
     double dold = d_old[i];
-    double ri = r[i] - dinv[i] * s;
-    r[i] = ri;
+    double ri = (double)r[i] - (double)dinv[i] * s;
+    r[i] = (TV)ri;
     double dn = c1 * dold + c2 * ri;
-    d_new[i] = dn;
-    x[i] = (x[i] + dold) + dn;
+    d_new[i] = (TV)dn;
+    x[i] = (TV)(((double)x[i] + dold) + dn);
   }
 };
 // Restriction b_c = R r that also takes the coarse level's first smoothing
 // step from x_c = 0 (as k_cheb_zero on b_c): r_c = dinv b_c,
 // d_c = r_c / theta, and x_c = 0, or x_c = 0 + d_c when that step is the
 // only one.
+template <typename TV>
 struct EpRestrictStart {
-  double *bc;
-  const double *dinv;
-  double *x, *r, *d;
+  TV *bc;
+  const TV *dinv;
+  TV *x, *r, *d;
   double inv_theta;
   int x_is_d;
   __device__ void operator()(int64_t i, double s) const {
-    bc[i] = s;
-    double ri = dinv[i] * s;
-    r[i] = ri;
+    bc[i] = (TV)s;
+    double ri = (double)dinv[i] * s;
+    r[i] = (TV)ri;
     double di = ri * inv_theta;
-    d[i] = di;
-    x[i] = x_is_d ? 0.0 + di : 0.0;
+    d[i] = (TV)di;
+    x[i] = (TV)(x_is_d ? 0.0 + di : 0.0);
   }
 };
 
-template <int G, typename T, typename Ep>
+// Column j of a CSR row: int32 absolute indices, or int16 offsets from the
+// row (square matrices whose bandwidth fits: half the index bytes).
+template <typename TI>
+__device__ __forceinline__ int32_t csr_col(int64_t row, TI v);
+template <>
+__device__ __forceinline__ int32_t csr_col<int32_t>(int64_t, int32_t v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ int32_t csr_col<int16_t>(int64_t row, int16_t v) {
+  return (int32_t)row + (int32_t)v;
+}
+
+template <int G, typename T, typename TI, typename TX, typename Ep>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
                                                  const int32_t *__restrict__ ptr,
-                                                 const int32_t *__restrict__ idx,
+                                                 const TI *__restrict__ idx,
                                                  const T *__restrict__ val,
-                                                 const double *__restrict__ x,
+                                                 const TX *__restrict__ x,
                                                  Ep ep) {
   constexpr int RPB = kBlock / G;
   int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / G;
@@ -106,17 +133,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
   if (row >= n_rows) return;
   int32_t j0 = ptr[row], j1 = ptr[row + 1];
   double s = 0.0;
-  for (int32_t j = j0 + lane; j < j1; j += G) s += (double)val[j] * x[idx[j]];
+  for (int32_t j = j0 + lane; j < j1; j += G)
+    s += (double)val[j] * (double)x[csr_col<TI>(row, idx[j])];
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
   if (lane == 0) ep(row, s);
 }
 
 // SpMV + Term-3 gather + Dirichlet identity rows (free_mask nullable)
-template <int G, typename T>
+template <int G, typename T, typename TI>
 __global__ __launch_bounds__(kBlock) void k_spmv_t3(
     int64_t n_rows, const int32_t *__restrict__ ptr,
-    const int32_t *__restrict__ idx, const T *__restrict__ val,
+    const TI *__restrict__ idx, const T *__restrict__ val,
     const double *__restrict__ x, const int32_t *__restrict__ t3ptr,
     const int32_t *__restrict__ t3ent,
     const double *__restrict__ W,  // weight of entry q, beside t3ent[q]
@@ -128,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t3(
   if (row >= n_rows) return;
   int32_t j0 = ptr[row], j1 = ptr[row + 1];
   double s = 0.0;
-  for (int32_t j = j0 + lane; j < j1; j += G) s += (double)val[j] * x[idx[j]];
+  for (int32_t j = j0 + lane; j < j1; j += G)
+    s += (double)val[j] * x[csr_col<TI>(row, idx[j])];
   // the row's Term-3 entries are strided over the same G lanes (fixed lane
   // assignment and reduction tree: deterministic)
   if (t3ptr) {
@@ -153,32 +182,43 @@ inline int lanes_for(int64_t n_rows, int64_t nnz) {
   return 64;
 }
 
-template <typename T, typename Ep>
+template <typename T, typename TI, typename TX, typename Ep>
 int launch_spmv(int64_t n_rows, int64_t nnz, const int32_t *ptr,
-                const int32_t *idx, const T *val, const double *x, Ep ep,
+                const TI *idx, const T *val, const TX *x, Ep ep,
                 hipStream_t s) {
   if (n_rows <= 0) return FP3D_OK;
   int G = lanes_for(n_rows, nnz);
   unsigned grid = grid_for(n_rows, kBlock / G);
   switch (G) {
-    case 4: hipLaunchKernelGGL((k_spmv<4, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
-    case 8: hipLaunchKernelGGL((k_spmv<8, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
-    case 16: hipLaunchKernelGGL((k_spmv<16, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
-    case 32: hipLaunchKernelGGL((k_spmv<32, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
-    default: hipLaunchKernelGGL((k_spmv<64, T, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 4: hipLaunchKernelGGL((k_spmv<4, T, TI, TX, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 8: hipLaunchKernelGGL((k_spmv<8, T, TI, TX, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 16: hipLaunchKernelGGL((k_spmv<16, T, TI, TX, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    case 32: hipLaunchKernelGGL((k_spmv<32, T, TI, TX, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
+    default: hipLaunchKernelGGL((k_spmv<64, T, TI, TX, Ep>), dim3(grid), dim3(kBlock), 0, s, n_rows, ptr, idx, val, x, ep); break;
   }
   return check_launch("spmv");
 }
 
-template <typename Ep>
-int spmv_any(int64_t n_rows, int64_t nnz, const int32_t *ptr,
-             const int32_t *idx, const void *val, int is_f32, const double *x,
-             Ep ep, hipStream_t s) {
-  if (is_f32)
-    return launch_spmv<float, Ep>(n_rows, nnz, ptr, idx, (const float *)val, x,
-                                  ep, s);
-  return launch_spmv<double, Ep>(n_rows, nnz, ptr, idx, (const double *)val, x,
-                                 ep, s);
+// Matrix format bits of the V-cycle and Jacobian entry points.
+constexpr int kFmtF32Values = 1, kFmtF32Vectors = 2, kFmtIdx16 = 4;
+
+// SpMV with the matrix format of `fmt` (values float32 or float64, int32 or
+// int16 row-relative columns) and an x of the epilogue's vector type.
+template <typename TX, typename Ep>
+int spmv_fmt(int64_t n_rows, int64_t nnz, const int32_t *ptr, const void *idx,
+             const void *val, int fmt, const TX *x, Ep ep, hipStream_t s) {
+  const bool v32 = fmt & kFmtF32Values, i16 = fmt & kFmtIdx16;
+  if (v32 && i16)
+    return launch_spmv<float, int16_t, TX, Ep>(n_rows, nnz, ptr, (const int16_t *)idx,
+                                               (const float *)val, x, ep, s);
+  if (v32)
+    return launch_spmv<float, int32_t, TX, Ep>(n_rows, nnz, ptr, (const int32_t *)idx,
+                                               (const float *)val, x, ep, s);
+  if (i16)
+    return launch_spmv<double, int16_t, TX, Ep>(n_rows, nnz, ptr, (const int16_t *)idx,
+                                                (const double *)val, x, ep, s);
+  return launch_spmv<double, int32_t, TX, Ep>(n_rows, nnz, ptr, (const int32_t *)idx,
+                                              (const double *)val, x, ep, s);
 }
 
 // ------------------------------------------------------------ BLAS-1 etc.
@@ -470,38 +510,40 @@ __global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
 
 // x = M b, M dense row-major: one wave per row, lanes stride the row
 // (coalesced), fixed shuffle-tree sum
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_dense_mv(int64_t n,
                                                     const double *__restrict__ M,
-                                                    const double *__restrict__ b,
-                                                    double *__restrict__ x) {
+                                                    const TV *__restrict__ b,
+                                                    TV *__restrict__ x) {
   const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
   const double *__restrict__ row = M + i * n;
   double s = 0.0;
-  for (int64_t j = lane; j < n; j += 64) s += row[j] * b[j];
+  for (int64_t j = lane; j < n; j += 64) s += row[j] * (double)b[j];
   s = wsum(s);
-  if (lane == 0) x[i] = s;
+  if (lane == 0) x[i] = (TV)s;
 }
 
 // Chebyshev first step from x = 0: x = 0; r = dinv b; d = r/theta (x = 0 + d
 // when this step is the only one)
-__global__ void k_cheb_zero(int64_t n, const double *__restrict__ b,
-                            const double *__restrict__ dinv, double inv_theta,
-                            double *__restrict__ x, double *__restrict__ r,
-                            double *__restrict__ d, int x_is_d) {
+template <typename TV>
+__global__ void k_cheb_zero(int64_t n, const TV *__restrict__ b,
+                            const TV *__restrict__ dinv, double inv_theta,
+                            TV *__restrict__ x, TV *__restrict__ r,
+                            TV *__restrict__ d, int x_is_d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double ri = dinv[i] * b[i];
-  r[i] = ri;
+  double ri = (double)dinv[i] * (double)b[i];
+  r[i] = (TV)ri;
   double di = ri * inv_theta;
-  d[i] = di;
-  x[i] = x_is_d ? 0.0 + di : 0.0;
+  d[i] = (TV)di;
+  x[i] = (TV)(x_is_d ? 0.0 + di : 0.0);
 }
-__global__ void k_add(int64_t n, const double *__restrict__ d,
-                      double *__restrict__ x) {
+template <typename TV>
+__global__ void k_add(int64_t n, const TV *__restrict__ d, TV *__restrict__ x) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += d[i];
+  if (i < n) x[i] = (TV)((double)x[i] + (double)d[i]);
 }
 
 // ----------------------------------------------------------- Term 3 (device)
@@ -837,6 +879,55 @@ __global__ void k_t3_tval(int64_t n_active, const int32_t *__restrict__ act,
 
 using namespace fp3d;
 
+// Chebyshev smoothing of one level on vectors of type TV
+template <typename TV>
+static int cheb_smooth_t(int64_t n, int64_t nnz, const int32_t *indptr,
+                         const void *indices, const void *data, int fmt,
+                         const TV *dinv, const TV *b, TV *x, TV *r, TV *d0, TV *d1,
+                         int degree, double lmin, double lmax, int zero_init,
+                         hipStream_t s) {
+  double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  double sigma = theta / delta;
+  double rho0 = 1.0 / sigma;
+  int rc = FP3D_OK;
+  // the last direction goes into x inside the last step (EpChebLast), or in
+  // the first step itself from x = 0 when degree == 1; only the first step
+  // from a nonzero x (which reads x in its SpMV) needs a separate x += d
+  if (zero_init == 1) {
+    hipLaunchKernelGGL(k_cheb_zero<TV>, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                       b, dinv, 1.0 / theta, x, r, d0, degree == 1 ? 1 : 0);
+    rc = check_launch("cheb_zero");
+  } else if (zero_init == 0) {
+    rc = spmv_fmt(n, nnz, indptr, indices, data, fmt, (const TV *)x,
+                  EpChebFirst<TV>{b, dinv, r, d0, 1.0 / theta}, s);
+  }
+  // zero_init == 2: the caller took the first step from x = 0 (x, r, d0 set,
+  // as fp3d_amg_restrict_start does)
+  if (rc) return rc;
+  TV *dold = d0, *dnew = d1;
+  for (int k = 1; k < degree; ++k) {
+    double rho1 = 1.0 / (2.0 * sigma - rho0);
+    double c1 = rho1 * rho0, c2 = 2.0 * rho1 / delta;
+    if (k == degree - 1)
+      rc = spmv_fmt(n, nnz, indptr, indices, data, fmt, (const TV *)dold,
+                    EpChebLast<TV>{dinv, dold, x, r, dnew, c1, c2}, s);
+    else
+      rc = spmv_fmt(n, nnz, indptr, indices, data, fmt, (const TV *)dold,
+                    EpChebIter<TV>{dinv, dold, x, r, dnew, c1, c2}, s);
+    if (rc) return rc;
+    rho0 = rho1;
+    TV *t = dold;
+    dold = dnew;
+    dnew = t;
+  }
+  if (degree == 1 && zero_init == 0) {
+    hipLaunchKernelGGL(k_add<TV>, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
+                       (const TV *)dold, x);
+    return check_launch("cheb_smooth");
+  }
+  return FP3D_OK;
+}
+
 // ======================================================================
 // C-ABI
 // ======================================================================
@@ -919,7 +1010,8 @@ int fp3d_csr_spmv(int64_t n_rows, int64_t nnz, const int32_t *indptr,
   FP3D_REQUIRE(n_rows == 0 || (indptr && indices && data && x && y),
                "csr_spmv: NULL pointer");
   FP3D_REQUIRE(x != y, "csr_spmv: x and y must not alias");
-  return launch_spmv<double>(n_rows, nnz, indptr, indices, data, x, EpStore{y},
+  return launch_spmv<double, int32_t, double>(n_rows, nnz, indptr, indices, data, x,
+                                              EpStore<double>{y},
                              FP3D_STREAM(stream));
 }
 
@@ -929,7 +1021,8 @@ int fp3d_csr_spmv_f32(int64_t n_rows, int64_t nnz, const int32_t *indptr,
   FP3D_REQUIRE(n_rows == 0 || (indptr && indices && data && x && y),
                "csr_spmv_f32: NULL pointer");
   FP3D_REQUIRE(x != y, "csr_spmv_f32: x and y must not alias");
-  return launch_spmv<float>(n_rows, nnz, indptr, indices, data, x, EpStore{y},
+  return launch_spmv<float, int32_t, double>(n_rows, nnz, indptr, indices, data, x,
+                                             EpStore<double>{y},
                             FP3D_STREAM(stream));
 }
 
@@ -1006,13 +1099,15 @@ int fp3d_term3_active_count(const void *t3, int64_t n_tets, int64_t n_nodes,
 }
 
 int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
-                        const int32_t *indices, const void *data, int is_f32,
+                        const void *indices, const void *data, int fmt,
                         const int32_t *elements, const int32_t *upstream,
                         const void *t3, int64_t n_active, int64_t n_tets,
                         const uint8_t *free_mask, const double *x, double *y,
                         void *stream) {
   FP3D_REQUIRE(n_rows > 0 && indptr && indices && data && x && y,
                "csr_spmv_term3: NULL pointer");
+  FP3D_REQUIRE((fmt & ~(kFmtF32Values | kFmtIdx16)) == 0,
+               "csr_spmv_term3: fmt takes bits 1 (float32 values) and 4 (int16 columns)");
   FP3D_REQUIRE(x != y, "csr_spmv_term3: x and y must not alias");
   hipStream_t s = FP3D_STREAM(stream);
   const int32_t *t3ptr = nullptr, *t3ent = nullptr;
@@ -1034,27 +1129,29 @@ int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
   }
   int G = lanes_for(n_rows, nnz);
   unsigned grid = grid_for(n_rows, kBlock / G);
-#define T3_LAUNCH(GG, T)                                                     \
-  hipLaunchKernelGGL((k_spmv_t3<GG, T>), dim3(grid), dim3(kBlock), 0, s,     \
-                     n_rows, indptr, indices, (const T *)data, x, t3ptr,     \
-                     t3ent, W, tval, free_mask, y)
-  if (is_f32) {
-    switch (G) {
-      case 4: T3_LAUNCH(4, float); break;
-      case 8: T3_LAUNCH(8, float); break;
-      case 16: T3_LAUNCH(16, float); break;
-      case 32: T3_LAUNCH(32, float); break;
-      default: T3_LAUNCH(64, float); break;
-    }
-  } else {
-    switch (G) {
-      case 4: T3_LAUNCH(4, double); break;
-      case 8: T3_LAUNCH(8, double); break;
-      case 16: T3_LAUNCH(16, double); break;
-      case 32: T3_LAUNCH(32, double); break;
-      default: T3_LAUNCH(64, double); break;
-    }
+#define T3_LAUNCH(GG, T, TI)                                                 \
+  hipLaunchKernelGGL((k_spmv_t3<GG, T, TI>), dim3(grid), dim3(kBlock), 0, s, \
+                     n_rows, indptr, (const TI *)indices, (const T *)data, x, \
+                     t3ptr, t3ent, W, tval, free_mask, y)
+#define T3_SWITCH(T, TI)                       \
+  switch (G) {                                 \
+    case 4: T3_LAUNCH(4, T, TI); break;        \
+    case 8: T3_LAUNCH(8, T, TI); break;        \
+    case 16: T3_LAUNCH(16, T, TI); break;      \
+    case 32: T3_LAUNCH(32, T, TI); break;      \
+    default: T3_LAUNCH(64, T, TI); break;      \
   }
+  const bool v32 = fmt & kFmtF32Values, i16 = fmt & kFmtIdx16;
+  if (v32 && i16) {
+    T3_SWITCH(float, int16_t)
+  } else if (v32) {
+    T3_SWITCH(float, int32_t)
+  } else if (i16) {
+    T3_SWITCH(double, int16_t)
+  } else {
+    T3_SWITCH(double, int32_t)
+  }
+#undef T3_SWITCH
 #undef T3_LAUNCH
   return check_launch("csr_spmv_term3");
 }
@@ -1244,108 +1341,111 @@ int fp3d_norms(int64_t n, const double *x, double *work, double *out2,
 }
 
 int fp3d_amg_cheb_smooth(int64_t n, int64_t nnz, const int32_t *indptr,
-                         const int32_t *indices, const void *data, int is_f32,
-                         const double *dinv, const double *b, double *x,
-                         double *r, double *d0, double *d1, int degree,
-                         double lmin, double lmax, int zero_init,
-                         void *stream) {
+                         const void *indices, const void *data, int fmt,
+                         const void *dinv, const void *b, void *x, void *r,
+                         void *d0, void *d1, int degree, double lmin, double lmax,
+                         int zero_init, void *stream) {
   FP3D_REQUIRE(n > 0 && indptr && indices && data && dinv && b && x && r &&
                    d0 && d1 && degree >= 1 && lmax > lmin && lmin > 0.0,
                "amg_cheb_smooth: bad arguments");
   FP3D_REQUIRE(zero_init >= 0 && zero_init <= 2,
                "amg_cheb_smooth: zero_init must be 0, 1 or 2");
+  FP3D_REQUIRE((fmt & ~7) == 0, "amg_cheb_smooth: fmt takes bits 1, 2 and 4");
   hipStream_t s = FP3D_STREAM(stream);
-  double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-  double sigma = theta / delta;
-  double rho0 = 1.0 / sigma;
-  int rc = FP3D_OK;
-  // the last direction goes into x inside the last step (EpChebLast), or in
-  // the first step itself from x = 0 when degree == 1; only the first step
-  // from a nonzero x (which reads x in its SpMV) needs a separate x += d
-  if (zero_init == 1) {
-    hipLaunchKernelGGL(k_cheb_zero, dim3(grid_for(n)), dim3(kBlock), 0, s, n,
-                       b, dinv, 1.0 / theta, x, r, d0, degree == 1 ? 1 : 0);
-    rc = check_launch("cheb_zero");
-  } else if (zero_init == 0) {
-    rc = spmv_any(n, nnz, indptr, indices, data, is_f32, x,
-                  EpChebFirst{b, dinv, r, d0, 1.0 / theta}, s);
-  }
-  // zero_init == 2: the caller took the first step from x = 0 (x, r, d0 set,
-  // as fp3d_amg_restrict_start does)
-  if (rc) return rc;
-  double *dold = d0, *dnew = d1;
-  for (int k = 1; k < degree; ++k) {
-    double rho1 = 1.0 / (2.0 * sigma - rho0);
-    double c1 = rho1 * rho0, c2 = 2.0 * rho1 / delta;
-    if (k == degree - 1)
-      rc = spmv_any(n, nnz, indptr, indices, data, is_f32, dold,
-                    EpChebLast{dinv, dold, x, r, dnew, c1, c2}, s);
-    else
-      rc = spmv_any(n, nnz, indptr, indices, data, is_f32, dold,
-                    EpChebIter{dinv, dold, x, r, dnew, c1, c2}, s);
-    if (rc) return rc;
-    rho0 = rho1;
-    double *t = dold;
-    dold = dnew;
-    dnew = t;
-  }
-  if (degree == 1 && zero_init == 0) {
-    hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dold, x);
-    return check_launch("cheb_smooth");
-  }
-  return FP3D_OK;
+  if (fmt & kFmtF32Vectors)
+    return cheb_smooth_t<float>(n, nnz, indptr, indices, data, fmt, (const float *)dinv,
+                                (const float *)b, (float *)x, (float *)r, (float *)d0,
+                                (float *)d1, degree, lmin, lmax, zero_init, s);
+  return cheb_smooth_t<double>(n, nnz, indptr, indices, data, fmt, (const double *)dinv,
+                               (const double *)b, (double *)x, (double *)r,
+                               (double *)d0, (double *)d1, degree, lmin, lmax,
+                               zero_init, s);
 }
 
 int fp3d_amg_restrict_start(int64_t nc, int64_t nnz, const int32_t *indptr,
-                            const int32_t *indices, const void *data,
-                            int is_f32, const double *r, double *bc,
-                            const double *dinv_c, int degree_c, double lmin_c,
-                            double lmax_c, double *x_c, double *r_c,
-                            double *d_c, void *stream) {
+                            const void *indices, const void *data, int fmt,
+                            const void *r, void *bc, const void *dinv_c,
+                            int degree_c, double lmin_c, double lmax_c, void *x_c,
+                            void *r_c, void *d_c, void *stream) {
   FP3D_REQUIRE(nc > 0 && indptr && indices && data && r && bc && dinv_c &&
                    x_c && r_c && d_c && degree_c >= 1 && lmax_c > lmin_c &&
                    lmin_c > 0.0,
                "amg_restrict_start: bad arguments");
+  FP3D_REQUIRE((fmt & ~7) == 0, "amg_restrict_start: fmt takes bits 1, 2 and 4");
   const double theta = 0.5 * (lmax_c + lmin_c);
-  return spmv_any(nc, nnz, indptr, indices, data, is_f32, r,
-                  EpRestrictStart{bc, dinv_c, x_c, r_c, d_c, 1.0 / theta,
-                                  degree_c == 1 ? 1 : 0},
+  const int xd = degree_c == 1 ? 1 : 0;
+  if (fmt & kFmtF32Vectors)
+    return spmv_fmt(nc, nnz, indptr, indices, data, fmt, (const float *)r,
+                    EpRestrictStart<float>{(float *)bc, (const float *)dinv_c,
+                                           (float *)x_c, (float *)r_c, (float *)d_c,
+                                           1.0 / theta, xd},
+                    FP3D_STREAM(stream));
+  return spmv_fmt(nc, nnz, indptr, indices, data, fmt, (const double *)r,
+                  EpRestrictStart<double>{(double *)bc, (const double *)dinv_c,
+                                          (double *)x_c, (double *)r_c, (double *)d_c,
+                                          1.0 / theta, xd},
                   FP3D_STREAM(stream));
 }
 
 int fp3d_amg_residual(int64_t n, int64_t nnz, const int32_t *indptr,
-                      const int32_t *indices, const void *data, int is_f32,
-                      const double *b, const double *x, double *r,
-                      void *stream) {
+                      const void *indices, const void *data, int fmt,
+                      const void *b, const void *x, void *r, void *stream) {
   FP3D_REQUIRE(n > 0 && indptr && indices && data && b && x && r,
                "amg_residual: bad arguments");
-  return spmv_any(n, nnz, indptr, indices, data, is_f32, x, EpResidual{b, r},
-                  FP3D_STREAM(stream));
+  FP3D_REQUIRE((fmt & ~7) == 0, "amg_residual: fmt takes bits 1, 2 and 4");
+  if (fmt & kFmtF32Vectors)
+    return spmv_fmt(n, nnz, indptr, indices, data, fmt, (const float *)x,
+                    EpResidual<float>{(const float *)b, (float *)r}, FP3D_STREAM(stream));
+  return spmv_fmt(n, nnz, indptr, indices, data, fmt, (const double *)x,
+                  EpResidual<double>{(const double *)b, (double *)r}, FP3D_STREAM(stream));
 }
 
 int fp3d_amg_restrict(int64_t nc, int64_t nnz, const int32_t *indptr,
-                      const int32_t *indices, const void *data, int is_f32,
-                      const double *r, double *bc, void *stream) {
+                      const void *indices, const void *data, int fmt,
+                      const void *r, void *bc, void *stream) {
   FP3D_REQUIRE(nc > 0 && indptr && indices && data && r && bc,
                "amg_restrict: bad arguments");
-  return spmv_any(nc, nnz, indptr, indices, data, is_f32, r, EpStore{bc},
-                  FP3D_STREAM(stream));
+  FP3D_REQUIRE((fmt & ~7) == 0, "amg_restrict: fmt takes bits 1, 2 and 4");
+  if (fmt & kFmtF32Vectors)
+    return spmv_fmt(nc, nnz, indptr, indices, data, fmt, (const float *)r,
+                    EpStore<float>{(float *)bc}, FP3D_STREAM(stream));
+  return spmv_fmt(nc, nnz, indptr, indices, data, fmt, (const double *)r,
+                  EpStore<double>{(double *)bc}, FP3D_STREAM(stream));
 }
 
 int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
-                         const int32_t *indices, const void *data, int is_f32,
-                         const double *xc, double *x, void *stream) {
+                         const void *indices, const void *data, int fmt,
+                         const void *xc, void *x, void *stream) {
   FP3D_REQUIRE(n > 0 && indptr && indices && data && xc && x,
                "amg_prolong_add: bad arguments");
-  return spmv_any(n, nnz, indptr, indices, data, is_f32, xc, EpAdd{x},
-                  FP3D_STREAM(stream));
+  FP3D_REQUIRE((fmt & ~7) == 0, "amg_prolong_add: fmt takes bits 1, 2 and 4");
+  if (fmt & kFmtF32Vectors)
+    return spmv_fmt(n, nnz, indptr, indices, data, fmt, (const float *)xc,
+                    EpAdd<float>{(float *)x}, FP3D_STREAM(stream));
+  return spmv_fmt(n, nnz, indptr, indices, data, fmt, (const double *)xc,
+                  EpAdd<double>{(double *)x}, FP3D_STREAM(stream));
+}
+
+int fp3d_dense_matvec_v(int fmt, int64_t n, const double *M, const void *b, void *x,
+                        void *stream) {
+  FP3D_REQUIRE(n > 0 && M && b && x, "dense_matvec_v: bad arguments");
+  FP3D_REQUIRE((fmt & ~kFmtF32Vectors) == 0, "dense_matvec_v: fmt takes bit 2 only");
+  const int64_t rows_per_block = kBlock / 64;
+  const unsigned grid = (unsigned)((n + rows_per_block - 1) / rows_per_block);
+  if (fmt & kFmtF32Vectors)
+    hipLaunchKernelGGL(k_dense_mv<float>, dim3(grid), dim3(kBlock), 0,
+                       FP3D_STREAM(stream), n, M, (const float *)b, (float *)x);
+  else
+    hipLaunchKernelGGL(k_dense_mv<double>, dim3(grid), dim3(kBlock), 0,
+                       FP3D_STREAM(stream), n, M, (const double *)b, (double *)x);
+  return check_launch("dense_matvec_v");
 }
 
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
                       void *stream) {
   FP3D_REQUIRE(n > 0 && M && b && x, "dense_matvec: bad arguments");
   const int64_t rows_per_block = kBlock / 64;
-  hipLaunchKernelGGL(k_dense_mv, dim3((n + rows_per_block - 1) / rows_per_block),
+  hipLaunchKernelGGL(k_dense_mv<double>, dim3((n + rows_per_block - 1) / rows_per_block),
                      dim3(kBlock), 0, FP3D_STREAM(stream), n, M, b, x);
   return check_launch("dense_matvec");
 }

@@ -163,9 +163,12 @@ def sa_setup(A: sp.csr_matrix, max_levels: int = 10, max_coarse: int = 10,
 
 class DeviceCSR:
     """CSR matrix resident in HBM (int32 indices, float64 and/or float32
-    values)."""
+    values; with compact_index a square matrix whose bandwidth fits also
+    keeps its columns as int16 offsets from the row, which the V-cycle
+    kernels read instead)."""
 
-    def __init__(self, M: sp.csr_matrix, f32: bool = False, keep_f64: bool = True):
+    def __init__(self, M: sp.csr_matrix, f32: bool = False, keep_f64: bool = True,
+                 compact_index: bool = False):
         M = M.tocsr()
         self.shape = M.shape
         self.n = M.shape[0]
@@ -176,10 +179,25 @@ class DeviceCSR:
             if (keep_f64 or not f32) else None
         self.data32 = to_dev(M.data.astype(np.float32), torch.float32) if f32 else None
         self.is_f32 = 1 if f32 else 0
+        self.indices16 = None
+        if compact_index and M.shape[0] == M.shape[1] and self.nnz:
+            from ..kernels.store import csr_offsets16
+
+            self.indices16 = csr_offsets16(M.indptr, M.indices)
 
     @property
     def values(self):
         return self.data32 if self.is_f32 else self.data
+
+    @property
+    def fmt(self) -> int:
+        """Matrix format bits of the V-cycle kernels (1 float32 values,
+        4 int16 row-relative columns)."""
+        return self.is_f32 | (4 if self.indices16 is not None else 0)
+
+    @property
+    def index_array(self):
+        return self.indices16 if self.indices16 is not None else self.indices
 
     def matvec(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
         if self.is_f32:
@@ -194,39 +212,52 @@ class DeviceCSR:
 
 
 class _DevLevel:
-    def __init__(self, h: HostLevel, f32: bool, degree: int, eig_ratio: float):
+    def __init__(self, h: HostLevel, f32: bool, degree: int, eig_ratio: float,
+                 v32: bool = False, compact_index: bool = True):
         self.n = h.A.shape[0]
-        self.A = DeviceCSR(h.A, f32=f32, keep_f64=not f32)
+        self.A = DeviceCSR(h.A, f32=f32, keep_f64=not f32, compact_index=compact_index)
         self.P = DeviceCSR(h.P, f32=f32, keep_f64=not f32) if h.P is not None else None
         self.R = DeviceCSR(h.R, f32=f32, keep_f64=not f32) if h.R is not None else None
-        self.dinv = to_dev(h.dinv, torch.float64)
+        vt = torch.float32 if v32 else torch.float64
+        self.dinv = to_dev(h.dinv, torch.float64).to(vt)
         self.lmax = 1.1 * h.rho
         self.lmin = self.lmax / eig_ratio
         self.degree = degree
         n = self.n
-        self.x = empty(n)
-        self.b = empty(n)
-        self.r = empty(n)
-        self.d0 = empty(n)
-        self.d1 = empty(n)
+        self.x = empty(n, vt)
+        self.b = empty(n, vt)
+        self.r = empty(n, vt)
+        self.d0 = empty(n, vt)
+        self.d1 = empty(n, vt)
 
 
 class AMGHierarchy:
     """Device V-cycle of a host-built SA hierarchy (module docstring).
 
     `precision` "f64" or "f32" selects the matrix value type of every level
-    (the coarsest dense solve is float64 in both)."""
+    and, by default, the type of the level vectors (`vectors`); the
+    arithmetic of every step and the coarsest dense solve are float64 in
+    both.  compact_index: square levels whose bandwidth fits keep int16
+    row-relative columns for the V-cycle (the same sums, half the index
+    bytes)."""
 
     def __init__(self, A: sp.csr_matrix, precision: str = "f64",
                  degree: int = 2, eig_ratio: float = 20.0, max_levels: int = 10,
-                 max_coarse: int = 10, seed: int = 0, use_graph: bool = True):
+                 max_coarse: int = 10, seed: int = 0, use_graph: bool = True,
+                 vectors: Optional[str] = None, compact_index: bool = True):
         if precision not in ("f64", "f32"):
             raise ValueError(f"unknown AMG precision {precision!r}")
+        vectors = precision if vectors is None else vectors
+        if vectors not in ("f64", "f32"):
+            raise ValueError(f"unknown AMG vector precision {vectors!r}")
         host, coarse = sa_setup(A, max_levels=max_levels, max_coarse=max_coarse,
                                 seed=seed)
         f32 = precision == "f32"
         self.precision = precision
-        self.levels = [_DevLevel(h, f32, degree, eig_ratio) for h in host]
+        self.v32 = vectors == "f32"
+        self.vfmt = 2 if self.v32 else 0
+        self.levels = [_DevLevel(h, f32, degree, eig_ratio, self.v32, compact_index)
+                       for h in host]
         self.coarse = to_dev(np.ascontiguousarray(coarse), torch.float64)
         self.shapes = [lv.n for lv in self.levels]
         self.nnzs = [lv.A.nnz for lv in self.levels]
@@ -237,6 +268,10 @@ class AMGHierarchy:
         self._graphs = {}
         self._gb = empty(self.levels[0].n)
         self._gx = empty(self.levels[0].n)
+        if self.v32:
+            # float32 copies of the float64 right-hand side and solution
+            self._b32 = empty(self.levels[0].n, torch.float32)
+            self._x32 = empty(self.levels[0].n, torch.float32)
         self._eager_calls = 0
 
     @property
@@ -245,55 +280,69 @@ class AMGHierarchy:
 
     def _vcycle(self, l: int, b: torch.Tensor, x: torch.Tensor,
                 started: bool = False) -> None:
-        """V-cycle from level l on b into x (x = 0 initially); started: the
-        restriction into this level already took the smoother's first step."""
+        """V-cycle from level l on b into x (x = 0 initially; both of the
+        level vector type); started: the restriction into this level already
+        took the smoother's first step."""
         L = self.levels[l]
         s = stream_handle()
+        vf = self.vfmt
         if l == len(self.levels) - 1:
-            _native.call("fp3d_dense_matvec", L.n, ptr(self.coarse), ptr(b),
+            _native.call("fp3d_dense_matvec_v", vf, L.n, ptr(self.coarse), ptr(b),
                          ptr(x), s)
             return
         A = L.A
         _native.call("fp3d_amg_cheb_smooth", L.n, A.nnz, ptr(A.indptr),
-                     ptr(A.indices), ptr(A.values), A.is_f32, ptr(L.dinv),
+                     ptr(A.index_array), ptr(A.values), A.fmt | vf, ptr(L.dinv),
                      ptr(b), ptr(x), ptr(L.r), ptr(L.d0), ptr(L.d1), L.degree,
                      L.lmin, L.lmax, 2 if started else 1, s)
         _native.call("fp3d_amg_residual", L.n, A.nnz, ptr(A.indptr),
-                     ptr(A.indices), ptr(A.values), A.is_f32, ptr(b), ptr(x),
+                     ptr(A.index_array), ptr(A.values), A.fmt | vf, ptr(b), ptr(x),
                      ptr(L.r), s)
         Lc = self.levels[l + 1]
         R = L.R
         coarse_next = l + 1 == len(self.levels) - 1
         if coarse_next:
             _native.call("fp3d_amg_restrict", R.n, R.nnz, ptr(R.indptr),
-                         ptr(R.indices), ptr(R.values), R.is_f32, ptr(L.r),
+                         ptr(R.index_array), ptr(R.values), R.fmt | vf, ptr(L.r),
                          ptr(Lc.b), s)
         else:
             # the restriction's epilogue also takes the coarse smoother's
             # first step from x = 0 (one launch fewer per level)
             _native.call("fp3d_amg_restrict_start", R.n, R.nnz, ptr(R.indptr),
-                         ptr(R.indices), ptr(R.values), R.is_f32, ptr(L.r),
+                         ptr(R.index_array), ptr(R.values), R.fmt | vf, ptr(L.r),
                          ptr(Lc.b), ptr(Lc.dinv), Lc.degree, Lc.lmin, Lc.lmax,
                          ptr(Lc.x), ptr(Lc.r), ptr(Lc.d0), s)
         self._vcycle(l + 1, Lc.b, Lc.x, started=not coarse_next)
         P = L.P
         _native.call("fp3d_amg_prolong_add", P.n, P.nnz, ptr(P.indptr),
-                     ptr(P.indices), ptr(P.values), P.is_f32, ptr(Lc.x), ptr(x),
+                     ptr(P.index_array), ptr(P.values), P.fmt | vf, ptr(Lc.x), ptr(x),
                      s)
         _native.call("fp3d_amg_cheb_smooth", L.n, A.nnz, ptr(A.indptr),
-                     ptr(A.indices), ptr(A.values), A.is_f32, ptr(L.dinv),
+                     ptr(A.index_array), ptr(A.values), A.fmt | vf, ptr(L.dinv),
                      ptr(b), ptr(x), ptr(L.r), ptr(L.d0), ptr(L.d1), L.degree,
                      L.lmin, L.lmax, 0, s)
 
+    def _vcycle_top(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        """One V-cycle on the float64 b into the float64 x (through the
+        float32 copies when the level vectors are float32)."""
+        if not self.v32:
+            self._vcycle(0, b, x)
+            return
+        n = self.levels[0].n
+        s = stream_handle()
+        _native.call("fp3d_scale_to_f32", n, 1.0, ptr(b), ptr(self._b32), s)
+        self._vcycle(0, self._b32, self._x32)
+        _native.call("fp3d_f32_to_f64", n, ptr(self._x32), ptr(x), s)
+
     def apply_eager(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
         """x = M b with one launch per kernel (no graph)."""
-        self._vcycle(0, b, x)
+        self._vcycle_top(b, x)
         return x
 
     def _capture(self, b: torch.Tensor, x: torch.Tensor):
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self._vcycle(0, b, x)
+            self._vcycle_top(b, x)
         return g
 
     # graphs captured directly on callers' (b, x) buffers; a Krylov solve

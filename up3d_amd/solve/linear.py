@@ -77,6 +77,39 @@ def scale(a: float, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
     return y
 
 
+_GMRES_WS: "OrderedDict" = None
+
+
+def _gmres_workspace(n: int, restart: int, f32: bool, device):
+    """Work buffers of gmres_dev for one (n, restart, basis type), kept
+    between calls (at most four shapes): the same buffers every call, so a
+    preconditioner that replays HIP graphs captured on its callers' buffers
+    keeps replaying them instead of copying in and out."""
+    global _GMRES_WS
+    from collections import OrderedDict
+
+    if _GMRES_WS is None:
+        _GMRES_WS = OrderedDict()
+    key = (str(device), int(n), int(restart), bool(f32))
+    ws = _GMRES_WS.get(key)
+    if ws is None:
+        nout = 2 * restart + 4
+        ws = {"V": torch.empty((restart + 1, n),
+                               dtype=torch.float32 if f32 else torch.float64, device=device),
+              "v64": empty(n) if f32 else None, "tmp": empty(n), "r": empty(n),
+              "w": [empty(n), empty(n)], "outs": [empty(nout), empty(nout)],
+              "hbuf": [torch.empty(nout, dtype=torch.float64, pin_memory=True)
+                       for _ in range(2)],
+              "evs": [torch.cuda.Event(), torch.cuda.Event()],
+              "red": _Red(restart + 1)}
+        _GMRES_WS[key] = ws
+        while len(_GMRES_WS) > 4:
+            _GMRES_WS.popitem(last=False)
+    else:
+        _GMRES_WS.move_to_end(key)
+    return ws
+
+
 def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
               x0: Optional[torch.Tensor] = None, rtol: float = 1e-5,
               atol: float = 0.0, restart: int = 20, maxiter: Optional[int] = None,
@@ -101,9 +134,10 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
     if maxiter is None:
         maxiter = 10 * n
     restart = int(min(restart, n))
-    red = _Red(restart + 1)
-    tmp = empty(n)
-    r = empty(n)
+    gw = _gmres_workspace(n, restart, f32, b.device)
+    red = gw["red"]
+    tmp = gw["tmp"]
+    r = gw["r"]
 
     def prec(src, dst):
         if psolve is None:
@@ -122,16 +156,12 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
     ptol_max_factor = 1.0
     ptol = Mb_nrm2 * min(ptol_max_factor, atol / bnrm2)
     presid = 0.0
-    if f32:
-        V = torch.empty((restart + 1, n), dtype=torch.float32, device=b.device)
-        v64 = empty(n)
-    else:
-        V = empty((restart + 1, n))
-    w_buf = [empty(n), empty(n)]
-    nout = 2 * restart + 4
-    outs = [empty(nout), empty(nout)]
-    hbuf = [torch.empty(nout, dtype=torch.float64, pin_memory=True) for _ in range(2)]
-    evs = [torch.cuda.Event(), torch.cuda.Event()]
+    V = gw["V"]
+    v64 = gw["v64"]
+    w_buf = gw["w"]
+    outs = gw["outs"]
+    hbuf = gw["hbuf"]
+    evs = gw["evs"]
     h = np.zeros((restart, restart + 1))
     givens = np.zeros((restart, 2))
     inner = 0
@@ -366,8 +396,11 @@ class NewtonOperator:
         st = self.store
         vals = self.data32 if self.is_f32 else self.data
         use_t3 = self.n_active > 0
+        i16 = getattr(st, "indices16", None)
+        fmt = self.is_f32 | (4 if i16 is not None else 0)
         _native.call("fp3d_csr_spmv_term3", st.n_nodes, st.nnz, ptr(st.indptr),
-                     ptr(st.indices), ptr(vals), self.is_f32, ptr(st.el),
+                     ptr(i16 if i16 is not None else st.indices), ptr(vals), fmt,
+                     ptr(st.el),
                      ptr(self.upstream) if use_t3 else None,
                      ptr(self.t3) if use_t3 else None, self.n_active,
                      st.n_tets, ptr(self.free_mask), ptr(x), ptr(y),
