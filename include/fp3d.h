@@ -369,6 +369,36 @@ int fp3d_multi_dot_norm_f32v(int64_t n, int nv, const float *V, int64_t ld,
  * caller. */
 int fp3d_multi_dot_norm_noalloc(int64_t n, int nv, const double *V, int64_t ld,
                                 const double *w, double *out, void *stream);
+/* Batched Arnoldi steps (no reference counterpart: the reference solves one
+ * flow condition per call, pyfp3d/solve/newton.py:1144-1307).  One Krylov
+ * step of each of nb GMRES solves on the same mesh, one launch per kernel for
+ * all of them (blockIdx.y = solve).  `jobs` is a device array of nb job
+ * records of fp3d_batch_job_bytes() = 128 bytes, sixteen 8-byte words each:
+ *   0 V (basis rows, float64 or float32)   1 ld (row stride)
+ *   2 nv (= col + 1)                        3 w (float64, orthogonalised in place)
+ *   4 out (float64[3 + 2 nv]: [0] ||w|| after, [1] max|w|, [2, 2+nv) h,
+ *     [2+nv] ||w||^2 before, [3+nv, 3+2nv) second pass)
+ *   5 work (fp3d_multi_dot_work_doubles(nv + 1) doubles)
+ *   6 vnext (basis row col + 1)             7 src32 (basis row col, float32 basis)
+ *   8 x (float64 matvec input)              9 y (float64 matvec output)
+ *  10 Jacobian values (fmt)                11 upstream (int32[n_tets])
+ *  12 Term-3 workspace of fp3d_term3_build 13 its active count (0: none)
+ *  14 free mask (uint8[n_rows])            15 reserved (0).
+ * batch_jacobian_matvec: widen ? x = (double) src32 : -, then y = J x as
+ * fp3d_csr_spmv_term3 (the mesh arguments are shared; max_active = the
+ * largest word 13).  batch_orthonormalise (after the caller has applied its
+ * preconditioner, w = M^-1 y): the CGS pass into out[2..], w -= V h, the
+ * DGKS-conditional second pass into out[3+nv..] (eta2), w -= V h2,
+ * out[0..1] = norms of w, vnext = w / out[0] (float32 when f32); max_nv = the
+ * largest word 2, at most 256.  Per solve the same kernel bodies on the same
+ * grids as the single-solve calls: the same bits. */
+int fp3d_batch_job_bytes(void);
+int fp3d_batch_jacobian_matvec(int nb, const void *jobs, int64_t max_active,
+                               int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                               const void *indices, int fmt, const int32_t *elements,
+                               int64_t n_tets, int widen, void *stream);
+int fp3d_batch_orthonormalise(int nb, const void *jobs, int max_nv, int64_t n, int f32,
+                              double eta2, void *stream);
 /* w += sign * sum_k h[k] v_k  (h on device, nv <= 256) */
 /* Re-orthogonalisation pass of CGS2 (no reference counterpart; the
  * reference's GMRES is scipy's): out[k] = V_k . w only when the first pass

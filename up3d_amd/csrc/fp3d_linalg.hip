@@ -141,9 +141,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
 }
 
 // SpMV + Term-3 gather + Dirichlet identity rows (free_mask nullable)
+// (the body takes its block index, so the batched kernel below runs the
+// same code per instance)
 template <int G, typename T, typename TI>
-__global__ __launch_bounds__(kBlock) void k_spmv_t3(
-    int64_t n_rows, const int32_t *__restrict__ ptr,
+__device__ __forceinline__ void spmv_t3_body(
+    int64_t bx, int64_t n_rows, const int32_t *__restrict__ ptr,
     const TI *__restrict__ idx, const T *__restrict__ val,
     const double *__restrict__ x, const int32_t *__restrict__ t3ptr,
     const int32_t *__restrict__ t3ent,
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t3(
     const double *__restrict__ tval, const uint8_t *__restrict__ fmask,
     double *__restrict__ y) {
   constexpr int RPB = kBlock / G;
-  int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / G;
+  int64_t row = bx * RPB + threadIdx.x / G;
   int lane = threadIdx.x % G;
   if (row >= n_rows) return;
   int32_t j0 = ptr[row], j1 = ptr[row + 1];
@@ -171,6 +173,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv_t3(
     if (fmask && !fmask[row]) s = x[row];
     y[row] = s;
   }
+}
+template <int G, typename T, typename TI>
+__global__ __launch_bounds__(kBlock) void k_spmv_t3(
+    int64_t n_rows, const int32_t *__restrict__ ptr,
+    const TI *__restrict__ idx, const T *__restrict__ val,
+    const double *__restrict__ x, const int32_t *__restrict__ t3ptr,
+    const int32_t *__restrict__ t3ent, const double *__restrict__ W,
+    const double *__restrict__ tval, const uint8_t *__restrict__ fmask,
+    double *__restrict__ y) {
+  spmv_t3_body<G, T, TI>(blockIdx.x, n_rows, ptr, idx, val, x, t3ptr, t3ent, W, tval,
+                         fmask, y);
 }
 
 inline int lanes_for(int64_t n_rows, int64_t nnz) {
@@ -341,30 +354,27 @@ inline int rblocks(int64_t n) {
 constexpr int kMdotLds = 2048;
 constexpr int64_t kMdotSegMin = 1024;
 template <bool STAGED, typename TV>
-__global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
-                                                   const TV *__restrict__ V,
-                                                   int64_t ld,
-                                                   const double *__restrict__ w,
-                                                   double *__restrict__ part,
-                                                   const double *__restrict__ cond,
-                                                   double eta2, int64_t seg,
-                                                   int with_self) {
+__device__ __forceinline__ void mdot_p1_body(int64_t bx, int nb, int64_t n, int nv,
+                                             const TV *__restrict__ V, int64_t ld,
+                                             const double *__restrict__ w,
+                                             double *__restrict__ part,
+                                             const double *__restrict__ cond,
+                                             double eta2, int64_t seg, int with_self) {
   // STAGED reads the slice of w from LDS (a separate instantiation, so the
   // loads stay in the LDS address space rather than generic).  with_self:
   // row nv is w itself (w . w, summed exactly as a one-row call on w)
   __shared__ double ws[STAGED ? kMdotLds : 1];
-  const int nb = gridDim.x;
   if (cond) {
     double h2 = 0.0;
     for (int k = 0; k < nv; ++k) h2 += cond[k] * cond[k];
     double w2 = cond[nv];
     if (!(w2 - h2 < eta2 * w2)) {
       for (int k = threadIdx.x; k < nv; k += blockDim.x)
-        part[(int64_t)k * nb + blockIdx.x] = 0.0;
+        part[(int64_t)k * nb + bx] = 0.0;
       return;
     }
   }
-  const int64_t r0 = (int64_t)blockIdx.x * seg;
+  const int64_t r0 = (int64_t)bx * seg;
   const int64_t r1 = r0 + seg < n ? r0 + seg : n;
   const int len = (int)(r1 - r0);
   if (STAGED) {
@@ -393,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
     }
     for (; i < len; i += 64) a0 += (double)v[i] * (STAGED ? ws[i] : wg[i]);
     double t = wsum((a0 + a1) + (a2 + a3));
-    if (lane == 0) part[(int64_t)k * nb + blockIdx.x] = t;
+    if (lane == 0) part[(int64_t)k * nb + bx] = t;
   }
   if (with_self && wave == nv % nwave) {
     // the row after the last basis vector: w itself, read from global as
@@ -415,8 +425,20 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
     }
     for (; i < len; i += 64) a0 += wg[i] * (STAGED ? ws[i] : wg[i]);
     double t = wsum((a0 + a1) + (a2 + a3));
-    if (lane == 0) part[(int64_t)nv * nb + blockIdx.x] = t;
+    if (lane == 0) part[(int64_t)nv * nb + bx] = t;
   }
+}
+template <bool STAGED, typename TV>
+__global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
+                                                   const TV *__restrict__ V,
+                                                   int64_t ld,
+                                                   const double *__restrict__ w,
+                                                   double *__restrict__ part,
+                                                   const double *__restrict__ cond,
+                                                   double eta2, int64_t seg,
+                                                   int with_self) {
+  mdot_p1_body<STAGED, TV>(blockIdx.x, gridDim.x, n, nv, V, ld, w, part, cond, eta2, seg,
+                           with_self);
 }
 inline int64_t mdot_seg(int64_t n) {
   int64_t seg = (n + kReduceBlocks - 1) / kReduceBlocks;
@@ -428,25 +450,29 @@ inline int mdot_blocks(int64_t n, int64_t seg) {
 }
 // one block per k, sums the nb partials in index order (wave-parallel then
 // fixed-order combine)
-__global__ __launch_bounds__(kBlock) void k_mdot_p2(int nb,
-                                                   const double *__restrict__ part,
-                                                   double *__restrict__ out) {
+__device__ __forceinline__ void mdot_p2_body(int k, int nb,
+                                             const double *__restrict__ part,
+                                             double *__restrict__ out) {
   __shared__ double sh[kBlock / 64];
-  int k = blockIdx.x;
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(int64_t)k * nb + b];
   double t = block_sum(s, sh);
   if (threadIdx.x == 0) out[k] = t;
 }
+__global__ __launch_bounds__(kBlock) void k_mdot_p2(int nb,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ out) {
+  mdot_p2_body(blockIdx.x, nb, part, out);
+}
 
-__global__ __launch_bounds__(kBlock) void k_norms_p1(int64_t n,
-                                                    const double *__restrict__ x,
-                                                    double *__restrict__ part) {
+__device__ __forceinline__ void norms_p1_body(int64_t bx, int nb, int64_t n,
+                                              const double *__restrict__ x,
+                                              double *__restrict__ part) {
   __shared__ double sh[kBlock / 64];
   __shared__ double shm[kBlock / 64];
   double s = 0.0, m = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = bx * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)nb * blockDim.x) {
     double v = x[i];
     s += v * v;
     double a = fabs(v);
@@ -458,12 +484,17 @@ __global__ __launch_bounds__(kBlock) void k_norms_p1(int64_t n,
   if (threadIdx.x == 0) {
     double mm = 0.0;
     for (int k = 0; k < kBlock / 64; ++k) mm = shm[k] > mm ? shm[k] : mm;
-    part[2 * blockIdx.x] = t;
-    part[2 * blockIdx.x + 1] = mm;
+    part[2 * bx] = t;
+    part[2 * bx + 1] = mm;
   }
 }
-__global__ __launch_bounds__(kBlock) void k_norms_p2(
-    int nb, const double *__restrict__ part, double *__restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_norms_p1(int64_t n,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ part) {
+  norms_p1_body(blockIdx.x, gridDim.x, n, x, part);
+}
+__device__ __forceinline__ void norms_p2_body(int nb, const double *__restrict__ part,
+                                              double *__restrict__ out) {
   __shared__ double sh[kBlock / 64];
   __shared__ double shm[kBlock / 64];
   double s = 0.0, m = 0.0;
@@ -481,15 +512,17 @@ __global__ __launch_bounds__(kBlock) void k_norms_p2(
     out[1] = mm;
   }
 }
+__global__ __launch_bounds__(kBlock) void k_norms_p2(
+    int nb, const double *__restrict__ part, double *__restrict__ out) {
+  norms_p2_body(nb, part, out);
+}
 
 // w -= sum_k h[k] V_k   (h in LDS; V float64 or float32, arithmetic float64)
 template <typename TV>
-__global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
-                                                 const TV *__restrict__ V,
-                                                 int64_t ld,
-                                                 const double *__restrict__ h,
-                                                 double *__restrict__ w,
-                                                 double sign) {
+__device__ __forceinline__ void maxpy_body(int64_t bx, int64_t n, int nv,
+                                           const TV *__restrict__ V, int64_t ld,
+                                           const double *__restrict__ h,
+                                           double *__restrict__ w, double sign) {
   __shared__ double hs[256];
   __shared__ int any_nz;
   if (threadIdx.x == 0) any_nz = 0;
@@ -501,11 +534,20 @@ __global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
   __syncthreads();
   // all-zero coefficients (a skipped re-orthogonalisation): w unchanged
   if (!any_nz) return;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = bx * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
   for (int k = 0; k < nv; ++k) acc += hs[k] * (double)V[(int64_t)k * ld + i];
   w[i] += sign * acc;
+}
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_maxpy(int64_t n, int nv,
+                                                 const TV *__restrict__ V,
+                                                 int64_t ld,
+                                                 const double *__restrict__ h,
+                                                 double *__restrict__ w,
+                                                 double sign) {
+  maxpy_body<TV>(blockIdx.x, n, nv, V, ld, h, w, sign);
 }
 
 // x = M b, M dense row-major: one wave per row, lanes stride the row
@@ -860,19 +902,126 @@ __global__ __launch_bounds__(kBlock) void k_t3_fill(
   }
 }
 // t_e = sum_k (grad_u . B_u,k) x[elements[u,k]] for the active elements
-__global__ void k_t3_tval(int64_t n_active, const int32_t *__restrict__ act,
-                          const int32_t *__restrict__ elements,
-                          const int32_t *__restrict__ up,
-                          const double *__restrict__ Cc,
-                          const double *__restrict__ x,
-                          double *__restrict__ tval) {
-  int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void t3_tval_body(int64_t bx, int64_t n_active,
+                                             const int32_t *__restrict__ act,
+                                             const int32_t *__restrict__ elements,
+                                             const int32_t *__restrict__ up,
+                                             const double *__restrict__ Cc,
+                                             const double *__restrict__ x,
+                                             double *__restrict__ tval) {
+  int64_t a = bx * blockDim.x + threadIdx.x;
   if (a >= n_active) return;
   int64_t e = act[a];
   int64_t u = up[e];
   const int4 nd = *reinterpret_cast<const int4 *>(elements + 4 * u);
   tval[e] = Cc[4 * e] * x[nd.x] + Cc[4 * e + 1] * x[nd.y] +
             Cc[4 * e + 2] * x[nd.z] + Cc[4 * e + 3] * x[nd.w];
+}
+__global__ void k_t3_tval(int64_t n_active, const int32_t *__restrict__ act,
+                          const int32_t *__restrict__ elements,
+                          const int32_t *__restrict__ up,
+                          const double *__restrict__ Cc,
+                          const double *__restrict__ x,
+                          double *__restrict__ tval) {
+  t3_tval_body(blockIdx.x, n_active, act, elements, up, Cc, x, tval);
+}
+
+// ------------------------------------------------ batched Arnoldi steps
+// One Krylov step of each of nb GMRES solves on the same mesh, one launch
+// per kernel for all of them: blockIdx.y selects the solve, whose buffers
+// come from its entry of a device-resident job table; every block then runs
+// the same body, with the same grid, as the solve's own launch would, so
+// each solve gets the same bits as when it runs alone.
+struct BatchJob {          // 16 x 8 bytes; layout shared with the host
+  const void *V;           // Krylov basis (float64 or float32 rows)
+  int64_t ld;              // its row stride
+  int64_t nv;              // basis vectors to orthogonalise against (col + 1)
+  double *w;               // M^-1 A v_col, orthogonalised in place
+  double *out;             // [0] ||w||, [1] max|w|, [2, 2+nv) h, [2+nv] ||w||^2,
+                           // [3+nv, 3+2nv) second pass
+  double *work;            // reduction partials
+  void *vnext;             // basis row col + 1
+  const float *src32;      // basis row col (float32 basis) or null
+  double *x;               // Jacobian matvec input
+  double *y;               // Jacobian matvec output
+  const void *vals;        // Jacobian values (float64 or float32)
+  const int32_t *up;       // upstream elements (Term 3)
+  const void *t3;          // Term-3 workspace
+  int64_t n_active;        // active Term-3 elements (0: no Term 3)
+  const uint8_t *fmask;    // free mask
+  int64_t reserved;
+};
+static_assert(sizeof(BatchJob) == 128, "BatchJob layout");
+constexpr int kMaxBatch = 1024;
+
+__global__ void k_b_widen(int64_t n, const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) j.x[i] = (double)j.src32[i];
+}
+__global__ void k_b_t3_tval(const BatchJob *__restrict__ jobs,
+                            const int32_t *__restrict__ elements, size_t off_act,
+                            size_t off_cc, size_t off_tval) {
+  const BatchJob &j = jobs[blockIdx.y];
+  if (j.n_active <= 0) return;
+  const char *base = (const char *)j.t3;
+  t3_tval_body(blockIdx.x, j.n_active, (const int32_t *)(base + off_act), elements, j.up,
+               (const double *)(base + off_cc), j.x, (double *)(base + off_tval));
+}
+template <int G, typename T, typename TI>
+__global__ __launch_bounds__(kBlock) void k_b_spmv_t3(
+    int64_t n_rows, const int32_t *__restrict__ ptr, const TI *__restrict__ idx,
+    const BatchJob *__restrict__ jobs, size_t off_rowptr, size_t off_ent, size_t off_wq,
+    size_t off_tval) {
+  const BatchJob &j = jobs[blockIdx.y];
+  const char *base = (const char *)j.t3;
+  const bool t3 = j.n_active > 0;
+  spmv_t3_body<G, T, TI>(blockIdx.x, n_rows, ptr, idx, (const T *)j.vals, j.x,
+                         t3 ? (const int32_t *)(base + off_rowptr) : nullptr,
+                         t3 ? (const int32_t *)(base + off_ent) : nullptr,
+                         t3 ? (const double *)(base + off_wq) : nullptr,
+                         t3 ? (const double *)(base + off_tval) : nullptr, j.fmask, j.y);
+}
+// pass 0: h = V^T w and ||w||^2 into out[2..]; pass 1: the conditional
+// second CGS pass into out[3 + nv..]
+template <bool STAGED, typename TV>
+__global__ __launch_bounds__(kBlock) void k_b_mdot_p1(int64_t n, int64_t seg, int pass,
+                                                     double eta2,
+                                                     const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  mdot_p1_body<STAGED, TV>(blockIdx.x, gridDim.x, n, (int)j.nv, (const TV *)j.V, j.ld,
+                           j.w, j.work, pass ? j.out + 2 : nullptr, eta2, seg,
+                           pass ? 0 : 1);
+}
+__global__ __launch_bounds__(kBlock) void k_b_mdot_p2(int nb, int pass,
+                                                     const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  const int rows = (int)j.nv + (pass ? 0 : 1);
+  if ((int)blockIdx.x >= rows) return;
+  mdot_p2_body(blockIdx.x, nb, j.work, pass ? j.out + 3 + j.nv : j.out + 2);
+}
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_b_maxpy(int64_t n, int pass,
+                                                   const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  maxpy_body<TV>(blockIdx.x, n, (int)j.nv, (const TV *)j.V, j.ld,
+                 pass ? j.out + 3 + j.nv : j.out + 2, j.w, -1.0);
+}
+__global__ __launch_bounds__(kBlock) void k_b_norms_p1(int64_t n,
+                                                      const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  norms_p1_body(blockIdx.x, gridDim.x, n, j.w, j.work);
+}
+__global__ __launch_bounds__(kBlock) void k_b_norms_p2(int nb,
+                                                      const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  norms_p2_body(nb, j.work, j.out);
+}
+template <typename TY>
+__global__ void k_b_scale_inv(int64_t n, const BatchJob *__restrict__ jobs) {
+  const BatchJob &j = jobs[blockIdx.y];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ((TY *)j.vnext)[i] = (TY)((1.0 / j.out[0]) * j.w[i]);
 }
 
 }  // namespace fp3d
@@ -1154,6 +1303,98 @@ int fp3d_csr_spmv_term3(int64_t n_rows, int64_t nnz, const int32_t *indptr,
 #undef T3_SWITCH
 #undef T3_LAUNCH
   return check_launch("csr_spmv_term3");
+}
+
+int fp3d_batch_job_bytes(void) { return (int)sizeof(BatchJob); }
+
+int fp3d_batch_jacobian_matvec(int nb, const void *jobs, int64_t max_active,
+                               int64_t n_rows, int64_t nnz, const int32_t *indptr,
+                               const void *indices, int fmt, const int32_t *elements,
+                               int64_t n_tets, int widen, void *stream) {
+  FP3D_REQUIRE(nb >= 1 && nb <= kMaxBatch, "batch_jacobian_matvec: nb in [1, 1024]");
+  FP3D_REQUIRE(jobs && n_rows > 0 && indptr && indices, "batch_jacobian_matvec: NULL");
+  FP3D_REQUIRE((fmt & ~(kFmtF32Values | kFmtIdx16)) == 0,
+               "batch_jacobian_matvec: fmt takes bits 1 and 4");
+  FP3D_REQUIRE(max_active >= 0 && max_active <= n_tets,
+               "batch_jacobian_matvec: max_active in [0, n_tets]");
+  FP3D_REQUIRE(max_active == 0 || (elements && FP3D_ALIGNED16(elements)),
+               "batch_jacobian_matvec: Term 3 needs 16-byte aligned elements");
+  hipStream_t s = FP3D_STREAM(stream);
+  const BatchJob *J = (const BatchJob *)jobs;
+  if (widen)
+    hipLaunchKernelGGL(k_b_widen, dim3(grid_for(n_rows), nb), dim3(kBlock), 0, s, n_rows,
+                       J);
+  const T3Layout L = t3_layout(n_tets, n_rows);
+  if (max_active > 0)
+    hipLaunchKernelGGL(k_b_t3_tval, dim3(grid_for(max_active), nb), dim3(kBlock), 0, s,
+                       J, elements, L.act, L.Cc, L.tval);
+  int G = lanes_for(n_rows, nnz);
+  unsigned grid = grid_for(n_rows, kBlock / G);
+#define B3_LAUNCH(GG, T, TI)                                                       \
+  hipLaunchKernelGGL((k_b_spmv_t3<GG, T, TI>), dim3(grid, nb), dim3(kBlock), 0, s,  \
+                     n_rows, indptr, (const TI *)indices, J, L.rowptr, L.ent, L.Wq, \
+                     L.tval)
+#define B3_SWITCH(T, TI)                  \
+  switch (G) {                            \
+    case 4: B3_LAUNCH(4, T, TI); break;   \
+    case 8: B3_LAUNCH(8, T, TI); break;   \
+    case 16: B3_LAUNCH(16, T, TI); break; \
+    case 32: B3_LAUNCH(32, T, TI); break; \
+    default: B3_LAUNCH(64, T, TI); break; \
+  }
+  const bool v32 = fmt & kFmtF32Values, i16 = fmt & kFmtIdx16;
+  if (v32 && i16) {
+    B3_SWITCH(float, int16_t)
+  } else if (v32) {
+    B3_SWITCH(float, int32_t)
+  } else if (i16) {
+    B3_SWITCH(double, int16_t)
+  } else {
+    B3_SWITCH(double, int32_t)
+  }
+#undef B3_SWITCH
+#undef B3_LAUNCH
+  return check_launch("batch_jacobian_matvec");
+}
+
+}  // extern "C"
+
+template <typename TV>
+static void batch_ortho_t(int nb, const BatchJob *J, int max_nv, int64_t n, double eta2,
+                          hipStream_t s) {
+  const int64_t seg = mdot_seg(n);
+  const int mb = mdot_blocks(n, seg);
+  const unsigned gn = grid_for(n);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (seg <= kMdotLds)
+      hipLaunchKernelGGL((k_b_mdot_p1<true, TV>), dim3(mb, nb), dim3(kBlock), 0, s, n, seg,
+                         pass, eta2, J);
+    else
+      hipLaunchKernelGGL((k_b_mdot_p1<false, TV>), dim3(mb, nb), dim3(kBlock), 0, s, n, seg,
+                         pass, eta2, J);
+    hipLaunchKernelGGL(k_b_mdot_p2, dim3(max_nv + 1, nb), dim3(kBlock), 0, s, mb, pass, J);
+    hipLaunchKernelGGL((k_b_maxpy<TV>), dim3(gn, nb), dim3(kBlock), 0, s, n, pass, J);
+  }
+  const int rb = rblocks(n);
+  hipLaunchKernelGGL(k_b_norms_p1, dim3(rb, nb), dim3(kBlock), 0, s, n, J);
+  hipLaunchKernelGGL(k_b_norms_p2, dim3(1, nb), dim3(kBlock), 0, s, rb, J);
+  hipLaunchKernelGGL((k_b_scale_inv<TV>), dim3(gn, nb), dim3(kBlock), 0, s, n, J);
+}
+
+extern "C" {
+
+int fp3d_batch_orthonormalise(int nb, const void *jobs, int max_nv, int64_t n, int f32,
+                              double eta2, void *stream) {
+  FP3D_REQUIRE(nb >= 1 && nb <= kMaxBatch, "batch_orthonormalise: nb in [1, 1024]");
+  FP3D_REQUIRE(jobs && n > 0, "batch_orthonormalise: NULL");
+  FP3D_REQUIRE(max_nv >= 1 && max_nv <= 256, "batch_orthonormalise: max_nv in [1, 256]");
+  FP3D_REQUIRE(eta2 > 0.0 && eta2 <= 1.0, "batch_orthonormalise: eta2 in (0, 1]");
+  hipStream_t s = FP3D_STREAM(stream);
+  if (f32)
+    batch_ortho_t<float>(nb, (const BatchJob *)jobs, max_nv, n, eta2, s);
+  else
+    batch_ortho_t<double>(nb, (const BatchJob *)jobs, max_nv, n, eta2, s);
+  return check_launch("batch_orthonormalise");
 }
 
 int fp3d_csr_mask_dirichlet(int64_t n_rows, const int32_t *indptr,

@@ -1,0 +1,220 @@
+"""Lockstep execution of many Newton solves (the batched sweep of flow
+conditions; no reference counterpart: the reference solves one condition
+per call, pyfp3d/solve/newton.py:1144-1307).
+
+Every solve is the single-condition driver itself run as a coroutine
+(newton_steps / transonic_steps with stepped=True): it yields an
+EvalRequest for each residual evaluation and an ArnoldiStep for each Krylov
+step instead of running them.  The executor advances all live solves one
+request at a time, in rounds:
+
+  * the Arnoldi steps of a round are queued back to back -- through the
+    batched C-ABI (one launch per kernel for all instances of one mesh:
+    the Jacobian matvec, the orthogonalisation, the norm and the
+    normalisation), with only the V-cycles left per instance (each
+    instance has its own hierarchy) -- then ONE stream synchronisation
+    serves all of them;
+  * the residual evaluations of a round run per instance.
+
+A solve that returns leaves the round structure (the convergence mask is
+the set of live coroutines).  Each solve queues exactly the device work, in
+the same order, that it queues when run alone, so its result is the same
+bits (tests/test_gpu_batch.py).
+"""
+
+from __future__ import annotations
+
+import time
+from collections import defaultdict
+from typing import Dict, Hashable, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .linear import ArnoldiStep, batch_key, batched_arnoldi
+from .newton import EvalRequest
+
+
+class BatchExecutor:
+    """Runs coroutines that yield EvalRequest / ArnoldiStep to completion.
+
+    batched=True queues the Arnoldi steps of instances that share a mesh
+    through the batched entry points (linear.batched_arnoldi); False queues
+    each step by itself (the same kernels, one instance per launch).
+    Statistics: rounds, Arnoldi steps and evaluations executed, and the
+    largest number of steps served by one synchronisation."""
+
+    def __init__(self, batched: bool = True):
+        self.batched = bool(batched)
+        self.n_rounds = 0
+        self.n_steps = 0
+        self.n_evals = 0
+        self.max_width = 0
+        self.n_batched_launch_groups = 0
+
+    def run(self, coros: Dict[Hashable, object],
+            on_done=None) -> Dict[Hashable, object]:
+        """Advance every coroutine to its return value; on_done(key, value)
+        is called as each finishes (it may add new coroutines with
+        self.add)."""
+        self._coros = dict(coros)
+        self._pending: Dict[Hashable, object] = {}
+        self._results: Dict[Hashable, object] = {}
+        self._on_done = on_done
+        for k in list(self._coros):
+            self._advance(k, None)
+        while self._pending:
+            self._round()
+        return self._results
+
+    def add(self, key, coro) -> None:
+        """Start another coroutine (from on_done)."""
+        if key in self._coros:
+            raise KeyError(f"coroutine {key!r} already running")
+        self._coros[key] = coro
+        self._advance(key, None)
+
+    def _advance(self, key, value) -> None:
+        try:
+            req = self._coros[key].send(value)
+        except StopIteration as stop:
+            del self._coros[key]
+            self._results[key] = stop.value
+            if self._on_done is not None:
+                self._on_done(key, stop.value)
+            return
+        if not isinstance(req, (EvalRequest, ArnoldiStep)):
+            raise TypeError(f"solve {key!r} yielded {type(req).__name__}")
+        self._pending[key] = req
+
+    def _round(self) -> None:
+        self.n_rounds += 1
+        items = list(self._pending.items())
+        self._pending.clear()
+        steps = [(k, r) for k, r in items if isinstance(r, ArnoldiStep)]
+        evals = [(k, r) for k, r in items if isinstance(r, EvalRequest)]
+        if steps:
+            self.n_steps += len(steps)
+            self.max_width = max(self.max_width, len(steps))
+            if self.batched:
+                groups = defaultdict(list)
+                for k, r in steps:
+                    groups[batch_key(r)].append(r)
+                for key, rs in groups.items():
+                    if key is not None and len(rs) > 1:
+                        batched_arnoldi(rs)
+                        self.n_batched_launch_groups += 1
+                    else:
+                        for r in rs:
+                            r.run()
+            else:
+                for _, r in steps:
+                    r.run()
+            torch.cuda.current_stream().synchronize()
+            for k, _ in steps:
+                self._advance(k, None)
+        for k, r in evals:
+            self.n_evals += 1
+            r.ws.eval_residual(r.s, *r.args)
+            self._advance(k, r.s)
+
+    def stats(self) -> dict:
+        return {"rounds": self.n_rounds, "arnoldi_steps": self.n_steps,
+                "evaluations": self.n_evals, "max_steps_per_sync": self.max_width,
+                "batched_groups": self.n_batched_launch_groups}
+
+
+# ---------------------------------------------------------------- the sweep
+def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]],
+                        chains_per_mesh: int = 4, precision: str = "fp32",
+                        upwind_c: float = 2.0, dm: float = 0.05,
+                        newton_kw: Optional[dict] = None, batched: bool = True,
+                        log=None) -> Dict[str, object]:
+    """Every (mesh index, M_inf, alpha_deg) condition, solved in lockstep.
+
+    The conditions of each mesh are ordered by (M_inf, alpha) and dealt
+    round-robin into chains_per_mesh chains; a chain solves its conditions
+    one after another, each warm-started from the chain's last converged
+    condition as in solve_sweep (alpha continuation, then a Mach ramp; a
+    cold ramp when that fails), and all chains of all meshes run side by
+    side through one BatchExecutor.  A condition whose chain could not
+    converge it is retried once at the end from the nearest converged
+    condition of its mesh with a finer Mach step (as solve_sweep).
+    Returns solve_sweep's keys plus "executor" statistics."""
+    from ..mesh.channel import boundary_nodes
+    from ..post.surface import wall_force_coefficients
+    from .sweep import _record, _solve_one_steps
+
+    kw = {"precision": precision, "gmres_restart": 200, "gmres_maxiter": 5,
+          "n_newton_max": 30, "amg_rebuild_every": 0,
+          "precond_matrix": "jacobian_spd", "amg_max_coarse": 1000}
+    kw.update(newton_kw or {})
+    t_all = time.perf_counter()
+    inst: List[Optional[dict]] = [None] * len(conditions)
+    phis: Dict[int, np.ndarray] = {}
+    dns = [boundary_nodes(m, ["inflow", "outflow"]) for m in meshes]
+    t_mesh0 = {}
+    t_mesh1 = {}
+
+    def chain(k, ids):
+        last = None
+        for i in ids:
+            _, m, a = conditions[i]
+            t0 = time.perf_counter()
+            r, route, subs = yield from _solve_one_steps(
+                meshes[k], dns[k], m, a, last, upwind_c, dm, kw, False, stepped=True)
+            inst[i] = _record(meshes[k], k, m, a, r, subs, route,
+                              time.perf_counter() - t0, False, wall_force_coefficients)
+            if r["converged"]:
+                last = (r["phi"], m, a)
+                phis[i] = r["phi"]
+            if log is not None:
+                log(f"[batch] mesh {k} M={m:.4f} alpha={a:.3f} "
+                    f"converged={r['converged']} {inst[i]['seconds']:.2f}s")
+        t_mesh1[k] = time.perf_counter()
+        return None
+
+    coros = {}
+    for k in range(len(meshes)):
+        ids = [i for i, c in enumerate(conditions) if c[0] == k]
+        ids.sort(key=lambda i: (conditions[i][1], conditions[i][2]))
+        t_mesh0[k] = time.perf_counter()
+        for c in range(min(chains_per_mesh, len(ids))):
+            coros[(k, c)] = chain(k, ids[c::chains_per_mesh])
+    ex = BatchExecutor(batched=batched)
+    ex.run(coros)
+
+    # retries: from the nearest converged condition of the mesh, finer step
+    retry = {}
+    for i, rec in enumerate(inst):
+        if rec is None or rec["converged"]:
+            continue
+        k, m, a = conditions[i]
+        done = [j for j in phis if conditions[j][0] == k and conditions[j][1] <= m]
+        src = (min(done, key=lambda j: (m - conditions[j][1]) / 0.05
+                   + abs(a - conditions[j][2]) / 0.5) if done else None)
+        seed = None if src is None else (phis[src], conditions[src][1], conditions[src][2])
+
+        def one(i=i, k=k, m=m, a=a, seed=seed):
+            t0 = time.perf_counter()
+            r, route, subs = yield from _solve_one_steps(
+                meshes[k], dns[k], m, a, seed, upwind_c, dm / 4.0, kw, False,
+                dalpha_max=0.25, stepped=True)
+            inst[i] = _record(meshes[k], k, m, a, r, subs, route,
+                              time.perf_counter() - t0, True, wall_force_coefficients)
+            if r["converged"]:
+                phis[i] = r["phi"]
+            return None
+
+        retry[("retry", i)] = one()
+    if retry:
+        ex.run(retry)
+    mask = [bool(r["converged"]) for r in inst]
+    per_mesh = [{"mesh": k, "n_tets": int(len(meshes[k].elements)),
+                 "n_conditions": sum(1 for c in conditions if c[0] == k),
+                 "seconds": t_mesh1.get(k, t_mesh0[k]) - t_mesh0[k]}
+                for k in range(len(meshes))]
+    return {"instances": inst, "converged_mask": mask,
+            "n_converged": int(sum(mask)), "wall_s": time.perf_counter() - t_all,
+            "per_mesh": per_mesh, "executor": ex.stats(),
+            "chains_per_mesh": int(chains_per_mesh)}

@@ -80,6 +80,21 @@ def scale(a: float, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
 _GMRES_WS: "OrderedDict" = None
 
 
+def gmres_buffers(n: int, restart: int, f32: bool, device, V=None) -> dict:
+    """A fresh set of gmres_dev work buffers (V: an optional preallocated
+    (restart + 1, n) basis, e.g. a slice of a batch's bases)."""
+    nout = 2 * restart + 4
+    if V is None:
+        V = torch.empty((restart + 1, n), dtype=torch.float32 if f32 else torch.float64,
+                        device=device)
+    return {"V": V, "v64": empty(n) if f32 else None, "tmp": empty(n), "r": empty(n),
+            "w": [empty(n), empty(n)], "outs": [empty(nout), empty(nout)],
+            "hbuf": [torch.empty(nout, dtype=torch.float64, pin_memory=True)
+                     for _ in range(2)],
+            "evs": [torch.cuda.Event(), torch.cuda.Event()],
+            "red": _Red(restart + 1)}
+
+
 def _gmres_workspace(n: int, restart: int, f32: bool, device):
     """Work buffers of gmres_dev for one (n, restart, basis type), kept
     between calls (at most four shapes): the same buffers every call, so a
@@ -93,21 +108,110 @@ def _gmres_workspace(n: int, restart: int, f32: bool, device):
     key = (str(device), int(n), int(restart), bool(f32))
     ws = _GMRES_WS.get(key)
     if ws is None:
-        nout = 2 * restart + 4
-        ws = {"V": torch.empty((restart + 1, n),
-                               dtype=torch.float32 if f32 else torch.float64, device=device),
-              "v64": empty(n) if f32 else None, "tmp": empty(n), "r": empty(n),
-              "w": [empty(n), empty(n)], "outs": [empty(nout), empty(nout)],
-              "hbuf": [torch.empty(nout, dtype=torch.float64, pin_memory=True)
-                       for _ in range(2)],
-              "evs": [torch.cuda.Event(), torch.cuda.Event()],
-              "red": _Red(restart + 1)}
+        ws = gmres_buffers(n, restart, f32, device)
         _GMRES_WS[key] = ws
         while len(_GMRES_WS) > 4:
             _GMRES_WS.popitem(last=False)
     else:
         _GMRES_WS.move_to_end(key)
     return ws
+
+
+def drive(gen):
+    """Run a solver coroutine whose device steps need no outside executor
+    (it yields nothing) and return its result."""
+    try:
+        req = next(gen)
+    except StopIteration as stop:
+        return stop.value
+    raise RuntimeError(f"solver coroutine yielded {req!r} without an executor")
+
+
+class ArnoldiStep:
+    """One Arnoldi step of a stepped GMRES (gmres_steps with
+    pipelined=False), handed to the executor: run() queues the step's device
+    work (enqueue(col)); after the executor's stream synchronisation the
+    coroutine reads the step's coefficients from pinned host memory."""
+
+    __slots__ = ("enqueue", "col", "matvec", "psolve", "gw", "f32")
+
+    def __init__(self, enqueue, col, matvec, psolve, gw, f32):
+        self.enqueue, self.col = enqueue, col
+        self.matvec, self.psolve, self.gw, self.f32 = matvec, psolve, gw, f32
+
+    def run(self):
+        self.enqueue(self.col)
+
+
+_JOB_FIELDS = 16       # int64 words per fp3d_batch job (include/fp3d.h)
+
+
+def batch_key(step: ArnoldiStep):
+    """Arnoldi steps that one batched launch can serve share this key: a
+    Newton-Jacobian matvec on the same mesh store and value format, the
+    same basis type and shape.  None for any other operator."""
+    op = getattr(step.matvec, "__self__", None)
+    if not isinstance(op, NewtonOperator):
+        return None
+    V = step.gw["V"]
+    return (id(op.store), op.fmt(), step.f32, tuple(V.shape))
+
+
+def batched_arnoldi(steps) -> None:
+    """Queue the Arnoldi steps of several stepped GMRES solves that share a
+    batch_key: the Jacobian matvecs in one batched launch per kernel, each
+    solve's preconditioner, then the orthogonalisation, norm and
+    normalisation in one batched launch per kernel, then each step's
+    coefficient copy to pinned host memory.  Per solve this is the device
+    work ArnoldiStep.run queues, in the same order, with the same kernel
+    bodies and grids."""
+    s0 = steps[0]
+    op0 = s0.matvec.__self__
+    st = op0.store
+    n = st.n_nodes
+    f32 = s0.f32
+    jobs = np.zeros((len(steps), _JOB_FIELDS), dtype=np.int64)
+    max_active = 0
+    max_nv = 0
+    for i, stp in enumerate(steps):
+        op = stp.matvec.__self__
+        gw = stp.gw
+        col = stp.col
+        V = gw["V"]
+        o = gw["outs"][col % 2]
+        w = gw["w"][col % 2]
+        na = op.n_active
+        max_active = max(max_active, na)
+        max_nv = max(max_nv, col + 1)
+        x = gw["v64"] if f32 else V[col]
+        jobs[i] = (ptr(V), V.stride(0), col + 1, ptr(w), ptr(o), ptr(gw["red"].work),
+                   ptr(V[col + 1]), ptr(V[col]) if f32 else 0, ptr(x), ptr(gw["tmp"]),
+                   ptr(op.data32 if op.is_f32 else op.data),
+                   ptr(op.upstream) if na > 0 else 0, ptr(op.t3) if na > 0 else 0, na,
+                   ptr(op.free_mask), 0)
+    host = torch.from_numpy(jobs).pin_memory()
+    dev = torch.empty_like(host, device=st.device)
+    dev.copy_(host, non_blocking=True)
+    sh = stream_handle()
+    i16 = getattr(st, "indices16", None)
+    _native.call("fp3d_batch_jacobian_matvec", len(steps), ptr(dev), max_active, n, st.nnz,
+                 ptr(st.indptr), ptr(i16 if i16 is not None else st.indices), op0.fmt(),
+                 ptr(st.el), st.n_tets, 1 if f32 else 0, sh)
+    for stp in steps:
+        gw = stp.gw
+        w = gw["w"][stp.col % 2]
+        if stp.psolve is None:
+            w.copy_(gw["tmp"])
+        else:
+            stp.psolve(gw["tmp"], w)
+    _native.call("fp3d_batch_orthonormalise", len(steps), ptr(dev), max_nv, n,
+                 1 if f32 else 0, 0.5, stream_handle())
+    for stp in steps:
+        gw = stp.gw
+        col, k = stp.col, stp.col + 1
+        gw["hbuf"][col % 2][: 3 + 2 * k].copy_(gw["outs"][col % 2][: 3 + 2 * k],
+                                               non_blocking=True)
+        gw["evs"][col % 2].record()
 
 
 def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
@@ -127,6 +231,22 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
     basis="f32" stores the Krylov basis in float32 (half the
     orthogonalisation traffic); dot products, the Hessenberg matrix, the
     Givens rotations and the solution update stay float64."""
+    return drive(gmres_steps(matvec, b, psolve, x0, rtol, atol, restart, maxiter,
+                             callback, basis))
+
+
+def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
+                x0: Optional[torch.Tensor] = None, rtol: float = 1e-5,
+                atol: float = 0.0, restart: int = 20, maxiter: Optional[int] = None,
+                callback: Optional[Callable] = None, basis: str = "f64",
+                gw: Optional[dict] = None, pipelined: bool = True):
+    """gmres_dev as a coroutine.  pipelined=True runs the one-step-ahead
+    loop and yields nothing; pipelined=False yields an ArnoldiStep per
+    Krylov step and resumes once its executor has run the step and
+    synchronised the stream (an executor can queue the steps of many
+    solves before one synchronisation).  Both queue the same device work in
+    the same order per solve, so they return the same bits.  gw: work
+    buffers from gmres_buffers (default: the shared per-shape set)."""
     if basis not in ("f64", "f32"):
         raise ValueError(f"basis={basis!r}: 'f64' or 'f32'")
     f32 = basis == "f32"
@@ -134,7 +254,8 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
     if maxiter is None:
         maxiter = 10 * n
     restart = int(min(restart, n))
-    gw = _gmres_workspace(n, restart, f32, b.device)
+    if gw is None:
+        gw = _gmres_workspace(n, restart, f32, b.device)
     red = gw["red"]
     tmp = gw["tmp"]
     r = gw["r"]
@@ -226,11 +347,15 @@ def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = No
         S[0] = t0
         breakdown = False
         col = 0
-        enqueue(0)
+        if pipelined:
+            enqueue(0)
         for col in range(restart):
-            if col + 1 < restart:
-                enqueue(col + 1)            # runs while the host does step col
-            evs[col % 2].synchronize()
+            if pipelined:
+                if col + 1 < restart:
+                    enqueue(col + 1)        # runs while the host does step col
+                evs[col % 2].synchronize()
+            else:
+                yield ArnoldiStep(enqueue, col, matvec, psolve, gw, f32)
             k = col + 1
             host = hbuf[col % 2][: 3 + 2 * k].numpy()
             h0 = float(np.sqrt(max(host[2 + k], 0.0)))
@@ -392,12 +517,17 @@ class NewtonOperator:
         self._n_active = None
         self.upstream = upstream
 
+    def fmt(self) -> int:
+        """Value / index format bits of the Terms 1+2 CSR the matvec reads
+        (1: float32 values, 4: int16 row-relative columns)."""
+        return self.is_f32 | (4 if getattr(self.store, "indices16", None) is not None else 0)
+
     def matvec(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
         st = self.store
         vals = self.data32 if self.is_f32 else self.data
         use_t3 = self.n_active > 0
         i16 = getattr(st, "indices16", None)
-        fmt = self.is_f32 | (4 if i16 is not None else 0)
+        fmt = self.fmt()
         _native.call("fp3d_csr_spmv_term3", st.n_nodes, st.nnz, ptr(st.indptr),
                      ptr(i16 if i16 is not None else st.indices), ptr(vals), fmt,
                      ptr(st.el),

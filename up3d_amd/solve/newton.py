@@ -41,7 +41,8 @@ from ..physics.isentropic import q2_at_mach
 from .common import (Reducer, amg_from_device, dirichlet_setup, mask_dirichlet,
                      masked)
 from .continuation import mach_schedule
-from .linear import NewtonOperator, axpby, gmres_dev, scale
+from .linear import (NewtonOperator, axpby, drive, gmres_buffers, gmres_steps,
+                     scale)
 from .timing import finalize, new_timings, snapshot, step_delta, sum_timings
 
 
@@ -135,6 +136,17 @@ class NewtonWorkspace:
             raise ValueError(f"Newton driver needs 0 <= M_inf < 1, got {m_inf}")
         self.m_inf = float(m_inf)
         self.q2cap = float(q2_at_mach(m_cap, m_inf, self.gamma_air))
+
+    def gmres_work(self, restart: int, f32: bool) -> dict:
+        """GMRES buffers of this workspace alone (stepped solves of several
+        workspaces run side by side, so they cannot share the per-shape
+        set)."""
+        key = (int(min(restart, self.n)), bool(f32))
+        gws = getattr(self, "_gmres_gw", None)
+        if gws is None or gws[0] != key:
+            self._gmres_gw = gws = (key, gmres_buffers(self.n, key[0], key[1],
+                                                       self.st.device))
+        return gws[1]
 
     def free_state(self, *busy):
         for s in self.states:
@@ -292,6 +304,56 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                  amg_max_coarse: Optional[int] = None, verbose: bool = False
                  ) -> Dict[str, object]:
     """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
+    n_st = 0): newton_steps run to completion; see newton_steps for the
+    arguments."""
+    kw = dict(locals())
+    return drive(newton_steps(**kw))
+
+
+class EvalRequest:
+    """A residual evaluation of a stepped Newton solve (newton_steps with
+    stepped=True), handed to the executor, which runs
+    ws.eval_residual(s, *args) -- or a batched equivalent -- and resumes the
+    solve."""
+
+    __slots__ = ("ws", "s", "args")
+
+    def __init__(self, ws, s, args):
+        self.ws, self.s, self.args = ws, s, args
+
+
+def _eval(ws, stepped, s, *args):
+    """ws.eval_residual(s, *args), inline or through the executor."""
+    if stepped:
+        yield EvalRequest(ws, s, args)
+    else:
+        ws.eval_residual(s, *args)
+    return s
+
+
+def newton_steps(nodes, elements, dirichlet_nodes, dirichlet_values,
+                 m_inf: float, u_inf: float = 1.0, gamma_air: float = 1.4,
+                 upwind_c: float = 1.5, m_crit: float = 0.95,
+                 m_cap: float = 3.0, rho_floor: float = 0.05,
+                 phi_init: Optional[np.ndarray] = None, n_picard_seed: int = 5,
+                 n_newton_max: int = 30, tol_residual: float = 1e-10,
+                 ew_eta0: float = 1e-2, ew_gamma: float = 0.9,
+                 ew_eta_max: float = 1e-2, amg_rebuild_every: int = 2,
+                 gmres_restart: int = 60, gmres_maxiter: int = 10,
+                 line_search: bool = True, rtol_seed: float = 1e-7,
+                 tol_residual_loose: Optional[float] = None,
+                 tol_residual_rel: Optional[float] = None,
+                 accept_on_stall: bool = False,
+                 freeze_tol: Optional[float] = None,
+                 freeze_refresh_max: int = 2, freeze_max_reverts: int = 3,
+                 workspace: Optional[NewtonWorkspace] = None,
+                 external_rhs: Optional[np.ndarray] = None,
+                 precision: str = "fp64", precond_matrix: str = "picard",
+                 precond_clamp: float = 0.9, ptc_dt0: Optional[float] = None,
+                 ptc_dt_max: float = 1e6, amg_degree: Optional[int] = None,
+                 amg_max_coarse: Optional[int] = None, verbose: bool = False,
+                 stepped: bool = False):
+    """Newton at ONE Mach level (pyfp3d/solve/newton.py:516-1141 with
     n_st = 0; precond is always the device AMG).  Converged when
     ||R_free||_inf < tol_residual with 0 limited and 0 floored elements.
     amg_rebuild_every = k > 0 rebuilds the AMG hierarchy every k Newton
@@ -314,7 +376,13 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     amg_degree / amg_max_coarse (None: the hierarchy's defaults, Chebyshev
     degree 2 and a coarsest level of at most 10 rows) set the AMG smoother
     degree and the size at which the hierarchy stops and solves densely --
-    the knobs `up3d_amd.tune` searches."""
+    the knobs `up3d_amd.tune` searches.
+
+    As a coroutine: stepped=False runs every device step inline and yields
+    nothing; stepped=True yields an EvalRequest per residual evaluation and
+    an ArnoldiStep per Krylov step (GMRES on the workspace's own buffers),
+    so an executor can advance many solves in lockstep.  The device work
+    per solve, and so the result, is the same either way."""
     if m_inf > 0.0 and upwind_c <= 0.0:
         raise ValueError("the Newton driver runs with the walk upwind machinery "
                          "active for m_inf > 0; pass upwind_c > 0")
@@ -371,7 +439,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
     accept_reason = None
     eta_prev = None
     t0 = time.perf_counter()
-    ws.eval_residual(cur, upwind_c, m_crit, rho_floor)
+    yield from _eval(ws, stepped, cur, upwind_c, m_crit, rho_floor)
     sync()
     timings["residual"] += time.perf_counter() - t0
 
@@ -439,7 +507,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
                 if n_freeze_reverts >= freeze_max_reverts:
                     freeze_armed = False
                 t0 = time.perf_counter()
-                ws.eval_residual(cur, upwind_c, m_crit, rho_floor)
+                yield from _eval(ws, stepped, cur, upwind_c, m_crit, rho_floor)
                 sync()
                 timings["residual"] += time.perf_counter() - t0
                 continue
@@ -479,7 +547,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
             t0 = time.perf_counter()
             live = ws.free_state(cur)
             live.phi.copy_(cur.phi)
-            ws.eval_residual(live, upwind_c, m_crit, rho_floor)
+            yield from _eval(ws, stepped, live, upwind_c, m_crit, rho_floor)
             residual_unfrozen = live.r_norm
             up_new, br_new = ws.freeze(live, upwind_c, m_crit, rho_floor)
             sync()
@@ -550,9 +618,12 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
         rec["n_lin_solves"] = 1
         t0 = time.perf_counter()
         scale(-1.0, cur.R, rhs)
-        x, info, n_it = gmres_dev(ws.jac.matvec, rhs, M_pre.apply, rtol=eta,
-                                  restart=gmres_restart, maxiter=gmres_maxiter,
-                                  basis="f32" if ws.precision == "fp32" else "f64")
+        f32b = ws.precision == "fp32"
+        x, info, n_it = yield from gmres_steps(
+            ws.jac.matvec, rhs, M_pre.apply, rtol=eta, restart=gmres_restart,
+            maxiter=gmres_maxiter, basis="f32" if f32b else "f64",
+            gw=ws.gmres_work(gmres_restart, f32b) if stepped else None,
+            pipelined=not stepped)
         dphi.copy_(x)
         sync()
         timings["linsolve"] += time.perf_counter() - t0
@@ -579,7 +650,7 @@ def solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
             trial = ws.free_state(cur, best[1] if best else None)
             trial.phi.copy_(cur.phi)
             axpby(lam, dphi, 1.0, trial.phi)
-            ws.eval_residual(trial, upwind_c, m_crit, rho_floor, frozen)
+            yield from _eval(ws, stepped, trial, upwind_c, m_crit, rho_floor, frozen)
             mt = trial.merit
             if np.isfinite(mt) and (not line_search or mt <= cur.merit):
                 accepted = True
@@ -645,14 +716,30 @@ def solve_newton_transonic(nodes, elements, dirichlet_nodes, dirichlet_values,
     """Upward Mach continuation of solve_newton
     (pyfp3d/solve/newton.py:1144-1307): warm starts from the last CONVERGED
     level, Mach step halved on failure down to dm_min, optional loose
-    intermediate levels and post-ramp upwind_c stages."""
+    intermediate levels and post-ramp upwind_c stages (transonic_steps run
+    to completion)."""
+    kw = dict(locals())
+    return drive(transonic_steps(**kw))
+
+
+def transonic_steps(nodes, elements, dirichlet_nodes, dirichlet_values,
+                    m_inf: float, m_start: float = 0.70, dm: float = 0.05,
+                    dm_min: float = 0.01, upwind_c: float = 1.5,
+                    m_crit: float = 0.95, upwind_c_post=None,
+                    freeze_tol: Optional[float] = 1e-6,
+                    intermediate_tol: Optional[float] = None,
+                    newton_kw: Optional[dict] = None,
+                    verbose: bool = False, stepped: bool = False,
+                    workspace: Optional[NewtonWorkspace] = None):
+    """solve_newton_transonic as a coroutine (stepped as in newton_steps);
+    workspace: reuse this workspace for every level."""
     if m_inf < m_start:
         raise ValueError(f"upward ramp only: m_inf {m_inf} < m_start {m_start}")
     kw = dict(newton_kw or {})
     kw.setdefault("upwind_c", upwind_c)
     kw.setdefault("m_crit", m_crit)
     kw.setdefault("freeze_tol", freeze_tol)
-    ws = None
+    ws = workspace
     result = None
     last_good = None
     level_history, level_results = [], []
@@ -661,8 +748,13 @@ def solve_newton_transonic(nodes, elements, dirichlet_nodes, dirichlet_values,
 
     def run_level(m, lvl_kw):
         t_lvl = time.perf_counter()
-        res = solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
-                           m_inf=m, workspace=ws, verbose=verbose, **lvl_kw)
+        if stepped:
+            res = yield from newton_steps(nodes, elements, dirichlet_nodes,
+                                          dirichlet_values, m_inf=m, workspace=ws,
+                                          verbose=verbose, stepped=True, **lvl_kw)
+        else:
+            res = solve_newton(nodes, elements, dirichlet_nodes, dirichlet_values,
+                               m_inf=m, workspace=ws, verbose=verbose, **lvl_kw)
         wall = time.perf_counter() - t_lvl
         level_history.append((m, res["n_newton"], res["residual_history"][-1]))
         level_results.append({
@@ -687,7 +779,7 @@ def solve_newton_transonic(nodes, elements, dirichlet_nodes, dirichlet_values,
             lvl_kw.update(phi_init=last_good["phi_dev"], n_picard_seed=0)
         if verbose:
             print(f"newton continuation level M = {m:.4f}", flush=True)
-        result = run_level(m, lvl_kw)
+        result = yield from run_level(m, lvl_kw)
         ws = result["workspace"]
         if not result["converged"]:
             m_prev = m_start if i == 0 else levels[i - 1]
@@ -704,7 +796,7 @@ def solve_newton_transonic(nodes, elements, dirichlet_nodes, dirichlet_values,
             lvl_kw = dict(kw)
             lvl_kw.update(upwind_c=c_post, phi_init=result["phi_dev"],
                           n_picard_seed=0)
-            result = run_level(m_inf, lvl_kw)
+            result = yield from run_level(m_inf, lvl_kw)
             if not result["converged"]:
                 break
     result = dict(result)

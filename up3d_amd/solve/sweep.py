@@ -31,7 +31,8 @@ import torch
 
 from ..constraints.dirichlet import freestream_phi
 from ..mesh.channel import boundary_nodes, bump_channel
-from .newton import solve_newton_transonic
+from .linear import drive
+from .newton import transonic_steps
 
 SWEEP_SHAPES = ((60, 20, 20), (96, 32, 32), (120, 40, 40), (153, 51, 51))
 
@@ -57,47 +58,58 @@ def sweep_conditions(n_meshes: int = 4, per_mesh: int = 16, seed: int = 20260105
     return out
 
 
-def _level(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm, newton_kw, verbose):
-    """One Mach ramp m_start -> m_inf at alpha from phi (None: freestream)."""
+def _level(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm, newton_kw, verbose,
+           stepped=False):
+    """One Mach ramp m_start -> m_inf at alpha from phi (None: freestream),
+    as a coroutine (transonic_steps)."""
     dv = freestream_phi(mesh.nodes[dn], alpha)
     kw = dict(newton_kw)
     phi0 = freestream_phi(mesh.nodes, alpha) if phi is None else np.array(phi, dtype=np.float64)
     phi0[dn] = dv
     kw["phi_init"] = phi0
     kw["n_picard_seed"] = 0
-    return solve_newton_transonic(mesh.nodes, mesh.elements, dn, dv, m_inf,
-                                  m_start=min(m_inf, m_start), dm=dm, dm_min=0.005,
-                                  upwind_c=upwind_c, newton_kw=kw, verbose=verbose)
+    return (yield from transonic_steps(mesh.nodes, mesh.elements, dn, dv, m_inf,
+                                       m_start=min(m_inf, m_start), dm=dm, dm_min=0.005,
+                                       upwind_c=upwind_c, newton_kw=kw, verbose=verbose,
+                                       stepped=stepped))
 
 
-def _solve_one(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verbose,
-               dalpha_max: float = 0.5):
-    """(result, route, sub-results).  From a converged neighbour
-    seed = (phi, M, alpha): alpha continuation at the neighbour's Mach in
-    steps of at most dalpha_max degrees, then a Mach ramp to m_inf; when that
-    fails (or without a seed), a cold Mach ramp from 0.6 at this alpha."""
+def _solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verbose,
+                     dalpha_max: float = 0.5, stepped: bool = False):
+    """(result, route, sub-results), as a coroutine.  From a converged
+    neighbour seed = (phi, M, alpha): alpha continuation at the neighbour's
+    Mach in steps of at most dalpha_max degrees, then a Mach ramp to m_inf;
+    when that fails (or without a seed), a cold Mach ramp from 0.6 at this
+    alpha."""
     subs = []
     if seed is not None:
         phi, m0, a0 = seed
         n_a = int(np.ceil(abs(alpha - a0) / dalpha_max - 1e-12))
         ok = True
         for j in range(1, n_a + 1):
-            r = _level(mesh, dn, m0, a0 + (alpha - a0) * j / n_a, phi, m0,
-                       upwind_c, dm, newton_kw, verbose)
+            r = yield from _level(mesh, dn, m0, a0 + (alpha - a0) * j / n_a, phi, m0,
+                                  upwind_c, dm, newton_kw, verbose, stepped)
             subs.append(r)
             if not r["converged"]:
                 ok = False
                 break
             phi = r["phi"]
         if ok:
-            r = _level(mesh, dn, m_inf, alpha, phi, max(m0, 0.6), upwind_c, dm,
-                       newton_kw, verbose)
+            r = yield from _level(mesh, dn, m_inf, alpha, phi, max(m0, 0.6), upwind_c, dm,
+                                  newton_kw, verbose, stepped)
             subs.append(r)
             if r["converged"]:
                 return r, "warm", subs
-    r = _level(mesh, dn, m_inf, alpha, None, 0.6, upwind_c, dm, newton_kw, verbose)
+    r = yield from _level(mesh, dn, m_inf, alpha, None, 0.6, upwind_c, dm, newton_kw,
+                          verbose, stepped)
     subs.append(r)
     return r, "cold", subs
+
+
+def _solve_one(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verbose,
+               dalpha_max: float = 0.5):
+    return drive(_solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw,
+                                  verbose, dalpha_max))
 
 
 def solve_sweep(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]],
