@@ -98,6 +98,16 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="also run the batched sweep: 64 conditions on 4 channel meshes "
                          "(about 3 minutes)")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only time K lockstep Krylov solves per sweep mesh (batched "
+                         "launches, per-solve launches, one after another)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only run the sweep (--sweep-mode) and print its JSON")
+    ap.add_argument("--sweep-mode", default="batched",
+                    choices=["batched", "sequential", "both"])
+    ap.add_argument("--sweep-chains", default="4,2,1,1",
+                    help="warm-start chains per mesh of the batched sweep: one count, "
+                         "or one per mesh separated by commas")
     ap.add_argument("--no-plate", action="store_true",
                     help="skip the embedded-plate lifting case (128x96x96 box)")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -123,6 +133,29 @@ def main():
               "multi-GPU sharding of this path is a design note, not a run.",
               file=sys.stderr)
         sys.exit(2)
+    if args.batch_profile:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(run_batch_profile()), flush=True)
+        return
+    if args.sweep_only:
+        import torch
+
+        from up3d_amd.solve.sweep import sweep_meshes
+
+        torch.cuda.set_device(0)
+        meshes = sweep_meshes()
+        modes = ["sequential", "batched"] if args.sweep_mode == "both" else [args.sweep_mode]
+        out = {}
+        for mode in modes:
+            for ch in (args.sweep_chains.split(";") if mode == "batched" else ["1"]):
+                key = mode if mode == "sequential" else f"batched[{ch}]"
+                out[key] = run_sweep(mode, _chains(ch), meshes)
+                log(f"[bench] sweep {key}: wall {out[key]['wall_s']} s, "
+                    f"{out[key]['n_converged']} converged, executor {out[key]['executor']}")
+        print(json.dumps(out), flush=True)
+        return
     if args.solve_profile:
         if args.calib:
             _calibration_copy()
@@ -314,7 +347,8 @@ def main():
 
     sweep = None
     if args.sweep:
-        sweep = run_sweep()
+        sweep = run_sweep(args.sweep_mode if args.sweep_mode != "both" else "batched",
+                          _chains(args.sweep_chains.split(";")[0]))
         log(f"[bench] sweep {json.dumps({k: v for k, v in sweep.items() if k != 'instances'})}")
 
     out = {
@@ -777,26 +811,121 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
             "rel_err_free": rel_free, "fd_seconds": round(fd_s, 1)}
 
 
-def run_sweep():
-    """Batched sweep: 16 seeded (M_inf in [0.60, 0.86], alpha in [0, 3] deg)
-    conditions on each of 4 jittered RCM bump channels (144,000 / 589,824 /
-    1,152,000 / 2,387,718 tets), FP32-preconditioned Newton, every
-    instance to ||R_free||_inf < 1e-10 with 0 limited / floored."""
+def run_batch_profile(ks=(1, 4, 16), iters: int = 120, restart: int = 60,
+                      mesh_ids=(0, 1, 3)):
+    """Krylov steps per second of K lockstep GMRES solves on each sweep
+    mesh: K copies of one Newton system (M_inf 0.80, the constructed state;
+    each copy its own workspace, Jacobian and AMG hierarchy) advanced by the
+    BatchExecutor with batched Arnoldi launches, with per-solve launches,
+    and as K pipelined solves one after another.  The solves run a fixed
+    number of iterations (tolerance out of reach)."""
     import torch
 
+    from up3d_amd.device import to_dev
+    from up3d_amd.mesh.channel import boundary_nodes
+    from up3d_amd.solve.batch import BatchExecutor
+    from up3d_amd.solve.linear import gmres_dev, gmres_steps, scale
+    from up3d_amd.solve.newton import NewtonWorkspace
+    from up3d_amd.solve.sweep import sweep_meshes
+
+    meshes = sweep_meshes()
+    out = {"workload": f"GMRES({restart}) x {iters // restart} cycles of one Newton "
+                       "system at M_inf 0.80 per solve, K solves per mesh",
+           "meshes": []}
+    for mi in mesh_ids:
+        mesh = meshes[mi]
+        dn = boundary_nodes(mesh, ["inflow", "outflow"])
+        kmax = max(ks)
+        wss, rhss, Ms = [], [], []
+        for _ in range(kmax):
+            ws = NewtonWorkspace(mesh.nodes, mesh.elements, dn, mesh.nodes[dn, 0].copy(),
+                                 precision="fp32")
+            ws.amg_kw = {"degree": 2, "max_coarse": 1000}
+            ws.set_mach(0.80)
+            s = ws.states[0]
+            s.phi.copy_(to_dev(constructed_phi(mesh.nodes)))
+            s.phi.index_copy_(0, ws.dn, ws.dv)
+            ws.eval_residual(s, 2.0, M_CRIT, RHO_FLOOR)
+            ws.assemble_coupled(s, 2.0, M_CRIT, RHO_FLOOR, None)
+            M = ws.build_preconditioner(s, "jacobian_spd", 0.9, se=ws.se_used)
+            rhs = torch.empty_like(s.R)
+            scale(-1.0, s.R, rhs)
+            wss.append(ws)
+            rhss.append(rhs)
+            Ms.append(M)
+        maxiter = max(1, iters // restart)
+        rec = {"mesh": mi, "n_tets": int(len(mesh.elements)), "n_nodes": int(wss[0].n)}
+        for k in ks:
+            def coros():
+                return {i: gmres_steps(wss[i].jac.matvec, rhss[i], Ms[i].apply, rtol=1e-30,
+                                       restart=restart, maxiter=maxiter, basis="f32",
+                                       gw=wss[i].gmres_work(restart, True), pipelined=False)
+                        for i in range(k)}
+            row = {}
+            for label, batched in (("batched", True), ("per_solve", False)):
+                BatchExecutor(batched).run(coros())          # warm-up (graphs)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                res = BatchExecutor(batched).run(coros())
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                n_steps = sum(r[2] for r in res.values())
+                row[label + "_us_per_step"] = round(dt / n_steps * 1e6, 1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            n_steps = 0
+            for i in range(k):
+                _, _, it = gmres_dev(wss[i].jac.matvec, rhss[i], Ms[i].apply, rtol=1e-30,
+                                     restart=restart, maxiter=maxiter, basis="f32")
+                n_steps += it
+            torch.cuda.synchronize()
+            row["alone_us_per_step"] = round((time.perf_counter() - t0) / n_steps * 1e6, 1)
+            rec[f"K{k}"] = row
+            log(f"[batchprof] mesh {mi} K={k}: {row}")
+        out["meshes"].append(rec)
+        del wss, rhss, Ms
+        torch.cuda.empty_cache()
+    return out
+
+
+def _chains(text: str):
+    """'4' -> 4, '4,2,1,1' -> [4, 2, 1, 1]."""
+    parts = [int(p) for p in text.split(",")]
+    return parts[0] if len(parts) == 1 else parts
+
+
+def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None):
+    """Sweep of 16 seeded (M_inf in [0.60, 0.86], alpha in [0, 3] deg)
+    conditions on each of 4 jittered RCM bump channels (144,000 / 589,824 /
+    1,152,000 / 2,387,718 tets), FP32-preconditioned Newton, every
+    instance to ||R_free||_inf < 1e-10 with 0 limited / floored.
+    mode "batched": `chains` warm-start chains per mesh, all advanced in
+    lockstep with batched Arnoldi launches (solve/batch.py); "sequential":
+    one condition after another (solve/sweep.py)."""
+    import torch
+
+    from up3d_amd.solve.batch import solve_sweep_batched
     from up3d_amd.solve.sweep import solve_sweep, sweep_conditions, sweep_meshes
 
     t0 = time.perf_counter()
-    meshes = sweep_meshes()
+    if meshes is None:
+        meshes = sweep_meshes()
     mesh_s = time.perf_counter() - t0
     conds = sweep_conditions(len(meshes), 16)
-    r = solve_sweep(meshes, conds, log=log)
+    if mode == "batched":
+        r = solve_sweep_batched(meshes, conds, chains_per_mesh=chains, log=log)
+    elif mode == "sequential":
+        r = solve_sweep(meshes, conds, log=log)
+    else:
+        raise ValueError(f"sweep mode {mode!r}")
     torch.cuda.synchronize()
     inst = r["instances"]
     tet_steps = sum(i["n_tets"] * i["n_newton"] for i in inst)
     return {
         "workload": "64 conditions: M_inf U[0.60,0.86], alpha U[0,3] deg, 16 on each of "
                     "4 jittered bump channels (144,000 / 589,824 / 1,152,000 / 2,387,718 tets)",
+        "mode": mode, "chains_per_mesh": chains if mode == "batched" else None,
+        "executor": r.get("executor"),
         "mesh_build_s": round(mesh_s, 2), "wall_s": round(r["wall_s"], 2),
         "n_conditions": len(conds), "n_converged": r["n_converged"],
         "conditions_per_s": round(len(conds) / r["wall_s"], 4),

@@ -59,7 +59,7 @@ def test_lockstep_solves_equal_solves_alone(channel, precision, batched):
                   for i, (m, a) in enumerate(CONDS)})
     st = ex.stats()
     print(f"[batch] executor {st}", flush=True)
-    assert st["max_steps_per_sync"] == len(CONDS)
+    assert st["max_steps_per_round"] == len(CONDS)
     if batched:
         assert st["batched_groups"] > 0
     for i, ref in enumerate(alone):

@@ -12,8 +12,10 @@ request at a time, in rounds:
     batched C-ABI (one launch per kernel for all instances of one mesh:
     the Jacobian matvec, the orthogonalisation, the norm and the
     normalisation), with only the V-cycles left per instance (each
-    instance has its own hierarchy) -- then ONE stream synchronisation
-    serves all of them;
+    instance has its own hierarchy); each solve then waits only for its
+    own previous step, queued a round earlier, and does that step's Givens
+    update on the host while the device runs the new steps (GMRES's
+    one-step-ahead pipeline, shared by all solves);
   * the residual evaluations of a round run per instance.
 
 A solve that returns leaves the round structure (the convergence mask is
@@ -41,8 +43,8 @@ class BatchExecutor:
     batched=True queues the Arnoldi steps of instances that share a mesh
     through the batched entry points (linear.batched_arnoldi); False queues
     each step by itself (the same kernels, one instance per launch).
-    Statistics: rounds, Arnoldi steps and evaluations executed, and the
-    largest number of steps served by one synchronisation."""
+    Statistics: rounds, Arnoldi steps and evaluations executed, the
+    largest number of steps queued in one round and the batched groups."""
 
     def __init__(self, batched: bool = True):
         self.batched = bool(batched)
@@ -110,7 +112,8 @@ class BatchExecutor:
             else:
                 for _, r in steps:
                     r.run()
-            torch.cuda.current_stream().synchronize()
+            # no synchronisation: each solve waits for its own earlier step
+            # (queued a round before) while these run on the device
             for k, _ in steps:
                 self._advance(k, None)
         for k, r in evals:
@@ -120,20 +123,20 @@ class BatchExecutor:
 
     def stats(self) -> dict:
         return {"rounds": self.n_rounds, "arnoldi_steps": self.n_steps,
-                "evaluations": self.n_evals, "max_steps_per_sync": self.max_width,
+                "evaluations": self.n_evals, "max_steps_per_round": self.max_width,
                 "batched_groups": self.n_batched_launch_groups}
 
 
 # ---------------------------------------------------------------- the sweep
 def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]],
-                        chains_per_mesh: int = 4, precision: str = "fp32",
+                        chains_per_mesh=4, precision: str = "fp32",
                         upwind_c: float = 2.0, dm: float = 0.05,
                         newton_kw: Optional[dict] = None, batched: bool = True,
                         log=None) -> Dict[str, object]:
     """Every (mesh index, M_inf, alpha_deg) condition, solved in lockstep.
 
     The conditions of each mesh are ordered by (M_inf, alpha) and dealt
-    round-robin into chains_per_mesh chains; a chain solves its conditions
+    round-robin into chains_per_mesh chains (one count, or one per mesh); a chain solves its conditions
     one after another, each warm-started from the chain's last converged
     condition as in solve_sweep (alpha continuation, then a Mach ramp; a
     cold ramp when that fails), and all chains of all meshes run side by
@@ -174,13 +177,17 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
         t_mesh1[k] = time.perf_counter()
         return None
 
+    nch = ([int(chains_per_mesh)] * len(meshes) if np.isscalar(chains_per_mesh)
+           else [int(c) for c in chains_per_mesh])
+    if len(nch) != len(meshes) or min(nch) < 1:
+        raise ValueError("chains_per_mesh: a positive count, or one per mesh")
     coros = {}
     for k in range(len(meshes)):
         ids = [i for i, c in enumerate(conditions) if c[0] == k]
         ids.sort(key=lambda i: (conditions[i][1], conditions[i][2]))
         t_mesh0[k] = time.perf_counter()
-        for c in range(min(chains_per_mesh, len(ids))):
-            coros[(k, c)] = chain(k, ids[c::chains_per_mesh])
+        for c in range(min(nch[k], len(ids))):
+            coros[(k, c)] = chain(k, ids[c::nch[k]])
     ex = BatchExecutor(batched=batched)
     ex.run(coros)
 
@@ -217,4 +224,4 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
     return {"instances": inst, "converged_mask": mask,
             "n_converged": int(sum(mask)), "wall_s": time.perf_counter() - t_all,
             "per_mesh": per_mesh, "executor": ex.stats(),
-            "chains_per_mesh": int(chains_per_mesh)}
+            "chains_per_mesh": nch}

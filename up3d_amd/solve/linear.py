@@ -130,8 +130,8 @@ def drive(gen):
 class ArnoldiStep:
     """One Arnoldi step of a stepped GMRES (gmres_steps with
     pipelined=False), handed to the executor: run() queues the step's device
-    work (enqueue(col)); after the executor's stream synchronisation the
-    coroutine reads the step's coefficients from pinned host memory."""
+    work (enqueue(col)), ending in the copy of its coefficients to pinned
+    host memory and an event the coroutine waits on before it reads them."""
 
     __slots__ = ("enqueue", "col", "matvec", "psolve", "gw", "f32")
 
@@ -241,12 +241,14 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
                 callback: Optional[Callable] = None, basis: str = "f64",
                 gw: Optional[dict] = None, pipelined: bool = True):
     """gmres_dev as a coroutine.  pipelined=True runs the one-step-ahead
-    loop and yields nothing; pipelined=False yields an ArnoldiStep per
-    Krylov step and resumes once its executor has run the step and
-    synchronised the stream (an executor can queue the steps of many
-    solves before one synchronisation).  Both queue the same device work in
-    the same order per solve, so they return the same bits.  gw: work
-    buffers from gmres_buffers (default: the shared per-shape set)."""
+    loop itself and yields nothing; pipelined=False yields an ArnoldiStep
+    wherever that loop queues a step, and resumes once its executor has
+    queued it (an executor can queue the steps of many solves, batched, and
+    then let each solve wait for its own previous step, so the device works
+    on the new steps while the host does the Givens updates).  Both queue
+    the same device work in the same order per solve, so they return the
+    same bits.  gw: work buffers from gmres_buffers (default: the shared
+    per-shape set)."""
     if basis not in ("f64", "f32"):
         raise ValueError(f"basis={basis!r}: 'f64' or 'f32'")
     f32 = basis == "f32"
@@ -349,13 +351,15 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
         col = 0
         if pipelined:
             enqueue(0)
+        else:
+            yield ArnoldiStep(enqueue, 0, matvec, psolve, gw, f32)
         for col in range(restart):
-            if pipelined:
-                if col + 1 < restart:
-                    enqueue(col + 1)        # runs while the host does step col
-                evs[col % 2].synchronize()
-            else:
-                yield ArnoldiStep(enqueue, col, matvec, psolve, gw, f32)
+            if col + 1 < restart:           # runs while the host does step col
+                if pipelined:
+                    enqueue(col + 1)
+                else:
+                    yield ArnoldiStep(enqueue, col + 1, matvec, psolve, gw, f32)
+            evs[col % 2].synchronize()
             k = col + 1
             host = hbuf[col % 2][: 3 + 2 * k].numpy()
             h0 = float(np.sqrt(max(host[2 + k], 0.0)))
