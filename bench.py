@@ -639,7 +639,17 @@ def run_solves(mesh, config, tune=True):
     res = {"method": ("grid-sequenced " + " -> ".join(f"{len(m.elements)} tets" for m in chain)
                       + "; Newton + GMRES(200) + SA-AMG of the clamped Jacobian, adaptive "
                       "AMG rebuild; upwind_c 3 Mach ramp, 3 -> 2 -> 1.5 on the middle mesh, "
-                      "fine mesh at 1.5")}
+                      "fine mesh at 1.5"),
+           "differs_from_stated_recipe": [
+               "GMRES(200), not GMRES(50): GMRES(50) with the Picard AMG stalls on the "
+               "fine mesh at M 0.84",
+               "AMG built on the clamped (SPD) Newton Terms 1+2, not on the Picard matrix",
+               "grid sequencing 48x16x16 -> 96x32x32 -> 192x64x64 instead of a Mach ramp "
+               "on the fine mesh",
+               "upwind_c ramp 3 -> 2 -> 1.5 (the reference's upwind_c_post stages)",
+               "Dirichlet phi = x on the inflow and outflow planes",
+               "acceptance by the reference's frozen-selection rule: the frozen system "
+               "below 1e-10, live residual reported as residual_inf_live"]}
     phis, cps = {}, {}
 
     def one(prec):

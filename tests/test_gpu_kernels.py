@@ -210,6 +210,48 @@ def test_gather_assembly_bitwise(mesh, ora):
         <= 1e-13 * np.abs(Jc).max()
 
 
+def test_coloured_scatter_fallback(mesh, ora):
+    """A mesh the node gather cannot take (a CSR row block beyond the LDS
+    bound) assembles its matrices by the coloured scatter: forced here on
+    the test channel, the Picard matrix and Terms 1+2 equal the oracle's
+    colour-order assembly bit for bit, the SPD variant the serial one to
+    1e-13."""
+    import torch
+
+    from up3d_amd.device import to_dev, to_host
+    from up3d_amd.kernels.jacobian import PicardOperator
+
+    op = PicardOperator(mesh.nodes, mesh.elements)
+    st = op.store
+    oc = O.OraclePicardOperator(mesh.nodes, mesh.elements, order="colored")
+    oc.B = np.ascontiguousarray(st.B_host.reshape(-1, 4, 3))
+    oc.V = np.ascontiguousarray(st.V_host)
+    assert np.array_equal(oc.color_offsets, st.color_offsets)
+    os_ = O.OraclePicardOperator(mesh.nodes, mesh.elements)
+    os_.B, os_.V = oc.B, oc.V
+    n = mesh.n_tets
+    rt = np.linspace(0.7, 1.3, n)
+    g = ora["g"]
+    se = np.sin(np.arange(n)) * (np.arange(n) % 3 != 0)
+    lim = (np.arange(n) % 5 != 0).astype(np.uint8)
+    rtd, gd, sed = to_dev(rt), to_dev(g.reshape(-1, 3)), to_dev(se)
+    limd = to_dev(lim, torch.uint8)
+    out = torch.full((op.nnz,), np.nan, dtype=torch.float64, device=rtd.device)
+    assert st.gather_ok
+    st.gather_ok = False
+    try:
+        A = to_host(op.matrix_dev(rtd, out))
+        J = to_host(op.jacobian_dev(rtd, gd, sed, limd, 0.5, out))
+        Js = to_host(op.jacobian_spd_dev(rtd, gd, -(sed.abs() * 40.0), limd, 1.0, 0.9,
+                                         out))
+    finally:
+        st.gather_ok = True
+    assert np.array_equal(A, oc.assemble_matrix(rt).data)
+    assert np.array_equal(J, oc.jacobian_terms12(g, rt, se * lim.astype(np.float64), 0.5))
+    ref = os_.jacobian_spd_terms12(g, rt, -np.abs(se) * 40.0, 1.0, 0.9, lim)
+    assert np.abs(Js - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
 def test_picard_operator_api(mesh, ora):
     from up3d_amd.kernels.jacobian import PicardOperator
 

@@ -187,6 +187,8 @@ class PicardOperator:
 
     def _gather_matrix(self, rho_tilde, grad, s_e, lim, inv_u2, clamp, out):
         st = self.store
+        if not st.gather_ok:
+            return self._scatter_matrix(rho_tilde, grad, s_e, lim, inv_u2, clamp, out)
         if self.d_K is None:
             self.d_K = empty(16 * self.n_tets)
         _native.call("fp3d_element_matrix_terms", self.n_tets, ptr(st.B),
@@ -198,6 +200,28 @@ class PicardOperator:
         _native.call("fp3d_gather_csr_terms", self.n_nodes, st.gather_max_seg,
                      ptr(st.indptr), ptr(st.inc_ptr), None,
                      ptr(st.rel8), ptr(self.d_K), ptr(out), stream_handle())
+        return out
+
+    def _scatter_matrix(self, rho_tilde, grad, s_e, lim, inv_u2, clamp, out):
+        """The same matrices by the coloured scatter (the reference's
+        colour-class order; no LDS or row-length bound), for meshes the
+        node gather cannot take (a CSR row longer than 256 entries, or a
+        row block longer than 8,192)."""
+        st = self.store
+        out.zero_()
+        common = (st.n_colors, st.offsets_ct, ptr(st.celems), ptr(st.e2c), ptr(st.B),
+                  ptr(st.V), ptr(rho_tilde))
+        p = lambda t: None if t is None else ptr(t)  # noqa: E731
+        if s_e is None:
+            _native.call("fp3d_assemble_matrix_colored", *common, ptr(out),
+                         stream_handle())
+        elif clamp == 0.0:
+            _native.call("fp3d_assemble_jacobian_colored", *common, ptr(grad), ptr(s_e),
+                         p(lim), float(inv_u2), ptr(out), stream_handle())
+        else:
+            _native.call("fp3d_assemble_jacobian_spd_colored", *common, ptr(grad),
+                         ptr(s_e), p(lim), float(inv_u2), float(clamp), ptr(out),
+                         stream_handle())
         return out
 
     def matrix_dev(self, rho_tilde: torch.Tensor, out: torch.Tensor) -> torch.Tensor:

@@ -144,10 +144,11 @@ class MeshStore:
         ends = np.minimum(starts + rows, self.n_nodes)
         self.gather_max_seg = int((indptr[ends].astype(np.int64)
                                    - indptr[starts]).max()) if self.n_nodes else 1
-        if self.gather_max_seg * 8 > 65536:
-            raise ValueError(f"node valence too high for the gather assembly "
-                             f"({self.gather_max_seg} CSR entries per "
-                             f"{rows}-row block)")
+        # the node-gather matrix assembly stages a row block's CSR slice in
+        # LDS (at most 8,192 entries) and addresses a slot by its byte offset
+        # inside the row (rows of at most 256 entries); a mesh beyond either
+        # bound assembles its matrices with the coloured scatter instead
+        self.gather_ok = self.gather_max_seg * 8 <= 65536
         # ---- upload
         self.el = to_dev(el, torch.int32)
         self.B = to_dev(B.reshape(self.n_tets, 12), torch.float64)
@@ -156,11 +157,14 @@ class MeshStore:
         # slot offset inside its row, one byte per (e, a, b)
         rel = e2c.reshape(self.n_tets, 4, 4) - indptr[el][:, :, None]
         if self.n_tets and (rel.min() < 0 or rel.max() > 255):
-            raise ValueError("a CSR row is longer than 256 entries")
-        # stored in node-incidence order (entry q holds the four offsets of
-        # incidence inc_ent[q]), so a row's gather reads them contiguously
-        rel_inc = np.ascontiguousarray(rel.astype(np.uint8)).reshape(-1, 4)[order]
-        self.rel8 = to_dev(np.ascontiguousarray(rel_inc).reshape(-1), torch.uint8)
+            self.gather_ok = False
+        self.rel8 = None
+        if self.gather_ok:
+            # stored in node-incidence order (entry q holds the four offsets
+            # of incidence inc_ent[q]), so a row's gather reads them
+            # contiguously
+            rel_inc = np.ascontiguousarray(rel.astype(np.uint8)).reshape(-1, 4)[order]
+            self.rel8 = to_dev(np.ascontiguousarray(rel_inc).reshape(-1), torch.uint8)
         self.fn = to_dev(fn.astype(np.int32), torch.int32)
         self.cen = to_dev(cen, torch.float64)
         # per-face centroid displacement (0 on the boundary)

@@ -58,10 +58,15 @@ def sweep_conditions(n_meshes: int = 4, per_mesh: int = 16, seed: int = 20260105
     return out
 
 
-def _level(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm, newton_kw, verbose,
-           stepped=False):
-    """One Mach ramp m_start -> m_inf at alpha from phi (None: freestream),
-    as a coroutine (transonic_steps)."""
+def _level(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm, newton_kw, verbose):
+    """One Mach ramp m_start -> m_inf at alpha from phi (None: freestream)."""
+    return drive(_level_steps(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm,
+                              newton_kw, verbose))
+
+
+def _level_steps(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm, newton_kw, verbose,
+                 stepped=False):
+    """_level as a coroutine (transonic_steps)."""
     dv = freestream_phi(mesh.nodes[dn], alpha)
     kw = dict(newton_kw)
     phi0 = freestream_phi(mesh.nodes, alpha) if phi is None else np.array(phi, dtype=np.float64)
@@ -87,7 +92,7 @@ def _solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verb
         n_a = int(np.ceil(abs(alpha - a0) / dalpha_max - 1e-12))
         ok = True
         for j in range(1, n_a + 1):
-            r = yield from _level(mesh, dn, m0, a0 + (alpha - a0) * j / n_a, phi, m0,
+            r = yield from _level_steps(mesh, dn, m0, a0 + (alpha - a0) * j / n_a, phi, m0,
                                   upwind_c, dm, newton_kw, verbose, stepped)
             subs.append(r)
             if not r["converged"]:
@@ -95,12 +100,12 @@ def _solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verb
                 break
             phi = r["phi"]
         if ok:
-            r = yield from _level(mesh, dn, m_inf, alpha, phi, max(m0, 0.6), upwind_c, dm,
+            r = yield from _level_steps(mesh, dn, m_inf, alpha, phi, max(m0, 0.6), upwind_c, dm,
                                   newton_kw, verbose, stepped)
             subs.append(r)
             if r["converged"]:
                 return r, "warm", subs
-    r = yield from _level(mesh, dn, m_inf, alpha, None, 0.6, upwind_c, dm, newton_kw,
+    r = yield from _level_steps(mesh, dn, m_inf, alpha, None, 0.6, upwind_c, dm, newton_kw,
                           verbose, stepped)
     subs.append(r)
     return r, "cold", subs
