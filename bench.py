@@ -105,6 +105,8 @@ def main():
                     help="only run the sweep (--sweep-mode) and print its JSON")
     ap.add_argument("--sweep-mode", default="batched",
                     choices=["batched", "sequential", "both"])
+    ap.add_argument("--no-coarse", action="store_true",
+                    help="sweep without the coarse-companion route")
     ap.add_argument("--sweep-chains", default="4,2,1,1",
                     help="warm-start chains per mesh of the batched sweep: one count, "
                          "or one per mesh separated by commas")
@@ -151,7 +153,9 @@ def main():
         for mode in modes:
             for ch in (args.sweep_chains.split(";") if mode == "batched" else ["1"]):
                 key = mode if mode == "sequential" else f"batched[{ch}]"
-                out[key] = run_sweep(mode, _chains(ch), meshes)
+                if args.no_coarse:
+                    key += "/no-coarse"
+                out[key] = run_sweep(mode, _chains(ch), meshes, coarse=not args.no_coarse)
                 log(f"[bench] sweep {key}: wall {out[key]['wall_s']} s, "
                     f"{out[key]['n_converged']} converged, executor {out[key]['executor']}")
         print(json.dumps(out), flush=True)
@@ -904,28 +908,32 @@ def _chains(text: str):
     return parts[0] if len(parts) == 1 else parts
 
 
-def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None):
+def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None, coarse: bool = True):
     """Sweep of 16 seeded (M_inf in [0.60, 0.86], alpha in [0, 3] deg)
     conditions on each of 4 jittered RCM bump channels (144,000 / 589,824 /
     1,152,000 / 2,387,718 tets), FP32-preconditioned Newton, every
     instance to ||R_free||_inf < 1e-10 with 0 limited / floored.
     mode "batched": `chains` warm-start chains per mesh, all advanced in
     lockstep with batched Arnoldi launches (solve/batch.py); "sequential":
-    one condition after another (solve/sweep.py)."""
+    one condition after another (solve/sweep.py).  coarse: supercritical
+    conditions on the two large meshes start from a solve on a 73,728-tet
+    companion channel (solve/sweep.py coarse_route)."""
     import torch
 
     from up3d_amd.solve.batch import solve_sweep_batched
-    from up3d_amd.solve.sweep import solve_sweep, sweep_conditions, sweep_meshes
+    from up3d_amd.solve.sweep import (companion_mesh, solve_sweep, sweep_conditions,
+                                      sweep_meshes)
 
     t0 = time.perf_counter()
     if meshes is None:
         meshes = sweep_meshes()
     mesh_s = time.perf_counter() - t0
     conds = sweep_conditions(len(meshes), 16)
+    comp = companion_mesh() if coarse else None
     if mode == "batched":
-        r = solve_sweep_batched(meshes, conds, chains_per_mesh=chains, log=log)
+        r = solve_sweep_batched(meshes, conds, chains_per_mesh=chains, log=log, coarse=comp)
     elif mode == "sequential":
-        r = solve_sweep(meshes, conds, log=log)
+        r = solve_sweep(meshes, conds, log=log, coarse=comp)
     else:
         raise ValueError(f"sweep mode {mode!r}")
     torch.cuda.synchronize()
@@ -935,6 +943,7 @@ def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None):
         "workload": "64 conditions: M_inf U[0.60,0.86], alpha U[0,3] deg, 16 on each of "
                     "4 jittered bump channels (144,000 / 589,824 / 1,152,000 / 2,387,718 tets)",
         "mode": mode, "chains_per_mesh": chains if mode == "batched" else None,
+        "coarse_route": bool(coarse),
         "executor": r.get("executor"),
         "mesh_build_s": round(mesh_s, 2), "wall_s": round(r["wall_s"], 2),
         "n_conditions": len(conds), "n_converged": r["n_converged"],
@@ -942,11 +951,13 @@ def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None):
         "newton_steps": int(sum(i["n_newton"] for i in inst)),
         "gmres_iterations": int(sum(i["n_lin_iters"] for i in inst)),
         "mtet_newton_steps_per_s": round(tet_steps / r["wall_s"] / 1e6, 2),
-        "routes": {k: sum(1 for i in inst if i["route"] == k) for k in ("warm", "cold")},
+        "routes": {k: sum(1 for i in inst if i["route"] == k)
+                   for k in ("coarse", "warm", "cold")},
         "per_mesh": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in pm.items()}
                      for pm in r["per_mesh"]],
         "instances": [{"mesh": i["mesh"], "m_inf": round(i["m_inf"], 4),
                        "alpha_deg": round(i["alpha_deg"], 4), "converged": i["converged"],
+                       "route": i["route"],
                        "n_newton": i["n_newton"], "seconds": round(i["seconds"], 2),
                        "cf": [round(c, 8) for c in i["cf"][:2]]} for i in inst],
     }

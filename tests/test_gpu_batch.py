@@ -94,3 +94,30 @@ def test_batched_sweep_converges_and_matches_sequential(gpu):
         assert a["residual_inf"] < 1e-10
         if a["m_inf"] < 0.7:          # subcritical: one discrete solution
             assert np.abs(np.array(a["cf"]) - np.array(b["cf"])).max() < 1e-8
+
+
+def test_coarse_route_sweep_sequential_equals_lockstep(gpu):
+    """Supercritical conditions started from a companion-channel solve (the
+    coarse route): every condition converges by that route, and the
+    lockstep sweep with one chain per mesh reproduces the sequential sweep's
+    wall forces bit for bit."""
+    from up3d_amd.mesh.channel import bump_channel
+    from up3d_amd.solve.batch import solve_sweep_batched
+    from up3d_amd.solve.sweep import companion_mesh, solve_sweep
+
+    meshes = [bump_channel(36, 12, 4, jitter=0.15, seed=31, rcm=True)]
+    comp = companion_mesh((18, 6, 4), seed=32)
+    conds = [(0, 0.78, 1.0), (0, 0.74, 2.0), (0, 0.64, 0.5)]
+    kw = {"gmres_restart": 40, "amg_max_coarse": 200}
+    print(f"[batch] coarse route: {len(meshes[0].elements)} tets, companion "
+          f"{len(comp[0].elements)}", flush=True)
+    rs = solve_sweep(meshes, conds, newton_kw=kw, coarse=comp, coarse_min_tets=0)
+    rb = solve_sweep_batched(meshes, conds, chains_per_mesh=1, newton_kw=kw, coarse=comp,
+                             coarse_min_tets=0)
+    assert rs["n_converged"] == rb["n_converged"] == len(conds)
+    routes = [i["route"] for i in rs["instances"]]
+    print(f"[batch] routes {routes}", flush=True)
+    assert routes[:2] == ["coarse", "coarse"] and routes[2] != "coarse"
+    for a, b in zip(rs["instances"], rb["instances"]):
+        assert a["route"] == b["route"] and a["n_newton"] == b["n_newton"]
+        assert a["cf"] == b["cf"] and a["residual_inf"] < 1e-10

@@ -132,7 +132,8 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
                         chains_per_mesh=4, precision: str = "fp32",
                         upwind_c: float = 2.0, dm: float = 0.05,
                         newton_kw: Optional[dict] = None, batched: bool = True,
-                        log=None) -> Dict[str, object]:
+                        log=None, coarse=None,
+                        coarse_min_tets: int = 500_000) -> Dict[str, object]:
     """Every (mesh index, M_inf, alpha_deg) condition, solved in lockstep.
 
     The conditions of each mesh are ordered by (M_inf, alpha) and dealt
@@ -142,11 +143,12 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
     cold ramp when that fails), and all chains of all meshes run side by
     side through one BatchExecutor.  A condition whose chain could not
     converge it is retried once at the end from the nearest converged
-    condition of its mesh with a finer Mach step (as solve_sweep).
+    condition of its mesh with a finer Mach step (as solve_sweep).  coarse:
+    a companion_mesh() pair enabling solve_sweep's coarse route.
     Returns solve_sweep's keys plus "executor" statistics."""
     from ..mesh.channel import boundary_nodes
     from ..post.surface import wall_force_coefficients
-    from .sweep import _record, _solve_one_steps
+    from .sweep import _record, _solve_one_steps, coarse_route
 
     kw = {"precision": precision, "gmres_restart": 200, "gmres_maxiter": 5,
           "n_newton_max": 30, "amg_rebuild_every": 0,
@@ -165,7 +167,8 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
             _, m, a = conditions[i]
             t0 = time.perf_counter()
             r, route, subs = yield from _solve_one_steps(
-                meshes[k], dns[k], m, a, last, upwind_c, dm, kw, False, stepped=True)
+                meshes[k], dns[k], m, a, last, upwind_c, dm, kw, False, stepped=True,
+                coarse=coarse_route(meshes[k], coarse, m, coarse_min_tets))
             inst[i] = _record(meshes[k], k, m, a, r, subs, route,
                               time.perf_counter() - t0, False, wall_force_coefficients)
             if r["converged"]:

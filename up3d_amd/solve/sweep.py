@@ -80,13 +80,30 @@ def _level_steps(mesh, dn, m_inf, alpha, phi, m_start, upwind_c, dm, newton_kw, 
 
 
 def _solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verbose,
-                     dalpha_max: float = 0.5, stepped: bool = False):
-    """(result, route, sub-results), as a coroutine.  From a converged
-    neighbour seed = (phi, M, alpha): alpha continuation at the neighbour's
-    Mach in steps of at most dalpha_max degrees, then a Mach ramp to m_inf;
-    when that fails (or without a seed), a cold Mach ramp from 0.6 at this
-    alpha."""
+                     dalpha_max: float = 0.5, stepped: bool = False, coarse=None):
+    """(result, route, sub-results), as a coroutine.  With coarse =
+    (companion mesh, its Dirichlet nodes): first the "coarse" route -- a
+    cold Mach ramp to (m_inf, alpha) on the companion mesh, its P1
+    interpolant on this mesh, one Newton level at m_inf.  Then (or without
+    it) from a converged neighbour seed = (phi, M, alpha): alpha
+    continuation at the neighbour's Mach in steps of at most dalpha_max
+    degrees, then a Mach ramp to m_inf; when that fails (or without a
+    seed), a cold Mach ramp from 0.6 at this alpha."""
     subs = []
+    if coarse is not None:
+        from ..mesh.transfer import interpolate_p1
+
+        cm, cdn = coarse
+        rc = yield from _level_steps(cm, cdn, m_inf, alpha, None, 0.6, upwind_c, dm,
+                                     newton_kw, verbose, stepped)
+        subs.append(rc)
+        if rc["converged"]:
+            phi0 = interpolate_p1(cm.nodes, cm.elements, rc["phi"], mesh.nodes)
+            r = yield from _level_steps(mesh, dn, m_inf, alpha, phi0, m_inf, upwind_c, dm,
+                                        newton_kw, verbose, stepped)
+            subs.append(r)
+            if r["converged"]:
+                return r, "coarse", subs
     if seed is not None:
         phi, m0, a0 = seed
         n_a = int(np.ceil(abs(alpha - a0) / dalpha_max - 1e-12))
@@ -112,16 +129,37 @@ def _solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verb
 
 
 def _solve_one(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verbose,
-               dalpha_max: float = 0.5):
+               dalpha_max: float = 0.5, coarse=None):
     return drive(_solve_one_steps(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw,
-                                  verbose, dalpha_max))
+                                  verbose, dalpha_max, coarse=coarse))
+
+
+def coarse_route(mesh, coarse, m_inf, min_tets: int = 500_000, m_min: float = 0.72):
+    """The companion (mesh, Dirichlet nodes) pair for the coarse route of a
+    condition, or None: used for supercritical conditions (m_inf >= m_min)
+    on meshes of at least min_tets tets, where a shock has to form or move
+    across a fine mesh."""
+    if coarse is None or len(mesh.elements) < min_tets or m_inf < m_min:
+        return None
+    return coarse
+
+
+def companion_mesh(shape=(48, 16, 16), jitter: float = 0.15, seed: int = 20260110,
+                   rcm: bool = True):
+    """(mesh, Dirichlet nodes) of the coarse channel the coarse route
+    solves on (73,728 tets)."""
+    m = bump_channel(*shape, jitter=jitter, seed=seed, rcm=rcm)
+    return m, boundary_nodes(m, ["inflow", "outflow"])
 
 
 def solve_sweep(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]],
                 precision: str = "fp32", upwind_c: float = 2.0, dm: float = 0.05,
                 newton_kw: Optional[dict] = None, retry: bool = True,
-                log=None, verbose: bool = False) -> Dict[str, object]:
-    """Solve every (mesh index, M_inf, alpha_deg) condition.  Returns
+                log=None, verbose: bool = False, coarse=None,
+                coarse_min_tets: int = 500_000) -> Dict[str, object]:
+    """Solve every (mesh index, M_inf, alpha_deg) condition (coarse: a
+    companion_mesh() pair that enables the coarse route on meshes of at
+    least coarse_min_tets tets, see coarse_route).  Returns
     {"instances": [...], "converged_mask", "n_converged", "wall_s",
     "per_mesh": [...]} (log: optional callable taking one progress line per
     instance) with, per instance, its mesh, M_inf, alpha, converged,
@@ -146,7 +184,9 @@ def solve_sweep(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]]
         for i in ids:
             _, m, a = conditions[i]
             t0 = time.perf_counter()
-            r, route, subs = _solve_one(mesh, dn, m, a, last, upwind_c, dm, kw, verbose)
+            r, route, subs = _solve_one(mesh, dn, m, a, last, upwind_c, dm, kw, verbose,
+                                        coarse=coarse_route(mesh, coarse, m,
+                                                            coarse_min_tets))
             if torch.cuda.is_available():
                 torch.cuda.synchronize()
             inst[i] = _record(mesh, k, m, a, r, subs, route,
