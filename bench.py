@@ -915,9 +915,9 @@ def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None, coarse: b
     instance to ||R_free||_inf < 1e-10 with 0 limited / floored.
     mode "batched": `chains` warm-start chains per mesh, all advanced in
     lockstep with batched Arnoldi launches (solve/batch.py); "sequential":
-    one condition after another (solve/sweep.py).  coarse: supercritical
-    conditions on the two large meshes start from a solve on a 73,728-tet
-    companion channel (solve/sweep.py coarse_route)."""
+    one condition after another (solve/sweep.py).  coarse: conditions on
+    the three meshes of at least 500,000 tets start from a solve on a
+    73,728-tet companion channel (solve/sweep.py coarse_route)."""
     import torch
 
     from up3d_amd.solve.batch import solve_sweep_batched

@@ -111,9 +111,10 @@ def test_coarse_route_sweep_sequential_equals_lockstep(gpu):
     kw = {"gmres_restart": 40, "amg_max_coarse": 200}
     print(f"[batch] coarse route: {len(meshes[0].elements)} tets, companion "
           f"{len(comp[0].elements)}", flush=True)
-    rs = solve_sweep(meshes, conds, newton_kw=kw, coarse=comp, coarse_min_tets=0)
+    rs = solve_sweep(meshes, conds, newton_kw=kw, coarse=comp, coarse_min_tets=0,
+                     coarse_min_mach=0.72)
     rb = solve_sweep_batched(meshes, conds, chains_per_mesh=1, newton_kw=kw, coarse=comp,
-                             coarse_min_tets=0)
+                             coarse_min_tets=0, coarse_min_mach=0.72)
     assert rs["n_converged"] == rb["n_converged"] == len(conds)
     routes = [i["route"] for i in rs["instances"]]
     print(f"[batch] routes {routes}", flush=True)

@@ -132,8 +132,8 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
                         chains_per_mesh=4, precision: str = "fp32",
                         upwind_c: float = 2.0, dm: float = 0.05,
                         newton_kw: Optional[dict] = None, batched: bool = True,
-                        log=None, coarse=None,
-                        coarse_min_tets: int = 500_000) -> Dict[str, object]:
+                        log=None, coarse=None, coarse_min_tets: int = 500_000,
+                        coarse_min_mach: float = 0.0) -> Dict[str, object]:
     """Every (mesh index, M_inf, alpha_deg) condition, solved in lockstep.
 
     The conditions of each mesh are ordered by (M_inf, alpha) and dealt
@@ -168,7 +168,8 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
             t0 = time.perf_counter()
             r, route, subs = yield from _solve_one_steps(
                 meshes[k], dns[k], m, a, last, upwind_c, dm, kw, False, stepped=True,
-                coarse=coarse_route(meshes[k], coarse, m, coarse_min_tets))
+                coarse=coarse_route(meshes[k], coarse, m, coarse_min_tets,
+                                    coarse_min_mach))
             inst[i] = _record(meshes[k], k, m, a, r, subs, route,
                               time.perf_counter() - t0, False, wall_force_coefficients)
             if r["converged"]:

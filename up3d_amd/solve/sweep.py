@@ -134,11 +134,12 @@ def _solve_one(mesh, dn, m_inf, alpha, seed, upwind_c, dm, newton_kw, verbose,
                                   verbose, dalpha_max, coarse=coarse))
 
 
-def coarse_route(mesh, coarse, m_inf, min_tets: int = 500_000, m_min: float = 0.72):
+def coarse_route(mesh, coarse, m_inf, min_tets: int = 500_000, m_min: float = 0.0):
     """The companion (mesh, Dirichlet nodes) pair for the coarse route of a
-    condition, or None: used for supercritical conditions (m_inf >= m_min)
-    on meshes of at least min_tets tets, where a shock has to form or move
-    across a fine mesh."""
+    condition, or None: used for conditions with m_inf >= m_min on meshes of
+    at least min_tets tets, where a Mach or alpha continuation on the mesh
+    itself costs many of its Newton steps (a shock forming and moving across
+    it, alpha jumps of several continuation levels)."""
     if coarse is None or len(mesh.elements) < min_tets or m_inf < m_min:
         return None
     return coarse
@@ -156,10 +157,12 @@ def solve_sweep(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]]
                 precision: str = "fp32", upwind_c: float = 2.0, dm: float = 0.05,
                 newton_kw: Optional[dict] = None, retry: bool = True,
                 log=None, verbose: bool = False, coarse=None,
-                coarse_min_tets: int = 500_000) -> Dict[str, object]:
+                coarse_min_tets: int = 500_000,
+                coarse_min_mach: float = 0.0) -> Dict[str, object]:
     """Solve every (mesh index, M_inf, alpha_deg) condition (coarse: a
     companion_mesh() pair that enables the coarse route on meshes of at
-    least coarse_min_tets tets, see coarse_route).  Returns
+    least coarse_min_tets tets for M_inf >= coarse_min_mach, see
+    coarse_route).  Returns
     {"instances": [...], "converged_mask", "n_converged", "wall_s",
     "per_mesh": [...]} (log: optional callable taking one progress line per
     instance) with, per instance, its mesh, M_inf, alpha, converged,
@@ -186,7 +189,8 @@ def solve_sweep(meshes: Sequence, conditions: Sequence[Tuple[int, float, float]]
             t0 = time.perf_counter()
             r, route, subs = _solve_one(mesh, dn, m, a, last, upwind_c, dm, kw, verbose,
                                         coarse=coarse_route(mesh, coarse, m,
-                                                            coarse_min_tets))
+                                                            coarse_min_tets,
+                                                            coarse_min_mach))
             if torch.cuda.is_available():
                 torch.cuda.synchronize()
             inst[i] = _record(mesh, k, m, a, r, subs, route,
