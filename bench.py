@@ -101,6 +101,12 @@ def main():
     ap.add_argument("--batch-profile", action="store_true",
                     help="only time K lockstep Krylov solves per sweep mesh (batched "
                          "launches, per-solve launches, one after another)")
+    ap.add_argument("--batch-meshes", default="0,1,3",
+                    help="with --batch-profile: sweep mesh indices")
+    ap.add_argument("--batch-k", default="1,4,16",
+                    help="with --batch-profile: solve counts K")
+    ap.add_argument("--batch-modes", default="batched,per_solve,alone",
+                    help="with --batch-profile: batched, per_solve and/or alone")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only run the sweep (--sweep-mode) and print its JSON")
     ap.add_argument("--sweep-mode", default="batched",
@@ -139,7 +145,10 @@ def main():
         import torch
 
         torch.cuda.set_device(0)
-        print(json.dumps(run_batch_profile()), flush=True)
+        print(json.dumps(run_batch_profile(
+            ks=tuple(int(x) for x in args.batch_k.split(",")),
+            mesh_ids=tuple(int(x) for x in args.batch_meshes.split(",")),
+            modes=tuple(args.batch_modes.split(",")))), flush=True)
         return
     if args.sweep_only:
         import torch
@@ -826,7 +835,7 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
 
 
 def run_batch_profile(ks=(1, 4, 16), iters: int = 120, restart: int = 60,
-                      mesh_ids=(0, 1, 3)):
+                      mesh_ids=(0, 1, 3), modes=("batched", "per_solve", "alone")):
     """Krylov steps per second of K lockstep GMRES solves on each sweep
     mesh: K copies of one Newton system (M_inf 0.80, the constructed state;
     each copy its own workspace, Jacobian and AMG hierarchy) advanced by the
@@ -877,6 +886,8 @@ def run_batch_profile(ks=(1, 4, 16), iters: int = 120, restart: int = 60,
                         for i in range(k)}
             row = {}
             for label, batched in (("batched", True), ("per_solve", False)):
+                if label not in modes:
+                    continue
                 BatchExecutor(batched).run(coros())          # warm-up (graphs)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -885,15 +896,18 @@ def run_batch_profile(ks=(1, 4, 16), iters: int = 120, restart: int = 60,
                 dt = time.perf_counter() - t0
                 n_steps = sum(r[2] for r in res.values())
                 row[label + "_us_per_step"] = round(dt / n_steps * 1e6, 1)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            n_steps = 0
-            for i in range(k):
-                _, _, it = gmres_dev(wss[i].jac.matvec, rhss[i], Ms[i].apply, rtol=1e-30,
-                                     restart=restart, maxiter=maxiter, basis="f32")
-                n_steps += it
-            torch.cuda.synchronize()
-            row["alone_us_per_step"] = round((time.perf_counter() - t0) / n_steps * 1e6, 1)
+            if "alone" in modes:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                n_steps = 0
+                for i in range(k):
+                    _, _, it = gmres_dev(wss[i].jac.matvec, rhss[i], Ms[i].apply,
+                                         rtol=1e-30, restart=restart, maxiter=maxiter,
+                                         basis="f32")
+                    n_steps += it
+                torch.cuda.synchronize()
+                row["alone_us_per_step"] = round((time.perf_counter() - t0) / n_steps * 1e6,
+                                                 1)
             rec[f"K{k}"] = row
             log(f"[batchprof] mesh {mi} K={k}: {row}")
         out["meshes"].append(rec)
