@@ -392,6 +392,30 @@ int fp3d_multi_dot_norm_noalloc(int64_t n, int nv, const double *V, int64_t ld,
  * out[0..1] = norms of w, vnext = w / out[0] (float32 when f32); max_nv = the
  * largest word 2, at most 256.  Per solve the same kernel bodies on the same
  * grids as the single-solve calls: the same bits. */
+/* Batched live residual evaluation (no reference counterpart, as the
+ * batched Arnoldi steps): for nb solves on one mesh, the chain of the
+ * Newton driver's residual evaluation -- fp3d_element_state (with grad.B),
+ * fp3d_count_limited, fp3d_upwind_rho_tilde_sens_d, fp3d_upwind_monitors,
+ * fp3d_element_residual_terms (incidence order) + fp3d_gather_node_terms,
+ * fp3d_mask_copy and fp3d_norms -- one launch per kernel for all solves.
+ * `jobs`: device array of nb records of fp3d_batch_eval_job_bytes() = 192
+ * bytes, 24 words: 0 phi, 1 grad, 2 q2n, 3 q2l, 4 rho, 5 lim (uint8),
+ * 6 grad.B, 7 upstream (int32), 8 nu, 9 rho_tilde, 10 s_e, 11 s_u,
+ * 12 element terms (4 n_tets), 13 R, 14 masked R, 15 free mask (uint8),
+ * 16 work (3 * 1024 doubles), 17 out (double[6]: limited count, nu max,
+ * count nu > 0, floored count, ||masked R||_2, ||masked R||_inf),
+ * 18 m_inf and 19 q2 cap (float64 bit patterns), 20-23 reserved (0).  The
+ * mesh arrays and u2, gamma, m_crit, C, rho_floor, max_hops are shared.  Per
+ * solve the same kernel bodies on the same grids as the single calls: the
+ * same bits. */
+int fp3d_batch_eval_job_bytes(void);
+int fp3d_batch_residual_eval(int nb, const void *jobs, int64_t n_tets, int64_t n_nodes,
+                             const int32_t *elements, const double *B, const double *V,
+                             const int32_t *face_neighbors, const double *face_disp,
+                             const double *nodes, const int32_t *inc_ptr,
+                             const int32_t *inc_pos, double u2, double gamma,
+                             double m_crit, double C, double rho_floor, int max_hops,
+                             void *stream);
 int fp3d_batch_job_bytes(void);
 int fp3d_batch_jacobian_matvec(int nb, const void *jobs, int64_t max_active,
                                int64_t n_rows, int64_t nnz, const int32_t *indptr,

@@ -3,10 +3,10 @@ run one at a time:
 
   * several transonic Mach ramps on one channel, at different M_inf and
     alpha, advanced together by the BatchExecutor -- with the batched
-    Arnoldi launches (one launch per kernel for every solve on the mesh) and
-    with per-solve launches -- give every solve's potential, residual
-    history and Krylov count bit for bit as the solve run alone, for the
-    float64 and the float32-basis precisions;
+    Arnoldi steps and residual evaluations (one launch per kernel for every
+    solve on the mesh) and with per-solve launches -- give every solve's
+    potential, residual history and Krylov count bit for bit as the solve
+    run alone, for the float64 and the float32-basis precisions;
   * the batched sweep converges every condition of a small two-mesh sweep
     and agrees with the sequential sweep on its subcritical conditions."""
 
@@ -61,7 +61,7 @@ def test_lockstep_solves_equal_solves_alone(channel, precision, batched):
     print(f"[batch] executor {st}", flush=True)
     assert st["max_steps_per_round"] == len(CONDS)
     if batched:
-        assert st["batched_groups"] > 0
+        assert st["batched_groups"] > 0 and st["batched_eval_groups"] > 0
     for i, ref in enumerate(alone):
         r = res[i]
         assert ref["converged"] and r["converged"]

@@ -110,6 +110,10 @@ _SIGS = {
     "fp3d_multi_dot_norm_f32v": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _vp, _vp]),
     "fp3d_multi_dot_norm_noalloc": (_int, [_i64, _int, _vp, _i64, _vp, _vp, _vp]),
     "fp3d_batch_job_bytes": (_int, []),
+    "fp3d_batch_eval_job_bytes": (_int, []),
+    "fp3d_batch_residual_eval": (_int, [_int, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _dbl, _int,
+                                        _vp]),
     "fp3d_batch_jacobian_matvec": (_int, [_int, _vp, _i64, _i64, _i64, _vp, _vp, _int,
                                           _vp, _i64, _int, _vp]),
     "fp3d_batch_orthonormalise": (_int, [_int, _vp, _int, _i64, _int, _dbl, _vp]),

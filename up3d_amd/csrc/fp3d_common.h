@@ -40,6 +40,26 @@ inline unsigned xcd_grid(int64_t n_blocks) {
 // repeatable.
 constexpr int kReduceBlocks = 1024;
 
+// One solve's buffers in a batched residual evaluation (layout shared with
+// the host: 24 words of 8 bytes; see fp3d_batch_residual_eval).
+struct EvalJob {
+  const double *phi;
+  double *grad, *q2n, *q2l, *rho;
+  uint8_t *lim;
+  double *gdotb;
+  int32_t *up;
+  double *nu, *rt, *se, *su, *terms, *R, *sR;
+  const uint8_t *fmask;
+  double *work;     // 3 * kReduceBlocks doubles
+  double *out;      // [0] limited, [1] nu max, [2] nu > 0, [3] floored, [4] l2, [5] linf
+  double m_inf, q2cap;
+  int64_t reserved[4];
+};
+static_assert(sizeof(EvalJob) == 192, "EvalJob layout");
+
+// the masked copy R -> sR and the norms of sR of every job (fp3d_linalg.hip)
+int batch_masked_norms(int nb, const EvalJob *jobs, int64_t n, hipStream_t s);
+
 }  // namespace fp3d
 
 #define FP3D_REQUIRE(cond, msg)        \

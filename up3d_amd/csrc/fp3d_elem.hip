@@ -110,14 +110,14 @@ __global__ __launch_bounds__(kBlock) void k_velocity_q2(
   q2[e] = gx * gx + gy * gy + gz * gz;
 }
 
-__global__ __launch_bounds__(kBlock) void k_element_state(
+__device__ __forceinline__ void k_element_state_body(int64_t bx,
     int64_t n, const int32_t *__restrict__ elements,
     const double *__restrict__ B, const double *__restrict__ phi, double u2,
     double m, double g, double cap, double *__restrict__ grad,
     double *__restrict__ q2n_out, double *__restrict__ q2l_out,
     double *__restrict__ rho_out, uint8_t *__restrict__ lim_out,
     double *__restrict__ gdotb) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t e = (int64_t)bx * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double gx, gy, gz;
   elem_grad(e, elements, B, phi, gx, gy, gz);
@@ -143,6 +143,16 @@ __global__ __launch_bounds__(kBlock) void k_element_state(
   q2l_out[e] = q2l;
   rho_out[e] = d_density(q2l, m, g);
   if (lim_out) lim_out[e] = (q2l == q2n) ? 1 : 0;
+}
+__global__ __launch_bounds__(kBlock) void k_element_state(
+    int64_t n, const int32_t *__restrict__ elements,
+    const double *__restrict__ B, const double *__restrict__ phi, double u2,
+    double m, double g, double cap, double *__restrict__ grad,
+    double *__restrict__ q2n_out, double *__restrict__ q2l_out,
+    double *__restrict__ rho_out, uint8_t *__restrict__ lim_out,
+    double *__restrict__ gdotb) {
+  k_element_state_body(blockIdx.x, n, elements, B, phi, u2, m, g, cap, grad, q2n_out,
+                       q2l_out, rho_out, lim_out, gdotb);
 }
 
 // ---------------------------------------------------------------- upwinding
@@ -522,7 +532,7 @@ __device__ __forceinline__ void walk_hop(WalkState &w, const int4 nb4,
 // rho_tilde as k_upwind_fused_d); W trades chains per lane against
 // registers (occupancy), chosen by fp3d_set_walk_width.
 template <int W, bool SENS>
-__global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
+__device__ __forceinline__ void k_upwind_sens_dw_body(int64_t bx,
     int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
     const double *__restrict__ nodes, const int32_t *__restrict__ elements,
     const double *__restrict__ grad, const double *__restrict__ q2,
@@ -530,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
     double C, double g, double floor_, int32_t *__restrict__ up_out,
     double *__restrict__ nu_out, double *__restrict__ rt_out,
     double *__restrict__ se, double *__restrict__ su) {
-  const int64_t e0 = (int64_t)blockIdx.x * W * blockDim.x + threadIdx.x;
+  const int64_t e0 = (int64_t)bx * W * blockDim.x + threadIdx.x;
   if (e0 >= n) return;
   WalkState w[W];
 #pragma unroll
@@ -566,6 +576,19 @@ __global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
                               rt_out, se, su);
     }
   }
+}
+template <int W, bool SENS>
+__global__ __launch_bounds__(kBlock) void k_upwind_sens_dw(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ elements,
+    const double *__restrict__ grad, const double *__restrict__ q2,
+    const double *__restrict__ rho, int max_hops, double m, double mc2,
+    double C, double g, double floor_, int32_t *__restrict__ up_out,
+    double *__restrict__ nu_out, double *__restrict__ rt_out,
+    double *__restrict__ se, double *__restrict__ su) {
+  k_upwind_sens_dw_body<W, SENS>(blockIdx.x, n, fn, dcen, nodes, elements, grad, q2,
+                                 rho, max_hops, m, mc2, C, g, floor_, up_out, nu_out,
+                                 rt_out, se, su);
 }
 
 // pyfp3d/kernels/upwind.py:596-643
@@ -670,13 +693,13 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // pass 1: per-block {max nu, count nu>0, count rt==floor} -> work[3*b+k]
-__global__ __launch_bounds__(kBlock) void k_monitors_p1(
+__device__ __forceinline__ void k_monitors_p1_body(int64_t bx, int64_t nbx,
     int64_t n, const double *__restrict__ nu, const double *__restrict__ rt,
     double floor_, double *__restrict__ work) {
   __shared__ double s[3][kBlock / 64];
   double vmax = -INFINITY, cpos = 0.0, cflr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = (int64_t)bx * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)nbx * blockDim.x) {
     double v = nu[i];
     vmax = v > vmax ? v : vmax;
     cpos += v > 0.0 ? 1.0 : 0.0;
@@ -699,14 +722,19 @@ __global__ __launch_bounds__(kBlock) void k_monitors_p1(
       b += s[1][k];
       c += s[2][k];
     }
-    work[3 * blockIdx.x] = a;
-    work[3 * blockIdx.x + 1] = b;
-    work[3 * blockIdx.x + 2] = c;
+    work[3 * bx] = a;
+    work[3 * bx + 1] = b;
+    work[3 * bx + 2] = c;
   }
+}
+__global__ __launch_bounds__(kBlock) void k_monitors_p1(
+    int64_t n, const double *__restrict__ nu, const double *__restrict__ rt,
+    double floor_, double *__restrict__ work) {
+  k_monitors_p1_body(blockIdx.x, gridDim.x, n, nu, rt, floor_, work);
 }
 // second pass: 256 lanes each fold a fixed strided subset, then a fixed
 // shuffle tree and a fixed-order combine of the 4 wavefront results
-__global__ __launch_bounds__(kBlock) void k_monitors_p2(
+__device__ __forceinline__ void k_monitors_p2_body(int64_t bx,
     int nb, const double *__restrict__ work, double *__restrict__ out) {
   __shared__ double s[3][kBlock / 64];
   double a = -INFINITY, b = 0.0, c = 0.0;
@@ -737,12 +765,16 @@ __global__ __launch_bounds__(kBlock) void k_monitors_p2(
     out[2] = s[2][0];
   }
 }
-__global__ __launch_bounds__(kBlock) void k_count_zero_p1(
+__global__ __launch_bounds__(kBlock) void k_monitors_p2(
+    int nb, const double *__restrict__ work, double *__restrict__ out) {
+  k_monitors_p2_body(blockIdx.x, nb, work, out);
+}
+__device__ __forceinline__ void k_count_zero_p1_body(int64_t bx, int64_t nbx,
     int64_t n, const uint8_t *__restrict__ m, double *__restrict__ work) {
   __shared__ double s[kBlock / 64];
   double c = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t i = (int64_t)bx * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)nbx * blockDim.x)
     c += m[i] == 0 ? 1.0 : 0.0;
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
@@ -750,10 +782,14 @@ __global__ __launch_bounds__(kBlock) void k_count_zero_p1(
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int k = 0; k < kBlock / 64; ++k) t += s[k];
-    work[blockIdx.x] = t;
+    work[bx] = t;
   }
 }
-__global__ __launch_bounds__(kBlock) void k_sum_p2(
+__global__ __launch_bounds__(kBlock) void k_count_zero_p1(
+    int64_t n, const uint8_t *__restrict__ m, double *__restrict__ work) {
+  k_count_zero_p1_body(blockIdx.x, gridDim.x, n, m, work);
+}
+__device__ __forceinline__ void k_sum_p2_body(int64_t bx,
     int nb, const double *__restrict__ work, double *__restrict__ out) {
   __shared__ double s[kBlock / 64];
   double t = 0.0;
@@ -766,6 +802,10 @@ __global__ __launch_bounds__(kBlock) void k_sum_p2(
     for (int k = 0; k < kBlock / 64; ++k) r += s[k];
     out[0] = r;
   }
+}
+__global__ __launch_bounds__(kBlock) void k_sum_p2(
+    int nb, const double *__restrict__ work, double *__restrict__ out) {
+  k_sum_p2_body(blockIdx.x, nb, work, out);
 }
 
 inline int reduce_blocks(int64_t n) {
@@ -904,12 +944,12 @@ __global__ __launch_bounds__(kBlock) void k_jacobian_color(
 // the node pass then reads one short record per incidence entry instead of
 // the element's geometry and state (reading those per entry costs about
 // 1.6x more on the 4.7M-tet channel).
-__global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
+__device__ __forceinline__ void k_elem_residual_terms_body(int64_t bx,
     int64_t n, const double *__restrict__ B, const double *__restrict__ V,
     const double *__restrict__ rt, const double *__restrict__ grad,
     const double *__restrict__ gdotb, const int32_t *__restrict__ pos,
     double *__restrict__ c) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t e = (int64_t)bx * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double w = rt[e] * V[e];
   double4 o;
@@ -938,6 +978,13 @@ __global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
   } else {
     *reinterpret_cast<double4 *>(c + 4 * e) = o;
   }
+}
+__global__ __launch_bounds__(kBlock) void k_elem_residual_terms(
+    int64_t n, const double *__restrict__ B, const double *__restrict__ V,
+    const double *__restrict__ rt, const double *__restrict__ grad,
+    const double *__restrict__ gdotb, const int32_t *__restrict__ pos,
+    double *__restrict__ c) {
+  k_elem_residual_terms_body(blockIdx.x, n, B, V, rt, grad, gdotb, pos, c);
 }
 
 // The loads of kUnroll consecutive entries are issued together (they are
@@ -971,12 +1018,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_terms(
 // one is summed), and each lane adds its own node's terms in entry order:
 // the same sums, in the same order, as k_gather_terms.
 constexpr int kTermChunk = 2048;               // doubles per chunk (16 KiB)
-__global__ __launch_bounds__(kBlock) void k_gather_terms_staged(
+__device__ __forceinline__ void k_gather_terms_staged_body(int64_t bx,
     int64_t nn, int64_t nb_pad, const int32_t *__restrict__ iptr,
     const double *__restrict__ c, double *__restrict__ R) {
   __shared__ double buf[kTermChunk];
   constexpr int P = kTermChunk / kBlock;
-  const int64_t v0 = xcd_block(blockIdx.x, nb_pad) * kBlock;
+  const int64_t v0 = xcd_block(bx, nb_pad) * kBlock;
   if (v0 >= nn) return;                        // uniform over the block
   const int64_t v1 = v0 + kBlock < nn ? v0 + kBlock : nn;
   const int64_t v = v0 + threadIdx.x;
@@ -1007,6 +1054,11 @@ __global__ __launch_bounds__(kBlock) void k_gather_terms_staged(
     for (; q < hi; ++q) r += buf[q - c0];
   }
   if (v < v1) R[v] = r;
+}
+__global__ __launch_bounds__(kBlock) void k_gather_terms_staged(
+    int64_t nn, int64_t nb_pad, const int32_t *__restrict__ iptr,
+    const double *__restrict__ c, double *__restrict__ R) {
+  k_gather_terms_staged_body(blockIdx.x, nn, nb_pad, iptr, c, R);
 }
 
 __global__ __launch_bounds__(kBlock) void k_elem_matrix_terms(
@@ -1236,6 +1288,64 @@ __global__ __launch_bounds__(kBlock) void k_term3_coo(
 __global__ void k_fill(int64_t n, double v, double *x) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;
+}
+
+// ------------------------------------------ batched residual evaluation
+// The live residual evaluation of nb solves on one mesh, one launch per
+// kernel for all of them (blockIdx.y = solve, its buffers from an EvalJob):
+// every block runs the single-solve kernel's body on the same grid, so each
+// solve gets the bits of its own evaluation.
+__global__ __launch_bounds__(kBlock) void k_be_state(int64_t n,
+                                                    const int32_t *__restrict__ el,
+                                                    const double *__restrict__ B, double u2,
+                                                    double g,
+                                                    const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_element_state_body(blockIdx.x, n, el, B, j.phi, u2, j.m_inf, g, j.q2cap, j.grad, j.q2n,
+                       j.q2l, j.rho, j.lim, j.gdotb);
+}
+__global__ __launch_bounds__(kBlock) void k_be_count_p1(int64_t n,
+                                                       const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_count_zero_p1_body(blockIdx.x, gridDim.x, n, j.lim, j.work);
+}
+__global__ __launch_bounds__(kBlock) void k_be_count_p2(int nb,
+                                                       const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_sum_p2_body(0, nb, j.work, j.out);
+}
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_be_walk(
+    int64_t n, const int32_t *__restrict__ fn, const double *__restrict__ dcen,
+    const double *__restrict__ nodes, const int32_t *__restrict__ el, int max_hops,
+    double mc2, double C, double g, double floor_, const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_upwind_sens_dw_body<W, true>(blockIdx.x, n, fn, dcen, nodes, el, j.grad, j.q2l, j.rho,
+                                 max_hops, j.m_inf, mc2, C, g, floor_, j.up, j.nu, j.rt,
+                                 j.se, j.su);
+}
+__global__ __launch_bounds__(kBlock) void k_be_mon_p1(int64_t n, double floor_,
+                                                     const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_monitors_p1_body(blockIdx.x, gridDim.x, n, j.nu, j.rt, floor_, j.work);
+}
+__global__ __launch_bounds__(kBlock) void k_be_mon_p2(int nb,
+                                                     const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_monitors_p2_body(0, nb, j.work, j.out + 1);
+}
+__global__ __launch_bounds__(kBlock) void k_be_terms(int64_t n, const double *__restrict__ B,
+                                                    const double *__restrict__ V,
+                                                    const int32_t *__restrict__ pos,
+                                                    const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_elem_residual_terms_body(blockIdx.x, n, B, V, j.rt, j.grad, j.gdotb, pos, j.terms);
+}
+__global__ __launch_bounds__(kBlock) void k_be_gather(int64_t nn, int64_t nb_pad,
+                                                     const int32_t *__restrict__ iptr,
+                                                     const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  k_gather_terms_staged_body(blockIdx.x, nn, nb_pad, iptr, j.terms, j.R);
 }
 
 }  // namespace fp3d
@@ -1509,6 +1619,46 @@ int fp3d_rho_tilde_sensitivities(int64_t n, const double *q2, const double *rho,
                "rho_tilde_sensitivities: NULL pointer");
   ELEMWISE(k_sens, n, q2, rho, upstream, m_inf, m_crit * m_crit, C, gamma,
            rho_floor, se_out, su_out);
+}
+
+int fp3d_batch_eval_job_bytes(void) { return (int)sizeof(EvalJob); }
+
+int fp3d_batch_residual_eval(int nb, const void *jobs, int64_t n_tets, int64_t n_nodes,
+                             const int32_t *elements, const double *B, const double *V,
+                             const int32_t *face_neighbors, const double *face_disp,
+                             const double *nodes, const int32_t *inc_ptr,
+                             const int32_t *inc_pos, double u2, double gamma,
+                             double m_crit, double C, double rho_floor, int max_hops,
+                             void *stream) {
+  FP3D_REQUIRE(nb >= 1 && nb <= 1024, "batch_residual_eval: nb in [1, 1024]");
+  FP3D_REQUIRE(jobs && n_tets > 0 && n_nodes > 0 && elements && B && V && face_neighbors &&
+                   face_disp && nodes && inc_ptr && inc_pos,
+               "batch_residual_eval: NULL pointer");
+  FP3D_REQUIRE(FP3D_ALIGNED16(elements) && FP3D_ALIGNED16(face_neighbors) &&
+                   FP3D_ALIGNED16(face_disp) && FP3D_ALIGNED16(inc_pos),
+               "batch_residual_eval: arrays must be 16-byte aligned");
+  FP3D_REQUIRE(u2 > 0.0 && max_hops >= 0, "batch_residual_eval: u2 > 0, max_hops >= 0");
+  hipStream_t s = FP3D_STREAM(stream);
+  const EvalJob *J = (const EvalJob *)jobs;
+  const int eb = g_elem_block;
+  hipLaunchKernelGGL(k_be_state, dim3(grid_for(n_tets, eb), nb), dim3(eb), 0, s, n_tets,
+                     elements, B, u2, gamma, J);
+  const int rb = reduce_blocks(n_tets);
+  hipLaunchKernelGGL(k_be_count_p1, dim3(rb, nb), dim3(kBlock), 0, s, n_tets, J);
+  hipLaunchKernelGGL(k_be_count_p2, dim3(1, nb), dim3(kBlock), 0, s, rb, J);
+  hipLaunchKernelGGL(k_be_walk<2>, dim3(grid_for(n_tets, 2 * eb), nb), dim3(eb), 0, s,
+                     n_tets, face_neighbors, face_disp, nodes, elements, max_hops,
+                     m_crit * m_crit, C, gamma, rho_floor, J);
+  hipLaunchKernelGGL(k_be_mon_p1, dim3(rb, nb), dim3(kBlock), 0, s, n_tets, rho_floor, J);
+  hipLaunchKernelGGL(k_be_mon_p2, dim3(1, nb), dim3(kBlock), 0, s, rb, J);
+  hipLaunchKernelGGL(k_be_terms, dim3(grid_for(n_tets, eb), nb), dim3(eb), 0, s, n_tets, B,
+                     V, inc_pos, J);
+  const unsigned g = xcd_grid((n_nodes + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_be_gather, dim3(g, nb), dim3(kBlock), 0, s, n_nodes, (int64_t)g,
+                     inc_ptr, J);
+  int rc = check_launch("batch_residual_eval");
+  if (rc) return rc;
+  return batch_masked_norms(nb, J, n_nodes, s);
 }
 
 int fp3d_upwind_monitors(int64_t n, const double *nu, const double *rho_tilde,

@@ -926,6 +926,23 @@ __global__ void k_t3_tval(int64_t n_active, const int32_t *__restrict__ act,
   t3_tval_body(blockIdx.x, n_active, act, elements, up, Cc, x, tval);
 }
 
+// ------------------------------------------ batched residual evaluation
+__global__ void k_be_mask(int64_t n, const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) j.sR[i] = j.fmask[i] ? j.R[i] : 0.0;
+}
+__global__ __launch_bounds__(kBlock) void k_be_norms_p1(int64_t n,
+                                                       const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  norms_p1_body(blockIdx.x, gridDim.x, n, j.sR, j.work);
+}
+__global__ __launch_bounds__(kBlock) void k_be_norms_p2(int nb,
+                                                       const EvalJob *__restrict__ jobs) {
+  const EvalJob &j = jobs[blockIdx.y];
+  norms_p2_body(nb, j.work, j.out + 4);
+}
+
 // ------------------------------------------------ batched Arnoldi steps
 // One Krylov step of each of nb GMRES solves on the same mesh, one launch
 // per kernel for all of them: blockIdx.y selects the solve, whose buffers
@@ -1358,6 +1375,14 @@ int fp3d_batch_jacobian_matvec(int nb, const void *jobs, int64_t max_active,
 }
 
 }  // extern "C"
+
+int fp3d::batch_masked_norms(int nb, const EvalJob *jobs, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_be_mask, dim3(grid_for(n), nb), dim3(kBlock), 0, s, n, jobs);
+  const int rb = rblocks(n);
+  hipLaunchKernelGGL(k_be_norms_p1, dim3(rb, nb), dim3(kBlock), 0, s, n, jobs);
+  hipLaunchKernelGGL(k_be_norms_p2, dim3(1, nb), dim3(kBlock), 0, s, rb, jobs);
+  return check_launch("batch_masked_norms");
+}
 
 template <typename TV>
 static void batch_ortho_t(int nb, const BatchJob *J, int max_nv, int64_t n, double eta2,

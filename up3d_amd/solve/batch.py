@@ -16,7 +16,9 @@ request at a time, in rounds:
     own previous step, queued a round earlier, and does that step's Givens
     update on the host while the device runs the new steps (GMRES's
     one-step-ahead pipeline, shared by all solves);
-  * the residual evaluations of a round run per instance.
+  * the live residual evaluations of a round that share a mesh go through
+    fp3d_batch_residual_eval (one launch per kernel of the evaluation chain
+    for all of them, one synchronisation); the others run per instance.
 
 A solve that returns leaves the round structure (the convergence mask is
 the set of live coroutines).  Each solve queues exactly the device work, in
@@ -53,6 +55,7 @@ class BatchExecutor:
         self.n_evals = 0
         self.max_width = 0
         self.n_batched_launch_groups = 0
+        self.n_batched_eval_groups = 0
 
     def run(self, coros: Dict[Hashable, object],
             on_done=None) -> Dict[Hashable, object]:
@@ -116,15 +119,88 @@ class BatchExecutor:
             # (queued a round before) while these run on the device
             for k, _ in steps:
                 self._advance(k, None)
+        groups = defaultdict(list)
         for k, r in evals:
             self.n_evals += 1
-            r.ws.eval_residual(r.s, *r.args)
-            self._advance(k, r.s)
+            groups[eval_key(r) if self.batched else None].append((k, r))
+        for key, items in groups.items():
+            if key is not None and len(items) > 1:
+                batched_eval([r for _, r in items])
+                self.n_batched_eval_groups += 1
+            else:
+                for _, r in items:
+                    r.ws.eval_residual(r.s, *r.args)
+            for k, r in items:
+                self._advance(k, r.s)
 
     def stats(self) -> dict:
         return {"rounds": self.n_rounds, "arnoldi_steps": self.n_steps,
                 "evaluations": self.n_evals, "max_steps_per_round": self.max_width,
-                "batched_groups": self.n_batched_launch_groups}
+                "batched_groups": self.n_batched_launch_groups,
+                "batched_eval_groups": self.n_batched_eval_groups}
+
+
+_EVAL_FIELDS = 24      # int64 words per fp3d_batch_residual_eval job
+
+
+def eval_key(req: EvalRequest):
+    """Residual evaluations that one batched launch can serve share this
+    key: live evaluations (no frozen selection) with M_inf > 0 and no
+    external right-hand side, on the same mesh store with the same u_inf,
+    gamma, upwind_c, m_crit and rho_floor.  None otherwise."""
+    ws = req.ws
+    args = req.args
+    frozen = args[3] if len(args) > 3 else None
+    if frozen is not None or ws.m_inf is None or ws.m_inf <= 0.0 or ws.b_ext is not None:
+        return None
+    return (id(ws.st), float(ws.u_inf), float(ws.gamma_air)) + tuple(float(a) for a in args[:3])
+
+
+def _bits(x: float) -> int:
+    return int(np.array([x], dtype=np.float64).view(np.int64)[0])
+
+
+def batched_eval(reqs) -> None:
+    """Run the live residual evaluations of several solves that share an
+    eval_key through fp3d_batch_residual_eval and fill each state as
+    NewtonWorkspace.eval_residual does (one synchronisation for all)."""
+    from .. import _native
+    from ..device import empty, ptr, stream_handle, to_host
+
+    ws0 = reqs[0].ws
+    st = ws0.st
+    upwind_c, m_crit, rho_floor = (float(a) for a in reqs[0].args[:3])
+    outs = empty((len(reqs), 8))
+    jobs = np.zeros((len(reqs), _EVAL_FIELDS), dtype=np.int64)
+    for i, r in enumerate(reqs):
+        ws, s = r.ws, r.s
+        work = getattr(ws, "_batch_eval_work", None)
+        if work is None:
+            work = ws._batch_eval_work = empty(3 * 1024)
+        jobs[i, :20] = (ptr(s.phi), ptr(s.grad), ptr(s.q2n), ptr(s.q2l), ptr(s.rho),
+                        ptr(s.lim), ptr(s.gb), ptr(s.up), ptr(ws.upw.d_nu), ptr(s.rt),
+                        ptr(s.se), ptr(s.su), ptr(ws.op.d_terms), ptr(ws.op.d_R), ptr(s.R),
+                        ptr(ws.fm), ptr(work), ptr(outs[i]), _bits(ws.m_inf),
+                        _bits(ws.q2cap))
+    host = torch.from_numpy(jobs).pin_memory()
+    dev = torch.empty_like(host, device=st.device)
+    dev.copy_(host, non_blocking=True)
+    _native.call("fp3d_batch_residual_eval", len(reqs), ptr(dev), st.n_tets, st.n_nodes,
+                 ptr(st.el), ptr(st.B), ptr(st.V), ptr(st.fn), ptr(st.dcen), ptr(st.nodes),
+                 ptr(st.inc_ptr), ptr(st.inc_pos), float(ws0.u_inf ** 2),
+                 float(ws0.gamma_air), m_crit, upwind_c, rho_floor, 4, stream_handle())
+    h = to_host(outs)
+    for i, r in enumerate(reqs):
+        ws, s = r.ws, r.s
+        upw = ws.upw
+        s.n_limited = int(h[i, 0])
+        upw.nu_max, upw.n_supersonic, upw.n_floored = float(h[i, 1]), int(h[i, 2]), int(h[i, 3])
+        s.sens_key = (ws.m_inf, upwind_c, m_crit, ws.gamma_air, rho_floor)
+        s.n_floored = upw.n_floored
+        s.nu_max = upw.nu_max
+        s.n_nu_active = upw.n_supersonic
+        s.r_norm = float(h[i, 5])
+        s.merit = float(h[i, 4]) * float(h[i, 4])
 
 
 # ---------------------------------------------------------------- the sweep
