@@ -133,9 +133,27 @@ class ChannelForceProblem:
         r["nodes"] = nodes
         r["m_inf"] = float(m_inf)
         r["alpha_deg"] = float(alpha_deg)
+        # the solve's last AMG hierarchy: the adjoint and tangent solves at the
+        # converged state precondition with it instead of building their own
+        ws = r.get("workspace")
+        r["precond"] = getattr(ws, "precond", None)
         if r["converged"]:
             self._settle(r, tol_residual)
         return r
+
+    @staticmethod
+    def _preconditioner(sol, st, A, fm):
+        """The converged solve's AMG hierarchy when it is on this mesh, else
+        one of the masked Picard matrix A."""
+        from .amg import AMGHierarchy
+        from .common import csr_to_host, mask_dirichlet
+
+        ml = sol.get("precond")
+        if ml is not None and ml.shapes[0] == st.n_nodes:
+            return ml
+        A = A.clone()
+        mask_dirichlet(st, fm, A)
+        return AMGHierarchy(csr_to_host(st, A), precision="f64", max_coarse=1000)
 
     def _settle(self, r, tol_residual: float) -> None:
         """Put the accepted state on the root of the selection it is
@@ -284,9 +302,6 @@ class ChannelForceProblem:
     def gradients(self, sol, weights=(1.0, 0.0), rtol: float = 1e-12,
                   upwind_c: float = 1.5, m_crit: float = 0.95) -> Dict[str, object]:
         """dF/d(y_floor, M_inf, alpha) for F = w0 cf_x + w1 cl at `sol`."""
-        from .common import csr_to_host, mask_dirichlet
-        from .amg import AMGHierarchy
-
         L = self._linearisation(sol, upwind_c, m_crit)
         if L["n_limited"]:
             raise RuntimeError("adjoint needs a state without speed-limited elements")
@@ -309,9 +324,7 @@ class ChannelForceProblem:
         # adjoint solve J_ff^T lam = (dF/dphi)_free
         b = to_dev(dFphi)
         b.mul_(fm.to(torch.float64))
-        A = L["A"].clone()
-        mask_dirichlet(st, fm, A)
-        ml = AMGHierarchy(csr_to_host(st, A), precision="f64")
+        ml = self._preconditioner(sol, st, L["A"], fm)
         fmd = fm.to(torch.float64)
 
         def mvT(x, y):
@@ -319,8 +332,8 @@ class ChannelForceProblem:
             y.mul_(fmd)
             return y
 
-        lam, info, n_it = gmres_dev(mvT, b, ml.apply, rtol=rtol, restart=60,
-                                    maxiter=40)
+        lam, info, n_it = gmres_dev(mvT, b, ml.apply, rtol=rtol, restart=200,
+                                    maxiter=12)
         red_norm = Reducer()
         # dR/dM_inf = scatter of d rho_tilde/dM
         drt = empty(nt)
@@ -371,9 +384,6 @@ class ChannelForceProblem:
                    rtol: float = 1e-12, upwind_c=1.5, m_crit=0.95) -> float:
         """Forward tangent dF for a perturbation of (M_inf, alpha):
         J_ff dphi_f = -(dR/dM dm + J_fD dphi_D), dF = F_p + F_phi . dphi."""
-        from .common import csr_to_host, mask_dirichlet
-        from .amg import AMGHierarchy
-
         L = self._linearisation(sol, upwind_c, m_crit)
         st = L["st"]
         n, nt = st.n_nodes, st.n_tets
@@ -394,16 +404,14 @@ class ChannelForceProblem:
         rhs.mul_(dm)
         axpby(1.0, Jv, 1.0, rhs)
         rhs.mul_(-fmd)
-        A = L["A"].clone()
-        mask_dirichlet(st, L["fm"], A)
-        ml = AMGHierarchy(csr_to_host(st, A), precision="f64")
+        ml = self._preconditioner(sol, st, L["A"], L["fm"])
 
         def mv(x, y):
             self._jmv(L, x, y, transpose=False)
             y.mul_(fmd)
             return y
 
-        dphi, info, _ = gmres_dev(mv, rhs, ml.apply, rtol=rtol, restart=60, maxiter=40)
+        dphi, info, _ = gmres_dev(mv, rhs, ml.apply, rtol=rtol, restart=200, maxiter=12)
         dphi_h = to_host(dphi) + vd
         t = lambda v, g=False: torch.tensor(v, dtype=torch.float64, requires_grad=g)  # noqa: E731
         with torch.enable_grad():
