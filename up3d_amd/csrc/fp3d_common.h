@@ -12,8 +12,10 @@ void set_error(const std::string &msg);
 
 // Launch geometry: 256 work-items = 4 wave64 wavefronts per workgroup.
 constexpr int kBlock = 256;
-// rows per workgroup of the LDS-staged node-gather matrix assembly
-constexpr int kGatherRows = 128;
+// rows per workgroup of the LDS-staged node-gather matrix assembly, and its
+// lanes (four per row)
+constexpr int kGatherRows = 64;
+constexpr int kGatherLanes = 4 * kGatherRows;
 
 inline unsigned grid_for(int64_t n, int per_block = kBlock) {
   int64_t g = (n + per_block - 1) / per_block;

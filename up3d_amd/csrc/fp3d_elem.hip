@@ -1175,28 +1175,38 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr(
 
 // Incidence-ordered matrix rows: the rows and slot offsets of a block's
 // kGatherRows CSR rows are one contiguous range [iptr[r0], iptr[r1]),
-// streamed through LDS in coalesced chunks of kRowChunk rows (next chunk
-// in flight while the current one is added; 1024 rows = 32 KiB by
-// default, 256, 512 or 2048 by fp3d_set_gather_chunk); each lane adds its own row's
-// entries in entry order into the LDS-staged CSR block, as k_gather_csr.
-// The chunk size changes only the staging, never the sums or their order.
+// streamed through LDS in coalesced chunks of kRowChunk entries (the next
+// chunk in flight while the current one is added; 1024 by default, 256,
+// 512 or 2048 by fp3d_set_gather_chunk).  Four lanes per CSR row: lane c
+// adds component c of each entry (the value for the element's c-th node)
+// into that column's slot of the LDS-staged CSR block.  The four slots of
+// one entry are distinct, and the four lanes of a row sit in one wavefront
+// and step through the row's entries together (one LDS read and one LDS
+// write per entry, issued in program order, which the LDS executes in
+// order for a wavefront), so every slot still receives its additions in
+// entry order -- the sums of k_gather_csr, bit for bit.  The chunk size
+// changes only the staging, never the sums or their order.
 template <int kRowChunk>
-__global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
+__global__ __launch_bounds__(kGatherLanes) void k_gather_csr_staged(
     int64_t nn, int64_t nb_pad, int max_seg, const int32_t *__restrict__ indptr,
     const int32_t *__restrict__ iptr, const uint32_t *__restrict__ rel8,
     const double4 *__restrict__ K4, double *__restrict__ data) {
   extern __shared__ double seg[];
   __shared__ double4 kb[kRowChunk];
   __shared__ uint32_t ob[kRowChunk];
-  constexpr int P = kRowChunk / kGatherRows;
+  constexpr int NT = kGatherLanes;
+  constexpr int P = kRowChunk / NT;
+  static_assert(P >= 1 && P * NT == kRowChunk, "chunk must be a multiple of the lanes");
+  static_assert(NT == 4 * kGatherRows, "four lanes per row");
   const int64_t r0 = xcd_block(blockIdx.x, nb_pad) * kGatherRows;
   if (r0 >= nn) return;                        // uniform over the block
   const int64_t r1 = r0 + kGatherRows < nn ? r0 + kGatherRows : nn;
   const int32_t base = indptr[r0];
   const int32_t len = indptr[r1] - base;
   if (len > max_seg) return;
-  for (int t = threadIdx.x; t < len; t += kGatherRows) seg[t] = 0.0;
-  const int64_t v = r0 + threadIdx.x;
+  for (int t = threadIdx.x; t < len; t += NT) seg[t] = 0.0;
+  const int c = threadIdx.x & 3;
+  const int64_t v = r0 + (threadIdx.x >> 2);
   const int64_t qb = iptr[r0], qe = iptr[r1];
   int64_t q = 0, q1 = 0;
   int32_t rb = 0;
@@ -1209,7 +1219,7 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
   uint32_t no[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t i = qb + threadIdx.x + k * kGatherRows;
+    const int64_t i = qb + threadIdx.x + k * NT;
     if (i < qe) {
       nk[k] = K4[i];
       no[k] = rel8[i];
@@ -1218,17 +1228,19 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
       no[k] = 0u;
     }
   }
+  const double *kbc = reinterpret_cast<const double *>(kb) + c;
+  const int sh = 8 * c;
   for (int64_t c0 = qb; c0 < qe; c0 += kRowChunk) {
     __syncthreads();                 // previous chunk consumed, seg zeroed
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      kb[threadIdx.x + k * kGatherRows] = nk[k];
-      ob[threadIdx.x + k * kGatherRows] = no[k];
+      kb[threadIdx.x + k * NT] = nk[k];
+      ob[threadIdx.x + k * NT] = no[k];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      const int64_t i = c0 + kRowChunk + threadIdx.x + k * kGatherRows;
+      const int64_t i = c0 + kRowChunk + threadIdx.x + k * NT;
       if (i < qe) {
         nk[k] = K4[i];
         no[k] = rel8[i];
@@ -1236,22 +1248,14 @@ __global__ __launch_bounds__(kGatherRows) void k_gather_csr_staged(
     }
     const int64_t hi = q1 < c0 + kRowChunk ? q1 : c0 + kRowChunk;
     for (; q < hi; ++q) {
-      const double4 kv = kb[q - c0];
-      const uint32_t of = ob[q - c0];
-      // the four slots are those of one element's four distinct nodes in
-      // row v, so they never alias: read all four, then write the sums (the
-      // same additions, without four chained read-modify-writes)
-      const int o0 = rb + (int)(of & 0xffu), o1 = rb + (int)((of >> 8) & 0xffu);
-      const int o2 = rb + (int)((of >> 16) & 0xffu), o3 = rb + (int)(of >> 24);
-      const double s0 = seg[o0], s1 = seg[o1], s2 = seg[o2], s3 = seg[o3];
-      seg[o0] = s0 + kv.x;
-      seg[o1] = s1 + kv.y;
-      seg[o2] = s2 + kv.z;
-      seg[o3] = s3 + kv.w;
+      const int j = (int)(q - c0);
+      const double kv = kbc[4 * j];
+      const int o = rb + (int)((ob[j] >> sh) & 0xffu);
+      seg[o] = seg[o] + kv;
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < len; t += kGatherRows) data[base + t] = seg[t];
+  for (int t = threadIdx.x; t < len; t += NT) data[base + t] = seg[t];
 }
 
 // pyfp3d/kernels/jacobian.py:197-228 (fixed slot slice [16a, 16a+16))
@@ -1866,7 +1870,7 @@ int fp3d_gather_csr_terms(int64_t n_nodes, int max_seg, const int32_t *indptr,
                        (const uint32_t *)rel8, K, data);
   } else {
 #define FP3D_STAGED(CH)                                                      \
-  hipLaunchKernelGGL(k_gather_csr_staged<CH>, dim3(g), dim3(kGatherRows),    \
+  hipLaunchKernelGGL(k_gather_csr_staged<CH>, dim3(g), dim3(kGatherLanes),   \
                      (size_t)max_seg * sizeof(double), FP3D_STREAM(stream),  \
                      n_nodes, (int64_t)g, max_seg, indptr, inc_ptr,          \
                      (const uint32_t *)rel8, (const double4 *)K, data)

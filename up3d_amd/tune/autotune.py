@@ -6,9 +6,9 @@ stream, on the caller's own mesh and state:
   * tune_element_block: threads per block of the element-wise kernels
     (state, walk + rho_tilde, residual element terms, sensitivities,
     Jacobian element terms) over {64, 128, 192, 256}; the node gathers keep
-    their fixed 128-row LDS blocks.  The winner is applied process-wide
+    their fixed LDS row blocks.  The winner is applied process-wide
     (fp3d_set_element_block).
-  * tune_gather_chunk: rows per LDS-staged chunk of the CSR matrix gather
+  * tune_gather_chunk: entries per LDS-staged chunk of the CSR matrix gather
     over {512, 1024} (fp3d_set_gather_chunk; the sums are unchanged).
   * tune_walk_width: upstream walks per lane over {1, 2, 3}
     (fp3d_set_walk_width; the selections are unchanged).
@@ -162,9 +162,11 @@ def tune_linear_solver(matvec: Callable, rhs: torch.Tensor, A_pre,
             "trials": trials}
 
 
-# Version of the cache entries' layout; entries of another version, or
-# missing a tuned dimension, are misses and are tuned again.
-CACHE_VERSION = 2
+# Version of the cache entries (their layout, and the kernels whose launch
+# parameters they hold: 3 = the four-lanes-per-row CSR gather); entries of
+# another version, or missing a tuned dimension, are misses and are tuned
+# again.
+CACHE_VERSION = 3
 TUNED_KEYS = ("element_block", "gather_chunk", "walk_width", "solver")
 
 
