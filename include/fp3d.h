@@ -43,6 +43,10 @@ int fp3d_version(void);
 /* number of HIP devices visible (0 on a machine without a GPU) */
 int fp3d_device_count(int *count);
 int fp3d_stream_sync(void *stream);
+/* Launch an instantiated HIP graph (hipGraphExec_t, e.g. a captured AMG
+ * V-cycle) on a stream: the one launch, without a framework's replay
+ * bookkeeping.  No reference counterpart. */
+int fp3d_graph_launch(void *graph_exec, void *stream);
 
 /* ------------------------------------------------- physics (elementwise)
  * pyfp3d/physics/isentropic.py:235-258 density_field,
@@ -613,6 +617,15 @@ int fp3d_host_spgemm_fill(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
                           const int32_t *indptr, int32_t *indices, double *data);
 int fp3d_host_csr_matvec(int64_t n_rows, const int32_t *Ap, const int32_t *Aj,
                          const double *Ax, const double *x, double *y);
+/* GMRES Hessenberg update (pyfp3d/solve/linear.py:182-193 calls
+ * scipy.sparse.linalg.gmres, whose Arnoldi loop rotates each new column by
+ * the previous Givens rotations): rotations j = 0 .. col-1 applied in order
+ * to hrow, the new column's entries h[0 .. col+1]:
+ * (hrow[j], hrow[j+1]) <- (c_j hrow[j] + s_j hrow[j+1],
+ *                          -s_j hrow[j] + c_j hrow[j+1]),
+ * with (c_j, s_j) = givens[2j], givens[2j + 1].  The same IEEE operations,
+ * in the same order, as the host loop it replaces. */
+int fp3d_host_apply_givens(int col, double *hrow, const double *givens);
 /* pyfp3d/mesh/metrics.py:207-278 build_face_adjacency (int32 (n,4)). */
 int fp3d_host_face_adjacency(int64_t n_tets, const int32_t *elements,
                              int32_t *face_neighbors);

@@ -189,3 +189,30 @@ def test_p1_transfer_exact_for_linear_fields():
     # interior fine nodes lie inside their located coarse tet
     inside = bary.min(axis=1) >= -1e-10
     assert inside.mean() > 0.95
+
+
+def test_apply_givens_matches_rotation_product():
+    """fp3d_host_apply_givens: the chain of rotations equals the product of
+    the 2x2 rotation matrices (an orthogonal map: the column norm is kept),
+    and gives the bits of the host loop it replaced in gmres_steps."""
+    rng = np.random.default_rng(3)
+    for col in (0, 1, 7, 60):
+        hrow = rng.standard_normal(col + 2)
+        ang = rng.uniform(0, 2 * np.pi, col)
+        givens = np.ascontiguousarray(np.stack([np.cos(ang), np.sin(ang)], axis=1))
+        out = hrow.copy()
+        _native.call("fp3d_host_apply_givens", col, out.ctypes.data, givens.ctypes.data)
+        Q = np.eye(col + 2)
+        for j in range(col):
+            c, s = givens[j]
+            G = np.eye(col + 2)
+            G[j:j + 2, j:j + 2] = [[c, s], [-s, c]]
+            Q = G @ Q
+        assert np.allclose(out, Q @ hrow, rtol=0, atol=1e-13)
+        assert abs(np.linalg.norm(out) - np.linalg.norm(hrow)) < 1e-13
+        ref = hrow.copy()
+        for j in range(col):
+            c, s = givens[j]
+            n0, n1 = ref[j], ref[j + 1]
+            ref[j], ref[j + 1] = c * n0 + s * n1, -s * n0 + c * n1
+        assert np.array_equal(out, ref)

@@ -1364,6 +1364,12 @@ extern "C" {
 const char *fp3d_last_error(void) { return fp3d::g_err.c_str(); }
 int fp3d_version(void) { return 1; }
 
+int fp3d_graph_launch(void *graph_exec, void *stream) {
+  FP3D_REQUIRE(graph_exec, "graph_launch: NULL graph");
+  FP3D_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, FP3D_STREAM(stream)));
+  return FP3D_OK;
+}
+
 int fp3d_device_count(int *count) {
   FP3D_REQUIRE(count, "count is NULL");
   int c = 0;

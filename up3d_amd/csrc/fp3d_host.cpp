@@ -175,6 +175,20 @@ int fp3d_host_spgemm_fill(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
   return FP3D_OK;
 }
 
+// Rotations 0..col-1 of the Hessenberg matrix, applied in order to one new
+// column (a dependent chain: rotation j rewrites entry j + 1, which rotation
+// j + 1 then reads).
+int fp3d_host_apply_givens(int col, double *hrow, const double *givens) {
+  HREQ(col >= 0 && (col == 0 || (hrow && givens)), "apply_givens: NULL");
+  for (int j = 0; j < col; ++j) {
+    const double c = givens[2 * j], s = givens[2 * j + 1];
+    const double n0 = hrow[j], n1 = hrow[j + 1];
+    hrow[j] = c * n0 + s * n1;
+    hrow[j + 1] = -s * n0 + c * n1;
+  }
+  return FP3D_OK;
+}
+
 /* y = A x on the host, row-parallel; row i sums from 0 in stored order, as
  * scipy.sparse's csr_matvec. */
 int fp3d_host_csr_matvec(int64_t n_rows, const int32_t *Ap, const int32_t *Aj,

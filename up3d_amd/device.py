@@ -21,7 +21,18 @@ def require_gpu() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+# the current stream's raw handle without building a torch Stream object
+# (every C-ABI launch asks for it: torch.cuda.current_stream() costs several
+# microseconds per call, this accessor well under one)
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_cur_device = getattr(torch._C, "_cuda_getDevice", None)
+
+
 def stream_handle() -> int:
+    """HIP stream the C-ABI launches go to: torch's current stream of the
+    current device (the one `with torch.cuda.stream(...)` selects)."""
+    if _raw_stream is not None and _cur_device is not None:
+        return int(_raw_stream(_cur_device()))
     return int(torch.cuda.current_stream().cuda_stream)
 
 

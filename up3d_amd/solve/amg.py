@@ -345,6 +345,17 @@ class AMGHierarchy:
             self._vcycle_top(b, x)
         return g
 
+    @staticmethod
+    def _launch(g) -> None:
+        """Replay a captured V-cycle: one hipGraphLaunch on the current
+        stream through the library (the V-cycle draws no random numbers, so
+        torch's replay bookkeeping has nothing to do)."""
+        h = g.raw_cuda_graph_exec()
+        if not isinstance(h, int):
+            g.replay()
+            return
+        _native.call("fp3d_graph_launch", h, stream_handle())
+
     # graphs captured directly on callers' (b, x) buffers; a Krylov solve
     # passes one source and two alternating destinations, and the caching
     # allocator hands the same buffers to the next solve of the same size
@@ -369,12 +380,12 @@ class AMGHierarchy:
             if g is None and len(self._graphs) < self._MAX_DIRECT_GRAPHS:
                 g = self._graphs[key] = self._capture(b, x)
             if g is not None:
-                g.replay()
+                self._launch(g)
                 return x
         if self._graph is None:
             self._graph = self._capture(self._gb, self._gx)
         self._gb.copy_(b)
-        self._graph.replay()
+        self._launch(self._graph)
         x.copy_(self._gx)
         return x
 

@@ -287,6 +287,9 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
     evs = gw["evs"]
     h = np.zeros((restart, restart + 1))
     givens = np.zeros((restart, 2))
+    # the rotation chain of each new column runs in the library's host code
+    apply_givens = _native.lib().fp3d_host_apply_givens
+    h_ptr, h_row, g_ptr = h.ctypes.data, h.strides[0], givens.ctypes.data
     inner = 0
     rnorm = np.inf
     x_nonzero = x0 is not None and bool(torch.any(x != 0).item())
@@ -370,10 +373,9 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
             if h1 <= eps * h0:
                 h[col, col + 1] = 0.0
                 breakdown = True
-            for j in range(col):
-                c, s_ = givens[j, 0], givens[j, 1]
-                n0, n1 = h[col, j], h[col, j + 1]
-                h[col, j], h[col, j + 1] = c * n0 + s_ * n1, -s_ * n0 + c * n1
+            if col:
+                _native.check(apply_givens(col, h_ptr + col * h_row, g_ptr),
+                              "apply_givens")
             c, s_, mag = _lartg(h[col, col], h[col, col + 1])
             givens[col, 0], givens[col, 1] = c, s_
             h[col, col], h[col, col + 1] = mag, 0.0
