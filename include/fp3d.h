@@ -508,6 +508,21 @@ int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
  * float64: the coarsest solve is float64 in both precisions) */
 int fp3d_dense_matvec_v(int fmt, int64_t n, const double *M, const void *b, void *x,
                         void *stream);
+/* One V-cycle operation for nb solves at once (no reference counterpart:
+ * the reference applies one pyamg hierarchy per call,
+ * pyfp3d/solve/linear.py:61-81): recs points to nb 128-byte device records,
+ * one per solve (row count, CSR arrays, operand, operation kind, five
+ * pointers, two scalars, a flag; the layout up3d_amd/solve/amg.py writes).
+ * op 0: SpMV with G lanes per row and an epilogue (store, residual, add,
+ * the three Chebyshev steps, restriction + coarse first step), fmt bits
+ * 1 float32 values, 2 float32 vectors, 4 int16 row offsets; op 1:
+ * element-wise (Chebyshev first step from 0, x += d, float32 / float64
+ * conversions); op 2: dense coarsest matvec.  Each solve runs the body,
+ * grid and arithmetic of its own fp3d_amg_* / fp3d_dense_matvec_v /
+ * conversion launch (max_n = the largest row or entry count), so the
+ * results are bitwise those of its own V-cycle. */
+int fp3d_grp_vcycle_op(int nb, const void *recs, int op, int64_t max_n, int G, int fmt,
+                       void *stream);
 
 /* ---------------------------------------------------- discrete adjoint
  * (new functionality: the reference has no adjoint.)  At a frozen upstream

@@ -62,6 +62,7 @@ def test_lockstep_solves_equal_solves_alone(channel, precision, batched):
     assert st["max_steps_per_round"] == len(CONDS)
     if batched:
         assert st["batched_groups"] > 0 and st["batched_eval_groups"] > 0
+        assert st["grouped_vcycles"] > 0
     for i, ref in enumerate(alone):
         r = res[i]
         assert ref["converged"] and r["converged"]

@@ -120,6 +120,23 @@ __device__ __forceinline__ int32_t csr_col<int16_t>(int64_t row, int16_t v) {
   return (int32_t)row + (int32_t)v;
 }
 
+// Row `row` of A x on its G lanes: lane-strided products, then a fixed
+// shuffle tree (lane 0 holds the sum).
+template <int G, typename T, typename TI, typename TX>
+__device__ __forceinline__ double spmv_row_sum(int64_t row, int lane,
+                                               const int32_t *__restrict__ ptr,
+                                               const TI *__restrict__ idx,
+                                               const T *__restrict__ val,
+                                               const TX *__restrict__ x) {
+  int32_t j0 = ptr[row], j1 = ptr[row + 1];
+  double s = 0.0;
+  for (int32_t j = j0 + lane; j < j1; j += G)
+    s += (double)val[j] * (double)x[csr_col<TI>(row, idx[j])];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+  return s;
+}
+
 template <int G, typename T, typename TI, typename TX, typename Ep>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
                                                  const int32_t *__restrict__ ptr,
@@ -131,12 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n_rows,
   int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / G;
   int lane = threadIdx.x % G;
   if (row >= n_rows) return;
-  int32_t j0 = ptr[row], j1 = ptr[row + 1];
-  double s = 0.0;
-  for (int32_t j = j0 + lane; j < j1; j += G)
-    s += (double)val[j] * (double)x[csr_col<TI>(row, idx[j])];
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+  const double s = spmv_row_sum<G, T, TI, TX>(row, lane, ptr, idx, val, x);
   if (lane == 0) ep(row, s);
 }
 
@@ -1094,6 +1106,130 @@ static int cheb_smooth_t(int64_t n, int64_t nnz, const int32_t *indptr,
   return FP3D_OK;
 }
 
+// ------------------------------------------------ grouped V-cycle steps
+// One V-cycle operation for many solves at once (their own hierarchies,
+// buffers and Chebyshev scalars), one launch: blockIdx.y picks the solve's
+// 128-byte record, and every block runs the body, grid and epilogue of the
+// solve's own launch (fp3d_amg_*, fp3d_dense_matvec_v, fp3d_scale_to_f32,
+// fp3d_f32_to_f64), so each solve gets the bits of its own V-cycle.
+struct GrpRec {              // 16 x 8 bytes; layout shared with the host
+  int64_t n;                 // rows (SpMV) or entries (element-wise, dense)
+  const int32_t *ptr;        // CSR row pointer
+  const void *idx;           // CSR columns (int32, or int16 row offsets)
+  const void *val;           // CSR values (float64 or float32)
+  const void *x;             // SpMV operand
+  int64_t kind;              // kGrp* below
+  void *p[5];                // epilogue / operation pointers
+  double d0, d1;             // epilogue scalars
+  int64_t flag;
+  int64_t reserved[2];
+};
+static_assert(sizeof(GrpRec) == 128, "GrpRec layout");
+
+enum : int64_t {
+  kGrpStore = 0,         // y[p0] = s
+  kGrpResidual = 1,      // r[p1] = b[p0] - s
+  kGrpAdd = 2,           // x[p0] += s
+  kGrpChebFirst = 3,     // EpChebFirst{b p0, dinv p1, r p2, d p3, 1/theta d0}
+  kGrpChebIter = 4,      // EpChebIter{dinv p0, d_old p1, x p2, r p3, d_new p4, c1 d0, c2 d1}
+  kGrpChebLast = 5,      // EpChebLast, same fields
+  kGrpRestrictStart = 6, // EpRestrictStart{bc p0, dinv p1, x p2, r p3, d p4, 1/theta d0, flag}
+  kGrpChebZero = 10,     // k_cheb_zero{b p0, dinv p1, x p2, r p3, d p4, 1/theta d0, flag}
+  kGrpAddVec = 11,       // k_add{d p0, x p1}
+  kGrpToF32 = 12,        // k_scale_cvt{a d0, x p0, y p1}
+  kGrpToF64 = 13,        // k_widen{x p0, y p1}
+  kGrpDense = 14,        // k_dense_mv{M p0, b p1, x p2}
+};
+
+template <typename TV>
+__device__ __forceinline__ void grp_epilogue(const GrpRec &r, int64_t row, double s) {
+  switch (r.kind) {
+    case kGrpStore:
+      EpStore<TV>{(TV *)r.p[0]}(row, s);
+      break;
+    case kGrpResidual:
+      EpResidual<TV>{(const TV *)r.p[0], (TV *)r.p[1]}(row, s);
+      break;
+    case kGrpAdd:
+      EpAdd<TV>{(TV *)r.p[0]}(row, s);
+      break;
+    case kGrpChebFirst:
+      EpChebFirst<TV>{(const TV *)r.p[0], (const TV *)r.p[1], (TV *)r.p[2], (TV *)r.p[3],
+                      r.d0}(row, s);
+      break;
+    case kGrpChebIter:
+      EpChebIter<TV>{(const TV *)r.p[0], (const TV *)r.p[1], (TV *)r.p[2], (TV *)r.p[3],
+                     (TV *)r.p[4], r.d0, r.d1}(row, s);
+      break;
+    case kGrpChebLast:
+      EpChebLast<TV>{(const TV *)r.p[0], (const TV *)r.p[1], (TV *)r.p[2], (TV *)r.p[3],
+                     (TV *)r.p[4], r.d0, r.d1}(row, s);
+      break;
+    default:  // kGrpRestrictStart
+      EpRestrictStart<TV>{(TV *)r.p[0], (const TV *)r.p[1], (TV *)r.p[2], (TV *)r.p[3],
+                          (TV *)r.p[4], r.d0, (int)r.flag}(row, s);
+      break;
+  }
+}
+
+template <int G, typename T, typename TI, typename TV>
+__global__ __launch_bounds__(kBlock) void k_grp_spmv(const GrpRec *__restrict__ recs) {
+  const GrpRec &r = recs[blockIdx.y];
+  constexpr int RPB = kBlock / G;
+  int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / G;
+  int lane = threadIdx.x % G;
+  if (row >= r.n) return;
+  const double s = spmv_row_sum<G, T, TI, TV>(row, lane, r.ptr, (const TI *)r.idx,
+                                              (const T *)r.val, (const TV *)r.x);
+  if (lane == 0) grp_epilogue<TV>(r, row, s);
+}
+
+template <typename TV>
+__global__ void k_grp_elem(const GrpRec *__restrict__ recs) {
+  const GrpRec &r = recs[blockIdx.y];
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r.n) return;
+  switch (r.kind) {
+    case kGrpChebZero: {
+      const TV *b = (const TV *)r.p[0], *dinv = (const TV *)r.p[1];
+      TV *x = (TV *)r.p[2], *rr = (TV *)r.p[3], *d = (TV *)r.p[4];
+      double ri = (double)dinv[i] * (double)b[i];
+      rr[i] = (TV)ri;
+      double di = ri * r.d0;
+      d[i] = (TV)di;
+      x[i] = (TV)(r.flag ? 0.0 + di : 0.0);
+      break;
+    }
+    case kGrpAddVec: {
+      const TV *d = (const TV *)r.p[0];
+      TV *x = (TV *)r.p[1];
+      x[i] = (TV)((double)x[i] + (double)d[i]);
+      break;
+    }
+    case kGrpToF32:
+      ((float *)r.p[1])[i] = (float)(r.d0 * ((const double *)r.p[0])[i]);
+      break;
+    default:  // kGrpToF64
+      ((double *)r.p[1])[i] = (double)((const float *)r.p[0])[i];
+      break;
+  }
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_grp_dense(const GrpRec *__restrict__ recs) {
+  const GrpRec &r = recs[blockIdx.y];
+  const int64_t n = r.n;
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double *__restrict__ row = (const double *)r.p[0] + i * n;
+  const TV *__restrict__ b = (const TV *)r.p[1];
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s += row[j] * (double)b[j];
+  s = wsum(s);
+  if (lane == 0) ((TV *)r.p[2])[i] = (TV)s;
+}
+
 // ======================================================================
 // C-ABI
 // ======================================================================
@@ -1705,6 +1841,66 @@ int fp3d_dense_matvec_v(int fmt, int64_t n, const double *M, const void *b, void
     hipLaunchKernelGGL(k_dense_mv<double>, dim3(grid), dim3(kBlock), 0,
                        FP3D_STREAM(stream), n, M, (const double *)b, (double *)x);
   return check_launch("dense_matvec_v");
+}
+
+int fp3d_grp_vcycle_op(int nb, const void *recs, int op, int64_t max_n, int G, int fmt,
+                       void *stream) {
+  FP3D_REQUIRE(nb >= 1 && nb <= kMaxBatch, "grp_vcycle_op: nb in [1, 1024]");
+  FP3D_REQUIRE(recs && max_n > 0, "grp_vcycle_op: bad arguments");
+  FP3D_REQUIRE(op >= 0 && op <= 2, "grp_vcycle_op: op is 0 (SpMV), 1 (element-wise) or 2 (dense)");
+  FP3D_REQUIRE((fmt & ~7) == 0, "grp_vcycle_op: fmt takes bits 1, 2 and 4");
+  hipStream_t s = FP3D_STREAM(stream);
+  const GrpRec *R = (const GrpRec *)recs;
+  const bool v32 = fmt & kFmtF32Vectors;
+  if (op == 1) {
+    if (v32)
+      hipLaunchKernelGGL(k_grp_elem<float>, dim3(grid_for(max_n), nb), dim3(kBlock), 0, s, R);
+    else
+      hipLaunchKernelGGL(k_grp_elem<double>, dim3(grid_for(max_n), nb), dim3(kBlock), 0, s, R);
+    return check_launch("grp_vcycle_op");
+  }
+  if (op == 2) {
+    const int64_t rpb = kBlock / 64;
+    const unsigned grid = (unsigned)((max_n + rpb - 1) / rpb);
+    if (v32)
+      hipLaunchKernelGGL(k_grp_dense<float>, dim3(grid, nb), dim3(kBlock), 0, s, R);
+    else
+      hipLaunchKernelGGL(k_grp_dense<double>, dim3(grid, nb), dim3(kBlock), 0, s, R);
+    return check_launch("grp_vcycle_op");
+  }
+  FP3D_REQUIRE(G == 4 || G == 8 || G == 16 || G == 32 || G == 64,
+               "grp_vcycle_op: G must be 4, 8, 16, 32 or 64");
+  const unsigned grid = grid_for(max_n, kBlock / G);
+  const bool f32v = fmt & kFmtF32Values, i16 = fmt & kFmtIdx16;
+#define GRP_L(GG, T, TI, TV) \
+  hipLaunchKernelGGL((k_grp_spmv<GG, T, TI, TV>), dim3(grid, nb), dim3(kBlock), 0, s, R)
+#define GRP_G(T, TI, TV)                     \
+  switch (G) {                               \
+    case 4: GRP_L(4, T, TI, TV); break;      \
+    case 8: GRP_L(8, T, TI, TV); break;      \
+    case 16: GRP_L(16, T, TI, TV); break;    \
+    case 32: GRP_L(32, T, TI, TV); break;    \
+    default: GRP_L(64, T, TI, TV); break;    \
+  }
+#define GRP_V(T, TI)             \
+  if (v32) {                     \
+    GRP_G(T, TI, float)          \
+  } else {                       \
+    GRP_G(T, TI, double)         \
+  }
+  if (f32v && i16) {
+    GRP_V(float, int16_t)
+  } else if (f32v) {
+    GRP_V(float, int32_t)
+  } else if (i16) {
+    GRP_V(double, int16_t)
+  } else {
+    GRP_V(double, int32_t)
+  }
+#undef GRP_V
+#undef GRP_G
+#undef GRP_L
+  return check_launch("grp_vcycle_op");
 }
 
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,

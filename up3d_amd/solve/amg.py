@@ -389,5 +389,195 @@ class AMGHierarchy:
         x.copy_(self._gx)
         return x
 
+    def program(self, b: torch.Tensor, x: torch.Tensor) -> "VCycleProgram":
+        """The V-cycle from b into x as device-operation records (cached per
+        buffer pair), for grouped launches with other hierarchies."""
+        key = (ptr(b), ptr(x))
+        progs = self.__dict__.setdefault("_programs", {})
+        p = progs.get(key)
+        if p is None:
+            p = progs[key] = VCycleProgram(self, b, x)
+        return p
+
     def aspreconditioner(self):
         return self
+
+
+# ------------------------------------------------------- grouped V-cycles
+# Record layout of fp3d_grp_vcycle_op (16 int64 words per solve and
+# operation): n, row pointer, columns, values, SpMV operand, kind,
+# five pointers, two float64 scalars (as their bits), flag, two spare.
+_GRP_WORDS = 16
+(_K_STORE, _K_RESIDUAL, _K_ADD, _K_CHEB_FIRST, _K_CHEB_ITER, _K_CHEB_LAST,
+ _K_RESTRICT_START) = range(7)
+_K_CHEB_ZERO, _K_ADD_VEC, _K_TO_F32, _K_TO_F64, _K_DENSE = 10, 11, 12, 13, 14
+_OP_SPMV, _OP_ELEM, _OP_DENSE = 0, 1, 2
+
+
+def _lanes_for(n_rows: int, nnz: int) -> int:
+    """Lanes per CSR row of the library's SpMV launches (its lanes_for)."""
+    avg = nnz / n_rows if n_rows > 0 else 1.0
+    if avg <= 6:
+        return 4
+    if avg <= 20:
+        return 8
+    if avg <= 48:
+        return 16
+    if avg <= 128:
+        return 32
+    return 64
+
+
+def _bits(v: float) -> int:
+    return int(np.float64(v).view(np.int64))
+
+
+class VCycleProgram:
+    """One V-cycle of an AMGHierarchy from b into x as the list of device
+    operations AMGHierarchy.apply_eager launches, in the same order with
+    the same parameters, each written as a record for fp3d_grp_vcycle_op.
+
+    `signature` holds every operation's launch shape (operation, lanes per
+    row, format bits, row / entry count); programs with equal signatures
+    run grouped, one launch per operation for all of them (run_grouped),
+    and each gets the bits of its own V-cycle."""
+
+    def __init__(self, amg: "AMGHierarchy", b: torch.Tensor, x: torch.Tensor):
+        self.amg = amg
+        self._ops = []
+        self._recs = []
+        vf = amg.vfmt
+        if amg.v32:
+            n0 = amg.levels[0].n
+            self._elem(_K_TO_F32, n0, vf, (ptr(b), ptr(amg._b32)), d0=1.0)
+            self._level(0, amg._b32, amg._x32, False)
+            self._elem(_K_TO_F64, n0, vf, (ptr(amg._x32), ptr(x)))
+        else:
+            self._level(0, b, x, False)
+        self.signature = tuple(self._ops)
+        self.table = np.stack(self._recs) if self._recs else np.zeros((0, _GRP_WORDS),
+                                                                       np.int64)
+
+    # -- record builders
+    def _rec(self, op, G, fmt, n, words):
+        self._ops.append((op, G, fmt, int(n)))
+        w = np.zeros(_GRP_WORDS, dtype=np.int64)
+        for i, v in words.items():
+            w[i] = v
+        self._recs.append(w)
+
+    def _spmv(self, M: DeviceCSR, xin, kind, p=(), d0=0.0, d1=0.0, flag=0):
+        fmt = M.fmt | self.amg.vfmt
+        words = {0: M.n, 1: ptr(M.indptr), 2: ptr(M.index_array), 3: ptr(M.values),
+                 4: ptr(xin), 5: kind, 11: _bits(d0), 12: _bits(d1), 13: int(flag)}
+        for i, q in enumerate(p):
+            words[6 + i] = ptr(q) if isinstance(q, torch.Tensor) else int(q)
+        self._rec(_OP_SPMV, _lanes_for(M.n, M.nnz), fmt, M.n, words)
+
+    def _elem(self, kind, n, fmt, p, d0=0.0, flag=0):
+        words = {0: n, 5: kind, 11: _bits(d0), 13: int(flag)}
+        for i, q in enumerate(p):
+            words[6 + i] = ptr(q) if isinstance(q, torch.Tensor) else int(q)
+        self._rec(_OP_ELEM, 0, fmt & 2, n, words)
+
+    # -- the V-cycle (AMGHierarchy._vcycle and the library's cheb_smooth_t)
+    def _cheb(self, L, b, x, zero_init):
+        A = L.A
+        theta = 0.5 * (L.lmax + L.lmin)
+        delta = 0.5 * (L.lmax - L.lmin)
+        sigma = theta / delta
+        rho0 = 1.0 / sigma
+        if zero_init == 1:
+            self._elem(_K_CHEB_ZERO, L.n, self.amg.vfmt, (b, L.dinv, x, L.r, L.d0),
+                       d0=1.0 / theta, flag=1 if L.degree == 1 else 0)
+        elif zero_init == 0:
+            self._spmv(A, x, _K_CHEB_FIRST, (b, L.dinv, L.r, L.d0), d0=1.0 / theta)
+        dold, dnew = L.d0, L.d1
+        for k in range(1, L.degree):
+            rho1 = 1.0 / (2.0 * sigma - rho0)
+            c1 = rho1 * rho0
+            c2 = 2.0 * rho1 / delta
+            kind = _K_CHEB_LAST if k == L.degree - 1 else _K_CHEB_ITER
+            self._spmv(A, dold, kind, (L.dinv, dold, x, L.r, dnew), d0=c1, d1=c2)
+            rho0 = rho1
+            dold, dnew = dnew, dold
+        if L.degree == 1 and zero_init == 0:
+            self._elem(_K_ADD_VEC, L.n, self.amg.vfmt, (dold, x))
+
+    def _level(self, l, b, x, started):
+        amg = self.amg
+        L = amg.levels[l]
+        if l == len(amg.levels) - 1:
+            words = {0: L.n, 5: _K_DENSE, 6: ptr(amg.coarse), 7: ptr(b), 8: ptr(x)}
+            self._rec(_OP_DENSE, 0, amg.vfmt, L.n, words)
+            return
+        self._cheb(L, b, x, 2 if started else 1)
+        self._spmv(L.A, x, _K_RESIDUAL, (b, L.r))
+        Lc = amg.levels[l + 1]
+        coarse_next = l + 1 == len(amg.levels) - 1
+        if coarse_next:
+            self._spmv(L.R, L.r, _K_STORE, (Lc.b,))
+        else:
+            theta_c = 0.5 * (Lc.lmax + Lc.lmin)
+            self._spmv(L.R, L.r, _K_RESTRICT_START, (Lc.b, Lc.dinv, Lc.x, Lc.r, Lc.d0),
+                       d0=1.0 / theta_c, flag=1 if Lc.degree == 1 else 0)
+        self._level(l + 1, Lc.b, Lc.x, not coarse_next)
+        self._spmv(L.P, Lc.x, _K_ADD, (x,))
+        self._cheb(L, b, x, 0)
+
+
+def run_grouped(programs) -> None:
+    """One V-cycle per program, all of equal signature and on distinct
+    hierarchies, queued as one fp3d_grp_vcycle_op launch per operation."""
+    p0 = programs[0]
+    if any(p.signature != p0.signature for p in programs[1:]):
+        raise ValueError("run_grouped: programs of different signatures")
+    if len({id(p.amg) for p in programs}) != len(programs):
+        raise ValueError("run_grouped: two programs share a hierarchy's buffers")
+    nb = len(programs)
+    n_ops = len(p0.signature)
+    table = np.ascontiguousarray(np.stack([p.table for p in programs], axis=1))
+    host = torch.from_numpy(table.reshape(-1)).pin_memory()
+    dev = torch.empty_like(host, device=p0.amg.coarse.device)
+    dev.copy_(host, non_blocking=True)
+    base = ptr(dev)
+    s = stream_handle()
+    for i, (op, G, fmt, n) in enumerate(p0.signature):
+        _native.call("fp3d_grp_vcycle_op", nb, base + i * nb * 8 * _GRP_WORDS, op, n, G,
+                     fmt, s)
+
+
+def apply_preconditioners(calls) -> int:
+    """psolve(b, x) for each (psolve, b, x) of one Krylov round.  AMG
+    V-cycles (psolve = AMGHierarchy.apply) whose programs have equal
+    signatures run grouped (run_grouped, one launch per operation for all
+    of them); any other preconditioner, and a V-cycle with no partner, runs
+    by itself.  Every V-cycle gives the bits it gives alone.  Returns the
+    number of V-cycles that ran grouped."""
+    groups: dict = {}
+    for ps, b, x in calls:
+        amg = getattr(ps, "__self__", None)
+        if (isinstance(amg, AMGHierarchy)
+                and getattr(ps, "__func__", None) is AMGHierarchy.apply):
+            prog = amg.program(b, x)
+            slots = groups.setdefault(prog.signature, [[]])
+            # two V-cycles of one hierarchy share its level buffers: never
+            # in the same grouped launch
+            for g in slots:
+                if all(q[0].amg is not amg for q in g):
+                    g.append((prog, b, x))
+                    break
+            else:
+                slots.append([(prog, b, x)])
+        else:
+            ps(b, x)
+    n_grouped = 0
+    for slots in groups.values():
+        for g in slots:
+            if len(g) == 1:
+                prog, b, x = g[0]
+                prog.amg.apply(b, x)
+            else:
+                run_grouped([q[0] for q in g])
+                n_grouped += len(g)
+    return n_grouped

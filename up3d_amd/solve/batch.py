@@ -55,6 +55,7 @@ class BatchExecutor:
         self.n_evals = 0
         self.max_width = 0
         self.n_batched_launch_groups = 0
+        self.n_grouped_vcycles = 0
         self.n_batched_eval_groups = 0
 
     def run(self, coros: Dict[Hashable, object],
@@ -107,7 +108,7 @@ class BatchExecutor:
                     groups[batch_key(r)].append(r)
                 for key, rs in groups.items():
                     if key is not None and len(rs) > 1:
-                        batched_arnoldi(rs)
+                        self.n_grouped_vcycles += batched_arnoldi(rs)
                         self.n_batched_launch_groups += 1
                     else:
                         for r in rs:
@@ -137,6 +138,7 @@ class BatchExecutor:
         return {"rounds": self.n_rounds, "arnoldi_steps": self.n_steps,
                 "evaluations": self.n_evals, "max_steps_per_round": self.max_width,
                 "batched_groups": self.n_batched_launch_groups,
+                "grouped_vcycles": self.n_grouped_vcycles,
                 "batched_eval_groups": self.n_batched_eval_groups}
 
 

@@ -28,7 +28,7 @@ from scipy.linalg import get_lapack_funcs
 
 from .. import _native
 from ..device import empty, ptr, require_gpu, stream_handle, to_dev, to_host, zeros
-from .amg import AMGHierarchy, DeviceCSR
+from .amg import AMGHierarchy, DeviceCSR, apply_preconditioners
 
 _lartg = get_lapack_funcs("lartg", dtype=np.float64)
 
@@ -157,14 +157,16 @@ def batch_key(step: ArnoldiStep):
     return (id(op.store), op.fmt(), step.f32, tuple(V.shape))
 
 
-def batched_arnoldi(steps) -> None:
+def batched_arnoldi(steps) -> int:
     """Queue the Arnoldi steps of several stepped GMRES solves that share a
-    batch_key: the Jacobian matvecs in one batched launch per kernel, each
-    solve's preconditioner, then the orthogonalisation, norm and
-    normalisation in one batched launch per kernel, then each step's
-    coefficient copy to pinned host memory.  Per solve this is the device
-    work ArnoldiStep.run queues, in the same order, with the same kernel
-    bodies and grids."""
+    batch_key: the Jacobian matvecs in one batched launch per kernel, the
+    solves' preconditioners (AMG V-cycles of equally shaped hierarchies in
+    one launch per V-cycle operation, amg.apply_preconditioners), then the
+    orthogonalisation, norm and normalisation in one batched launch per
+    kernel, then each step's coefficient copy to pinned host memory.  Per
+    solve this is the device work ArnoldiStep.run queues, in the same
+    order, with the same kernel bodies and grids.  Returns the number of
+    V-cycles that ran grouped."""
     s0 = steps[0]
     op0 = s0.matvec.__self__
     st = op0.store
@@ -197,13 +199,17 @@ def batched_arnoldi(steps) -> None:
     _native.call("fp3d_batch_jacobian_matvec", len(steps), ptr(dev), max_active, n, st.nnz,
                  ptr(st.indptr), ptr(i16 if i16 is not None else st.indices), op0.fmt(),
                  ptr(st.el), st.n_tets, 1 if f32 else 0, sh)
+    pre = []
     for stp in steps:
         gw = stp.gw
         w = gw["w"][stp.col % 2]
         if stp.psolve is None:
             w.copy_(gw["tmp"])
         else:
-            stp.psolve(gw["tmp"], w)
+            pre.append((stp.psolve, gw["tmp"], w))
+    # the solves' V-cycles, grouped where their hierarchies have the same
+    # shape (one launch per V-cycle operation for all of them)
+    n_grouped = apply_preconditioners(pre)
     _native.call("fp3d_batch_orthonormalise", len(steps), ptr(dev), max_nv, n,
                  1 if f32 else 0, 0.5, stream_handle())
     for stp in steps:
@@ -212,6 +218,7 @@ def batched_arnoldi(steps) -> None:
         gw["hbuf"][col % 2][: 3 + 2 * k].copy_(gw["outs"][col % 2][: 3 + 2 * k],
                                                non_blocking=True)
         gw["evs"][col % 2].record()
+    return n_grouped
 
 
 def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
