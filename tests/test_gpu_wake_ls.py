@@ -142,11 +142,21 @@ def test_side_graphs_do_not_cross_the_wake(gpu):
 
     mesh = _mesh("coarse")
     _, mv = _setup(mesh)
-    for (fn_d, _), keep in zip(mv._side_graphs(), (mv.in_upper, mv.in_lower)):
+    st = mv.op.store
+    fn_h = np.asarray(st.face_neighbors_host, dtype=np.int64)
+    cen = np.asarray(st.centroids_host, dtype=np.float64)
+    for (fn_d, d_d), keep in zip(mv._side_graphs(), (mv.in_upper, mv.in_lower)):
         fn = to_host(fn_d)
         nb = fn[keep][fn[keep] >= 0]
         assert keep[nb].all(), "upwind graph leaks across the wake"
         assert (fn[~keep] == -1).all()
+        # the device construction is the host formula, bit for bit: a face
+        # is kept when both its elements are on the side, its displacement
+        # the centroid difference, 0 where dropped
+        ok = (fn_h >= 0) & keep[:, None] & keep[np.maximum(fn_h, 0)]
+        assert np.array_equal(fn, np.where(ok, fn_h, -1))
+        disp = np.where(ok[:, :, None], cen[np.maximum(fn_h, 0)] - cen[:, None, :], 0.0)
+        assert np.array_equal(to_host(d_d), disp.reshape(len(fn_h), 12))
 
 
 def test_supercritical_pocket_m070(gpu):

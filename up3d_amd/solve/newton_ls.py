@@ -383,7 +383,8 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
                              phi_init: Optional[np.ndarray] = None,
                              gamma_init: float = 0.0, n_seed: int = 40,
                              seed_damping_theta: Optional[float] = 0.2,
-                             seed_omega_rho: float = 0.5, n_newton_max: int = 40,
+                             seed_omega_rho: float = 0.5, seed_amg_rebuild_every: int = 5,
+                             n_newton_max: int = 40,
                              tol_residual: float = 1e-10, lam_min: float = 0.05,
                              freeze: bool = True, freeze_tol: Optional[float] = None,
                              freeze_refresh_max: int = 8,
@@ -414,7 +415,8 @@ def solve_multivalued_newton(mvop, mesh, m_inf: float, alpha_deg: float = 1.25,
         upwind_c=upwind_c, m_crit=m_crit, damping_theta=seed_damping_theta,
         omega_rho=seed_omega_rho, n_outer_max=n_seed, gamma_init=gamma_init,
         phi_init=phi_init, vortex_center=vortex_center, farfield=farfield,
-        lift_axis=lift_axis, external_rhs=external_rhs)
+        lift_axis=lift_axis, external_rhs=external_rhs,
+        amg_rebuild_every=seed_amg_rebuild_every)
     phi = seed["phi_ext"].copy()
     gamma = float(seed["gamma"])
     timings["seed"] = time.perf_counter() - t0

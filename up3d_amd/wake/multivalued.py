@@ -269,18 +269,21 @@ class MultivaluedOperator:
         """Face graphs restricted to each side's element set (the wake is a
         slip line: the upwind walk never crosses it), as device
         (face_neighbors, face displacement) pairs
-        (pyfp3d/wake/multivalued.py:608-640)."""
+        (pyfp3d/wake/multivalued.py:608-640), built on the device from the
+        store's face table: a face is kept when both its elements are on the
+        side, and its displacement is the store's centroid difference (the
+        same float64 subtraction), 0 where the face is dropped."""
         if self._graphs is None:
             st = self.op.store
-            fn = np.asarray(st.face_neighbors_host, dtype=np.int64)
-            cen = np.asarray(st.centroids_host, dtype=np.float64)
+            fn = st.fn.long()
+            d = st.dcen.view(-1, 4, 3)
             out = []
             for keep in (self.in_upper, self.in_lower):
-                ok = (fn >= 0) & keep[:, None] & keep[np.maximum(fn, 0)]
-                f = np.where(ok, fn, -1)
-                disp = np.where(ok[:, :, None], cen[np.maximum(f, 0)] - cen[:, None, :], 0.0)
-                out.append((to_dev(f.astype(np.int32), torch.int32),
-                            to_dev(np.ascontiguousarray(disp).reshape(len(fn), 12))))
+                kd = to_dev(keep.astype(np.uint8), torch.uint8).bool()
+                ok = (fn >= 0) & kd[:, None] & kd[fn.clamp(min=0)]
+                f = torch.where(ok, fn, torch.full_like(fn, -1)).to(torch.int32)
+                disp = torch.where(ok[:, :, None], d, torch.zeros_like(d))
+                out.append((f.contiguous(), disp.reshape(-1, 12).contiguous()))
             self._graphs = tuple(out)
         return self._graphs
 
