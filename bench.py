@@ -748,20 +748,44 @@ def run_config1(with_cpu: bool = True):
     return out
 
 
+def config3_state(m, dn, m_inf, alpha):
+    """Grid-sequenced Newton state of the 96x32x32 channel `m` at (M_inf,
+    alpha): Mach ramp 0.6 -> M_inf at upwind_c 3 on the 48x16x16 channel
+    (the M 0.84 workload's coarsest mesh), then on `m` from its P1
+    interpolant at upwind_c 3, 2 and 1.5; freestream Dirichlet values on the
+    inflow and outflow planes."""
+    from up3d_amd.constraints.dirichlet import freestream_phi
+    from up3d_amd.mesh.channel import boundary_nodes, bump_channel
+    from up3d_amd.solve.sequencing import solve_newton_sequenced
+
+    coarse = bump_channel(48, 16, 16, jitter=0.15, seed=20260103, rcm=True)
+    dc = boundary_nodes(coarse, ["inflow", "outflow"])
+    defs = [{"nodes": coarse.nodes, "elements": coarse.elements, "dirichlet_nodes": dc,
+             "dirichlet_values": freestream_phi(coarse.nodes[dc], alpha)},
+            {"nodes": m.nodes, "elements": m.elements, "dirichlet_nodes": dn,
+             "dirichlet_values": freestream_phi(m.nodes[dn], alpha)}]
+    return solve_newton_sequenced(
+        defs, m_inf, m_start=0.6, dm=0.05, upwind_c_ramp=3.0, upwind_c_post=(2.0, 1.5),
+        post_stages_on="finest",
+        newton_kw={"gmres_restart": 200, "gmres_maxiter": 5, "n_newton_max": 30,
+                   "amg_rebuild_every": 0, "precond_matrix": "jacobian_spd"})
+
+
 def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
     """The adjoint case on the 96x32x32 jittered channel (589,824 tets) at
-    M_inf 0.80, alpha 0: the converged Newton state (upward Mach ramp with
-    step halving), then d(cf_x)/d(M_inf, alpha, y of the 97x33 floor nodes)
+    M_inf 0.80, alpha 0: the converged Newton state (grid-sequenced as the
+    M 0.84 workload: Mach ramp at upwind_c 3 on the 48x16x16 channel, its P1
+    interpolant on the 96x32x32 one, upwind_c 3 -> 2 -> 1.5 there, then the
+    problem's own solve, which settles the state on its upwind selection),
+    then d(cf_x)/d(M_inf, alpha, y of the 97x33 floor nodes)
     from one adjoint solve, checked by central differences (step h) along
     n_pert seeded unit-norm floor perturbations: of the frozen-selection
     solve (the smooth piece the adjoint differentiates) and, for n_free of
     them, of free re-solves (which may switch upstream selections)."""
     import torch
 
-    from up3d_amd.constraints.dirichlet import freestream_phi
     from up3d_amd.mesh.channel import boundary_nodes, config_mesh
     from up3d_amd.solve.adjoint import ChannelForceProblem
-    from up3d_amd.solve.newton import solve_newton_transonic
 
     m = config_mesh("c3")
     dn = boundary_nodes(m, ["inflow", "outflow"])
@@ -769,15 +793,8 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
     prob = ChannelForceProblem(m.nodes, m.elements, dn, m.boundary_faces["floor"], floor)
     m_inf, alpha = 0.80, 0.0
     t0 = time.perf_counter()
-    sol = None
-    for mm in (0.5, 0.6, 0.65, 0.7):
-        sol = prob.solve(None, mm, alpha, phi_init=None if sol is None else sol["phi"])
-    rt = solve_newton_transonic(
-        m.nodes, m.elements, dn, freestream_phi(m.nodes[dn], alpha), m_inf,
-        m_start=0.7, dm=0.05, upwind_c=1.5,
-        newton_kw={"n_newton_max": 40, "tol_residual": 1e-13, "phi_init": sol["phi"],
-                   "n_picard_seed": 0})
-    sol = prob.solve(None, m_inf, alpha, phi_init=rt["phi"])
+    seq = config3_state(m, dn, m_inf, alpha)
+    sol = prob.solve(None, m_inf, alpha, phi_init=seq["phi"])
     solve_s = time.perf_counter() - t0
     if not sol["converged"]:
         return {"error": "no converged state at M 0.80"}
@@ -790,8 +807,12 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
     torch.cuda.synchronize()
     adjoint_s = time.perf_counter() - t0
     # the forward tangent of the same linearisation: exact derivatives that
-    # the adjoint must reproduce to round-off
+    # the adjoint must reproduce to round-off (and, timed, one forward
+    # linear solve at the converged state to the adjoint's tolerance)
+    t0 = time.perf_counter()
     tan_m = prob.jvp_params(sol, 1.0, 0.0, weights=(1.0, 0.0))
+    torch.cuda.synchronize()
+    tangent_s = time.perf_counter() - t0
     tan_a = prob.jvp_params(sol, 0.0, 1.0, weights=(1.0, 0.0))
     frozen = prob.frozen_selection(sol)
 
@@ -817,12 +838,23 @@ def run_adjoint(n_pert: int = 16, n_free: int = 4, h: float = 1e-6):
             fr = (f_free(y0 + h * d) - f_free(y0 - h * d)) / (2 * h)
             rel_free.append(abs(ad - fr) / max(abs(fr), 1e-30))
     fd_s = time.perf_counter() - t0
-    step_s = solve_s / max(1, sum(n for _, n, _ in rt["level_history"]))
+    # one forward Newton step on this mesh: the 96x32x32 stages of the
+    # sequence, per Newton step (residual, linearisation, AMG, GMRES to the
+    # Eisenstat-Walker tolerance)
+    fine = seq["mesh_results"][-1]
+    fine_steps = sum(lv[2] for lv in fine["levels"])
+    fine_its = sum(lv[5] for lv in fine["levels"])
+    step_s = fine["wall_s"] / max(1, fine_steps)
     return {"workload": "bump channel 96x32x32 hexes -> 589,824 tets, jitter 15%, RCM, "
                         "M_inf 0.80, alpha 0, output wall x-force",
             "n_tets": int(m.n_tets), "n_floor_nodes": int(len(floor)),
             "solve_s": round(solve_s, 2), "adjoint_s": round(adjoint_s, 3),
+            "newton_step_s": round(step_s, 4), "newton_steps_96": int(fine_steps),
+            "newton_gmres_per_step": round(fine_its / max(1, fine_steps), 1),
+            "adjoint_gmres": int(g["adjoint_iters"]),
             "adjoint_over_newton_step": round(adjoint_s / step_s, 2),
+            "tangent_s": round(tangent_s, 3),
+            "adjoint_over_tangent": round(adjoint_s / tangent_s, 2),
             "d_m_inf": g["d_m_inf"], "tangent_m_inf": tan_m, "fd_m_inf_frozen": fd_m,
             "d_alpha_deg": g["d_alpha_deg"], "tangent_alpha_deg": tan_a,
             "fd_alpha_frozen": fd_a,

@@ -49,9 +49,9 @@ def test_config1_picard_full_size_matches_oracle(gpu):
 
 def test_config3_adjoint_full_size(gpu):
     from up3d_amd.constraints.dirichlet import freestream_phi
-    from up3d_amd.mesh.channel import boundary_nodes, config_mesh
+    from up3d_amd.mesh.channel import boundary_nodes, bump_channel, config_mesh
     from up3d_amd.solve.adjoint import ChannelForceProblem
-    from up3d_amd.solve.newton import solve_newton_transonic
+    from up3d_amd.solve.sequencing import solve_newton_sequenced
 
     m = config_mesh("c3")
     dn = boundary_nodes(m, ["inflow", "outflow"])
@@ -59,15 +59,21 @@ def test_config3_adjoint_full_size(gpu):
     prob = ChannelForceProblem(m.nodes, m.elements, dn, m.boundary_faces["floor"], floor)
     print(f"[configs] c3 n_tets={m.n_tets} floor nodes={len(floor)}", flush=True)
     m_inf, alpha = 0.80, 0.0
-    sol = None
-    for mm in (0.5, 0.6, 0.65, 0.7):
-        sol = prob.solve(None, mm, alpha, phi_init=None if sol is None else sol["phi"])
-    rt = solve_newton_transonic(
-        m.nodes, m.elements, dn, freestream_phi(m.nodes[dn], alpha), m_inf,
-        m_start=0.7, dm=0.05, upwind_c=1.5,
-        newton_kw={"n_newton_max": 40, "tol_residual": 1e-13, "phi_init": sol["phi"],
-                   "n_picard_seed": 0})
-    sol = prob.solve(None, m_inf, alpha, phi_init=rt["phi"])
+    # grid-sequenced state: Mach ramp at upwind_c 3 on 48x16x16, then the
+    # 96x32x32 channel from its interpolant at upwind_c 3 -> 2 -> 1.5
+    coarse = bump_channel(48, 16, 16, jitter=0.15, seed=20260103, rcm=True)
+    dc = boundary_nodes(coarse, ["inflow", "outflow"])
+    seq = solve_newton_sequenced(
+        [{"nodes": coarse.nodes, "elements": coarse.elements, "dirichlet_nodes": dc,
+          "dirichlet_values": freestream_phi(coarse.nodes[dc], alpha)},
+         {"nodes": m.nodes, "elements": m.elements, "dirichlet_nodes": dn,
+          "dirichlet_values": freestream_phi(m.nodes[dn], alpha)}],
+        m_inf, m_start=0.6, dm=0.05, upwind_c_ramp=3.0, upwind_c_post=(2.0, 1.5),
+        post_stages_on="finest",
+        newton_kw={"gmres_restart": 200, "gmres_maxiter": 5, "n_newton_max": 30,
+                   "amg_rebuild_every": 0, "precond_matrix": "jacobian_spd"})
+    assert seq["converged"]
+    sol = prob.solve(None, m_inf, alpha, phi_init=seq["phi"])
     print(f"[configs] c3 M0.80: converged={sol['converged']} frozen residual at "
           f"acceptance {sol['frozen_residual_at_accept']:.1e}, "
           f"{sol['frozen_polish_steps']} settling steps", flush=True)
