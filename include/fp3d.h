@@ -452,6 +452,22 @@ int fp3d_multi_axpy_f32v(int64_t n, int nv, const float *V, int64_t ld,
                          const double *h, double sign, double *w, void *stream);
 int fp3d_scale_to_f32(int64_t n, double a, const double *x, float *y, void *stream);
 int fp3d_f32_to_f64(int64_t n, const float *x, double *y, void *stream);
+/* One Arnoldi step of the device GMRES in one call (no reference
+ * counterpart; pyfp3d/solve/linear.py:182-193 runs scipy's GMRES, whose
+ * step this is): w = M^-1 J v_col (fp3d_csr_spmv_term3, then the V-cycle as
+ * an instantiated HIP graph, or w = J v_col without one), w orthogonalised
+ * against v_0..v_col (fp3d_multi_dot_norm, fp3d_multi_axpy, the
+ * conditional second pass, the f32v variants for a float32 basis),
+ * normalised into v_{col+1} (fp3d_norms, fp3d_scale_inv_dev[_to_f32]), and
+ * the step's coefficients out[0 .. 3 + 2(col+1)) copied to pinned host
+ * memory.  The same launches, in the same order, as issued one by one, so
+ * the same bits.  a: 27 int64 words -- n, f32 basis flag, V, ld, v64 scratch,
+ * tmp, w[0], w[1], out[0], out[1], reduction work, pinned hbuf[0], hbuf[1];
+ * the Jacobian matvec's nnz, indptr, columns, values, fmt, elements,
+ * upstream, Term-3 workspace, n_active, n_tets, free_mask; V-cycle graph
+ * execs for w[0] and w[1]; 1 when a preconditioner is applied.  Buffers of
+ * parity col & 1 are used.  col <= 255. */
+int fp3d_arnoldi_step(const int64_t *a, int col, void *stream);
 /* y = x / nrm[0], the divisor read on the device (no reference
  * counterpart): GMRES normalises the new Arnoldi vector without a host
  * round trip, so the next step is queued while the host applies Givens. */

@@ -162,6 +162,7 @@ _SIGS = {
     "fp3d_host_csr_matvec": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_host_apply_givens": (_int, [_int, _vp, _vp]),
     "fp3d_graph_launch": (_int, [_vp, _vp]),
+    "fp3d_arnoldi_step": (_int, [_vp, _int, _vp]),
     "fp3d_host_face_adjacency": (_int, [_i64, _vp, _vp]),
     "fp3d_host_element_geometry": (_int, [_i64, _vp, _vp, _vp, _vp, _pi64]),
     "fp3d_host_locate_points": (_int, [_i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),

@@ -1843,6 +1843,73 @@ int fp3d_dense_matvec_v(int fmt, int64_t n, const double *M, const void *b, void
   return check_launch("dense_matvec_v");
 }
 
+int fp3d_arnoldi_step(const int64_t *a, int col, void *stream) {
+  FP3D_REQUIRE(a && col >= 0, "arnoldi_step: bad arguments");
+  const int64_t n = a[0];
+  const bool f32 = a[1] != 0;
+  const int64_t ld = a[3];
+  const int k = col + 1;
+  FP3D_REQUIRE(n > 0 && k <= 256, "arnoldi_step: n > 0 and col <= 255");
+  hipStream_t s = FP3D_STREAM(stream);
+  double *v64 = (double *)a[4], *tmp = (double *)a[5];
+  double *w = (double *)a[6 + (col & 1)];
+  double *o = (double *)a[8 + (col & 1)];
+  double *work = (double *)a[10];
+  double *hb = (double *)a[11 + (col & 1)];
+  const void *gexec = (const void *)a[24 + (col & 1)];
+  int rc;
+  // w = M^-1 A v_col
+  const double *x;
+  if (f32) {
+    const float *V = (const float *)a[2];
+    rc = fp3d_f32_to_f64(n, V + (int64_t)col * ld, v64, stream);
+    if (rc) return rc;
+    x = v64;
+  } else {
+    x = (const double *)a[2] + (int64_t)col * ld;
+  }
+  rc = fp3d_csr_spmv_term3(n, a[13], (const int32_t *)a[14], (const void *)a[15],
+                           (const void *)a[16], (int)a[17], (const int32_t *)a[18],
+                           (const int32_t *)a[19], (const void *)a[20], a[21], a[22],
+                           (const uint8_t *)a[23], x, tmp, stream);
+  if (rc) return rc;
+  if (a[26]) {
+    FP3D_REQUIRE(gexec, "arnoldi_step: no V-cycle graph for this column's parity");
+    FP3D_HIP(hipGraphLaunch((hipGraphExec_t)gexec, s));
+  } else {
+    FP3D_HIP(hipMemcpyAsync(w, tmp, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  // h = V^T w, ||w||^2; w -= V h; the conditional second pass; ||w||; v_{col+1}
+  if (f32) {
+    const float *V = (const float *)a[2];
+    if ((rc = fp3d_multi_dot_norm_f32v(n, k, V, ld, w, work, o + 2, stream))) return rc;
+    if ((rc = fp3d_multi_axpy_f32v(n, k, V, ld, o + 2, -1.0, w, stream))) return rc;
+    if ((rc = fp3d_multi_dot_reorth_f32v(n, k, V, ld, w, o + 2, 0.5, work, o + 3 + k,
+                                         stream)))
+      return rc;
+    if ((rc = fp3d_multi_axpy_f32v(n, k, V, ld, o + 3 + k, -1.0, w, stream))) return rc;
+    if ((rc = fp3d_norms(n, w, work, o, stream))) return rc;
+    if ((rc = fp3d_scale_inv_dev_to_f32(n, o, w, (float *)a[2] + (int64_t)(col + 1) * ld,
+                                        stream)))
+      return rc;
+  } else {
+    const double *V = (const double *)a[2];
+    if ((rc = fp3d_multi_dot_norm(n, k, V, ld, w, work, o + 2, stream))) return rc;
+    if ((rc = fp3d_multi_axpy(n, k, V, ld, o + 2, -1.0, w, stream))) return rc;
+    if ((rc = fp3d_multi_dot_reorth(n, k, V, ld, w, o + 2, 0.5, work, o + 3 + k, stream)))
+      return rc;
+    if ((rc = fp3d_multi_axpy(n, k, V, ld, o + 3 + k, -1.0, w, stream))) return rc;
+    if ((rc = fp3d_norms(n, w, work, o, stream))) return rc;
+    if ((rc = fp3d_scale_inv_dev(n, o, w, (double *)a[2] + (int64_t)(col + 1) * ld,
+                                 stream)))
+      return rc;
+  }
+  // the step's coefficients to pinned host memory
+  FP3D_HIP(hipMemcpyAsync(hb, o, (size_t)(3 + 2 * k) * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  return FP3D_OK;
+}
+
 int fp3d_grp_vcycle_op(int nb, const void *recs, int op, int64_t max_n, int G, int fmt,
                        void *stream) {
   FP3D_REQUIRE(nb >= 1 && nb <= kMaxBatch, "grp_vcycle_op: nb in [1, 1024]");

@@ -389,6 +389,32 @@ class AMGHierarchy:
         x.copy_(self._gx)
         return x
 
+    def graph_execs(self, b: torch.Tensor, xs) -> Optional[list]:
+        """Raw hipGraphExec handles of this hierarchy's V-cycle from b into
+        each x of xs (graphs on the callers' buffers, captured here when
+        missing), for a caller that launches them itself; None when graphs
+        are off, before the first eager V-cycle (which loads the kernels),
+        or when the buffers cannot take a direct graph."""
+        if not self.use_graph or self._eager_calls == 0:
+            return None
+        out = []
+        for x in xs:
+            if not (b.is_contiguous() and x.is_contiguous() and b.dtype == torch.float64
+                    and x.dtype == torch.float64 and b.numel() == x.numel() == self._gb.numel()
+                    and b.data_ptr() != x.data_ptr()):
+                return None
+            key = (b.data_ptr(), x.data_ptr())
+            g = self._graphs.get(key)
+            if g is None:
+                if len(self._graphs) >= self._MAX_DIRECT_GRAPHS:
+                    return None
+                g = self._graphs[key] = self._capture(b, x)
+            h = g.raw_cuda_graph_exec()
+            if not isinstance(h, int) or h == 0:
+                return None
+            out.append(h)
+        return out
+
     def program(self, b: torch.Tensor, x: torch.Tensor) -> "VCycleProgram":
         """The V-cycle from b into x as device-operation records (cached per
         buffer pair), for grouped launches with other hierarchies."""

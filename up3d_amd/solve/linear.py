@@ -221,6 +221,40 @@ def batched_arnoldi(steps) -> int:
     return n_grouped
 
 
+def _arnoldi_args(matvec, psolve, gw, n, f32):
+    """(fp3d_arnoldi_step argument words, stream) for a Newton-Jacobian
+    matvec and an AMG (or no) preconditioner, else None (the step is then
+    issued launch by launch)."""
+    op = getattr(matvec, "__self__", None)
+    if not isinstance(op, NewtonOperator) or \
+            getattr(matvec, "__func__", None) is not NewtonOperator.matvec:
+        return None
+    gexec, pre = [0, 0], 0
+    if psolve is not None:
+        amg = getattr(psolve, "__self__", None)
+        if not isinstance(amg, AMGHierarchy) or \
+                getattr(psolve, "__func__", None) is not AMGHierarchy.apply:
+            return None
+        hs = amg.graph_execs(gw["tmp"], gw["w"])
+        if hs is None:
+            return None
+        gexec, pre = hs, 1
+    st = op.store
+    V = gw["V"]
+    na = op.n_active
+    use_t3 = na > 0
+    i16 = getattr(st, "indices16", None)
+    vals = op.data32 if op.is_f32 else op.data
+    words = [n, 1 if f32 else 0, ptr(V), V.stride(0), ptr(gw["v64"]) if f32 else 0,
+             ptr(gw["tmp"]), ptr(gw["w"][0]), ptr(gw["w"][1]), ptr(gw["outs"][0]),
+             ptr(gw["outs"][1]), ptr(gw["red"].work), gw["hbuf"][0].data_ptr(),
+             gw["hbuf"][1].data_ptr(), st.nnz, ptr(st.indptr),
+             ptr(i16 if i16 is not None else st.indices), ptr(vals), op.fmt(), ptr(st.el),
+             ptr(op.upstream) if use_t3 else 0, ptr(op.t3) if use_t3 else 0, na, st.n_tets,
+             ptr(op.free_mask), gexec[0], gexec[1], pre]
+    return np.array(words, dtype=np.int64), torch.cuda.current_stream()
+
+
 def gmres_dev(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = None,
               x0: Optional[torch.Tensor] = None, rtol: float = 1e-5,
               atol: float = 0.0, restart: int = 20, maxiter: Optional[int] = None,
@@ -301,11 +335,25 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
     rnorm = np.inf
     x_nonzero = x0 is not None and bool(torch.any(x != 0).item())
 
+    # Newton-Jacobian matvec with an AMG V-cycle (or none): each step in one
+    # library call (fp3d_arnoldi_step, the same launches as below), set up at
+    # the first step, once the preconditioner has run eagerly
+    fast = [None, pipelined]
+    arnoldi = _native.lib().fp3d_arnoldi_step
+
     def enqueue(col):
         """Device work of Arnoldi step col: w = M^-1 A v_col orthogonalised
         against v_0..v_col and normalised into v_{col+1}; its coefficients
         (out[2:2+k] = h, out[2+k] = ||w||^2 before, out[3+k:3+2k] = second
         pass, out[0] = ||w|| after) copied to pinned host memory."""
+        if fast[1]:
+            fast[0] = _arnoldi_args(matvec, psolve, gw, n, f32)
+            fast[1] = False
+        if fast[0] is not None:
+            a, stream = fast[0]
+            _native.check(arnoldi(a.ctypes.data, col, stream.cuda_stream), "arnoldi_step")
+            evs[col % 2].record(stream)
+            return
         k = col + 1
         o = outs[col % 2]
         w = w_buf[col % 2]
