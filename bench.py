@@ -113,7 +113,10 @@ def main():
                     choices=["batched", "sequential", "both"])
     ap.add_argument("--no-coarse", action="store_true",
                     help="sweep without the coarse-companion route")
-    ap.add_argument("--sweep-chains", default="4,2,1,1",
+    ap.add_argument("--sweep-all-meshes", action="store_true",
+                    help="batched sweep: advance the chains of all four meshes together "
+                         "instead of one mesh after another")
+    ap.add_argument("--sweep-chains", default="4,16,16,16",
                     help="warm-start chains per mesh of the batched sweep: one count, "
                          "or one per mesh separated by commas")
     ap.add_argument("--no-plate", action="store_true",
@@ -164,7 +167,10 @@ def main():
                 key = mode if mode == "sequential" else f"batched[{ch}]"
                 if args.no_coarse:
                     key += "/no-coarse"
-                out[key] = run_sweep(mode, _chains(ch), meshes, coarse=not args.no_coarse)
+                if mode == "batched" and args.sweep_all_meshes:
+                    key += "/all-meshes"
+                out[key] = run_sweep(mode, _chains(ch), meshes, coarse=not args.no_coarse,
+                                     by_mesh=not args.sweep_all_meshes)
                 log(f"[bench] sweep {key}: wall {out[key]['wall_s']} s, "
                     f"{out[key]['n_converged']} converged, executor {out[key]['executor']}")
         print(json.dumps(out), flush=True)
@@ -361,7 +367,8 @@ def main():
     sweep = None
     if args.sweep:
         sweep = run_sweep(args.sweep_mode if args.sweep_mode != "both" else "batched",
-                          _chains(args.sweep_chains.split(";")[0]))
+                          _chains(args.sweep_chains.split(";")[0]),
+                          by_mesh=not args.sweep_all_meshes)
         log(f"[bench] sweep {json.dumps({k: v for k, v in sweep.items() if k != 'instances'})}")
 
     out = {
@@ -954,16 +961,19 @@ def _chains(text: str):
     return parts[0] if len(parts) == 1 else parts
 
 
-def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None, coarse: bool = True):
+def run_sweep(mode: str = "batched", chains=(4, 16, 16, 16), meshes=None, coarse: bool = True,
+              by_mesh: bool = True):
     """Sweep of 16 seeded (M_inf in [0.60, 0.86], alpha in [0, 3] deg)
     conditions on each of 4 jittered RCM bump channels (144,000 / 589,824 /
     1,152,000 / 2,387,718 tets), FP32-preconditioned Newton, every
     instance to ||R_free||_inf < 1e-10 with 0 limited / floored.
-    mode "batched": `chains` warm-start chains per mesh, all advanced in
-    lockstep with batched Arnoldi launches (solve/batch.py); "sequential":
-    one condition after another (solve/sweep.py).  coarse: conditions on
-    the three meshes of at least 500,000 tets start from a solve on a
-    73,728-tet companion channel (solve/sweep.py coarse_route)."""
+    mode "batched": `chains` warm-start chains per mesh, advanced in
+    lockstep with batched Arnoldi launches and grouped V-cycles
+    (solve/batch.py), one mesh after another (by_mesh) or all meshes
+    together; "sequential": one condition after another (solve/sweep.py).
+    coarse: conditions on the three meshes of at least 500,000 tets start
+    from a solve on a 73,728-tet companion channel (solve/sweep.py
+    coarse_route), which makes them independent of their chain."""
     import torch
 
     from up3d_amd.solve.batch import solve_sweep_batched
@@ -977,7 +987,8 @@ def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None, coarse: b
     conds = sweep_conditions(len(meshes), 16)
     comp = companion_mesh() if coarse else None
     if mode == "batched":
-        r = solve_sweep_batched(meshes, conds, chains_per_mesh=chains, log=log, coarse=comp)
+        r = solve_sweep_batched(meshes, conds, chains_per_mesh=chains, log=log, coarse=comp,
+                                mesh_by_mesh=by_mesh)
     elif mode == "sequential":
         r = solve_sweep(meshes, conds, log=log, coarse=comp)
     else:
@@ -989,6 +1000,7 @@ def run_sweep(mode: str = "batched", chains=(4, 2, 1, 1), meshes=None, coarse: b
         "workload": "64 conditions: M_inf U[0.60,0.86], alpha U[0,3] deg, 16 on each of "
                     "4 jittered bump channels (144,000 / 589,824 / 1,152,000 / 2,387,718 tets)",
         "mode": mode, "chains_per_mesh": chains if mode == "batched" else None,
+        "mesh_by_mesh": bool(by_mesh) if mode == "batched" else None,
         "coarse_route": bool(coarse),
         "executor": r.get("executor"),
         "mesh_build_s": round(mesh_s, 2), "wall_s": round(r["wall_s"], 2),

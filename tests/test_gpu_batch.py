@@ -73,7 +73,8 @@ def test_lockstep_solves_equal_solves_alone(channel, precision, batched):
         assert r["n_term3_active"] > 0
 
 
-def test_batched_sweep_converges_and_matches_sequential(gpu):
+@pytest.mark.parametrize("by_mesh", [False, True])
+def test_batched_sweep_converges_and_matches_sequential(gpu, by_mesh):
     from up3d_amd.mesh.channel import bump_channel
     from up3d_amd.solve.batch import solve_sweep_batched
     from up3d_amd.solve.sweep import solve_sweep
@@ -84,7 +85,8 @@ def test_batched_sweep_converges_and_matches_sequential(gpu):
              (1, 0.64, 1.5), (1, 0.76, 0.3), (1, 0.72, 2.2)]
     kw = {"gmres_restart": 40, "amg_max_coarse": 200}
     print("[batch] sweep: batched", flush=True)
-    rb = solve_sweep_batched(meshes, conds, chains_per_mesh=2, newton_kw=kw)
+    rb = solve_sweep_batched(meshes, conds, chains_per_mesh=2, newton_kw=kw,
+                             mesh_by_mesh=by_mesh)
     print(f"[batch] sweep executor {rb['executor']} wall {rb['wall_s']:.2f}s", flush=True)
     assert rb["n_converged"] == len(conds)
     assert rb["executor"]["batched_groups"] > 0
@@ -116,6 +118,14 @@ def test_coarse_route_sweep_sequential_equals_lockstep(gpu):
                      coarse_min_mach=0.72)
     rb = solve_sweep_batched(meshes, conds, chains_per_mesh=1, newton_kw=kw, coarse=comp,
                              coarse_min_tets=0, coarse_min_mach=0.72)
+    # coarse-route conditions do not depend on their chain: one chain per
+    # condition, mesh by mesh, gives them the same bits
+    rw = solve_sweep_batched(meshes, conds, chains_per_mesh=len(conds), newton_kw=kw,
+                             coarse=comp, coarse_min_tets=0, coarse_min_mach=0.72,
+                             mesh_by_mesh=True)
+    assert rw["executor"]["max_steps_per_round"] > 1
+    for a, b in zip(rs["instances"][:2], rw["instances"][:2]):
+        assert a["route"] == b["route"] == "coarse" and a["cf"] == b["cf"]
     assert rs["n_converged"] == rb["n_converged"] == len(conds)
     routes = [i["route"] for i in rs["instances"]]
     print(f"[batch] routes {routes}", flush=True)

@@ -211,7 +211,8 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
                         upwind_c: float = 2.0, dm: float = 0.05,
                         newton_kw: Optional[dict] = None, batched: bool = True,
                         log=None, coarse=None, coarse_min_tets: int = 500_000,
-                        coarse_min_mach: float = 0.0) -> Dict[str, object]:
+                        coarse_min_mach: float = 0.0,
+                        mesh_by_mesh: bool = False) -> Dict[str, object]:
     """Every (mesh index, M_inf, alpha_deg) condition, solved in lockstep.
 
     The conditions of each mesh are ordered by (M_inf, alpha) and dealt
@@ -223,6 +224,9 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
     converge it is retried once at the end from the nearest converged
     condition of its mesh with a finer Mach step (as solve_sweep).  coarse:
     a companion_mesh() pair enabling solve_sweep's coarse route.
+    mesh_by_mesh=True advances the chains of one mesh at a time (one
+    executor run per mesh, meshes in order), so a round never mixes a small
+    mesh's steps with a large mesh's.
     Returns solve_sweep's keys plus "executor" statistics."""
     from ..mesh.channel import boundary_nodes
     from ..post.surface import wall_force_coefficients
@@ -263,15 +267,17 @@ def solve_sweep_batched(meshes: Sequence, conditions: Sequence[Tuple[int, float,
            else [int(c) for c in chains_per_mesh])
     if len(nch) != len(meshes) or min(nch) < 1:
         raise ValueError("chains_per_mesh: a positive count, or one per mesh")
-    coros = {}
-    for k in range(len(meshes)):
-        ids = [i for i, c in enumerate(conditions) if c[0] == k]
-        ids.sort(key=lambda i: (conditions[i][1], conditions[i][2]))
-        t_mesh0[k] = time.perf_counter()
-        for c in range(min(nch[k], len(ids))):
-            coros[(k, c)] = chain(k, ids[c::nch[k]])
     ex = BatchExecutor(batched=batched)
-    ex.run(coros)
+    groups = [[k] for k in range(len(meshes))] if mesh_by_mesh else [list(range(len(meshes)))]
+    for group in groups:
+        coros = {}
+        for k in group:
+            ids = [i for i, c in enumerate(conditions) if c[0] == k]
+            ids.sort(key=lambda i: (conditions[i][1], conditions[i][2]))
+            t_mesh0[k] = time.perf_counter()
+            for c in range(min(nch[k], len(ids))):
+                coros[(k, c)] = chain(k, ids[c::nch[k]])
+        ex.run(coros)
 
     # retries: from the nearest converged condition of the mesh, finer step
     retry = {}
