@@ -87,11 +87,15 @@ def gmres_buffers(n: int, restart: int, f32: bool, device, V=None) -> dict:
     if V is None:
         V = torch.empty((restart + 1, n), dtype=torch.float32 if f32 else torch.float64,
                         device=device)
+    hbuf = [torch.empty(nout, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+    evs = [torch.cuda.Event(), torch.cuda.Event()]
+    # hbuf / evs hold, per parity, where the last queued step's coefficients
+    # land and the event that marks them: the solve's own buffers, or a row
+    # of a batched round's shared buffer (batched_arnoldi); *_own keep the
+    # solve's own, which a step queued by itself returns to
     return {"V": V, "v64": empty(n) if f32 else None, "tmp": empty(n), "r": empty(n),
             "w": [empty(n), empty(n)], "outs": [empty(nout), empty(nout)],
-            "hbuf": [torch.empty(nout, dtype=torch.float64, pin_memory=True)
-                     for _ in range(2)],
-            "evs": [torch.cuda.Event(), torch.cuda.Event()],
+            "hbuf": list(hbuf), "evs": list(evs), "hbuf_own": hbuf, "evs_own": evs,
             "red": _Red(restart + 1)}
 
 
@@ -172,7 +176,13 @@ def batched_arnoldi(steps) -> int:
     st = op0.store
     n = st.n_nodes
     f32 = s0.f32
-    jobs = np.zeros((len(steps), _JOB_FIELDS), dtype=np.int64)
+    nb = len(steps)
+    # the round's coefficients: one device block (row i = step i's out), one
+    # copy to one pinned block, one event; each solve reads its row
+    nout = s0.gw["outs"][0].numel()
+    rdev = torch.empty((nb, nout), dtype=torch.float64, device=st.device)
+    rhost = torch.empty((nb, nout), dtype=torch.float64, pin_memory=True)
+    jobs = np.zeros((nb, _JOB_FIELDS), dtype=np.int64)
     max_active = 0
     max_nv = 0
     for i, stp in enumerate(steps):
@@ -180,7 +190,7 @@ def batched_arnoldi(steps) -> int:
         gw = stp.gw
         col = stp.col
         V = gw["V"]
-        o = gw["outs"][col % 2]
+        o = rdev[i]
         w = gw["w"][col % 2]
         na = op.n_active
         max_active = max(max_active, na)
@@ -212,12 +222,14 @@ def batched_arnoldi(steps) -> int:
     n_grouped = apply_preconditioners(pre)
     _native.call("fp3d_batch_orthonormalise", len(steps), ptr(dev), max_nv, n,
                  1 if f32 else 0, 0.5, stream_handle())
-    for stp in steps:
+    rhost.copy_(rdev, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    for i, stp in enumerate(steps):
         gw = stp.gw
-        col, k = stp.col, stp.col + 1
-        gw["hbuf"][col % 2][: 3 + 2 * k].copy_(gw["outs"][col % 2][: 3 + 2 * k],
-                                               non_blocking=True)
-        gw["evs"][col % 2].record()
+        p = stp.col % 2
+        gw["hbuf"][p] = rhost[i]
+        gw["evs"][p] = ev
     return n_grouped
 
 
@@ -247,8 +259,8 @@ def _arnoldi_args(matvec, psolve, gw, n, f32):
     vals = op.data32 if op.is_f32 else op.data
     words = [n, 1 if f32 else 0, ptr(V), V.stride(0), ptr(gw["v64"]) if f32 else 0,
              ptr(gw["tmp"]), ptr(gw["w"][0]), ptr(gw["w"][1]), ptr(gw["outs"][0]),
-             ptr(gw["outs"][1]), ptr(gw["red"].work), gw["hbuf"][0].data_ptr(),
-             gw["hbuf"][1].data_ptr(), st.nnz, ptr(st.indptr),
+             ptr(gw["outs"][1]), ptr(gw["red"].work), gw["hbuf_own"][0].data_ptr(),
+             gw["hbuf_own"][1].data_ptr(), st.nnz, ptr(st.indptr),
              ptr(i16 if i16 is not None else st.indices), ptr(vals), op.fmt(), ptr(st.el),
              ptr(op.upstream) if use_t3 else 0, ptr(op.t3) if use_t3 else 0, na, st.n_tets,
              ptr(op.free_mask), gexec[0], gexec[1], pre]
@@ -349,10 +361,13 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
         if fast[1]:
             fast[0] = _arnoldi_args(matvec, psolve, gw, n, f32)
             fast[1] = False
+        p = col % 2
+        hbuf[p] = gw["hbuf_own"][p]
+        evs[p] = gw["evs_own"][p]
         if fast[0] is not None:
             a, stream = fast[0]
             _native.check(arnoldi(a.ctypes.data, col, stream.cuda_stream), "arnoldi_step")
-            evs[col % 2].record(stream)
+            evs[p].record(stream)
             return
         k = col + 1
         o = outs[col % 2]
