@@ -516,3 +516,58 @@ def test_amg_graph_replay_matches_eager(gpu):
             _ = (b * 2.0).sum().item()         # eager work between replays
         # nine buffer pairs were used: four direct graphs, then the copy graph
         assert len(ml._graphs) == ml._MAX_DIRECT_GRAPHS and ml._graph is not None
+
+
+def test_gather_long_rows_bitwise(gpu):
+    """A node of high valence (a star of 236 tets around the centre of a
+    120-point sphere: a 121-entry CSR row, 236 incidence entries, just under
+    the 256-colour limit the greedy colouring keeps): the node gathers still
+    equal the oracle's serial assembly bit for bit, with staged chunks of 256
+    and 1024 entries."""
+    import ctypes
+    import torch
+    from scipy.spatial import ConvexHull
+
+    from up3d_amd import _native
+    from up3d_amd.device import to_dev, to_host
+    from up3d_amd.kernels.jacobian import PicardOperator
+
+    k = np.arange(120) + 0.5
+    th = np.arccos(1 - 2 * k / 120)
+    ph = np.pi * (1 + 5 ** 0.5) * k
+    sph = np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)], 1)
+    nodes = np.vstack([np.zeros((1, 3)), sph])
+    tri = ConvexHull(sph).simplices + 1
+    el = np.hstack([np.zeros((len(tri), 1), dtype=np.int64), tri])
+    # positive orientation
+    d = nodes[el[:, 1:]] - nodes[el[:, :1]]
+    neg = np.einsum("ij,ij->i", d[:, 0], np.cross(d[:, 1], d[:, 2])) < 0
+    el[neg, 1], el[neg, 2] = el[neg, 2].copy(), el[neg, 1].copy()
+    el = el.astype(np.int32)
+    op = PicardOperator(nodes, el)
+    st = op.store
+    row0 = int(st.indptr_host[1] - st.indptr_host[0])
+    print(f"[long rows] {len(el)} tets, centre row {row0} entries", flush=True)
+    assert row0 == 121 and st.gather_ok
+    oo = O.OraclePicardOperator(nodes, el)
+    oo.B = np.ascontiguousarray(st.B_host.reshape(-1, 4, 3))
+    oo.V = np.ascontiguousarray(st.V_host)
+    n = len(el)
+    rng = np.random.default_rng(5)
+    rt = 0.6 + rng.random(n)
+    g = rng.standard_normal((n, 3))
+    se = rng.standard_normal(n)
+    rtd, gd, sed = to_dev(rt), to_dev(g), to_dev(se)
+    out = torch.full((op.nnz,), np.nan, dtype=torch.float64, device=rtd.device)
+    L = _native.lib()
+    prev = int(L.fp3d_get_gather_chunk())
+    try:
+        for chunk in (256, 1024):
+            assert L.fp3d_set_gather_chunk(ctypes.c_int(chunk)) == 0
+            out.fill_(np.nan)
+            assert np.array_equal(to_host(op.matrix_dev(rtd, out)), oo.assemble_matrix(rt).data)
+            out.fill_(np.nan)
+            assert np.array_equal(to_host(op.jacobian_dev(rtd, gd, sed, None, 1.0, out)),
+                                  oo.jacobian_terms12(g, rt, se, 1.0))
+    finally:
+        L.fp3d_set_gather_chunk(ctypes.c_int(prev))
