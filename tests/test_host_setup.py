@@ -216,3 +216,40 @@ def test_apply_givens_matches_rotation_product():
             n0, n1 = ref[j], ref[j + 1]
             ref[j], ref[j + 1] = c * n0 + s * n1, -s * n0 + c * n1
         assert np.array_equal(out, ref)
+
+
+def test_masked_sum_csr_is_the_masked_dense_sum():
+    """The adjoint's merged operator: rows of J12 + T with the Dirichlet rows
+    emptied, duplicate (row, column) pairs of the two kept as two entries,
+    equals diag(fm) (J12 + T) densely, and its row order is ascending."""
+    from types import SimpleNamespace
+
+    import scipy.sparse as sp
+    import torch
+
+    from up3d_amd.solve.adjoint import _csr_from_coo, _masked_sum_csr
+
+    rng = np.random.default_rng(5)
+    n = 23
+    A = sp.random(n, n, density=0.3, random_state=1, format="csr") + sp.eye(n)
+    A = A.tocsr()
+    A.sort_indices()
+    st = SimpleNamespace(n_nodes=n, indptr=torch.from_numpy(A.indptr.astype(np.int32)),
+                         indices=torch.from_numpy(A.indices.astype(np.int32)))
+    J12 = torch.from_numpy(A.data.astype(np.float64))
+    m = 60                       # COO with repeated pairs, some shared with A
+    r = rng.integers(0, n, m)
+    c = rng.integers(0, n, m)
+    r[:5], c[:5] = r[5:10], c[5:10]
+    v = rng.standard_normal(m)
+    T = _csr_from_coo(torch.from_numpy(r.astype(np.int32)), torch.from_numpy(c.astype(np.int32)),
+                      torch.from_numpy(v), n)
+    fm = torch.from_numpy((rng.random(n) > 0.3).astype(np.uint8))
+    M = _masked_sum_csr(st, J12, T, fm)
+    Tt = sp.coo_matrix((v, (r, c)), shape=(n, n)).toarray()
+    want = np.diag(fm.numpy().astype(float)) @ (A.toarray() + Tt)
+    ip, ix, dv = M.indptr.numpy(), M.indices.numpy(), M.data.numpy()
+    got = sp.csr_matrix((dv, ix, ip), shape=(n, n)).toarray()
+    assert np.allclose(got, want, rtol=0, atol=1e-14)
+    assert M.nnz == int(np.diff(A.indptr)[fm.numpy() > 0].sum() + (fm.numpy()[r] > 0).sum())
+    assert all(np.all(np.diff(ix[ip[i]:ip[i + 1]]) >= 0) for i in range(n))

@@ -365,24 +365,30 @@ inline int rblocks(int64_t n) {
 // cond[nv]); otherwise the partials are written as exact zeros.
 constexpr int kMdotLds = 2048;
 constexpr int64_t kMdotSegMin = 1024;
+constexpr int kMdotMinBlocks = 1024;
 template <bool STAGED, typename TV>
 __device__ __forceinline__ void mdot_p1_body(int64_t bx, int nb, int64_t n, int nv,
                                              const TV *__restrict__ V, int64_t ld,
                                              const double *__restrict__ w,
                                              double *__restrict__ part,
                                              const double *__restrict__ cond,
-                                             double eta2, int64_t seg, int with_self) {
+                                             double eta2, int64_t seg, int with_self,
+                                             int group = 0, int n_groups = 1) {
   // STAGED reads the slice of w from LDS (a separate instantiation, so the
   // loads stay in the LDS address space rather than generic).  with_self:
-  // row nv is w itself (w . w, summed exactly as a one-row call on w)
+  // row nv is w itself (w . w, summed exactly as a one-row call on w).
+  // The rows k are dealt to (group, wave) round-robin: row k belongs to
+  // wave k % nwave of group (k / nwave) % n_groups, so splitting the rows
+  // over more blocks leaves every (k, bx) partial computed by the same code
   __shared__ double ws[STAGED ? kMdotLds : 1];
   if (cond) {
     double h2 = 0.0;
     for (int k = 0; k < nv; ++k) h2 += cond[k] * cond[k];
     double w2 = cond[nv];
     if (!(w2 - h2 < eta2 * w2)) {
-      for (int k = threadIdx.x; k < nv; k += blockDim.x)
-        part[(int64_t)k * nb + bx] = 0.0;
+      if (group == 0)
+        for (int k = threadIdx.x; k < nv; k += blockDim.x)
+          part[(int64_t)k * nb + bx] = 0.0;
       return;
     }
   }
@@ -396,7 +402,7 @@ __device__ __forceinline__ void mdot_p1_body(int64_t bx, int nb, int64_t n, int 
   const double *__restrict__ wg = w + r0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwave = blockDim.x >> 6;
-  for (int k = wave; k < nv; k += nwave) {
+  for (int k = group * nwave + wave; k < nv; k += nwave * n_groups) {
     const TV *__restrict__ v = V + (int64_t)k * ld + r0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int i = lane;
@@ -417,7 +423,7 @@ __device__ __forceinline__ void mdot_p1_body(int64_t bx, int nb, int64_t n, int 
     double t = wsum((a0 + a1) + (a2 + a3));
     if (lane == 0) part[(int64_t)k * nb + bx] = t;
   }
-  if (with_self && wave == nv % nwave) {
+  if (with_self && wave == nv % nwave && group == (nv / nwave) % n_groups) {
     // the row after the last basis vector: w itself, read from global as
     // the V operand of a one-row call would be
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_mdot_p1(int64_t n, int nv,
                                                    double eta2, int64_t seg,
                                                    int with_self) {
   mdot_p1_body<STAGED, TV>(blockIdx.x, gridDim.x, n, nv, V, ld, w, part, cond, eta2, seg,
-                           with_self);
+                           with_self, blockIdx.y, gridDim.y);
 }
 inline int64_t mdot_seg(int64_t n) {
   int64_t seg = (n + kReduceBlocks - 1) / kReduceBlocks;
@@ -1241,11 +1247,18 @@ static int multi_dot_any(int64_t n, int nv, const TV *V, int64_t ld, const doubl
   hipStream_t s = FP3D_STREAM(stream);
   int64_t seg = mdot_seg(n);
   int nb = mdot_blocks(n, seg);
+  // a short basis (few row segments) also splits its rows k over block
+  // groups, so about kMdotMinBlocks blocks keep loads in flight
+  const int rows = nv + with_self, waves = kBlock / 64;
+  int ng = (kMdotMinBlocks + nb - 1) / nb;
+  const int ng_max = (rows + waves - 1) / waves;
+  if (ng > ng_max) ng = ng_max;
+  if (ng < 1) ng = 1;
   if (seg <= kMdotLds)
-    hipLaunchKernelGGL((k_mdot_p1<true, TV>), dim3(nb), dim3(kBlock), 0, s, n, nv,
+    hipLaunchKernelGGL((k_mdot_p1<true, TV>), dim3(nb, ng), dim3(kBlock), 0, s, n, nv,
                        V, ld, w, work, cond, eta2, seg, with_self);
   else
-    hipLaunchKernelGGL((k_mdot_p1<false, TV>), dim3(nb), dim3(kBlock), 0, s, n, nv,
+    hipLaunchKernelGGL((k_mdot_p1<false, TV>), dim3(nb, ng), dim3(kBlock), 0, s, n, nv,
                        V, ld, w, work, cond, eta2, seg, with_self);
   hipLaunchKernelGGL(k_mdot_p2, dim3(nv + with_self), dim3(kBlock), 0, s, nb, work,
                      out);

@@ -60,6 +60,25 @@ def _csr_from_coo(rows: torch.Tensor, cols: torch.Tensor, vals: torch.Tensor,
     return csr
 
 
+def _masked_sum_csr(st, J12: torch.Tensor, T: DeviceCSR, fm: torch.Tensor) -> DeviceCSR:
+    """One CSR of the free rows of J12 + T (J12 on the store's pattern, T a
+    CSR of the same size; rows with fm == 0 are empty, so its SpMV writes
+    zero there).  Entries of one (row, column) from both stay separate, J12's
+    first: the SpMV sums a row's entries in a fixed order."""
+    n = st.n_nodes
+    dev = J12.device
+
+    def rows_of(indptr):
+        return torch.repeat_interleave(torch.arange(n, device=dev),
+                                       (indptr[1:] - indptr[:-1]).long())
+
+    rows = torch.cat([rows_of(st.indptr), rows_of(T.indptr)])
+    cols = torch.cat([st.indices, T.indices])
+    vals = torch.cat([J12, T.data])
+    keep = fm.bool()[rows]
+    return _csr_from_coo(rows[keep], cols[keep], vals[keep], n)
+
+
 def _wall_functional(nodes, phi, m_inf, alpha_deg, wf, flip, wx, wl, s_ref=1.0,
                      gamma=1.4):
     """torch restatement of the wall force (triangle tangential gradient,
@@ -110,6 +129,11 @@ class ChannelForceProblem:
                       self.base_nodes[self.wall_faces[:, 2]] - self.base_nodes[self.wall_faces[:, 0]])
         n_out = wall_outward_normals(self.base_nodes, self.elements, self.wall_faces)
         self.flip = np.sign(np.einsum("ij,ij->i", nv, n_out))
+        # the functional touches only the wall nodes: its autograd runs on
+        # them (faces renumbered locally) and scatters back
+        self.wall_nodes, wl = np.unique(self.wall_faces, return_inverse=True)
+        self.wall_faces_local = wl.reshape(self.wall_faces.shape)
+        self._ops = None
 
     def nodes_for(self, y_floor) -> np.ndarray:
         nodes = self.base_nodes.copy()
@@ -234,13 +258,57 @@ class ChannelForceProblem:
                                        t(self.flip), weights[0], weights[1])
         return float(val), cf.numpy(), float(cl)
 
+    def _functional_grads(self, sol, weights):
+        """(value, cf, cl, dF/dnodes (n, 3), dF/dphi (n,), dF/dM_inf,
+        dF/dalpha) by host autograd over the wall nodes."""
+        wn = self.wall_nodes
+        n = len(sol["phi"])
+        with torch.enable_grad():
+            t = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)  # noqa: E731
+            nodes_t = t(np.asarray(sol["nodes"])[wn])
+            phi_t = t(np.asarray(sol["phi"])[wn])
+            m_t, a_t = t(sol["m_inf"]), t(sol["alpha_deg"])
+            val, cf, cl = _wall_functional(nodes_t, phi_t, m_t, a_t,
+                                           torch.from_numpy(self.wall_faces_local),
+                                           torch.tensor(self.flip), weights[0],
+                                           weights[1])
+            gx, gphi, gm, ga = torch.autograd.grad(val, (nodes_t, phi_t, m_t, a_t))
+        dFx = np.zeros((n, 3))
+        dFx[wn] = gx.numpy()
+        dFphi = np.zeros(n)
+        dFphi[wn] = gphi.numpy()
+        return (val.detach(), cf.detach(), cl.detach(), dFx, dFphi, float(gm),
+                float(ga))
+
     # ----------------------------------------------------------- adjoint
-    def _linearisation(self, sol, upwind_c=1.5, m_crit=0.95, rho_floor=0.05):
-        """Device state, frozen (upstream, branch), sensitivities, Terms 1+2
-        (unmasked) and Term 3 as row- and column-sorted CSRs."""
-        nodes = sol["nodes"]
+    def _operators(self, nodes):
+        """(PicardOperator, UpwindOperator, free mask) of these node
+        coordinates, kept for the last coordinates asked for."""
+        if self._ops is not None and np.array_equal(self._ops[0], nodes):
+            return self._ops[1:]
         op = PicardOperator(nodes, self.elements)
         upw = UpwindOperator(nodes, self.elements)
+        fm = dirichlet_setup(op.store.n_nodes, self.dn, np.zeros(len(self.dn)),
+                             op.store.device)[0]
+        self._ops = (np.array(nodes, copy=True), op, upw, fm)
+        return op, upw, fm
+
+    @staticmethod
+    def _term3(L, transpose: bool) -> DeviceCSR:
+        """Term 3 of the linearisation as a row-sorted (transpose: column-
+        sorted) CSR, built on first use."""
+        key = "T3t" if transpose else "T3"
+        if key not in L:
+            rows, cols, vals = L["t3_coo"]
+            n = L["st"].n_nodes
+            L[key] = _csr_from_coo(cols, rows, vals, n) if transpose else \
+                _csr_from_coo(rows, cols, vals, n)
+        return L[key]
+
+    def _linearisation(self, sol, upwind_c=1.5, m_crit=0.95, rho_floor=0.05):
+        """Device state, frozen (upstream, branch), sensitivities, Terms 1+2
+        (unmasked) and Term 3's COO (CSRs of it through _term3)."""
+        op, upw, fm = self._operators(sol["nodes"])
         st = op.store
         n, nt = st.n_nodes, st.n_tets
         m = sol["m_inf"]
@@ -276,15 +344,11 @@ class ChannelForceProblem:
                          ptr(st.B), ptr(st.V), ptr(grad), ptr(up), ptr(su),
                          float(inv_u2), ptr(rows), ptr(cols), ptr(vals),
                          stream_handle())
-        T3 = _csr_from_coo(rows, cols, vals, n)
-        T3t = _csr_from_coo(cols, rows, vals, n)
-        fm, dn, dv, phi_d = dirichlet_setup(n, self.dn, np.zeros(len(self.dn)),
-                                            st.device)
         A = empty(st.nnz)
         op.matrix_dev(rt, A)
         return dict(op=op, upw=upw, st=st, grad=grad, q2l=q2l, rho=rho, rt=rt,
-                    up=up, br=br, se=se, su=su, J12=J12, T3=T3, T3t=T3t, fm=fm,
-                    A=A, n_limited=int((lim == 0).sum().item()))
+                    up=up, br=br, se=se, su=su, J12=J12, t3_coo=(rows, cols, vals),
+                    fm=fm, A=A, n_limited=int((lim == 0).sum().item()))
 
     def _jmv(self, L, x, y, transpose):
         """y = J x (or J^T x), unmasked."""
@@ -293,7 +357,7 @@ class ChannelForceProblem:
         _native.call("fp3d_csr_spmv", st.n_nodes, st.nnz, ptr(st.indptr),
                      ptr(st.indices), ptr(L["J12"]), ptr(x), ptr(y),
                      stream_handle())
-        T = L["T3t"] if transpose else L["T3"]
+        T = self._term3(L, transpose)
         if T.nnz:
             T.matvec(x, tmp)
             axpby(1.0, tmp, 1.0, y)
@@ -309,30 +373,14 @@ class ChannelForceProblem:
         n, nt = st.n_nodes, st.n_tets
         fm = L["fm"]
         # functional derivatives (host autograd; also valid inside backward)
-        with torch.enable_grad():
-            nodes_t = torch.tensor(sol["nodes"], dtype=torch.float64, requires_grad=True)
-            phi_t = torch.tensor(sol["phi"], dtype=torch.float64, requires_grad=True)
-            m_t = torch.tensor(sol["m_inf"], dtype=torch.float64, requires_grad=True)
-            a_t = torch.tensor(sol["alpha_deg"], dtype=torch.float64, requires_grad=True)
-            val, cf, cl = _wall_functional(nodes_t, phi_t, m_t, a_t,
-                                           torch.from_numpy(self.wall_faces),
-                                           torch.tensor(self.flip), weights[0],
-                                           weights[1])
-            dFx, dFphi, dFm, dFa = torch.autograd.grad(val, (nodes_t, phi_t, m_t, a_t))
-        val, cf, cl = val.detach(), cf.detach(), cl.detach()
-        dFx, dFphi = dFx.numpy(), dFphi.numpy()
+        val, cf, cl, dFx, dFphi, dFm, dFa = self._functional_grads(sol, weights)
         # adjoint solve J_ff^T lam = (dF/dphi)_free
         b = to_dev(dFphi)
         b.mul_(fm.to(torch.float64))
         ml = self._preconditioner(sol, st, L["A"], fm)
-        fmd = fm.to(torch.float64)
-
-        def mvT(x, y):
-            self._jmv(L, x, y, transpose=True)
-            y.mul_(fmd)
-            return y
-
-        lam, info, n_it = gmres_dev(mvT, b, ml.apply, rtol=rtol, restart=200,
+        # J_ff^T (J12 is symmetric) with the Dirichlet rows emptied
+        JfT = _masked_sum_csr(st, L["J12"], self._term3(L, True), fm)
+        lam, info, n_it = gmres_dev(JfT.matvec, b, ml.apply, rtol=rtol, restart=200,
                                     maxiter=12)
         red_norm = Reducer()
         # dR/dM_inf = scatter of d rho_tilde/dM
@@ -342,14 +390,16 @@ class ChannelForceProblem:
                      float(upwind_c), 1.4, ptr(drt), stream_handle())
         RM = L["op"].residual_dev(drt, L["grad"], out=empty(n))
         dF_dm = float(dFm) - dot(lam, RM)
+        # J^T lam, unmasked: the Dirichlet-value chains of alpha and of the
+        # Dirichlet nodes' coordinates (lam^T J v = (J^T lam) . v)
+        JTl_h = to_host(self._jmv(L, lam, empty(n), transpose=True))
         # alpha: Dirichlet values phi_D = x cos a + y sin a (a in degrees)
         nodes = sol["nodes"]
         a = np.deg2rad(sol["alpha_deg"])
         vd = np.zeros(n)
         vd[self.dn] = (-nodes[self.dn, 0] * np.sin(a)
                        + nodes[self.dn, 1] * np.cos(a)) * (np.pi / 180.0)
-        Jv = self._jmv(L, to_dev(vd), empty(n), transpose=False)
-        dF_da = float(dFa) + float(dFphi @ vd) - dot(lam, Jv)
+        dF_da = float(dFa) + float(dFphi @ vd) - float(JTl_h @ vd)
         # node coordinates: lambda^T dR/dx (device) + Dirichlet-value chain
         work = empty(int(_native.lib().fp3d_adjoint_dx_work_doubles(nt)))
         up_l = L["up"].long()
@@ -366,9 +416,7 @@ class ChannelForceProblem:
                      ptr(L["grad"]), ptr(L["rt"]), ptr(L["se"]), ptr(L["su"]),
                      ptr(dptr), ptr(dsrc), ptr(st.inc_ptr), ptr(st.inc_ent),
                      ptr(lam), ptr(work), ptr(dRx), stream_handle())
-        JTl = self._jmv(L, lam, empty(n), transpose=True)
         dRx_h = to_host(dRx)
-        JTl_h = to_host(JTl)
         grad_x = dFx - dRx_h
         # Dirichlet nodes: phi_D = x cos a + y sin a
         coef = dFphi[self.dn] - JTl_h[self.dn]
@@ -405,23 +453,12 @@ class ChannelForceProblem:
         axpby(1.0, Jv, 1.0, rhs)
         rhs.mul_(-fmd)
         ml = self._preconditioner(sol, st, L["A"], L["fm"])
-
-        def mv(x, y):
-            self._jmv(L, x, y, transpose=False)
-            y.mul_(fmd)
-            return y
-
-        dphi, info, _ = gmres_dev(mv, rhs, ml.apply, rtol=rtol, restart=200, maxiter=12)
+        Jf = _masked_sum_csr(st, L["J12"], self._term3(L, False), L["fm"])
+        dphi, info, _ = gmres_dev(Jf.matvec, rhs, ml.apply, rtol=rtol, restart=200,
+                                  maxiter=12)
         dphi_h = to_host(dphi) + vd
-        t = lambda v, g=False: torch.tensor(v, dtype=torch.float64, requires_grad=g)  # noqa: E731
-        with torch.enable_grad():
-            m_t, a_t = t(sol["m_inf"], True), t(sol["alpha_deg"], True)
-            phi_t = t(sol["phi"], True)
-            val, _, _ = _wall_functional(t(nodes), phi_t, m_t, a_t,
-                                         torch.from_numpy(self.wall_faces),
-                                         t(self.flip), weights[0], weights[1])
-            gphi, gm, ga = torch.autograd.grad(val, (phi_t, m_t, a_t))
-        return float(gm) * dm + float(ga) * dalpha_deg + float(gphi.numpy() @ dphi_h)
+        _, _, _, _, gphi, gm, ga = self._functional_grads(sol, weights)
+        return gm * dm + ga * dalpha_deg + float(gphi @ dphi_h)
 
 
 class ChannelForce(torch.autograd.Function):

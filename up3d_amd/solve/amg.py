@@ -266,6 +266,7 @@ class AMGHierarchy:
         self.use_graph = use_graph
         self._graph = None
         self._graphs = {}
+        self._exec_graphs = {}
         self._gb = empty(self.levels[0].n)
         self._gx = empty(self.levels[0].n)
         if self.v32:
@@ -358,8 +359,12 @@ class AMGHierarchy:
 
     # graphs captured directly on callers' (b, x) buffers; a Krylov solve
     # passes one source and two alternating destinations, and the caching
-    # allocator hands the same buffers to the next solve of the same size
+    # allocator hands the same buffers to the next solve of the same size.
+    # apply() fills its own slots; graph_execs() (Krylov workspaces that
+    # launch the graphs themselves) has slots of its own, so one-off apply
+    # calls on other buffers cannot use them up
     _MAX_DIRECT_GRAPHS = 4
+    _MAX_EXEC_GRAPHS = 8
 
     def apply(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
         """x = M b (one V-cycle from a zero initial guess).  After the first
@@ -376,7 +381,7 @@ class AMGHierarchy:
                   and b.data_ptr() != x.data_ptr())
         if direct:
             key = (b.data_ptr(), x.data_ptr())
-            g = self._graphs.get(key)
+            g = self._graphs.get(key) or self._exec_graphs.get(key)
             if g is None and len(self._graphs) < self._MAX_DIRECT_GRAPHS:
                 g = self._graphs[key] = self._capture(b, x)
             if g is not None:
@@ -404,11 +409,11 @@ class AMGHierarchy:
                     and b.data_ptr() != x.data_ptr()):
                 return None
             key = (b.data_ptr(), x.data_ptr())
-            g = self._graphs.get(key)
+            g = self._exec_graphs.get(key) or self._graphs.get(key)
             if g is None:
-                if len(self._graphs) >= self._MAX_DIRECT_GRAPHS:
+                if len(self._exec_graphs) >= self._MAX_EXEC_GRAPHS:
                     return None
-                g = self._graphs[key] = self._capture(b, x)
+                g = self._exec_graphs[key] = self._capture(b, x)
             h = g.raw_cuda_graph_exec()
             if not isinstance(h, int) or h == 0:
                 return None

@@ -234,12 +234,13 @@ def batched_arnoldi(steps) -> int:
 
 
 def _arnoldi_args(matvec, psolve, gw, n, f32):
-    """(fp3d_arnoldi_step argument words, stream) for a Newton-Jacobian
-    matvec and an AMG (or no) preconditioner, else None (the step is then
-    issued launch by launch)."""
+    """(fp3d_arnoldi_step argument words, stream) for a Newton-Jacobian or a
+    plain float64 CSR matvec and an AMG (or no) preconditioner, else None
+    (the step is then issued launch by launch)."""
     op = getattr(matvec, "__self__", None)
-    if not isinstance(op, NewtonOperator) or \
-            getattr(matvec, "__func__", None) is not NewtonOperator.matvec:
+    func = getattr(matvec, "__func__", None)
+    plain = isinstance(op, DeviceCSR) and func is DeviceCSR.matvec and not op.is_f32
+    if not plain and (not isinstance(op, NewtonOperator) or func is not NewtonOperator.matvec):
         return None
     gexec, pre = [0, 0], 0
     if psolve is not None:
@@ -251,19 +252,24 @@ def _arnoldi_args(matvec, psolve, gw, n, f32):
         if hs is None:
             return None
         gexec, pre = hs, 1
-    st = op.store
     V = gw["V"]
-    na = op.n_active
-    use_t3 = na > 0
-    i16 = getattr(st, "indices16", None)
-    vals = op.data32 if op.is_f32 else op.data
-    words = [n, 1 if f32 else 0, ptr(V), V.stride(0), ptr(gw["v64"]) if f32 else 0,
-             ptr(gw["tmp"]), ptr(gw["w"][0]), ptr(gw["w"][1]), ptr(gw["outs"][0]),
-             ptr(gw["outs"][1]), ptr(gw["red"].work), gw["hbuf_own"][0].data_ptr(),
-             gw["hbuf_own"][1].data_ptr(), st.nnz, ptr(st.indptr),
-             ptr(i16 if i16 is not None else st.indices), ptr(vals), op.fmt(), ptr(st.el),
-             ptr(op.upstream) if use_t3 else 0, ptr(op.t3) if use_t3 else 0, na, st.n_tets,
-             ptr(op.free_mask), gexec[0], gexec[1], pre]
+    head = [n, 1 if f32 else 0, ptr(V), V.stride(0), ptr(gw["v64"]) if f32 else 0,
+            ptr(gw["tmp"]), ptr(gw["w"][0]), ptr(gw["w"][1]), ptr(gw["outs"][0]),
+            ptr(gw["outs"][1]), ptr(gw["red"].work), gw["hbuf_own"][0].data_ptr(),
+            gw["hbuf_own"][1].data_ptr()]
+    if plain:
+        # y = A x, no Term 3, no Dirichlet rows (the matrix carries them)
+        mv = [op.nnz, ptr(op.indptr), ptr(op.indices), ptr(op.data), 0, 0, 0, 0, 0, 0, 0]
+    else:
+        st = op.store
+        na = op.n_active
+        use_t3 = na > 0
+        i16 = getattr(st, "indices16", None)
+        vals = op.data32 if op.is_f32 else op.data
+        mv = [st.nnz, ptr(st.indptr), ptr(i16 if i16 is not None else st.indices), ptr(vals),
+              op.fmt(), ptr(st.el), ptr(op.upstream) if use_t3 else 0,
+              ptr(op.t3) if use_t3 else 0, na, st.n_tets, ptr(op.free_mask)]
+    words = head + mv + [gexec[0], gexec[1], pre]
     return np.array(words, dtype=np.int64), torch.cuda.current_stream()
 
 
