@@ -118,3 +118,48 @@ def test_spgemm_all_cancelling_rows():
     B = sp.csr_matrix(np.array([[3.0, 1.0], [3.0, 1.0]]))
     C = amg.spgemm(A, B, force_native=True)
     assert C.nnz == (A @ B).nnz == 0
+
+
+def _lap1d(n, dirichlet=True):
+    """Tridiagonal 1-D Laplacian: Dirichlet ends (SPD) or Neumann (singular,
+    constants in the kernel)."""
+    main = np.full(n, 2.0)
+    if not dirichlet:
+        main[0] = main[-1] = 1.0
+    return sp.diags([-np.ones(n - 1), main, -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+
+
+def test_coarse_inverse_spd_is_the_inverse():
+    """An SPD coarsest operator goes through its Cholesky factor: the
+    inverse (A X = I to rounding), equal to the pseudo-inverse."""
+    A = _lap1d(40)
+    X = amg.coarse_inverse(A)
+    assert np.allclose(A.toarray() @ X, np.eye(40), rtol=0, atol=1e-11)
+    assert np.allclose(X, np.linalg.pinv(A.toarray()), rtol=0, atol=1e-10)
+    assert np.array_equal(X, X.T)
+
+
+def test_coarse_inverse_singular_and_nonsymmetric_fall_back_to_pinv():
+    """Singular (pure Neumann) and non-symmetric operators keep the
+    pseudo-inverse: X A X = X, A X A = A, and X annihilates the kernel."""
+    A = _lap1d(30, dirichlet=False)
+    X = amg.coarse_inverse(A)
+    Ad = A.toarray()
+    assert np.array_equal(X, np.linalg.pinv(Ad))
+    assert np.allclose(Ad @ X @ Ad, Ad, atol=1e-10)
+    assert np.allclose(X @ np.ones(30), 0.0, atol=1e-10)
+    N = _lap1d(12).toarray()
+    N[0, 1] = -1.5                      # not symmetric
+    assert np.array_equal(amg.coarse_inverse(sp.csr_matrix(N)), np.linalg.pinv(N))
+    assert amg.coarse_inverse(sp.csr_matrix((0, 0))).shape == (0, 0)
+
+
+def test_sa_setup_takes_a_given_finest_rho():
+    """rho0 replaces only the finest level's power iteration."""
+    A = _lap1d(3000)
+    lv_a, _ = amg.sa_setup(A, max_coarse=10)
+    lv_b, _ = amg.sa_setup(A, max_coarse=10, rho0=lv_a[0].rho)
+    assert [l.rho for l in lv_a] == [l.rho for l in lv_b]
+    lv_c, _ = amg.sa_setup(A, max_coarse=10, rho0=1.5 * lv_a[0].rho)
+    assert lv_c[0].rho == 1.5 * lv_a[0].rho
+    assert not np.array_equal(lv_c[0].P.data, lv_a[0].P.data)

@@ -128,16 +128,71 @@ class HostLevel:
     rho: float
 
 
+def spectral_radius_dinv_a_dev(n: int, nnz: int, indptr: torch.Tensor,
+                               indices: torch.Tensor, data: torch.Tensor,
+                               dinv: torch.Tensor, iters: int = 12, seed: int = 0
+                               ) -> float:
+    """spectral_radius_dinv_a on a float64 device CSR (int32 indices): the
+    same seeded start vector and iteration, with device SpMVs and
+    deterministic device norms; one host read at the end."""
+    rng = np.random.default_rng(seed)
+    x = to_dev(rng.random(n) + 0.5)
+    y = empty(n)
+    work = empty(int(_native.lib().fp3d_reduce_work_doubles()))
+    norms = empty((iters, 2, 2))           # per iteration: (||y||, max), (||x||, max)
+    s = stream_handle()
+    for it in range(iters):
+        _native.call("fp3d_csr_spmv", n, nnz, ptr(indptr), ptr(indices), ptr(data),
+                     ptr(x), ptr(y), s)
+        y.mul_(dinv)
+        _native.call("fp3d_norms", n, ptr(y), ptr(work), ptr(norms[it, 0]), s)
+        _native.call("fp3d_norms", n, ptr(x), ptr(work), ptr(norms[it, 1]), s)
+        _native.call("fp3d_scale_inv_dev", n, ptr(norms[it, 0]), ptr(y), ptr(x), s)
+    h = norms.cpu().numpy()
+    if np.any(h[:, 0, 0] == 0.0):
+        return 1.0
+    return float(h[-1, 0, 0] / h[-1, 1, 0])
+
+
+def coarse_inverse(Ac: sp.csr_matrix) -> np.ndarray:
+    """Dense inverse of the coarsest operator: through a Cholesky factor when
+    the matrix is symmetric to rounding and positive definite with a
+    well-scaled factor (diagonal ratio above 1e-6), else the pseudo-inverse
+    (singular or indefinite operators, e.g. pure Neumann problems)."""
+    M = Ac.toarray()
+    n = M.shape[0]
+    if n:
+        scale = np.abs(M).max()
+        if scale > 0.0 and np.abs(M - M.T).max() <= 1e-12 * scale:
+            try:
+                L = np.linalg.cholesky(0.5 * (M + M.T))
+            except np.linalg.LinAlgError:
+                L = None
+            if L is not None:
+                dl = np.diag(L)
+                if dl.min() > 1e-6 * dl.max():
+                    import scipy.linalg as sla
+
+                    Li = sla.solve_triangular(L, np.eye(n), lower=True)
+                    return Li.T @ Li
+    return np.linalg.pinv(M)
+
+
 def sa_setup(A: sp.csr_matrix, max_levels: int = 10, max_coarse: int = 10,
-             omega: float = 4.0 / 3.0, seed: int = 0):
-    """Host SA hierarchy: list of HostLevel plus the dense coarse pinv."""
+             omega: float = 4.0 / 3.0, seed: int = 0, rho0: Optional[float] = None):
+    """Host SA hierarchy: list of HostLevel plus the dense coarse inverse.
+    rho0: rho(D^-1 A) of the finest level when already known (estimated on
+    the device from the same seeded start vector)."""
     A = A.tocsr()
     levels: List[HostLevel] = []
     while True:
         n = A.shape[0]
         d = A.diagonal()
         dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
-        rho = spectral_radius_dinv_a(A, dinv, seed=seed)
+        if rho0 is not None and not levels:
+            rho = float(rho0)
+        else:
+            rho = spectral_radius_dinv_a(A, dinv, seed=seed)
         if n <= max_coarse or len(levels) >= max_levels - 1:
             levels.append(HostLevel(A, None, None, dinv, rho))
             break
@@ -157,8 +212,7 @@ def sa_setup(A: sp.csr_matrix, max_levels: int = 10, max_coarse: int = 10,
         Ac = spgemm(R, spgemm(A, P))
         levels.append(HostLevel(A, P, R, dinv, rho))
         A = Ac
-    coarse = np.linalg.pinv(levels[-1].A.toarray())
-    return levels, coarse
+    return levels, coarse_inverse(levels[-1].A)
 
 
 class DeviceCSR:
@@ -184,6 +238,25 @@ class DeviceCSR:
             from ..kernels.store import csr_offsets16
 
             self.indices16 = csr_offsets16(M.indptr, M.indices)
+
+    @classmethod
+    def on_device(cls, n: int, indptr: torch.Tensor, indices: torch.Tensor,
+                  data: torch.Tensor, f32: bool = False, keep_f64: bool = True,
+                  indices16: Optional[torch.Tensor] = None) -> "DeviceCSR":
+        """A square CSR already in HBM (the pattern tensors are shared, the
+        float64 values copied), equal to constructing it from its host copy
+        with compact_index when indices16 is the pattern's int16 offsets."""
+        M = cls.__new__(cls)
+        M.shape = (n, n)
+        M.n = n
+        M.nnz = int(indices.numel())
+        M.indptr = indptr
+        M.indices = indices
+        M.data = data.clone() if (keep_f64 or not f32) else None
+        M.data32 = data.to(torch.float32) if f32 else None
+        M.is_f32 = 1 if f32 else 0
+        M.indices16 = indices16
+        return M
 
     @property
     def values(self):
@@ -213,13 +286,16 @@ class DeviceCSR:
 
 class _DevLevel:
     def __init__(self, h: HostLevel, f32: bool, degree: int, eig_ratio: float,
-                 v32: bool = False, compact_index: bool = True):
+                 v32: bool = False, compact_index: bool = True,
+                 A_dev: Optional[DeviceCSR] = None):
         self.n = h.A.shape[0]
-        self.A = DeviceCSR(h.A, f32=f32, keep_f64=not f32, compact_index=compact_index)
+        self.A = A_dev if A_dev is not None else \
+            DeviceCSR(h.A, f32=f32, keep_f64=not f32, compact_index=compact_index)
         self.P = DeviceCSR(h.P, f32=f32, keep_f64=not f32) if h.P is not None else None
         self.R = DeviceCSR(h.R, f32=f32, keep_f64=not f32) if h.R is not None else None
         vt = torch.float32 if v32 else torch.float64
         self.dinv = to_dev(h.dinv, torch.float64).to(vt)
+        self.rho = h.rho
         self.lmax = 1.1 * h.rho
         self.lmin = self.lmax / eig_ratio
         self.degree = degree
@@ -244,20 +320,40 @@ class AMGHierarchy:
     def __init__(self, A: sp.csr_matrix, precision: str = "f64",
                  degree: int = 2, eig_ratio: float = 20.0, max_levels: int = 10,
                  max_coarse: int = 10, seed: int = 0, use_graph: bool = True,
-                 vectors: Optional[str] = None, compact_index: bool = True):
+                 vectors: Optional[str] = None, compact_index: bool = True,
+                 device_pattern=None, device_values: Optional[torch.Tensor] = None,
+                 rho0: Optional[float] = None):
+        """device_pattern / device_values: A's CSR pattern (an object with
+        n_nodes, nnz, indptr, indices and optionally indices16 device
+        tensors) and float64 values already in HBM; the finest level is then
+        built from them (no upload) and its rho(D^-1 A) estimated there.
+        rho0: the finest level's rho(D^-1 A), when already known."""
         if precision not in ("f64", "f32"):
             raise ValueError(f"unknown AMG precision {precision!r}")
         vectors = precision if vectors is None else vectors
         if vectors not in ("f64", "f32"):
             raise ValueError(f"unknown AMG vector precision {vectors!r}")
-        host, coarse = sa_setup(A, max_levels=max_levels, max_coarse=max_coarse,
-                                seed=seed)
         f32 = precision == "f32"
+        A0 = None
+        if device_values is not None:
+            st = device_pattern
+            if rho0 is None:
+                d = A.diagonal()
+                dinv = to_dev(np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0))
+                rho0 = spectral_radius_dinv_a_dev(st.n_nodes, st.nnz, st.indptr,
+                                                  st.indices, device_values, dinv,
+                                                  seed=seed)
+            i16 = getattr(st, "indices16", None) if compact_index else None
+            A0 = DeviceCSR.on_device(st.n_nodes, st.indptr, st.indices, device_values,
+                                     f32=f32, keep_f64=not f32, indices16=i16)
+        host, coarse = sa_setup(A, max_levels=max_levels, max_coarse=max_coarse,
+                                seed=seed, rho0=rho0)
         self.precision = precision
         self.v32 = vectors == "f32"
         self.vfmt = 2 if self.v32 else 0
-        self.levels = [_DevLevel(h, f32, degree, eig_ratio, self.v32, compact_index)
-                       for h in host]
+        self.levels = [_DevLevel(h, f32, degree, eig_ratio, self.v32, compact_index,
+                                 A_dev=A0 if i == 0 else None)
+                       for i, h in enumerate(host)]
         self.coarse = to_dev(np.ascontiguousarray(coarse), torch.float64)
         self.shapes = [lv.n for lv in self.levels]
         self.nnzs = [lv.A.nnz for lv in self.levels]

@@ -59,6 +59,8 @@ def masked(free_mask, x, out):
 
 
 def amg_from_device(store, data, precision="f64", **amg_kw) -> AMGHierarchy:
-    """SA hierarchy (host set-up) of a masked device CSR matrix; amg_kw go to
-    AMGHierarchy (degree, max_coarse, ...)."""
-    return AMGHierarchy(csr_to_host(store, data), precision=precision, seed=0, **amg_kw)
+    """SA hierarchy (host set-up) of a masked device CSR matrix on the
+    store's pattern; the finest level keeps the device values and pattern.
+    amg_kw go to AMGHierarchy (degree, max_coarse, ...)."""
+    return AMGHierarchy(csr_to_host(store, data), precision=precision, seed=0,
+                        device_pattern=store, device_values=data, **amg_kw)
