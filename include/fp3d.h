@@ -648,6 +648,26 @@ int fp3d_host_spgemm_fill(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
                           const int32_t *indptr, int32_t *indices, double *data);
 int fp3d_host_csr_matvec(int64_t n_rows, const int32_t *Ap, const int32_t *Aj,
                          const double *Ax, const double *x, double *y);
+/* The smoothed-aggregation prolongator of the same set-up (pyamg's
+ * smoothed_aggregation_solver, called at pyfp3d/solve/linear.py:61-81):
+ * P = T - diag(s) (A T) for the tentative prolongator T of an aggregation
+ * (row j of T: tval[agg[j]] in column agg[j]; no entry when agg[j] = -1)
+ * and the per-row damping s_i = (omega / rho) / a_ii, with the values of
+ * the scipy recipe (A T accumulated as fp3d_host_spgemm, X = s_i (A T),
+ * T - X merged by column, exact zeros dropped).  Two passes as
+ * fp3d_host_spgemm (Pp: n + 1).  fp3d_host_csr_transpose: R = P^T (Tp:
+ * n_cols + 1), rows of R with ascending columns, values copied. */
+int fp3d_host_sa_prolongator(int64_t n, const int32_t *Ap, const int32_t *Aj,
+                             const double *Ax, const int32_t *agg, int64_t n_agg,
+                             const double *tval, const double *s, int32_t *Pp,
+                             int64_t *nnz_out);
+int fp3d_host_sa_prolongator_fill(int64_t n, const int32_t *Ap, const int32_t *Aj,
+                                  const double *Ax, const int32_t *agg, int64_t n_agg,
+                                  const double *tval, const double *s,
+                                  const int32_t *Pp, int32_t *Pj, double *Px);
+int fp3d_host_csr_transpose(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
+                            const int32_t *Aj, const double *Ax, int32_t *Tp,
+                            int32_t *Tj, double *Tx);
 /* GMRES Hessenberg update (pyfp3d/solve/linear.py:182-193 calls
  * scipy.sparse.linalg.gmres, whose Arnoldi loop rotates each new column by
  * the previous Givens rotations): rotations j = 0 .. col-1 applied in order

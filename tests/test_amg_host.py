@@ -77,12 +77,12 @@ def test_sa_setup_matches_scipy_recipe():
     A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
     A.sum_duplicates()
     A.sort_indices()
-    old = amg._HOST_PARALLEL_NNZ
-    amg._HOST_PARALLEL_NNZ = 0          # every product through the library
+    old = amg._HOST_PARALLEL_NNZ, amg._HOST_PROLONG_NNZ
+    amg._HOST_PARALLEL_NNZ = amg._HOST_PROLONG_NNZ = 0     # everything through the library
     try:
         levels, _ = amg.sa_setup(A, max_coarse=10)
     finally:
-        amg._HOST_PARALLEL_NNZ = old
+        amg._HOST_PARALLEL_NNZ, amg._HOST_PROLONG_NNZ = old
     # the scipy recipe, level by level
     Ar = A
     for lv in levels[:-1]:
@@ -163,3 +163,50 @@ def test_sa_setup_takes_a_given_finest_rho():
     lv_c, _ = amg.sa_setup(A, max_coarse=10, rho0=1.5 * lv_a[0].rho)
     assert lv_c[0].rho == 1.5 * lv_a[0].rho
     assert not np.array_equal(lv_c[0].P.data, lv_a[0].P.data)
+
+
+@pytest.mark.parametrize("threads", ["1", "5"])
+def test_sa_prolongator_bitwise_scipy_recipe(threads, monkeypatch):
+    """The library's P = T - diag(s) (A T) and R = P^T equal the scipy
+    recipe bit for bit, with isolated rows (aggregate -1, no T entry), rows
+    whose damping s is zero, empty aggregates' neighbours and cancelling
+    sums, at any thread count."""
+    monkeypatch.setenv("FP3D_HOST_THREADS", threads)
+    rng = np.random.default_rng(int(threads))
+    n = 60000
+    A = _rand_csr(rng, n, n, 6.0 / n)
+    A = (A + sp.eye(n)).tocsr()
+    A.sort_indices()
+    n_agg = 7000
+    agg = rng.integers(0, n_agg, n).astype(np.int32)
+    agg[rng.random(n) < 0.05] = -1                   # isolated rows
+    rows = np.nonzero(agg >= 0)[0]
+    cnt = np.bincount(agg[rows], minlength=n_agg).astype(np.float64)
+    cnt[cnt == 0] = 1.0
+    tval = 1.0 / np.sqrt(cnt)
+    s = rng.standard_normal(n)
+    s[rng.random(n) < 0.05] = 0.0                    # zero damping
+    # an exact cancellation t - x = 0 in one row: x = s_i (A T)_{i, agg_i}
+    P, R = amg.sa_prolongator(A, agg, n_agg, tval, s)
+    T = sp.csr_matrix((tval[agg[rows]], (rows, agg[rows])), shape=(n, n_agg))
+    Pr = (T - sp.diags(s) @ (A @ T)).tocsr()
+    Pr.sort_indices()
+    Rr = Pr.T.tocsr()
+    Rr.sort_indices()
+    for got, ref in ((P, Pr), (R, Rr)):
+        assert np.array_equal(got.indptr, ref.indptr)
+        assert np.array_equal(got.indices, ref.indices)
+        assert np.array_equal(got.data, ref.data)
+    # exact zeros of t - x are dropped: force one and compare again
+    i = int(rows[0])
+    at = (A @ T)[i, agg[i]]
+    if at != 0.0:
+        s2 = s.copy()
+        s2[i] = tval[agg[i]] / at
+        if s2[i] * at == tval[agg[i]]:
+            P2, _ = amg.sa_prolongator(A, agg, n_agg, tval, s2)
+            Pr2 = (T - sp.diags(s2) @ (A @ T)).tocsr()
+            Pr2.sort_indices()
+            assert np.array_equal(P2.indices, Pr2.indices)
+            assert np.array_equal(P2.data, Pr2.data)
+            assert P2.nnz == Pr2.nnz

@@ -160,6 +160,11 @@ _SIGS = {
     "fp3d_host_spgemm_fill": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp, _vp]),
     "fp3d_host_csr_matvec": (_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "fp3d_host_sa_prolongator": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                        _vp]),
+    "fp3d_host_sa_prolongator_fill": (_int, [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                             _vp, _vp, _vp]),
+    "fp3d_host_csr_transpose": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fp3d_host_apply_givens": (_int, [_int, _vp, _vp]),
     "fp3d_graph_launch": (_int, [_vp, _vp]),
     "fp3d_arnoldi_step": (_int, [_vp, _int, _vp]),

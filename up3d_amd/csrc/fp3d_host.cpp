@@ -119,6 +119,66 @@ struct RowProduct {
   }
 };
 
+// Row i of the smoothed prolongator P = T - diag(s) (A T) of an aggregation
+// (T: one entry tval[agg[j]] in column agg[j] of row j, none for agg[j] < 0),
+// with the operations of the scipy recipe it replaces: (A T) row sums from 0
+// in A's stored order, exact zeros dropped; X = s_i (A T); P = T - X merged
+// by ascending column (t - x where both are stored, t or 0 - x where one
+// is), exact zeros dropped.  emit(c, k, v) for the c-th entry, column k.
+struct ProlongRow {
+  std::vector<double> sums;
+  std::vector<uint8_t> seen;
+  std::vector<int32_t> touched;
+  explicit ProlongRow(int64_t n_agg) : sums(n_agg, 0.0), seen(n_agg, 0) {}
+  template <class Emit>
+  int32_t run(int64_t i, const int32_t *Ap, const int32_t *Aj, const double *Ax,
+              const int32_t *agg, const double *tval, const double *s, Emit emit) {
+    touched.clear();
+    for (int32_t jj = Ap[i]; jj < Ap[i + 1]; ++jj) {
+      const int32_t a = agg[Aj[jj]];
+      if (a < 0) continue;
+      sums[a] += Ax[jj] * tval[a];
+      if (!seen[a]) {
+        seen[a] = 1;
+        touched.push_back(a);
+      }
+    }
+    std::sort(touched.begin(), touched.end());
+    const int32_t ti = agg[i];
+    const double t = ti >= 0 ? tval[ti] : 0.0;
+    bool t_done = ti < 0;
+    int32_t cnt = 0;
+    auto put = [&](int32_t k, double v) {
+      if (v != 0.0) emit(cnt++, k, v);
+    };
+    for (const int32_t k : touched) {
+      const double at = sums[k];
+      sums[k] = 0.0;
+      seen[k] = 0;
+      if (!t_done && ti < k) {
+        put(ti, t);
+        t_done = true;
+      }
+      const double x = at != 0.0 ? s[i] * at : 0.0;
+      if (x == 0.0) {
+        if (!t_done && ti == k) {
+          put(ti, t);
+          t_done = true;
+        }
+        continue;
+      }
+      if (!t_done && ti == k) {
+        put(k, t - x);
+        t_done = true;
+      } else {
+        put(k, 0.0 - x);
+      }
+    }
+    if (!t_done) put(ti, t);
+    return cnt;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -186,6 +246,82 @@ int fp3d_host_apply_givens(int col, double *hrow, const double *givens) {
     hrow[j] = c * n0 + s * n1;
     hrow[j + 1] = -s * n0 + c * n1;
   }
+  return FP3D_OK;
+}
+
+/* Smoothed-aggregation prolongator P = T - diag(s) (A T) in two passes
+ * (row pointer and count, then indices and data), row-parallel. */
+int fp3d_host_sa_prolongator(int64_t n, const int32_t *Ap, const int32_t *Aj,
+                             const double *Ax, const int32_t *agg, int64_t n_agg,
+                             const double *tval, const double *s, int32_t *Pp,
+                             int64_t *nnz_out) {
+  HREQ(n >= 0 && n_agg >= 0 && Pp && nnz_out, "sa_prolongator: bad arguments");
+  HREQ(n == 0 || (Ap && agg && s && (n_agg == 0 || tval)), "sa_prolongator: NULL pointer");
+  for (int64_t i = 0; i < n; ++i)
+    HREQ(agg[i] >= -1 && agg[i] < n_agg, "sa_prolongator: aggregate id out of range");
+  const int64_t nnzA = n ? (int64_t)Ap[n] : 0;
+  for (int64_t q = 0; q < nnzA; ++q)
+    HREQ(Aj[q] >= 0 && Aj[q] < n, "sa_prolongator: column out of range");
+  Pp[0] = 0;
+  parallel_rows(n, nnzA, [&](int64_t lo, int64_t hi, unsigned) {
+    ProlongRow pr(n_agg);
+    for (int64_t i = lo; i < hi; ++i)
+      Pp[i + 1] = pr.run(i, Ap, Aj, Ax, agg, tval, s, [](int32_t, int32_t, double) {});
+  });
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    nnz += Pp[i + 1];
+    HREQ(nnz <= INT32_MAX, "sa_prolongator: more than 2^31 entries");
+    Pp[i + 1] = (int32_t)nnz;
+  }
+  *nnz_out = nnz;
+  return FP3D_OK;
+}
+
+int fp3d_host_sa_prolongator_fill(int64_t n, const int32_t *Ap, const int32_t *Aj,
+                                  const double *Ax, const int32_t *agg, int64_t n_agg,
+                                  const double *tval, const double *s,
+                                  const int32_t *Pp, int32_t *Pj, double *Px) {
+  HREQ(n >= 0 && n_agg >= 0 && Pp, "sa_prolongator_fill: bad arguments");
+  HREQ(n == 0 || Pp[n] == 0 || (Pj && Px), "sa_prolongator_fill: NULL output");
+  parallel_rows(n, n ? (int64_t)Ap[n] : 0, [&](int64_t lo, int64_t hi, unsigned) {
+    ProlongRow pr(n_agg);
+    for (int64_t i = lo; i < hi; ++i) {
+      int32_t *pj = Pj + Pp[i];
+      double *px = Px + Pp[i];
+      const int32_t c = pr.run(i, Ap, Aj, Ax, agg, tval, s,
+                               [pj, px](int32_t c, int32_t k, double v) {
+                                 pj[c] = k;
+                                 px[c] = v;
+                               });
+      (void)c;
+    }
+  });
+  return FP3D_OK;
+}
+
+/* Transpose of an (n_rows x n_cols) CSR: counting sort by column, rows
+ * visited in ascending order, so each row of the result has ascending
+ * columns (scipy's csr -> csc). */
+int fp3d_host_csr_transpose(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
+                            const int32_t *Aj, const double *Ax, int32_t *Tp,
+                            int32_t *Tj, double *Tx) {
+  HREQ(n_rows >= 0 && n_cols >= 0 && Tp, "csr_transpose: bad arguments");
+  const int64_t nnz = n_rows ? (int64_t)Ap[n_rows] : 0;
+  HREQ(nnz == 0 || (Aj && Ax && Tj && Tx), "csr_transpose: NULL pointer");
+  std::vector<int64_t> cnt(n_cols + 1, 0);
+  for (int64_t q = 0; q < nnz; ++q) {
+    HREQ(Aj[q] >= 0 && Aj[q] < n_cols, "csr_transpose: column out of range");
+    ++cnt[Aj[q] + 1];
+  }
+  for (int64_t k = 0; k < n_cols; ++k) cnt[k + 1] += cnt[k];
+  for (int64_t k = 0; k <= n_cols; ++k) Tp[k] = (int32_t)cnt[k];
+  for (int64_t i = 0; i < n_rows; ++i)
+    for (int32_t q = Ap[i]; q < Ap[i + 1]; ++q) {
+      const int64_t dst = cnt[Aj[q]]++;
+      Tj[dst] = (int32_t)i;
+      Tx[dst] = Ax[q];
+    }
   return FP3D_OK;
 }
 

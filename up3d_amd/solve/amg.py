@@ -56,6 +56,10 @@ def _csr32(M: sp.csr_matrix):
 # below this many stored entries of A the single-pass scipy product is the
 # faster of the two (the values are the same)
 _HOST_PARALLEL_NNZ = 1 << 21
+# the smoothed prolongator and its transpose go through the library above
+# this many entries of A (one pass instead of scipy's chain of products,
+# scalings, differences and transposes; the same values)
+_HOST_PROLONG_NNZ = 1 << 14
 
 
 def spgemm(A: sp.csr_matrix, B: sp.csr_matrix, force_native: bool = False
@@ -86,6 +90,41 @@ def spgemm(A: sp.csr_matrix, B: sp.csr_matrix, force_native: bool = False
     out = sp.csr_matrix((dv, ix, ip), shape=(A.shape[0], B.shape[1]))
     out.has_sorted_indices = True
     return out
+
+
+def sa_prolongator(A: sp.csr_matrix, agg: np.ndarray, n_agg: int, tval: np.ndarray,
+                   s: np.ndarray):
+    """(P, R = P^T) of smoothed aggregation, P = T - diag(s) (A T) with T the
+    tentative prolongator (row j: tval[agg[j]] in column agg[j]), by the
+    library's row-parallel host code: the values of the scipy recipe
+    (T - sp.diags(s) @ (A @ T), transposed), columns sorted."""
+    ap, aj, ax = _csr32(A)
+    n = A.shape[0]
+    ag = np.ascontiguousarray(agg, dtype=np.int32)
+    tv = np.ascontiguousarray(tval, dtype=np.float64)
+    sv = np.ascontiguousarray(s, dtype=np.float64)
+    L = _native.lib()
+    args = (n, ap.ctypes.data, aj.ctypes.data, ax.ctypes.data, ag.ctypes.data, int(n_agg),
+            tv.ctypes.data, sv.ctypes.data)
+    pp = np.empty(n + 1, dtype=np.int32)
+    nnz = C.c_int64(0)
+    _native.check(L.fp3d_host_sa_prolongator(*args, pp.ctypes.data, C.byref(nnz)),
+                  "host_sa_prolongator")
+    pj = np.empty(int(nnz.value), dtype=np.int32)
+    px = np.empty(int(nnz.value), dtype=np.float64)
+    _native.check(L.fp3d_host_sa_prolongator_fill(*args, pp.ctypes.data, pj.ctypes.data,
+                                                  px.ctypes.data), "host_sa_prolongator_fill")
+    P = sp.csr_matrix((px, pj, pp), shape=(n, int(n_agg)))
+    P.has_sorted_indices = True
+    rp = np.empty(int(n_agg) + 1, dtype=np.int32)
+    rj = np.empty(len(pj), dtype=np.int32)
+    rx = np.empty(len(pj), dtype=np.float64)
+    _native.check(L.fp3d_host_csr_transpose(n, int(n_agg), pp.ctypes.data, pj.ctypes.data,
+                                            px.ctypes.data, rp.ctypes.data, rj.ctypes.data,
+                                            rx.ctypes.data), "host_csr_transpose")
+    R = sp.csr_matrix((rx, rj, rp), shape=(int(n_agg), n))
+    R.has_sorted_indices = True
+    return P, R
 
 
 def csr_matvec(A: sp.csr_matrix, x: np.ndarray, force_native: bool = False
@@ -202,13 +241,16 @@ def sa_setup(A: sp.csr_matrix, max_levels: int = 10, max_coarse: int = 10,
             break
         rows = np.nonzero(agg >= 0)[0]
         cnt = np.bincount(agg[rows], minlength=n_agg).astype(np.float64)
-        T = sp.csr_matrix((1.0 / np.sqrt(cnt[agg[rows]]), (rows, agg[rows])),
-                          shape=(n, n_agg))
-        AT = spgemm(A, T)
-        P = (T - sp.diags((omega / rho) * dinv) @ AT).tocsr()
-        P.sort_indices()
-        R = P.T.tocsr()
-        R.sort_indices()
+        if A.nnz >= _HOST_PROLONG_NNZ:
+            P, R = sa_prolongator(A, agg, n_agg, 1.0 / np.sqrt(cnt), (omega / rho) * dinv)
+        else:
+            T = sp.csr_matrix((1.0 / np.sqrt(cnt[agg[rows]]), (rows, agg[rows])),
+                              shape=(n, n_agg))
+            AT = spgemm(A, T)
+            P = (T - sp.diags((omega / rho) * dinv) @ AT).tocsr()
+            P.sort_indices()
+            R = P.T.tocsr()
+            R.sort_indices()
         Ac = spgemm(R, spgemm(A, P))
         levels.append(HostLevel(A, P, R, dinv, rho))
         A = Ac
