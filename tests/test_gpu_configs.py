@@ -56,7 +56,8 @@ def test_config3_adjoint_full_size(gpu):
     m = config_mesh("c3")
     dn = boundary_nodes(m, ["inflow", "outflow"])
     floor = np.unique(m.boundary_faces["floor"])
-    prob = ChannelForceProblem(m.nodes, m.elements, dn, m.boundary_faces["floor"], floor)
+    prob = ChannelForceProblem(m.nodes, m.elements, dn, m.boundary_faces["floor"], floor,
+                               newton_kw={"amg_max_coarse": 1000})
     print(f"[configs] c3 n_tets={m.n_tets} floor nodes={len(floor)}", flush=True)
     m_inf, alpha = 0.80, 0.0
     # grid-sequenced state: Mach ramp at upwind_c 3 on 48x16x16, then the
@@ -71,7 +72,8 @@ def test_config3_adjoint_full_size(gpu):
         m_inf, m_start=0.6, dm=0.05, upwind_c_ramp=3.0, upwind_c_post=(2.0, 1.5),
         post_stages_on="finest",
         newton_kw={"gmres_restart": 200, "gmres_maxiter": 5, "n_newton_max": 30,
-                   "amg_rebuild_every": 0, "precond_matrix": "jacobian_spd"})
+                   "amg_rebuild_every": 0, "precond_matrix": "jacobian_spd",
+                   "amg_max_coarse": 1000})
     assert seq["converged"]
     sol = prob.solve(None, m_inf, alpha, phi_init=seq["phi"])
     print(f"[configs] c3 M0.80: converged={sol['converged']} frozen residual at "
