@@ -11,12 +11,14 @@ lift -- the gradient is
 
 taken at the FROZEN upstream selection and branch of the converged state
 (the same frozen linearisation the Newton finish uses).  Device pieces:
-J^T x = J12 x + T3^T x (Terms 1+2 are symmetric; Term 3 through its COO
-sorted by column), dR/dM_inf from the branch-wise d rho_tilde/dM kernel and
-the coloured residual scatter, lambda^T dR/dx from fp3d_adjoint_dx.  The
+J_ff^T as one CSR, the free rows of J12 + T3^T (Terms 1+2 are symmetric;
+Term 3 through its COO sorted by column), so the adjoint GMRES takes the
+one-call Arnoldi step; dR/dM_inf from the branch-wise d rho_tilde/dM
+kernel and the coloured residual scatter; lambda^T dR/dx from
+fp3d_adjoint_dx; the Dirichlet chains from the unmasked J^T lambda.  The
 functional's own derivatives (a 2-D wall sum) use torch autograd on the
-host.  alpha is in degrees (freestream_phi convention); d/d alpha is per
-degree.
+wall nodes, on the device.  alpha is in degrees (freestream_phi
+convention); d/d alpha is per degree.
 """
 
 from __future__ import annotations
@@ -258,27 +260,34 @@ class ChannelForceProblem:
                                        t(self.flip), weights[0], weights[1])
         return float(val), cf.numpy(), float(cl)
 
-    def _functional_grads(self, sol, weights):
+    def _functional_grads(self, sol, weights, device):
         """(value, cf, cl, dF/dnodes (n, 3), dF/dphi (n,), dF/dM_inf,
-        dF/dalpha) by host autograd over the wall nodes."""
+        dF/dalpha) by autograd over the wall nodes, on `device` (a few
+        thousand faces: on the GPU the sixty-odd small operations cost
+        about a millisecond; on the host each one wakes the intra-op
+        thread pool)."""
         wn = self.wall_nodes
         n = len(sol["phi"])
         with torch.enable_grad():
-            t = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)  # noqa: E731
+            def t(a):
+                return torch.tensor(a, dtype=torch.float64, device=device,
+                                    requires_grad=True)
+
             nodes_t = t(np.asarray(sol["nodes"])[wn])
             phi_t = t(np.asarray(sol["phi"])[wn])
             m_t, a_t = t(sol["m_inf"]), t(sol["alpha_deg"])
-            val, cf, cl = _wall_functional(nodes_t, phi_t, m_t, a_t,
-                                           torch.from_numpy(self.wall_faces_local),
-                                           torch.tensor(self.flip), weights[0],
-                                           weights[1])
+            val, cf, cl = _wall_functional(
+                nodes_t, phi_t, m_t, a_t,
+                torch.as_tensor(self.wall_faces_local, device=device),
+                torch.as_tensor(self.flip, dtype=torch.float64, device=device),
+                weights[0], weights[1])
             gx, gphi, gm, ga = torch.autograd.grad(val, (nodes_t, phi_t, m_t, a_t))
         dFx = np.zeros((n, 3))
-        dFx[wn] = gx.numpy()
+        dFx[wn] = gx.cpu().numpy()
         dFphi = np.zeros(n)
-        dFphi[wn] = gphi.numpy()
-        return (val.detach(), cf.detach(), cl.detach(), dFx, dFphi, float(gm),
-                float(ga))
+        dFphi[wn] = gphi.cpu().numpy()
+        return (val.detach().cpu(), cf.detach().cpu(), cl.detach().cpu(), dFx, dFphi,
+                float(gm), float(ga))
 
     # ----------------------------------------------------------- adjoint
     def _operators(self, nodes):
@@ -372,8 +381,9 @@ class ChannelForceProblem:
         st = L["st"]
         n, nt = st.n_nodes, st.n_tets
         fm = L["fm"]
-        # functional derivatives (host autograd; also valid inside backward)
-        val, cf, cl, dFx, dFphi, dFm, dFa = self._functional_grads(sol, weights)
+        # functional derivatives (device autograd; also valid inside backward)
+        val, cf, cl, dFx, dFphi, dFm, dFa = self._functional_grads(sol, weights,
+                                                                   st.device)
         # adjoint solve J_ff^T lam = (dF/dphi)_free
         b = to_dev(dFphi)
         b.mul_(fm.to(torch.float64))
@@ -457,7 +467,7 @@ class ChannelForceProblem:
         dphi, info, _ = gmres_dev(Jf.matvec, rhs, ml.apply, rtol=rtol, restart=200,
                                   maxiter=12)
         dphi_h = to_host(dphi) + vd
-        _, _, _, _, gphi, gm, ga = self._functional_grads(sol, weights)
+        _, _, _, _, gphi, gm, ga = self._functional_grads(sol, weights, st.device)
         return gm * dm + ga * dalpha_deg + float(gphi @ dphi_h)
 
 
