@@ -253,3 +253,17 @@ def test_masked_sum_csr_is_the_masked_dense_sum():
     assert np.allclose(got, want, rtol=0, atol=1e-14)
     assert M.nnz == int(np.diff(A.indptr)[fm.numpy() > 0].sum() + (fm.numpy()[r] > 0).sum())
     assert all(np.all(np.diff(ix[ip[i]:ip[i + 1]]) >= 0) for i in range(n))
+
+
+def test_gmres_rejects_restart_beyond_the_basis_limit():
+    """The device multi-dot orthogonalises against at most 256 vectors, so a
+    longer restart is refused before any device work."""
+    import torch
+
+    from up3d_amd.solve.linear import MAX_RESTART, gmres_dev
+
+    b = torch.ones(10, dtype=torch.float64)
+    with pytest.raises(ValueError, match="restart"):
+        gmres_dev(lambda x, y: y.copy_(x), b, restart=MAX_RESTART + 1)
+    with pytest.raises(ValueError, match="restart"):
+        gmres_dev(lambda x, y: y.copy_(x), b, restart=0)

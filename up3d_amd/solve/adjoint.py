@@ -390,7 +390,7 @@ class ChannelForceProblem:
         ml = self._preconditioner(sol, st, L["A"], fm)
         # J_ff^T (J12 is symmetric) with the Dirichlet rows emptied
         JfT = _masked_sum_csr(st, L["J12"], self._term3(L, True), fm)
-        lam, info, n_it = gmres_dev(JfT.matvec, b, ml.apply, rtol=rtol, restart=200,
+        lam, info, n_it = gmres_dev(JfT.matvec, b, ml.apply, rtol=rtol, restart=256,
                                     maxiter=12)
         red_norm = Reducer()
         # dR/dM_inf = scatter of d rho_tilde/dM
@@ -464,7 +464,7 @@ class ChannelForceProblem:
         rhs.mul_(-fmd)
         ml = self._preconditioner(sol, st, L["A"], L["fm"])
         Jf = _masked_sum_csr(st, L["J12"], self._term3(L, False), L["fm"])
-        dphi, info, _ = gmres_dev(Jf.matvec, rhs, ml.apply, rtol=rtol, restart=200,
+        dphi, info, _ = gmres_dev(Jf.matvec, rhs, ml.apply, rtol=rtol, restart=256,
                                   maxiter=12)
         dphi_h = to_host(dphi) + vd
         _, _, _, _, gphi, gm, ga = self._functional_grads(sol, weights, st.device)

@@ -80,6 +80,10 @@ def scale(a: float, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
 _GMRES_WS: "OrderedDict" = None
 
 
+# the device multi-dot / multi-axpy take at most 256 basis vectors
+MAX_RESTART = 256
+
+
 def gmres_buffers(n: int, restart: int, f32: bool, device, V=None) -> dict:
     """A fresh set of gmres_dev work buffers (V: an optional preallocated
     (restart + 1, n) basis, e.g. a slice of a batch's bases)."""
@@ -310,6 +314,9 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
     per-shape set)."""
     if basis not in ("f64", "f32"):
         raise ValueError(f"basis={basis!r}: 'f64' or 'f32'")
+    if not 1 <= int(restart) <= MAX_RESTART:
+        raise ValueError(f"restart={restart}: 1 to {MAX_RESTART} (the basis vectors one "
+                         "multi-dot launch orthogonalises against)")
     f32 = basis == "f32"
     n = b.numel()
     if maxiter is None:
