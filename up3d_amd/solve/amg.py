@@ -150,12 +150,19 @@ def spectral_radius_dinv_a(A: sp.csr_matrix, dinv: np.ndarray, iters: int = 12,
     lam = 1.0
     for _ in range(iters):
         y = dinv * csr_matvec(A, x)
-        ny = np.linalg.norm(y)
+        ny = _norm2(y)
         if ny == 0.0:
             return 1.0
-        lam = ny / np.linalg.norm(x)
+        lam = ny / _norm2(x)
         x = y / ny
     return float(lam)
+
+
+def _norm2(v: np.ndarray) -> float:
+    """2-norm by numpy's pairwise summation: one thread and a fixed order
+    (a BLAS dot splits the sum over its thread pool, whose wake-ups cost
+    about a millisecond per call on short vectors)."""
+    return float(np.sqrt(np.add.reduce(v * v)))
 
 
 @dataclass
@@ -203,17 +210,17 @@ def coarse_inverse(Ac: sp.csr_matrix) -> np.ndarray:
     if n:
         scale = np.abs(M).max()
         if scale > 0.0 and np.abs(M - M.T).max() <= 1e-12 * scale:
-            try:
-                L = np.linalg.cholesky(0.5 * (M + M.T))
-            except np.linalg.LinAlgError:
-                L = None
-            if L is not None:
+            from scipy.linalg import lapack
+
+            # LAPACK potrf (lower factor) then potri (the inverse from it)
+            L, info = lapack.dpotrf(0.5 * (M + M.T), lower=1, clean=1)
+            if info == 0:
                 dl = np.diag(L)
                 if dl.min() > 1e-6 * dl.max():
-                    import scipy.linalg as sla
-
-                    Li = sla.solve_triangular(L, np.eye(n), lower=True)
-                    return Li.T @ Li
+                    X, info = lapack.dpotri(L, lower=1)
+                    if info == 0:
+                        X = np.tril(X)
+                        return X + np.tril(X, -1).T
     return np.linalg.pinv(M)
 
 
