@@ -67,3 +67,68 @@ def test_plate_section_cp_uniform_flow():
     assert abs(c["x_le"] - 0.5 * np.tan(np.deg2rad(30.0))) < h
     with pytest.raises(ValueError):
         section_cp_curve(box, phi, eta=0.5, upper_hint=(0.0, 0.0, 1.0), span_axis=1)
+
+
+def _normals_full_search(nodes, elements, wall_faces):
+    """Wall normals with the owner found among all 4 N_t faces."""
+    el = np.asarray(elements, dtype=np.int64)
+    wf = np.asarray(wall_faces, dtype=np.int64)
+    n = int(max(el.max(), wf.max())) + 1
+
+    def keys(tri):
+        s = np.sort(tri, axis=1)
+        return (s[:, 0] * n + s[:, 1]) * n + s[:, 2]
+
+    faces = np.concatenate([el[:, [1, 2, 3]], el[:, [0, 2, 3]], el[:, [0, 1, 3]],
+                            el[:, [0, 1, 2]]])
+    owner = np.tile(np.arange(len(el)), 4)
+    order = np.argsort(keys(faces), kind="stable")
+    ks, os_ = keys(faces)[order], owner[order]
+    own = os_[np.searchsorted(ks, keys(wf))]
+    p0 = nodes[wf[:, 0]]
+    nv = np.cross(nodes[wf[:, 1]] - p0, nodes[wf[:, 2]] - p0)
+    nrm = nv / np.linalg.norm(nv, axis=1)[:, None]
+    flip = np.einsum("ij,ij->i", nrm, nodes[el[own]].mean(axis=1) - nodes[wf].mean(axis=1)) < 0
+    nrm[flip] *= -1.0
+    return nrm
+
+
+def test_wall_normals_candidate_search_equals_full_search():
+    """The owner search over tets with three wall nodes gives the normals of
+    the search over every face, bit for bit, on each channel wall; a face
+    that is not a boundary face is refused."""
+    from up3d_amd.post.surface import wall_outward_normals
+
+    m = bump_channel(12, 4, 4, jitter=0.2, seed=3, rcm=True)
+    for name, wf in m.boundary_faces.items():
+        got = wall_outward_normals(m.nodes, m.elements, wf)
+        assert np.array_equal(got, _normals_full_search(m.nodes, m.elements, wf)), name
+    interior = m.elements[:1, :3]            # a face of the first tet
+    faces = {tuple(sorted(f)) for f in np.concatenate(list(m.boundary_faces.values()))}
+    if tuple(sorted(interior[0])) not in faces:
+        with pytest.raises(ValueError, match="not owned"):
+            wall_outward_normals(m.nodes, m.elements, interior)
+
+
+def test_array_digest_and_cached_locations():
+    """The mesh content key separates different contents and matches equal
+    ones; repeated P1 transfers reuse the located points and give the same
+    values."""
+    from up3d_amd.kernels.store import array_digest
+    from up3d_amd.mesh.transfer import interpolate_p1, locate_points_cached
+
+    a = np.arange(12.0)
+    assert array_digest(a) == array_digest(a.copy())
+    b = a.copy()
+    b[3] += 1e-12
+    assert array_digest(a) != array_digest(b)
+    assert array_digest(a.reshape(3, 4)) == array_digest(a)          # bytes only
+    src = bump_channel(6, 2, 2, jitter=0.1, seed=1, rcm=True)
+    dst = bump_channel(12, 4, 4, jitter=0.1, seed=2, rcm=True)
+    f = 1.0 + 2.0 * src.nodes[:, 0] - src.nodes[:, 1] + 0.5 * src.nodes[:, 2]
+    v1 = interpolate_p1(src.nodes, src.elements, f, dst.nodes)
+    h1 = locate_points_cached(src.nodes, src.elements, dst.nodes)
+    v2 = interpolate_p1(src.nodes, src.elements, 2.0 * f, dst.nodes)
+    h2 = locate_points_cached(src.nodes, src.elements, dst.nodes)
+    assert h1[0] is h2[0] and not h1[0].flags.writeable
+    assert np.array_equal(v2, 2.0 * v1)

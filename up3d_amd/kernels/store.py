@@ -62,6 +62,23 @@ def csr_pattern(elements: np.ndarray, n_nodes: int):
     return indptr, indices
 
 
+def array_digest(*arrays) -> str:
+    """128-bit content key of numpy arrays (their bytes in order, read in
+    place): xxh3 where the xxhash module is present (about 15x faster than
+    blake2b on mesh-sized arrays), else blake2b.  A process-local cache key,
+    never stored."""
+    try:
+        import xxhash
+
+        h = xxhash.xxh3_128()
+    except ImportError:
+        h = hashlib.blake2b(digest_size=16)
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        h.update(memoryview(a).cast("B"))
+    return h.hexdigest()
+
+
 def csr_offsets16(indptr: np.ndarray, indices: np.ndarray):
     """int16 device array of col - row for a square CSR pattern whose every
     |col - row| fits in int16, else None."""
@@ -95,11 +112,7 @@ class MeshStore:
     def get(cls, nodes: np.ndarray, elements: np.ndarray) -> "MeshStore":
         nd = np.ascontiguousarray(nodes, dtype=np.float64)
         el = np.ascontiguousarray(elements, dtype=np.int32)
-        h = hashlib.blake2b(digest_size=16)
-        h.update(np.int64(len(nd)).tobytes())
-        h.update(nd.tobytes())
-        h.update(el.tobytes())
-        key = h.hexdigest()
+        key = array_digest(np.int64(len(nd)), nd, el)
         st = cls._cache.pop(key, None)
         if st is None:
             st = cls(nd, el)

@@ -43,14 +43,20 @@ def wall_outward_normals(nodes, elements, wall_faces):
     """Unit wall normals pointing into the fluid (the owning tet's side);
     ValueError if a face is not owned by exactly one tet."""
     nodes = np.asarray(nodes, dtype=np.float64)
-    el = np.asarray(elements, dtype=np.int64)
+    el_all = np.asarray(elements)
     wf = np.asarray(wall_faces, dtype=np.int64)
-    n = int(max(el.max(), wf.max())) + 1
+    n = int(max(el_all.max(), wf.max())) + 1
 
     def keys(tri):
         s = np.sort(tri, axis=1)
         return (s[:, 0] * n + s[:, 1]) * n + s[:, 2]
 
+    # only a tet with three wall nodes can own (or share) a wall face: key
+    # the faces of those tets, not all 4 N_t faces
+    on_wall = np.zeros(n, dtype=bool)
+    on_wall[wf.ravel()] = True
+    cand = np.flatnonzero(on_wall[el_all].sum(axis=1) >= 3)
+    el = np.asarray(el_all[cand], dtype=np.int64)
     faces = np.concatenate([el[:, [1, 2, 3]], el[:, [0, 2, 3]],
                             el[:, [0, 1, 3]], el[:, [0, 1, 2]]])
     owner = np.tile(np.arange(len(el)), 4)
