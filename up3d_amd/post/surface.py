@@ -11,6 +11,8 @@ set is a 2-D surface (24,576 triangles on the 4.7M-tet channel).
 
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import numpy as np
 
 from ..physics.isentropic import GAMMA
@@ -80,6 +82,30 @@ def wall_outward_normals(nodes, elements, wall_faces):
     return nrm
 
 
+# outward normals per (nodes, elements, wall faces) content, most recent
+# last: a sweep evaluates the force of every condition on a few meshes
+_NORMALS: "OrderedDict[str, np.ndarray]" = OrderedDict()
+_MAX_NORMALS = 8
+
+
+def _outward_normals_cached(nodes, elements, wall_faces) -> np.ndarray:
+    from ..kernels.store import array_digest
+
+    nd = np.ascontiguousarray(nodes, dtype=np.float64)
+    el = np.ascontiguousarray(elements)
+    wf = np.ascontiguousarray(wall_faces, dtype=np.int64)
+    key = array_digest(np.int64(len(nd)), np.int64(len(el)), np.int64(el.dtype.itemsize),
+                       nd, el, wf)
+    nrm = _NORMALS.pop(key, None)
+    if nrm is None:
+        nrm = wall_outward_normals(nd, el, wf)
+        nrm.setflags(write=False)
+    _NORMALS[key] = nrm
+    while len(_NORMALS) > _MAX_NORMALS:
+        _NORMALS.popitem(last=False)
+    return nrm
+
+
 def cp_from_q2(q2, m_inf, gamma=GAMMA):
     """Exact isentropic Cp (incompressible 1 - q2 at M_inf = 0)."""
     q2 = np.asarray(q2, dtype=np.float64)
@@ -95,7 +121,7 @@ def wall_force_coefficients(nodes, elements, wall_faces, phi, alpha_deg=0.0,
                             u_inf=1.0, s_ref=1.0, m_inf=0.0):
     """dict(cl, cd_pressure, cf (3,), cp_tri (n,))."""
     g, area, _ = triangle_tangential_gradients(nodes, wall_faces, phi)
-    n_out = wall_outward_normals(nodes, elements, wall_faces)
+    n_out = _outward_normals_cached(nodes, elements, wall_faces)
     q2 = np.einsum("ij,ij->i", g, g) / u_inf ** 2
     cp = cp_from_q2(q2, m_inf)
     cf = -(cp * area) @ n_out / s_ref
