@@ -77,12 +77,13 @@ def test_sa_setup_matches_scipy_recipe():
     A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
     A.sum_duplicates()
     A.sort_indices()
-    old = amg._HOST_PARALLEL_NNZ, amg._HOST_PROLONG_NNZ
-    amg._HOST_PARALLEL_NNZ = amg._HOST_PROLONG_NNZ = 0     # everything through the library
+    old = amg._HOST_PARALLEL_NNZ, amg._HOST_PROLONG_NNZ, amg._HOST_PARALLEL_FLOPS
+    # everything through the library
+    amg._HOST_PARALLEL_NNZ = amg._HOST_PROLONG_NNZ = amg._HOST_PARALLEL_FLOPS = 0
     try:
         levels, _ = amg.sa_setup(A, max_coarse=10)
     finally:
-        amg._HOST_PARALLEL_NNZ, amg._HOST_PROLONG_NNZ = old
+        amg._HOST_PARALLEL_NNZ, amg._HOST_PROLONG_NNZ, amg._HOST_PARALLEL_FLOPS = old
     # the scipy recipe, level by level
     Ar = A
     for lv in levels[:-1]:

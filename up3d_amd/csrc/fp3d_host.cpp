@@ -84,6 +84,23 @@ void parallel_rows(int64_t n, int64_t work, F f) {
   for (auto &x : th) x.join();
 }
 
+// Multiply-adds of C = A B, estimated for the thread count: A's entries
+// times the mean length of the B rows they select, sampled at up to 1,024
+// evenly spaced entries of A (an O(1) estimate; the products themselves do
+// not depend on it).
+int64_t product_work(int64_t n_rows, const int32_t *Ap, const int32_t *Aj,
+                     const int32_t *Bp) {
+  const int64_t nnz = n_rows ? (int64_t)Ap[n_rows] : 0;
+  if (nnz == 0) return 0;
+  const int64_t samples = nnz < 1024 ? nnz : 1024;
+  int64_t len = 0;
+  for (int64_t t = 0; t < samples; ++t) {
+    const int32_t j = Aj[t * nnz / samples];
+    len += Bp[j + 1] - Bp[j];
+  }
+  return nnz * (len / samples > 1 ? len / samples : 1);
+}
+
 // One row of C = A B accumulated as scipy.sparse's csr_matmat does (A
 // entries in stored order, then the B entries of that row, sums from 0);
 // emit(k, value) is called for the nonzero sums in ascending column k.
@@ -196,7 +213,7 @@ int fp3d_host_spgemm(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
   HREQ(n_rows >= 0 && n_cols >= 0 && indptr && nnz_out, "spgemm: bad arguments");
   HREQ(n_rows == 0 || (Ap && Bp), "spgemm: NULL pointer");
   indptr[0] = 0;
-  parallel_rows(n_rows, n_rows ? (int64_t)Ap[n_rows] : 0,
+  parallel_rows(n_rows, product_work(n_rows, Ap, Aj, Bp),
                 [&](int64_t lo, int64_t hi, unsigned) {
     RowProduct rp(n_cols);
     for (int64_t i = lo; i < hi; ++i)
@@ -220,7 +237,7 @@ int fp3d_host_spgemm_fill(int64_t n_rows, int64_t n_cols, const int32_t *Ap,
   HREQ(n_rows >= 0 && n_cols >= 0 && indptr, "spgemm_fill: bad arguments");
   HREQ(n_rows == 0 || indptr[n_rows] == 0 || (indices && data),
        "spgemm_fill: NULL output");
-  parallel_rows(n_rows, n_rows ? (int64_t)Ap[n_rows] : 0,
+  parallel_rows(n_rows, product_work(n_rows, Ap, Aj, Bp),
                 [&](int64_t lo, int64_t hi, unsigned) {
     RowProduct rp(n_cols);
     for (int64_t i = lo; i < hi; ++i) {

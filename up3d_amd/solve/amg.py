@@ -53,9 +53,13 @@ def _csr32(M: sp.csr_matrix):
             np.ascontiguousarray(M.data, dtype=np.float64))
 
 
-# below this many stored entries of A the single-pass scipy product is the
+# below this many stored entries of A the single-pass scipy matvec is the
 # faster of the two (the values are the same)
 _HOST_PARALLEL_NNZ = 1 << 21
+# sparse products with fewer estimated multiply-adds (A's entries times the
+# mean length of B's rows) stay in scipy; above, the library's threaded
+# product is faster (the same values)
+_HOST_PARALLEL_FLOPS = 1 << 20
 # the smoothed prolongator and its transpose go through the library above
 # this many entries of A (one pass instead of scipy's chain of products,
 # scalings, differences and transposes; the same values)
@@ -70,7 +74,8 @@ def spgemm(A: sp.csr_matrix, B: sp.csr_matrix, force_native: bool = False
     A, B = A.tocsr(), B.tocsr()
     if A.shape[1] != B.shape[0]:
         raise ValueError(f"spgemm: shapes {A.shape} and {B.shape} do not align")
-    if A.nnz < _HOST_PARALLEL_NNZ and not force_native:
+    flops = A.nnz * max(1.0, B.nnz / max(B.shape[0], 1))
+    if flops < _HOST_PARALLEL_FLOPS and not force_native:
         out = (A @ B).tocsr()
         out.sort_indices()
         return out
