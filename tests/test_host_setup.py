@@ -267,3 +267,35 @@ def test_gmres_rejects_restart_beyond_the_basis_limit():
         gmres_dev(lambda x, y: y.copy_(x), b, restart=MAX_RESTART + 1)
     with pytest.raises(ValueError, match="restart"):
         gmres_dev(lambda x, y: y.copy_(x), b, restart=0)
+
+
+def test_executor_host_tasks_on_workers_and_inline():
+    """HostTasks of stepped solves: on worker threads (host_workers > 0) or
+    inline (0, and drive_inline), the same results; a task's exception
+    reaches the caller."""
+    import threading
+
+    from up3d_amd.solve.batch import BatchExecutor
+    from up3d_amd.solve.newton import HostTask, drive_inline
+
+    main = threading.get_ident()
+    seen = []
+
+    def co(i):
+        a = yield HostTask(lambda: (seen.append(threading.get_ident()), i * i)[1])
+        b = yield HostTask(lambda: a + 1)
+        return b
+
+    for workers in (0, 4):
+        seen.clear()
+        ex = BatchExecutor(host_workers=workers)
+        assert ex.run({i: co(i) for i in range(8)}) == {i: i * i + 1 for i in range(8)}
+        assert ex.stats()["host_tasks"] == 16
+        assert all((t == main) == (workers == 0) for t in seen)
+    assert drive_inline(co(3)) == 10
+
+    def bad():
+        yield HostTask(lambda: 1 / 0)
+
+    with pytest.raises(ZeroDivisionError):
+        BatchExecutor(host_workers=2).run({0: bad()})

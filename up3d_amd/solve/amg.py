@@ -376,12 +376,14 @@ class AMGHierarchy:
                  max_coarse: int = 10, seed: int = 0, use_graph: bool = True,
                  vectors: Optional[str] = None, compact_index: bool = True,
                  device_pattern=None, device_values: Optional[torch.Tensor] = None,
-                 rho0: Optional[float] = None):
+                 rho0: Optional[float] = None, host_setup=None):
         """device_pattern / device_values: A's CSR pattern (an object with
         n_nodes, nnz, indptr, indices and optionally indices16 device
         tensors) and float64 values already in HBM; the finest level is then
         built from them (no upload) and its rho(D^-1 A) estimated there.
-        rho0: the finest level's rho(D^-1 A), when already known."""
+        rho0: the finest level's rho(D^-1 A), when already known.
+        host_setup: the (levels, coarse inverse) of sa_setup for these
+        arguments, when already computed (AMGPlan runs it apart)."""
         if precision not in ("f64", "f32"):
             raise ValueError(f"unknown AMG precision {precision!r}")
         vectors = precision if vectors is None else vectors
@@ -400,8 +402,10 @@ class AMGHierarchy:
             i16 = getattr(st, "indices16", None) if compact_index else None
             A0 = DeviceCSR.on_device(st.n_nodes, st.indptr, st.indices, device_values,
                                      f32=f32, keep_f64=not f32, indices16=i16)
-        host, coarse = sa_setup(A, max_levels=max_levels, max_coarse=max_coarse,
-                                seed=seed, rho0=rho0)
+        if host_setup is None:
+            host_setup = sa_setup(A, max_levels=max_levels, max_coarse=max_coarse,
+                                  seed=seed, rho0=rho0)
+        host, coarse = host_setup
         self.precision = precision
         self.v32 = vectors == "f32"
         self.vfmt = 2 if self.v32 else 0
@@ -762,3 +766,40 @@ def apply_preconditioners(calls) -> int:
                 run_grouped([q[0] for q in g])
                 n_grouped += len(g)
     return n_grouped
+
+
+class AMGPlan:
+    """An AMGHierarchy built in three parts, so the host set-up can run on
+    another thread while the caller's thread keeps the device busy:
+
+      plan = AMGPlan(A, ..., device_pattern=..., device_values=...)  # caller's
+          thread: the finest level's rho(D^-1 A) on the device
+      h = plan.host()        # any thread: sa_setup (host only, no device
+                             # or torch call)
+      M = plan.finish(h)     # caller's thread: the device levels
+
+    The hierarchy equals AMGHierarchy(A, ...) with the same arguments."""
+
+    def __init__(self, A: sp.csr_matrix, max_levels: int = 10, max_coarse: int = 10,
+                 seed: int = 0, device_pattern=None,
+                 device_values: Optional[torch.Tensor] = None, **kw):
+        self.A = A.tocsr()
+        self.kw = dict(kw, max_levels=max_levels, max_coarse=max_coarse, seed=seed,
+                       device_pattern=device_pattern, device_values=device_values)
+        self.rho0 = kw.pop("rho0", None)
+        if device_values is not None and self.rho0 is None:
+            st = device_pattern
+            d = self.A.diagonal()
+            dinv = to_dev(np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0))
+            self.rho0 = spectral_radius_dinv_a_dev(st.n_nodes, st.nnz, st.indptr,
+                                                   st.indices, device_values, dinv,
+                                                   seed=seed)
+        self.kw["rho0"] = self.rho0
+
+    def host(self):
+        return sa_setup(self.A, max_levels=self.kw["max_levels"],
+                        max_coarse=self.kw["max_coarse"], seed=self.kw["seed"],
+                        rho0=self.rho0)
+
+    def finish(self, host_setup) -> "AMGHierarchy":
+        return AMGHierarchy(self.A, host_setup=host_setup, **self.kw)

@@ -18,7 +18,11 @@ request at a time, in rounds:
     one-step-ahead pipeline, shared by all solves);
   * the live residual evaluations of a round that share a mesh go through
     fp3d_batch_residual_eval (one launch per kernel of the evaluation chain
-    for all of them, one synchronisation); the others run per instance.
+    for all of them, one synchronisation); the others run per instance;
+  * a solve's host-only work (HostTask: the SA set-up of a new AMG
+    hierarchy) runs on a worker thread while the other solves keep their
+    rounds going; the solve resumes with its result in the first round
+    after it finishes (host_workers=0 runs it inline).
 
 A solve that returns leaves the round structure (the convergence mask is
 the set of live coroutines).  Each solve queues exactly the device work, in
@@ -36,7 +40,7 @@ import numpy as np
 import torch
 
 from .linear import ArnoldiStep, batch_key, batched_arnoldi
-from .newton import EvalRequest
+from .newton import EvalRequest, HostTask
 
 
 class BatchExecutor:
@@ -48,8 +52,10 @@ class BatchExecutor:
     Statistics: rounds, Arnoldi steps and evaluations executed, the
     largest number of steps queued in one round and the batched groups."""
 
-    def __init__(self, batched: bool = True):
+    def __init__(self, batched: bool = True, host_workers: int = 4):
         self.batched = bool(batched)
+        self.host_workers = int(host_workers)
+        self.n_host_tasks = 0
         self.n_rounds = 0
         self.n_steps = 0
         self.n_evals = 0
@@ -63,14 +69,32 @@ class BatchExecutor:
         """Advance every coroutine to its return value; on_done(key, value)
         is called as each finishes (it may add new coroutines with
         self.add)."""
+        from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+
         self._coros = dict(coros)
         self._pending: Dict[Hashable, object] = {}
         self._results: Dict[Hashable, object] = {}
+        self._tasks: Dict[Hashable, object] = {}
         self._on_done = on_done
-        for k in list(self._coros):
-            self._advance(k, None)
-        while self._pending:
-            self._round()
+        self._pool = (ThreadPoolExecutor(max_workers=self.host_workers)
+                      if self.host_workers > 0 else None)
+        try:
+            for k in list(self._coros):
+                self._advance(k, None)
+            while self._pending or self._tasks:
+                if self._tasks:
+                    if not self._pending:     # nothing else to run: wait for one
+                        wait(list(self._tasks.values()), return_when=FIRST_COMPLETED)
+                    for k, fut in list(self._tasks.items()):
+                        if fut.done():
+                            del self._tasks[k]
+                            self._advance(k, fut.result())
+                if self._pending:
+                    self._round()
+        finally:
+            if self._pool is not None:
+                self._pool.shutdown(wait=True)
+                self._pool = None
         return self._results
 
     def add(self, key, coro) -> None:
@@ -88,6 +112,13 @@ class BatchExecutor:
             self._results[key] = stop.value
             if self._on_done is not None:
                 self._on_done(key, stop.value)
+            return
+        if isinstance(req, HostTask):
+            self.n_host_tasks += 1
+            if self._pool is None:
+                self._advance(key, req.fn())
+            else:
+                self._tasks[key] = self._pool.submit(req.fn)
             return
         if not isinstance(req, (EvalRequest, ArnoldiStep)):
             raise TypeError(f"solve {key!r} yielded {type(req).__name__}")
@@ -139,7 +170,8 @@ class BatchExecutor:
                 "evaluations": self.n_evals, "max_steps_per_round": self.max_width,
                 "batched_groups": self.n_batched_launch_groups,
                 "grouped_vcycles": self.n_grouped_vcycles,
-                "batched_eval_groups": self.n_batched_eval_groups}
+                "batched_eval_groups": self.n_batched_eval_groups,
+                "host_tasks": self.n_host_tasks}
 
 
 _EVAL_FIELDS = 24      # int64 words per fp3d_batch_residual_eval job

@@ -64,3 +64,12 @@ def amg_from_device(store, data, precision="f64", **amg_kw) -> AMGHierarchy:
     amg_kw go to AMGHierarchy (degree, max_coarse, ...)."""
     return AMGHierarchy(csr_to_host(store, data), precision=precision, seed=0,
                         device_pattern=store, device_values=data, **amg_kw)
+
+
+def amg_plan_from_device(store, data, precision="f64", **amg_kw):
+    """amg_from_device in parts (solve.amg.AMGPlan): the host set-up can
+    run on another thread."""
+    from .amg import AMGPlan
+
+    return AMGPlan(csr_to_host(store, data), precision=precision, seed=0,
+                   device_pattern=store, device_values=data, **amg_kw)

@@ -38,7 +38,7 @@ from ..device import empty, ptr, stream_handle, to_dev, to_host
 from ..kernels.jacobian import PicardOperator
 from ..kernels.upwind import UpwindOperator
 from ..physics.isentropic import q2_at_mach
-from .common import (Reducer, amg_from_device, dirichlet_setup, mask_dirichlet,
+from .common import (Reducer, amg_plan_from_device, dirichlet_setup, mask_dirichlet,
                      masked)
 from .continuation import mach_schedule
 from .linear import (NewtonOperator, axpby, drive, gmres_buffers, gmres_steps,
@@ -248,6 +248,30 @@ class NewtonWorkspace:
                              clamp: float = 0.9, se=None, ptc_scale: float = 0.0):
         """AMG of the Picard matrix A(rho_tilde) (the reference's choice) or
         of the SPD clamped Jacobian (precond_matrix="jacobian_spd")."""
+        return drive_inline(self.build_preconditioner_steps(s, precond_matrix, clamp, se,
+                                                            ptc_scale, stepped=False))
+
+    def build_preconditioner_steps(self, s: _State, precond_matrix: str = "picard",
+                                   clamp: float = 0.9, se=None, ptc_scale: float = 0.0,
+                                   stepped: bool = False):
+        """build_preconditioner as a coroutine: stepped=True yields the host
+        SA set-up as a HostTask (an executor may run it on another thread
+        while other solves use the device) and builds the device levels
+        when resumed with its result; the same hierarchy either way."""
+        self._precond_matrix_dev(s, precond_matrix, clamp, se, ptc_scale)
+        plan = amg_plan_from_device(
+            self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64",
+            **getattr(self, "amg_kw", {}))
+        if stepped:
+            host = yield HostTask(plan.host)
+        else:
+            host = plan.host()
+        self.precond = plan.finish(host)
+        return self.precond
+
+    def _precond_matrix_dev(self, s: _State, precond_matrix: str, clamp: float, se,
+                            ptc_scale: float) -> None:
+        """The masked preconditioner matrix into self.A_pic."""
         if precond_matrix == "jacobian_spd" and se is not None:
             self.op.jacobian_spd_dev(s.rt, s.grad, se, s.lim,
                                      1.0 / (self.u_inf * self.u_inf), clamp,
@@ -258,10 +282,6 @@ class NewtonWorkspace:
             raise ValueError(f"unknown precond_matrix {precond_matrix!r}")
         self._diag_shift(self.A_pic, ptc_scale)
         mask_dirichlet(self.st, self.fm, self.A_pic)
-        self.precond = amg_from_device(
-            self.st, self.A_pic, "f32" if self.precision == "fp32" else "f64",
-            **getattr(self, "amg_kw", {}))
-        return self.precond
 
     def freeze(self, s: _State, upwind_c, m_crit, rho_floor):
         return self.upw.freeze_dev(s.grad, s.q2l, s.rho, self.m_inf, upwind_c,
@@ -320,6 +340,29 @@ class EvalRequest:
 
     def __init__(self, ws, s, args):
         self.ws, self.s, self.args = ws, s, args
+
+
+class HostTask:
+    """Host-only work of a stepped solve (no device or torch call inside
+    fn), handed to the executor, which may run fn() on a worker thread and
+    resumes the solve with its return value."""
+
+    __slots__ = ("fn",)
+
+    def __init__(self, fn):
+        self.fn = fn
+
+
+def drive_inline(gen):
+    """Run a coroutine that may yield HostTasks, running each inline."""
+    try:
+        req = next(gen)
+        while True:
+            if not isinstance(req, HostTask):
+                raise TypeError(f"unexpected request {type(req).__name__}")
+            req = gen.send(req.fn())
+    except StopIteration as stop:
+        return stop.value
 
 
 def _eval(ws, stepped, s, *args):
@@ -604,8 +647,9 @@ def newton_steps(nodes, elements, dirichlet_nodes, dirichlet_values,
                             or last_lin[1] > 2 * max(last_lin[2], 10))
                            and it - last_lin[3] >= 3))
         if rebuild:
-            M_pre = ws.build_preconditioner(cur, precond_matrix, precond_clamp,
-                                            se=ws.se_used, ptc_scale=ptc_scale)
+            M_pre = yield from ws.build_preconditioner_steps(
+                cur, precond_matrix, precond_clamp, se=ws.se_used, ptc_scale=ptc_scale,
+                stepped=stepped)
             last_lin[2] = -1
             last_lin[3] = it
         sync()
