@@ -539,6 +539,13 @@ int fp3d_dense_matvec_v(int fmt, int64_t n, const double *M, const void *b, void
  * results are bitwise those of its own V-cycle. */
 int fp3d_grp_vcycle_op(int nb, const void *recs, int op, int64_t max_n, int G, int fmt,
                        void *stream);
+/* A whole grouped V-cycle: n_ops operations, operation i being
+ * fp3d_grp_vcycle_op(nb, recs + i * nb records, ops[4i], ops[4i + 1],
+ * ops[4i + 2], ops[4i + 3]) (ops: host array of (op, max_n, G, fmt)
+ * words), launched in order on the stream; one call instead of one per
+ * operation. */
+int fp3d_grp_vcycle_run(int nb, const void *recs, int n_ops, const int64_t *ops,
+                        void *stream);
 
 /* ---------------------------------------------------- discrete adjoint
  * (new functionality: the reference has no adjoint.)  At a frozen upstream

@@ -144,6 +144,7 @@ _SIGS = {
     "fp3d_dense_matvec": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "fp3d_dense_matvec_v": (_int, [_int, _i64, _vp, _vp, _vp, _vp]),
     "fp3d_grp_vcycle_op": (_int, [_int, _vp, _int, _i64, _int, _int, _vp]),
+    "fp3d_grp_vcycle_run": (_int, [_int, _vp, _int, _vp, _vp]),
     "fp3d_drho_tilde_dmach": (_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl,
                                      _dbl, _vp, _vp]),
     "fp3d_adjoint_dx": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -1987,6 +1987,20 @@ int fp3d_grp_vcycle_op(int nb, const void *recs, int op, int64_t max_n, int G, i
   return check_launch("grp_vcycle_op");
 }
 
+int fp3d_grp_vcycle_run(int nb, const void *recs, int n_ops, const int64_t *ops,
+                        void *stream) {
+  FP3D_REQUIRE(recs && n_ops >= 0 && (n_ops == 0 || ops), "grp_vcycle_run: bad arguments");
+  FP3D_REQUIRE(nb >= 1 && nb <= kMaxBatch, "grp_vcycle_run: nb in [1, 1024]");
+  const char *base = (const char *)recs;
+  for (int i = 0; i < n_ops; ++i) {
+    const int64_t *o = ops + 4 * (int64_t)i;
+    const int rc = fp3d_grp_vcycle_op(nb, base + (size_t)i * nb * sizeof(GrpRec), (int)o[0],
+                                      o[1], (int)o[2], (int)o[3], stream);
+    if (rc) return rc;
+  }
+  return FP3D_OK;
+}
+
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
                       void *stream) {
   FP3D_REQUIRE(n > 0 && M && b && x, "dense_matvec: bad arguments");

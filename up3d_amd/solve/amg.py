@@ -711,6 +711,10 @@ class VCycleProgram:
         self._cheb(L, b, x, 0)
 
 
+# (op, max_n, G, fmt) words of fp3d_grp_vcycle_run per program signature
+_SIGNATURE_OPS: dict = {}
+
+
 def run_grouped(programs) -> None:
     """One V-cycle per program, all of equal signature and on distinct
     hierarchies, queued as one fp3d_grp_vcycle_op launch per operation."""
@@ -725,11 +729,12 @@ def run_grouped(programs) -> None:
     host = torch.from_numpy(table.reshape(-1)).pin_memory()
     dev = torch.empty_like(host, device=p0.amg.coarse.device)
     dev.copy_(host, non_blocking=True)
-    base = ptr(dev)
-    s = stream_handle()
-    for i, (op, G, fmt, n) in enumerate(p0.signature):
-        _native.call("fp3d_grp_vcycle_op", nb, base + i * nb * 8 * _GRP_WORDS, op, n, G,
-                     fmt, s)
+    ops = _SIGNATURE_OPS.get(p0.signature)
+    if ops is None:
+        ops = _SIGNATURE_OPS[p0.signature] = np.array(
+            [(op, n, G, fmt) for op, G, fmt, n in p0.signature], dtype=np.int64).reshape(-1)
+    _native.call("fp3d_grp_vcycle_run", nb, ptr(dev), n_ops, ops.ctypes.data,
+                 stream_handle())
 
 
 def apply_preconditioners(calls) -> int:
