@@ -638,7 +638,9 @@ def run_solves(mesh, config, tune=True):
     (colouring, CSR, geometry, upload) and the autotuning of the finest mesh
     (element block size, GMRES restart, AMG degree and coarsest size, timed
     on its interpolated initial state; `tuning` in each entry), later
-    (warm) calls reuse both (the tuning from its cache)."""
+    (warm) calls reuse both (the tuning from its cache), and the location
+    of each finer mesh's nodes in the coarser mesh that the P1 transfer
+    uses (the interpolation itself runs every time)."""
     import torch
 
     from up3d_amd.mesh.channel import boundary_nodes, bump_channel, config_mesh
