@@ -129,13 +129,33 @@ class MeshStore:
         self.n_tets = len(el)
         self.nodes_host = nodes
         self.elements_host = el
-        # ---- host set-up, original element order
-        B, V = precompute_element_geometry(nodes, el)
-        off, celems, n_colors = color_partition_csr(el)
-        indptr, indices = csr_pattern(el, self.n_nodes)
-        e2c = elem_to_csr(el, indptr, indices)
-        fn, _ = build_face_adjacency(el)
-        cen = nodes[el].mean(axis=1)
+        # ---- host set-up, original element order.  The independent pieces
+        # (each a library call or a numpy pass that releases the GIL) run on
+        # worker threads beside the CSR pattern; each gives the values it
+        # gives alone
+        from concurrent.futures import ThreadPoolExecutor
+
+        with ThreadPoolExecutor(max_workers=4) as pool:
+            f_geo = pool.submit(precompute_element_geometry, nodes, el)
+            f_col = pool.submit(color_partition_csr, el)
+            f_fn = pool.submit(build_face_adjacency, el)
+            f_cen = pool.submit(lambda: nodes[el].mean(axis=1))
+            indptr, indices = csr_pattern(el, self.n_nodes)
+            e2c = elem_to_csr(el, indptr, indices)
+            B, V = f_geo.result()
+            off, celems, n_colors = f_col.result()
+            fn, _ = f_fn.result()
+            cen = f_cen.result()
+            # per-face centroid displacement (0 on the boundary), beside the
+            # incidence sort and the uploads below
+            f_dcen = pool.submit(lambda: np.ascontiguousarray(
+                np.where((fn >= 0)[:, :, None], cen[np.maximum(fn, 0)] - cen[:, None, :],
+                         0.0)).reshape(self.n_tets, 12))
+            self._build_rest(nodes, el, B, V, off, celems, n_colors, indptr, indices,
+                             e2c, fn, cen, f_dcen)
+
+    def _build_rest(self, nodes, el, B, V, off, celems, n_colors, indptr, indices, e2c,
+                    fn, cen, f_dcen):
         self.B_host, self.V_host = B, V
         self.color_offsets = np.ascontiguousarray(off, dtype=np.int64)
         self.color_elems = celems
@@ -180,11 +200,6 @@ class MeshStore:
             self.rel8 = to_dev(np.ascontiguousarray(rel_inc).reshape(-1), torch.uint8)
         self.fn = to_dev(fn.astype(np.int32), torch.int32)
         self.cen = to_dev(cen, torch.float64)
-        # per-face centroid displacement (0 on the boundary)
-        dcen = np.where((fn >= 0)[:, :, None],
-                        cen[np.maximum(fn, 0)] - cen[:, None, :], 0.0)
-        self.dcen = to_dev(np.ascontiguousarray(dcen).reshape(self.n_tets, 12),
-                           torch.float64)
         self.celems = to_dev(celems.astype(np.int32), torch.int32)
         self.nodes = to_dev(nodes, torch.float64)
         self.indptr = to_dev(indptr, torch.int32)
@@ -200,6 +215,7 @@ class MeshStore:
         inc_pos = np.empty(len(order), dtype=np.int32)
         inc_pos[order] = np.arange(len(order), dtype=np.int32)
         self.inc_pos = to_dev(inc_pos, torch.int32)
+        self.dcen = to_dev(f_dcen.result(), torch.float64)
         self.offsets_ct = self.color_offsets.ctypes.data_as(
             C.POINTER(C.c_int64))
 
