@@ -5,6 +5,7 @@
 // kernels.
 #include <algorithm>
 #include <array>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -447,39 +448,66 @@ int fp3d_host_face_adjacency(int64_t n_tets, const int32_t *elements,
   HREQ(n_tets == 0 || (elements && face_neighbors), "face_adjacency: NULL");
   static const int fd[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
   struct Face {
-    int32_t a, b, c;
+    int32_t b, c;    // the face's two larger node ids (the smallest is its bucket)
     int64_t packed;  // e*4 + f, the order the reference visits faces in
   };
-  std::vector<Face> faces((size_t)(4 * n_tets));
+  if (n_tets == 0) return FP3D_OK;
+  int64_t n_nodes = 0;
+  for (int64_t q = 0; q < 4 * n_tets; ++q) {
+    HREQ(elements[q] >= 0, "face_adjacency: negative node id");
+    if (elements[q] + 1 > n_nodes) n_nodes = elements[q] + 1;
+  }
+  // faces bucketed by their smallest node (a counting sort), then each
+  // bucket sorted by (b, c, packed): the order of one global sort by
+  // (a, b, c, packed), built in linear time and sorted bucket-parallel
+  std::vector<int64_t> start(n_nodes + 1, 0);
+  auto sorted3 = [&](int64_t e, int f, int32_t t[3]) {
+    t[0] = elements[4 * e + fd[f][0]];
+    t[1] = elements[4 * e + fd[f][1]];
+    t[2] = elements[4 * e + fd[f][2]];
+    std::sort(t, t + 3);
+  };
   for (int64_t e = 0; e < n_tets; ++e)
     for (int f = 0; f < 4; ++f) {
-      int32_t t[3] = {elements[4 * e + fd[f][0]], elements[4 * e + fd[f][1]],
-                      elements[4 * e + fd[f][2]]};
-      std::sort(t, t + 3);
-      faces[4 * e + f] = Face{t[0], t[1], t[2], 4 * e + f};
+      int32_t t[3];
+      sorted3(e, f, t);
+      ++start[t[0] + 1];
       face_neighbors[4 * e + f] = -1;
     }
-  std::sort(faces.begin(), faces.end(), [](const Face &x, const Face &y) {
-    if (x.a != y.a) return x.a < y.a;
-    if (x.b != y.b) return x.b < y.b;
-    if (x.c != y.c) return x.c < y.c;
-    return x.packed < y.packed;
-  });
+  for (int64_t v = 0; v < n_nodes; ++v) start[v + 1] += start[v];
+  // every slot is written below: no value-initialisation pass
+  std::unique_ptr<Face[]> faces(new Face[(size_t)(4 * n_tets)]);
+  {
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t e = 0; e < n_tets; ++e)
+      for (int f = 0; f < 4; ++f) {
+        int32_t t[3];
+        sorted3(e, f, t);
+        faces[fill[t[0]]++] = Face{t[1], t[2], 4 * e + f};
+      }
+  }
   // pyfp3d/mesh/metrics.py:248-276: the first owner waits in the map, the
   // next occurrence links to it and clears the entry -> consecutive pairs
-  size_t i = 0;
-  while (i < faces.size()) {
-    size_t j = i;
-    while (j < faces.size() && faces[j].a == faces[i].a &&
-           faces[j].b == faces[i].b && faces[j].c == faces[i].c)
-      ++j;
-    for (size_t k = i; k + 1 < j; k += 2) {
-      int64_t p = faces[k].packed, q = faces[k + 1].packed;
-      face_neighbors[p] = (int32_t)(q / 4);
-      face_neighbors[q] = (int32_t)(p / 4);
+  parallel_rows(n_nodes, 4 * n_tets, [&](int64_t lo, int64_t hi, unsigned) {
+    for (int64_t v = lo; v < hi; ++v) {
+      Face *f0 = faces.get() + start[v], *f1 = faces.get() + start[v + 1];
+      std::sort(f0, f1, [](const Face &x, const Face &y) {
+        if (x.b != y.b) return x.b < y.b;
+        if (x.c != y.c) return x.c < y.c;
+        return x.packed < y.packed;
+      });
+      for (Face *i = f0; i < f1;) {
+        Face *j = i;
+        while (j < f1 && j->b == i->b && j->c == i->c) ++j;
+        for (Face *k = i; k + 1 < j; k += 2) {
+          const int64_t p = k->packed, q = (k + 1)->packed;
+          face_neighbors[p] = (int32_t)(q / 4);
+          face_neighbors[q] = (int32_t)(p / 4);
+        }
+        i = j;
+      }
     }
-    i = j;
-  }
+  });
   return FP3D_OK;
 }
 
