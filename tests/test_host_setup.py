@@ -75,6 +75,17 @@ def test_csr_pattern_and_elem_to_csr_bit_identical():
     assert np.array_equal(ip, pat.indptr) and np.array_equal(ix, pat.indices)
     assert np.array_equal(elem_to_csr(el, ip, ix),
                           O.build_elem_to_csr(el, pat.indptr, pat.indices))
+    # a channel large enough for every host thread (row- and element-
+    # parallel passes), and a pattern missing one element's couplings
+    m = channel.bump_channel(24, 8, 8, jitter=0.1, seed=2, rcm=True)
+    ip2, ix2 = csr_pattern(m.elements, m.n_nodes)
+    pat2 = O.build_csr_pattern(m.elements, m.n_nodes)
+    assert np.array_equal(ip2, pat2.indptr) and np.array_equal(ix2, pat2.indices)
+    assert np.array_equal(elem_to_csr(m.elements, ip2, ix2),
+                          O.build_elem_to_csr(m.elements, pat2.indptr, pat2.indices))
+    ip3, ix3 = csr_pattern(m.elements[1:], m.n_nodes)
+    with pytest.raises(ValueError, match="missing"):
+        elem_to_csr(m.elements, ip3, ix3)
 
 
 def test_face_adjacency_identical_to_oracle():

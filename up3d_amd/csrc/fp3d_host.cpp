@@ -405,21 +405,39 @@ int fp3d_host_csr_pattern(int64_t n_tets, const int32_t *elements,
   std::vector<int64_t> off;
   std::vector<int32_t> lst;
   node_incidence(n_tets, elements, n_nodes, off, lst);
-  std::vector<int32_t> row;
-  int64_t nnz = 0;
-  indptr[0] = 0;
-  for (int64_t v = 0; v < n_nodes; ++v) {
+  // row v: the sorted, deduplicated nodes of v's incident elements.  Rows
+  // are independent: counted, then (with indices) written, row-parallel
+  auto build_row = [&](int64_t v, std::vector<int32_t> &row) {
     row.clear();
     for (int64_t k = off[v]; k < off[v + 1]; ++k)
       for (int j = 0; j < 4; ++j) row.push_back(elements[4 * (int64_t)lst[k] + j]);
     std::sort(row.begin(), row.end());
     row.erase(std::unique(row.begin(), row.end()), row.end());
-    if (indices)
-      std::memcpy(indices + nnz, row.data(), row.size() * sizeof(int32_t));
-    nnz += (int64_t)row.size();
+  };
+  const int64_t work = 16 * n_tets;
+  std::vector<int64_t> len(n_nodes);
+  parallel_rows(n_nodes, work, [&](int64_t lo, int64_t hi, unsigned) {
+    std::vector<int32_t> row;
+    for (int64_t v = lo; v < hi; ++v) {
+      build_row(v, row);
+      len[v] = (int64_t)row.size();
+    }
+  });
+  int64_t nnz = 0;
+  indptr[0] = 0;
+  for (int64_t v = 0; v < n_nodes; ++v) {
+    nnz += len[v];
     HREQ(nnz < INT32_MAX, "CSR pattern exceeds int32 nnz");
     indptr[v + 1] = (int32_t)nnz;
   }
+  if (indices)
+    parallel_rows(n_nodes, work, [&](int64_t lo, int64_t hi, unsigned) {
+      std::vector<int32_t> row;
+      for (int64_t v = lo; v < hi; ++v) {
+        build_row(v, row);
+        std::memcpy(indices + indptr[v], row.data(), row.size() * sizeof(int32_t));
+      }
+    });
   *nnz_out = nnz;
   return FP3D_OK;
 }
@@ -429,17 +447,27 @@ int fp3d_host_elem_to_csr(int64_t n_tets, const int32_t *elements,
                           int32_t *out) {
   HREQ(n_tets == 0 || (elements && indptr && indices && out),
        "elem_to_csr: NULL pointer");
-  for (int64_t e = 0; e < n_tets; ++e)
-    for (int i = 0; i < 4; ++i) {
-      int32_t r = elements[4 * e + i];
-      const int32_t *lo = indices + indptr[r], *hi = indices + indptr[r + 1];
-      for (int j = 0; j < 4; ++j) {
-        int32_t c = elements[4 * e + j];
-        const int32_t *p = std::lower_bound(lo, hi, c);
-        HREQ(p != hi && *p == c, "elem_to_csr: entry missing from pattern");
-        out[16 * e + 4 * i + j] = (int32_t)(p - indices);
+  // elements are independent: element-parallel, each thread noting an
+  // element with an entry absent from the pattern
+  std::vector<int64_t> bad(host_threads(), -1);
+  parallel_rows(n_tets, 16 * n_tets, [&](int64_t lo, int64_t hi, unsigned t) {
+    for (int64_t e = lo; e < hi; ++e)
+      for (int i = 0; i < 4; ++i) {
+        int32_t r = elements[4 * e + i];
+        const int32_t *lo_p = indices + indptr[r], *hi_p = indices + indptr[r + 1];
+        for (int j = 0; j < 4; ++j) {
+          int32_t c = elements[4 * e + j];
+          const int32_t *p = std::lower_bound(lo_p, hi_p, c);
+          if (p == hi_p || *p != c) {
+            if (bad[t] < 0) bad[t] = e;
+            out[16 * e + 4 * i + j] = -1;
+          } else {
+            out[16 * e + 4 * i + j] = (int32_t)(p - indices);
+          }
+        }
       }
-    }
+  });
+  for (int64_t b : bad) HREQ(b < 0, "elem_to_csr: entry missing from pattern");
   return FP3D_OK;
 }
 
