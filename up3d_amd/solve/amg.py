@@ -361,6 +361,10 @@ class _DevLevel:
         self.d1 = empty(n, vt)
 
 
+# side stream for V-cycle graph capture (default priority), made on first use
+_CAPTURE_STREAM = None
+
+
 class AMGHierarchy:
     """Device V-cycle of a host-built SA hierarchy (module docstring).
 
@@ -495,9 +499,25 @@ class AMGHierarchy:
         return x
 
     def _capture(self, b: torch.Tensor, x: torch.Tensor):
+        """The V-cycle from b into x captured as a graph, on a side stream of
+        default priority (capture needs a non-default stream).  Not through
+        torch.cuda.graph, whose set-up collects Python garbage and empties
+        the allocator cache before every capture (about a millisecond each,
+        hundreds of captures in a sweep): the V-cycle allocates nothing."""
+        global _CAPTURE_STREAM
+        if _CAPTURE_STREAM is None:
+            _CAPTURE_STREAM = torch.cuda.Stream()
+        side = _CAPTURE_STREAM
+        cur = torch.cuda.current_stream()
+        side.wait_stream(cur)
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self._vcycle_top(b, x)
+        with torch.cuda.stream(side):
+            g.capture_begin()
+            try:
+                self._vcycle_top(b, x)
+            finally:
+                g.capture_end()
+        cur.wait_stream(side)
         return g
 
     @staticmethod

@@ -100,7 +100,9 @@ def gmres_buffers(n: int, restart: int, f32: bool, device, V=None) -> dict:
     return {"V": V, "v64": empty(n) if f32 else None, "tmp": empty(n), "r": empty(n),
             "w": [empty(n), empty(n)], "outs": [empty(nout), empty(nout)],
             "hbuf": list(hbuf), "evs": list(evs), "hbuf_own": hbuf, "evs_own": evs,
-            "red": _Red(restart + 1)}
+            "red": _Red(restart + 1),
+            "ybuf": torch.empty(restart + 1, dtype=torch.float64, pin_memory=True),
+            "ydev": empty(restart + 1)}
 
 
 def _gmres_workspace(n: int, restart: int, f32: bool, device):
@@ -480,7 +482,13 @@ def gmres_steps(matvec: Callable, b: torch.Tensor, psolve: Optional[Callable] = 
                 y[:k2] -= y[k2] * h[k2, :k2]
         if y[0] != 0.0:
             y[0] /= h[0, 0]
-        yd = to_dev(y, torch.float64)
+        # the solution coefficients through the workspace's pinned buffer
+        # (rewritten only at the next restart, after the residual norm below
+        # has synchronised the stream)
+        yh = gw["ybuf"][: col + 1]
+        yh.numpy()[:] = y
+        yd = gw["ydev"][: col + 1]
+        yd.copy_(yh, non_blocking=True)
         _native.call("fp3d_multi_axpy_f32v" if f32 else "fp3d_multi_axpy", n, col + 1,
                      ptr(V), V.stride(0), ptr(yd), 1.0, ptr(x), stream_handle())
         matvec(x, tmp)
