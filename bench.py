@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--sweep-all-meshes", action="store_true",
                     help="batched sweep: advance the chains of all four meshes together "
                          "instead of one mesh after another")
+    ap.add_argument("--sweep-mesh-ids", default="0,1,2,3",
+                    help="sweep: run only the conditions of these meshes (indices 0-3)")
     ap.add_argument("--sweep-chains", default="4,16,16,16",
                     help="warm-start chains per mesh of the batched sweep: one count, "
                          "or one per mesh separated by commas")
@@ -170,7 +172,9 @@ def main():
                 if mode == "batched" and args.sweep_all_meshes:
                     key += "/all-meshes"
                 out[key] = run_sweep(mode, _chains(ch), meshes, coarse=not args.no_coarse,
-                                     by_mesh=not args.sweep_all_meshes)
+                                     by_mesh=not args.sweep_all_meshes,
+                                     mesh_ids=tuple(int(x) for x in
+                                                    args.sweep_mesh_ids.split(",")))
                 log(f"[bench] sweep {key}: wall {out[key]['wall_s']} s, "
                     f"{out[key]['n_converged']} converged, executor {out[key]['executor']}")
         print(json.dumps(out), flush=True)
@@ -991,7 +995,7 @@ def _chains(text: str):
 
 
 def run_sweep(mode: str = "batched", chains=(4, 16, 16, 16), meshes=None, coarse: bool = True,
-              by_mesh: bool = True):
+              by_mesh: bool = True, mesh_ids=(0, 1, 2, 3)):
     """Sweep of 16 seeded (M_inf in [0.60, 0.86], alpha in [0, 3] deg)
     conditions on each of 4 jittered RCM bump channels (144,000 / 589,824 /
     1,152,000 / 2,387,718 tets), FP32-preconditioned Newton, every
@@ -1002,7 +1006,8 @@ def run_sweep(mode: str = "batched", chains=(4, 16, 16, 16), meshes=None, coarse
     together; "sequential": one condition after another (solve/sweep.py).
     coarse: conditions on the three meshes of at least 500,000 tets start
     from a solve on a 73,728-tet companion channel (solve/sweep.py
-    coarse_route), which makes them independent of their chain."""
+    coarse_route), which makes them independent of their chain.
+    mesh_ids: run only the conditions of these meshes."""
     import torch
 
     from up3d_amd.solve.batch import solve_sweep_batched
@@ -1013,7 +1018,7 @@ def run_sweep(mode: str = "batched", chains=(4, 16, 16, 16), meshes=None, coarse
     if meshes is None:
         meshes = sweep_meshes()
     mesh_s = time.perf_counter() - t0
-    conds = sweep_conditions(len(meshes), 16)
+    conds = [c for c in sweep_conditions(len(meshes), 16) if c[0] in set(mesh_ids)]
     comp = companion_mesh() if coarse else None
     if mode == "batched":
         r = solve_sweep_batched(meshes, conds, chains_per_mesh=chains, log=log, coarse=comp,
@@ -1027,7 +1032,9 @@ def run_sweep(mode: str = "batched", chains=(4, 16, 16, 16), meshes=None, coarse
     tet_steps = sum(i["n_tets"] * i["n_newton"] for i in inst)
     return {
         "workload": "64 conditions: M_inf U[0.60,0.86], alpha U[0,3] deg, 16 on each of "
-                    "4 jittered bump channels (144,000 / 589,824 / 1,152,000 / 2,387,718 tets)",
+                    "4 jittered bump channels (144,000 / 589,824 / 1,152,000 / 2,387,718 tets)"
+                    + ("" if len(set(mesh_ids)) == 4 else
+                       f"; only the conditions of meshes {sorted(set(mesh_ids))}"),
         "mode": mode, "chains_per_mesh": chains if mode == "batched" else None,
         "mesh_by_mesh": bool(by_mesh) if mode == "batched" else None,
         "coarse_route": bool(coarse),
