@@ -517,7 +517,7 @@ int fp3d_amg_restrict(int64_t nc, int64_t nnz, const int32_t *indptr,
 int fp3d_amg_prolong_add(int64_t n, int64_t nnz, const int32_t *indptr,
                          const void *indices, const void *data, int fmt,
                          const void *xc, void *x, void *stream);
-/* x = M b, M dense row-major n x n (the coarsest-level pseudo-inverse) */
+/* x = M b, M dense row-major n x n (the coarsest level's dense inverse) */
 int fp3d_dense_matvec(int64_t n, const double *M, const double *b, double *x,
                       void *stream);
 /* the same with float32 b and x when fmt bit 2 is set (M and the sums stay

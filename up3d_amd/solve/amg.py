@@ -14,7 +14,8 @@ sequential Gauss-Seidel smoother by Chebyshev polynomial smoothing on
 D^-1 A, which is fully parallel on the GPU and keeps the V-cycle symmetric
 (same polynomial before and after the coarse correction), so it remains a
 valid CG preconditioner.  The coarsest level is applied as a dense
-float64 pseudo-inverse matvec.  Matrix values can be held in float32
+float64 inverse matvec (through a Cholesky factor for SPD operators, the
+pseudo-inverse otherwise).  Matrix values can be held in float32
 (precision="f32"); vectors and accumulation stay float64.
 """
 
