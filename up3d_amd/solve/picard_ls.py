@@ -184,8 +184,17 @@ class ExtendedSystem:
                 _native.call("fp3d_csr_spmv", C.n, C.nnz, ptr(C.indptr),
                              ptr(C.indices), ptr(C.data), ptr(z[:nm]), ptr(t),
                              stream_handle())
-            rhs = to_host(r[nm:]) - to_host(t)
-            z[nm:].copy_(to_dev(self.aux_lu.solve(rhs)))
+            # one read-back of r_aux - t, the exact aux solve on the host,
+            # and the solution back through a pinned staging buffer (reused
+            # only after the next read-back has synchronised the stream)
+            t.mul_(-1.0).add_(r[nm:])
+            rhs = to_host(t)
+            stage = getattr(self, "_aux_stage", None)
+            if stage is None:
+                stage = self._aux_stage = torch.empty(len(rhs), dtype=torch.float64,
+                                                      pin_memory=True)
+            stage.numpy()[:] = self.aux_lu.solve(rhs)
+            z[nm:].copy_(stage, non_blocking=True)
         return z
 
 
