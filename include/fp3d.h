@@ -710,6 +710,13 @@ int fp3d_host_locate_points(int64_t n_tets, const int32_t *elements,
                             int64_t n_pts, const double *pts,
                             int32_t *out_elem, double *out_bary);
 
+/* Shortest incident edge per node (pyfp3d/wake/cut_elements.py's node
+ * length scale for the level-set tolerances): h[v] = min over the six
+ * edges (i, j) of every tet containing v of |x_i - x_j|, the length summed
+ * as (dx^2 + dy^2) + dz^2 then square-rooted; +inf for a node in no tet. */
+int fp3d_host_node_min_edge_length(int64_t n_tets, const int32_t *elements,
+                                   int64_t n_nodes, const double *nodes, double *h);
+
 /* Threads per block of the element-wise kernels (state, walk, upwind,
  * sensitivity and element-term sweeps): 64, 128, 192 or 256, process-wide
  * (no reference counterpart; set by up3d_amd.tune). */

@@ -89,3 +89,25 @@ def test_farfield_jump_across_the_cut():
     assert pairs
     for s, ms in pairs:
         assert abs((val[s] - val[ms]) - 0.2) < 1e-12
+
+
+def test_node_min_edge_length_equals_numpy_formula():
+    """The library's shortest incident edge per node equals the numpy
+    restatement (edge norms, minimum.at) bit for bit; a node in no tet
+    keeps +inf."""
+    import numpy as np
+
+    from up3d_amd.mesh.channel import bump_channel
+    from up3d_amd.wake.cut_elements import _EDGES, node_min_edge_length
+
+    m = bump_channel(10, 4, 4, jitter=0.2, seed=7, rcm=True)
+    nodes = np.vstack([m.nodes, [[9.0, 9.0, 9.0]]])          # an unused node
+    el = np.asarray(m.elements, dtype=np.int64)
+    i, j = el[:, _EDGES[:, 0]].ravel(), el[:, _EDGES[:, 1]].ravel()
+    L = np.linalg.norm(nodes[i] - nodes[j], axis=1)
+    ref = np.full(len(nodes), np.inf)
+    np.minimum.at(ref, i, L)
+    np.minimum.at(ref, j, L)
+    got = node_min_edge_length(nodes, m.elements)
+    assert np.array_equal(got, ref)
+    assert np.isinf(got[-1])

@@ -172,6 +172,7 @@ _SIGS = {
     "fp3d_host_face_adjacency": (_int, [_i64, _vp, _vp]),
     "fp3d_host_element_geometry": (_int, [_i64, _vp, _vp, _vp, _vp, _pi64]),
     "fp3d_host_locate_points": (_int, [_i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "fp3d_host_node_min_edge_length": (_int, [_i64, _vp, _i64, _vp, _vp]),
     "fp3d_host_standard_aggregation": (_int, [_i64, _vp, _vp, _vp, _vp, _pi64]),
 }
 

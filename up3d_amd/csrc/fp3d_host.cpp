@@ -267,6 +267,26 @@ int fp3d_host_apply_givens(int col, double *hrow, const double *givens) {
   return FP3D_OK;
 }
 
+int fp3d_host_node_min_edge_length(int64_t n_tets, const int32_t *elements,
+                                   int64_t n_nodes, const double *nodes, double *h) {
+  int rc = check_elements(n_tets, elements, n_nodes);
+  if (rc) return rc;
+  HREQ(n_nodes == 0 || (nodes && h), "node_min_edge_length: NULL pointer");
+  static const int ed[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  for (int64_t v = 0; v < n_nodes; ++v) h[v] = INFINITY;
+  for (int64_t e = 0; e < n_tets; ++e)
+    for (int k = 0; k < 6; ++k) {
+      const int32_t i = elements[4 * e + ed[k][0]], j = elements[4 * e + ed[k][1]];
+      const double dx = nodes[3 * (int64_t)i] - nodes[3 * (int64_t)j];
+      const double dy = nodes[3 * (int64_t)i + 1] - nodes[3 * (int64_t)j + 1];
+      const double dz = nodes[3 * (int64_t)i + 2] - nodes[3 * (int64_t)j + 2];
+      const double L = std::sqrt((dx * dx + dy * dy) + dz * dz);
+      if (L < h[i]) h[i] = L;
+      if (L < h[j]) h[j] = L;
+    }
+  return FP3D_OK;
+}
+
 /* Smoothed-aggregation prolongator P = T - diag(s) (A T) in two passes
  * (row pointer and count, then indices and data), row-parallel. */
 int fp3d_host_sa_prolongator(int64_t n, const int32_t *Ap, const int32_t *Aj,

@@ -31,13 +31,16 @@ _EDGES = np.array([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)],
 
 
 def node_min_edge_length(nodes, elements) -> np.ndarray:
-    el = np.asarray(elements, dtype=np.int64)
-    i = el[:, _EDGES[:, 0]].ravel()
-    j = el[:, _EDGES[:, 1]].ravel()
-    L = np.linalg.norm(nodes[i] - nodes[j], axis=1)
-    h = np.full(len(nodes), np.inf)
-    np.minimum.at(h, i, L)
-    np.minimum.at(h, j, L)
+    """Shortest incident edge per node (+inf for a node in no tet), in the
+    library's host code: one pass over the tets' six edges."""
+    from .. import _native
+
+    nd = np.ascontiguousarray(nodes, dtype=np.float64)
+    el = np.ascontiguousarray(elements, dtype=np.int32)
+    h = np.empty(len(nd))
+    _native.check(_native.lib().fp3d_host_node_min_edge_length(
+        len(el), el.ctypes.data, len(nd), nd.ctypes.data, h.ctypes.data),
+        "node_min_edge_length")
     return h
 
 
